@@ -1,0 +1,312 @@
+"""ctypes view of the checker libraries (TEST INFRASTRUCTURE ONLY).
+
+``Oracle`` wraps liboracle.so, the CPU restatement of XPIR's NFLLWE reply path
+(xpir_oracle.c).  ``RefNFL`` wraps oracle/_ref/libxpir_ref.so, the reference's own
+NFLlib compiled from its sources by oracle/Makefile (absent where the reference
+tree is not available; tests that need it skip).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libxpir_ref.so")
+
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class XoParams(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("n", C.c_uint32), ("logn", C.c_uint32),
+                ("m", C.c_uint32), ("bits", C.c_uint32), ("Berr", C.c_uint32),
+                ("A", C.c_int32), ("p", C.c_uint64 * 4)]
+
+
+class XoTables(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("m", C.c_uint32)] + [
+        (nm, C.POINTER(C.c_uint64)) for nm in ("phi", "phiq", "w", "wq", "iw", "iwq", "ipt", "iptq")]
+
+
+def build():
+    """Compile liboracle.so (and the reference build when the reference tree exists)."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+    if os.path.isdir("/root/reference/crypto"):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+
+
+def _load(path):
+    return C.CDLL(path)
+
+
+class Oracle:
+    """Parameters + tables for one cryptosystem string and n0, with numpy helpers."""
+
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            if not os.path.exists(LIB):
+                build()
+            L = _load(LIB)
+            L.xo_params_init.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(XoParams)]
+            L.xo_tables_init.argtypes = [C.POINTER(XoParams), C.POINTER(XoTables)]
+            L.xo_tables_free.argtypes = [C.POINTER(XoTables)]
+            for f in ("xo_mulmod", "xo_shoup_quot"):
+                getattr(L, f).restype = C.c_uint64
+            L.xo_mulmod.argtypes = [C.c_uint64] * 3
+            L.xo_shoup_quot.argtypes = [C.c_uint64] * 2
+            L.xo_mulmod_shoup.restype = C.c_uint64
+            L.xo_mulmod_shoup.argtypes = [C.c_uint64] * 4
+            L.xo_mac_step.restype = C.c_uint64
+            L.xo_mac_step.argtypes = [C.c_uint64] * 5
+            L.xo_modulus.restype = C.c_uint64
+            L.xo_modulus.argtypes = [C.c_uint32]
+            L.xo_root.restype = C.c_uint64
+            L.xo_root.argtypes = [C.c_uint32]
+            L.xo_inv8192.restype = C.c_uint64
+            L.xo_inv8192.argtypes = [C.c_uint32]
+            L.xo_ntt.argtypes = [u64p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint32, C.c_uint64]
+            L.xo_ntt_powphi.argtypes = [C.POINTER(XoTables), C.POINTER(XoParams), u64p]
+            L.xo_invntt_powinvphi.argtypes = [C.POINTER(XoTables), C.POINTER(XoParams), u64p]
+            L.xo_absorb_polys.restype = C.c_uint64
+            L.xo_absorb_polys.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
+            L.xo_absorb.argtypes = [u8p, C.c_uint64, C.POINTER(XoParams), u64p]
+            L.xo_import.argtypes = [C.POINTER(XoTables), C.POINTER(XoParams), u8p, C.c_uint64, u64p]
+            L.xo_reply_level.argtypes = [C.POINTER(XoParams), u64p, u64p, C.c_uint64, C.c_uint64,
+                                         C.c_uint32, u64p, C.c_int]
+            L.xo_recursion.restype = C.c_uint32
+            L.xo_recursion.argtypes = [C.POINTER(XoTables), C.POINTER(XoParams), u64p, C.c_uint64,
+                                       C.c_uint32, u64p]
+            L.xo_next_polys.restype = C.c_uint32
+            L.xo_next_polys.argtypes = [C.POINTER(XoParams), C.c_uint32]
+            L.xo_rand_word.restype = C.c_uint64
+            L.xo_rand_word.argtypes = [C.c_uint64, C.c_uint64]
+            L.xo_rand_bytes.argtypes = [C.c_uint64, C.c_uint64, u8p, C.c_uint64]
+            L.xo_keygen.argtypes = [C.POINTER(XoParams), C.c_uint64, u64p]
+            L.xo_encrypt_bit.argtypes = [C.POINTER(XoTables), C.POINTER(XoParams), u64p, C.c_uint32,
+                                         C.c_uint64, C.c_uint64, u64p]
+            L.xo_decrypt.restype = C.c_int
+            L.xo_decrypt.argtypes = [C.POINTER(XoTables), C.POINTER(XoParams), u64p, u64p, u8p]
+            cls._lib = L
+        return cls._lib
+
+    def __init__(self, desc: str, n0: int):
+        L = self.lib()
+        self.prm = XoParams()
+        rc = L.xo_params_init(desc.encode(), n0, C.byref(self.prm))
+        if rc != 0:
+            raise ValueError(f"bad cryptosystem {desc!r} (n0={n0}): {rc}")
+        self.tab = XoTables()
+        if L.xo_tables_init(C.byref(self.prm), C.byref(self.tab)) != 0:
+            raise MemoryError("tables")
+        self.n, self.m, self.A = self.prm.n, self.prm.m, self.prm.A
+        self.p = [self.prm.p[i] for i in range(self.m)]
+        self.mn = self.n * self.m
+
+    def __del__(self):
+        try:
+            self.lib().xo_tables_free(C.byref(self.tab))
+        except Exception:
+            pass
+
+    # --- sizes ---
+    def polys(self, nbytes: int) -> int:
+        return int(self.lib().xo_absorb_polys(nbytes, self.A, self.n))
+
+    def next_polys(self, P: int) -> int:
+        return int(self.lib().xo_next_polys(C.byref(self.prm), P))
+
+    # --- kernels ---
+    def absorb(self, data: np.ndarray) -> np.ndarray:
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        out = np.zeros(self.polys(data.size) * self.mn, dtype=np.uint64)
+        self.lib().xo_absorb(data, data.size, C.byref(self.prm), out)
+        return out
+
+    def import_element(self, data: np.ndarray) -> np.ndarray:
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        out = np.zeros(self.polys(data.size) * self.mn, dtype=np.uint64)
+        self.lib().xo_import(C.byref(self.tab), C.byref(self.prm), data, data.size, out)
+        return out
+
+    def ntt_powphi(self, poly: np.ndarray) -> np.ndarray:
+        x = np.array(poly, dtype=np.uint64, copy=True)
+        self.lib().xo_ntt_powphi(C.byref(self.tab), C.byref(self.prm), x)
+        return x
+
+    def invntt(self, poly: np.ndarray) -> np.ndarray:
+        x = np.array(poly, dtype=np.uint64, copy=True)
+        self.lib().xo_invntt_powinvphi(C.byref(self.tab), C.byref(self.prm), x)
+        return x
+
+    def tables(self, name: str) -> np.ndarray:
+        ptr = getattr(self.tab, name)
+        return np.ctypeslib.as_array(ptr, shape=(self.mn,)).copy()
+
+    def reply_level(self, db: np.ndarray, q: np.ndarray, n_l: int, groups: int, P: int,
+                    threads: int = 0) -> np.ndarray:
+        db = np.ascontiguousarray(db, dtype=np.uint64)
+        q = np.ascontiguousarray(q, dtype=np.uint64)
+        assert db.size == groups * n_l * P * self.mn and q.size == n_l * 2 * self.mn
+        out = np.zeros(groups * P * 2 * self.mn, dtype=np.uint64)
+        self.lib().xo_reply_level(C.byref(self.prm), db, q, n_l, groups, P, out, threads)
+        return out
+
+    def recursion(self, reply: np.ndarray, groups: int, P: int) -> tuple[np.ndarray, int]:
+        reply = np.ascontiguousarray(reply, dtype=np.uint64)
+        P2 = self.next_polys(P)
+        out = np.zeros(groups * P2 * self.mn, dtype=np.uint64)
+        self.lib().xo_recursion(C.byref(self.tab), C.byref(self.prm), reply, groups, P, out)
+        return out, P2
+
+    # --- seeded stream + client side ---
+    def rand_bytes(self, seed: int, offset: int, nbytes: int) -> np.ndarray:
+        out = np.zeros(nbytes, dtype=np.uint8)
+        self.lib().xo_rand_bytes(seed, offset, out, nbytes)
+        return out
+
+    def keygen(self, seed: int) -> np.ndarray:
+        sk = np.zeros(self.mn, dtype=np.uint64)
+        self.lib().xo_keygen(C.byref(self.prm), seed, sk)
+        return sk
+
+    def encrypt_bit(self, sk: np.ndarray, bit: int, noise_seed: int, elem: int) -> np.ndarray:
+        out = np.zeros(2 * self.mn, dtype=np.uint64)
+        self.lib().xo_encrypt_bit(C.byref(self.tab), C.byref(self.prm), sk, bit, noise_seed, elem, out)
+        return out
+
+    def decrypt(self, sk: np.ndarray, ct: np.ndarray) -> np.ndarray:
+        out = np.zeros(self.A * self.n // 8, dtype=np.uint8)
+        rc = self.lib().xo_decrypt(C.byref(self.tab), C.byref(self.prm), sk,
+                                   np.ascontiguousarray(ct, dtype=np.uint64), out)
+        if rc != 0:
+            raise ValueError("decrypt")
+        return out
+
+    # --- whole query / reply (test helpers) ---
+    def make_query(self, sk, dims, index, noise_seed):
+        """Query for `index` over the grid `dims`: per level, dims[l] encryptions of 0/1
+        (PIRQueryGenerator_internal.cpp:37-70).  Element numbering is global."""
+        coords, x = [], index
+        for nl in dims:
+            coords.append(x % nl)
+            x //= nl
+        levels, elem = [], 0
+        for lvl, nl in enumerate(dims):
+            q = np.zeros((nl, 2 * self.mn), dtype=np.uint64)
+            for j in range(nl):
+                q[j] = self.encrypt_bit(sk, 1 if j == coords[lvl] else 0, noise_seed, elem)
+                elem += 1
+            levels.append(q.reshape(-1))
+        return levels
+
+    def generate_reply(self, db_words: np.ndarray, P0: int, dims, query_levels, threads: int = 0):
+        """Full d-level reply (PIRReplyGeneratorNFL_internal.cpp:340-445)."""
+        data, P = db_words, P0
+        for lvl, nl in enumerate(dims):
+            groups = int(np.prod(dims[lvl + 1:], dtype=np.int64)) if lvl + 1 < len(dims) else 1
+            rep = self.reply_level(data, query_levels[lvl], nl, groups, P, threads)
+            if lvl + 1 < len(dims):
+                data, P = self.recursion(rep, groups, P)
+            else:
+                return rep, P
+
+
+def extract_reply(orc: Oracle, sk, reply: np.ndarray, dims, elem_bytes: int) -> bytes:
+    """Decrypt a d-level reply back to the element bytes.  Level-d ciphertexts decrypt
+    to the bytes of level-(d-1) ciphertexts, and so on down to the element."""
+    ct_words = 2 * orc.mn
+    ct_bytes = ct_words * 8
+    abs_bytes = orc.A * orc.n // 8
+    d = len(dims)
+    # ciphertext count per level, bottom up (computeReplySizeInChunks)
+    counts = [-(-elem_bytes // abs_bytes)]
+    for _ in range(1, d):
+        counts.append(-(-counts[-1] * ct_bytes // abs_bytes))
+    data = reply.reshape(-1, ct_words)
+    for lvl in range(d - 1, -1, -1):
+        assert data.shape[0] == counts[lvl], (data.shape, counts, lvl)
+        plain = b"".join(orc.decrypt(sk, c).tobytes() for c in data)
+        if lvl == 0:
+            return plain[:elem_bytes]
+        need = counts[lvl - 1] * ct_bytes
+        data = np.frombuffer(plain[:need], dtype=np.uint64).reshape(-1, ct_words)
+    raise AssertionError("unreachable")
+
+
+class RefNFL:
+    """The reference's own NFLlib (oracle/_ref/libxpir_ref.so) for one (n, bits)."""
+
+    _lib = None
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.exists(REF_LIB)
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            L = _load(REF_LIB)
+            L.xr_create.restype = C.c_void_p
+            L.xr_create.argtypes = [C.c_uint, C.c_uint]
+            L.xr_destroy.argtypes = [C.c_void_p]
+            L.xr_import.restype = C.c_int64
+            L.xr_import.argtypes = [C.c_void_p, C.c_uint, u8p, C.c_uint64, C.c_uint64, u64p, C.c_uint64]
+            L.xr_ntt_powphi.argtypes = [C.c_void_p, u64p]
+            L.xr_invntt_powinvphi.argtypes = [C.c_void_p, u64p]
+            L.xr_mulandadd_shoup.restype = C.c_uint64
+            L.xr_mulandadd_shoup.argtypes = [C.c_uint64] * 5
+            L.xr_mulmod_shoup.restype = C.c_uint64
+            L.xr_mulmod_shoup.argtypes = [C.c_uint64] * 4
+            L.xr_modulus.restype = C.c_uint64
+            L.xr_modulus.argtypes = [C.c_void_p, C.c_uint]
+            L.xr_reply_level.argtypes = [C.c_void_p, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint,
+                                         u64p, C.c_int]
+            cls._lib = L
+        return cls._lib
+
+    def __init__(self, n: int, bits: int):
+        self.h = self.lib().xr_create(n, bits)
+        self.n, self.m = n, bits // 60
+        self.mn = n * self.m
+
+    def __del__(self):
+        try:
+            self.lib().xr_destroy(self.h)
+        except Exception:
+            pass
+
+    def import_buffers(self, data: np.ndarray, nbufs: int, A: int) -> np.ndarray:
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        assert data.size % nbufs == 0
+        per = data.size // nbufs
+        cap = (-(-data.size * 8 // (A * self.n)) + 1) * self.mn
+        out = np.zeros(cap, dtype=np.uint64)
+        P = self.lib().xr_import(self.h, A, data, nbufs, per, out, cap)
+        assert P >= 0
+        return out[:P * self.mn]
+
+    def ntt_powphi(self, poly):
+        x = np.array(poly, dtype=np.uint64, copy=True)
+        self.lib().xr_ntt_powphi(self.h, x)
+        return x
+
+    def invntt(self, poly):
+        x = np.array(poly, dtype=np.uint64, copy=True)
+        self.lib().xr_invntt_powinvphi(self.h, x)
+        return x
+
+    def reply_level(self, db, q, n_l, groups, P, threads=1):
+        db = np.ascontiguousarray(db, dtype=np.uint64)
+        q = np.ascontiguousarray(q, dtype=np.uint64)
+        out = np.zeros(groups * P * 2 * self.mn, dtype=np.uint64)
+        self.lib().xr_reply_level(self.h, db, q, n_l, groups, P, out, threads)
+        return out

@@ -1,0 +1,114 @@
+// ref_driver.cpp -- thin C entry points over the REFERENCE's own NFLlib, compiled from
+// the unmodified sources under the reference tree by oracle/Makefile into
+// oracle/_ref/libxpir_ref.so.  TEST INFRASTRUCTURE ONLY (the checker).
+//
+// The arithmetic all comes from the reference (crypto/NFLlib.hpp / NFLlib.cpp):
+//   NFLlib::setNewParameters, deserializeDataNFL (bitsplitter + nttAndPowPhi),
+//   nttAndPowPhi, invnttAndPowInvPhi, mulandaddShoup.
+// The reference's NFLLWE.cpp and reply generator need Boost, which the image lacks,
+// so the two loops they wrap are driven here in the reference's own order:
+//   NFLLWE::mulandadd (crypto/NFLLWE.cpp:176-205): per modulus, the a chain over all
+//   slots, then the b chain; and the element loop of
+//   PIRReplyGeneratorNFL_internal::generateReply (PIRReplyGeneratorNFL_internal.cpp:
+//   531-569): per poly (OpenMP), elements ascending.
+#include "NFLlib.hpp"
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+struct RefHandle {
+  NFLlib nfl;
+  unsigned n;
+  unsigned m;
+};
+
+extern "C" {
+
+void *xr_create(unsigned n, unsigned bits) {
+  RefHandle *h = new RefHandle();
+  h->nfl.setNewParameters(n, bits);
+  h->n = n;
+  h->m = h->nfl.getnbModuli();
+  return h;
+}
+
+void xr_destroy(void *hp) { delete static_cast<RefHandle *>(hp); }
+
+// deserializeDataNFL over `nbufs` buffers of `bytes_per_buf` bytes each, stored back to
+// back in `data`.  Each buffer gets its own allocation with slack, because the
+// reference reads whole 64-bit words at buffer ends.  Returns the poly count, or -1 if
+// `out` (cap in words) is too small.
+int64_t xr_import(void *hp, unsigned A, const uint8_t *data, uint64_t nbufs,
+                  uint64_t bytes_per_buf, uint64_t *out, uint64_t cap_words) {
+  RefHandle *h = static_cast<RefHandle *>(hp);
+  std::vector<unsigned char *> bufs(nbufs);
+  for (uint64_t i = 0; i < nbufs; i++) {
+    bufs[i] = static_cast<unsigned char *>(calloc(bytes_per_buf + 64, 1));
+    memcpy(bufs[i], data + i * bytes_per_buf, bytes_per_buf);
+  }
+  uint64_t polys = 0;
+  poly64 *res = h->nfl.deserializeDataNFL(bufs.data(), nbufs, bytes_per_buf * 8, A, polys);
+  int64_t ret = (int64_t)polys;
+  uint64_t words = polys * h->m * h->n;
+  if (words > cap_words) ret = -1;
+  else memcpy(out, res[0], words * sizeof(uint64_t));
+  free(res[0]);
+  free(res);
+  for (auto b : bufs) free(b);
+  return ret;
+}
+
+void xr_ntt_powphi(void *hp, uint64_t *poly) { static_cast<RefHandle *>(hp)->nfl.nttAndPowPhi(poly); }
+
+void xr_invntt_powinvphi(void *hp, uint64_t *poly) {
+  static_cast<RefHandle *>(hp)->nfl.invnttAndPowInvPhi(poly);
+}
+
+uint64_t xr_mulandadd_shoup(uint64_t rop, uint64_t x, uint64_t y, uint64_t yq, uint64_t p) {
+  NFLlib::mulandaddShoup(rop, x, y, yq, p);
+  return rop;
+}
+
+uint64_t xr_mulmod_shoup(uint64_t x, uint64_t y, uint64_t yq, uint64_t p) {
+  return NFLlib::mulmodShoup(x, y, yq, p);
+}
+
+uint64_t xr_modulus(void *hp, unsigned i) { return static_cast<RefHandle *>(hp)->nfl.getmoduli()[i]; }
+
+// One reply level.  db: (groups*n_l) x P x m x n; q: n_l x {a,b} x m x n (canonical);
+// out: groups x P x {a,b} x m x n.  Shoup quotients as pushQuery computes them.
+void xr_reply_level(void *hp, const uint64_t *db, const uint64_t *q, uint64_t n_l,
+                    uint64_t groups, unsigned P, uint64_t *out, int threads) {
+  RefHandle *h = static_cast<RefHandle *>(hp);
+  const uint64_t n = h->n, m = h->m, mn = n * m;
+  const uint64_t *moduli = h->nfl.getmoduli();
+  std::vector<uint64_t> qq(n_l * 2 * mn);
+  for (uint64_t e = 0; e < n_l; e++)
+    for (uint64_t c = 0; c < m; c++)
+      for (uint64_t s = 0; s < n; s++)
+        for (int ab = 0; ab < 2; ab++) {
+          uint64_t idx = (e * 2 + ab) * mn + c * n + s;
+          qq[idx] = ((uint128_t)q[idx] << 64) / moduli[c];
+        }
+  if (threads <= 0) threads = 1;
+  for (uint64_t g = 0; g < groups; g++) {
+#pragma omp parallel for num_threads(threads)
+    for (unsigned k = 0; k < P; k++) {
+      uint64_t *ra = out + (g * P + k) * 2 * mn, *rb = ra + mn;
+      memset(ra, 0, 2 * mn * sizeof(uint64_t));
+      for (uint64_t e = 0; e < n_l; e++) {
+        const uint64_t *x = db + ((g * n_l + e) * P + k) * mn;
+        const uint64_t *ya = q + e * 2 * mn, *yb = ya + mn;
+        const uint64_t *yaq = qq.data() + e * 2 * mn, *ybq = yaq + mn;
+        for (uint64_t c = 0; c < m; c++) {
+          for (uint64_t s = 0; s < n; s++)
+            NFLlib::mulandaddShoup(ra[c * n + s], x[c * n + s], ya[c * n + s], yaq[c * n + s], moduli[c]);
+          for (uint64_t s = 0; s < n; s++)
+            NFLlib::mulandaddShoup(rb[c * n + s], x[c * n + s], yb[c * n + s], ybq[c * n + s], moduli[c]);
+        }
+      }
+    }
+  }
+}
+
+}  // extern "C"
