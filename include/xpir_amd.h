@@ -1,0 +1,146 @@
+/*
+ * xpir_amd.h -- C ABI of the MI355X NFLLWE PIR reply engine (libxpir_amd.so).
+ *
+ * One shared library, plain pointers and sizes, integer status codes, opaque handles.
+ * Buffers are host memory unless the name ends in _dev (device memory on the
+ * context's GPU).  Word layouts are the reference's:
+ *   poly         m blocks of n uint64 (crypto/NFLlib.hpp:72, NFLLWEDatatypes.hpp:20-36)
+ *   ciphertext   a[m][n] || b[m][n], 16*n*m bytes (NFLLWEPublicParameters.cpp:148-151)
+ *   query level  dims[l] ciphertexts, canonical words (NFLLWE.cpp:355-356)
+ *   NTT DB       units x P x m x n lazy words in [0,2p) (NFLlib.cpp:375-401)
+ *   reply        P_d ciphertexts, lazy words (PIRReplyGeneratorNFL_internal.cpp:570-574)
+ * Every reply is bit-identical to the reference's generateReply on the same inputs.
+ *
+ * Threading: one in-flight call per context; contexts are independent.  The serving
+ * loop (xpg_server_*) owns its own concurrency.
+ */
+#ifndef XPIR_AMD_H
+#define XPIR_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference prints or exit()s; this ABI returns) ---- */
+#define XPG_OK 0
+#define XPG_EINVAL -1      /* malformed argument / unsupported parameters */
+#define XPG_ENOMEM -2      /* device or host allocation failed             */
+#define XPG_EDEVICE -3     /* HIP runtime error                            */
+#define XPG_ESIZE -4       /* buffer too small / size mismatch             */
+#define XPG_EBUSY -5       /* admission rejected (serving loop full)       */
+
+#define XPG_MAX_LEVELS 10  /* PIRParameters.n[MAX_REC_LVL], pir/PIRParameters.hpp:22 */
+
+typedef struct xpg_ctx xpg_ctx;
+typedef struct xpg_db xpg_db;
+
+/* Cryptosystem parameters for one "LWE:k:n:bits" string and first dimension n0.
+ * Replaces HomomorphicCryptoFactory::getCryptoMethod (pir/libpir.cpp:20-23) ->
+ * NFLLWE::setNewParameters (crypto/NFLLWE.cpp:55-120) and
+ * setandgetAbsBitPerCiphertext(n0) (crypto/NFLLWE.cpp:656-668). */
+typedef struct {
+  uint32_t k, n, logn, m, bits, Berr;
+  int32_t A;                  /* absorption bits per coefficient                 */
+  uint64_t n0;
+  uint64_t moduli[4];
+  uint64_t abs_bytes;         /* A*n/8: plaintext bytes per poly (getAbsorptionBitsize/8) */
+  uint64_t ct_bytes;          /* 16*n*m: ciphertext bytes (getCiphertextBitsize/8)        */
+} xpg_params;
+
+int xpg_params_from_string(const char *desc, uint64_t n0, xpg_params *out);
+
+/* polys per import unit of `unit_bytes` bytes: ceil(8*unit_bytes/(A*n)) (NFLlib.cpp:378) */
+uint32_t xpg_polys_for_bytes(const xpg_params *prm, uint64_t unit_bytes);
+/* reply ciphertexts for d levels: computeReplySizeInChunks (PIRReplyGeneratorNFL_internal.cpp:667-679) */
+uint32_t xpg_reply_polys(const xpg_params *prm, uint32_t P0, uint32_t d);
+
+/* ---- context ---- */
+int xpg_ctx_create(int device, xpg_ctx **out);
+void xpg_ctx_destroy(xpg_ctx *ctx);
+/* message of the last failure on this context ("" if none) */
+const char *xpg_last_error(const xpg_ctx *ctx);
+/* the context's HIP stream (hipStream_t); every call below is ordered on it */
+void *xpg_ctx_stream(xpg_ctx *ctx);
+int xpg_sync(xpg_ctx *ctx);
+
+/* device memory plumbing for callers without a GPU runtime of their own */
+int xpg_dev_alloc(xpg_ctx *ctx, uint64_t bytes, void **ptr_dev);
+int xpg_dev_free(xpg_ctx *ctx, void *ptr_dev);
+int xpg_h2d(xpg_ctx *ctx, void *dst_dev, const void *src, uint64_t bytes);
+int xpg_d2h(xpg_ctx *ctx, void *dst, const void *src_dev, uint64_t bytes);
+
+/* ---- database import (PIRReplyGenerator::importData, pir/libpir.cpp:90-103 ->
+ *      PIRReplyGeneratorNFL_internal::importDataNFL, :79-158) ----
+ * elems: n_elems elements of elem_bytes, back to back (shorter elements zero-padded by
+ * the caller, as DBVectorProcessor::readStream does, DBVectorProcessor.cpp:88-106).
+ * alpha elements form one import unit (DBHandler::readAggregatedStream,
+ * DBHandler.cpp:3-26).  dims[0..d-1] is the query grid; prod(dims) >= units; the grid's
+ * missing units are zero polys (PIRReplyGeneratorNFL_internal.cpp:146-154).
+ * The NTT-domain database stays resident in HBM until xpg_db_free. */
+int xpg_db_import(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems, uint64_t n_elems,
+                  uint64_t elem_bytes, uint32_t alpha, uint32_t d, const uint32_t *dims,
+                  xpg_db **out);
+/* same, with the element bytes already in HBM (elems_dev) */
+int xpg_db_import_dev(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems_dev,
+                      uint64_t n_elems, uint64_t elem_bytes, uint32_t alpha, uint32_t d,
+                      const uint32_t *dims, xpg_db **out);
+/* NTT-domain words produced elsewhere (units x P x m x n), e.g. by a CPU import */
+int xpg_db_import_ntt_words(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *words,
+                            uint64_t units, uint32_t P, uint64_t raw_elem_bytes, uint32_t d,
+                            const uint32_t *dims, xpg_db **out);
+/* copy the resident NTT words back (parity checks); cap in words */
+int xpg_db_export_ntt_words(xpg_ctx *ctx, const xpg_db *db, uint64_t *out, uint64_t cap_words);
+/* polys per unit at level 0, and the reply's ciphertext count */
+uint32_t xpg_db_polys(const xpg_db *db);
+uint32_t xpg_db_reply_polys(const xpg_db *db);
+/* query words the database expects: sum_l dims[l] * 2*m*n */
+uint64_t xpg_db_query_words(const xpg_db *db);
+/* raw database bytes (n_elems * elem_bytes), the numerator of DB GB/s */
+uint64_t xpg_db_raw_bytes(const xpg_db *db);
+void xpg_db_free(xpg_db *db);
+
+/* ---- reply generation (PIRReplyGenerator::generateReply, pir/libpir.cpp:106-135 ->
+ *      PIRReplyGeneratorNFL_internal::generateReply, :340-445, with pushQuery's Shoup
+ *      precompute (:738-781) folded into the kernels and popReply (libpir.cpp:144-154)
+ *      replaced by one contiguous output buffer) ----
+ * query: xpg_db_query_words(db) words, levels back to back; reply: P_d*2*m*n words. */
+int xpg_reply(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query, uint64_t query_words,
+              uint64_t *reply, uint64_t reply_cap_words);
+/* device-resident query and reply, asynchronous on the context stream */
+int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev);
+
+/* ---- batched replies: many queries over resident databases, the DB tiles of each
+ *      database streamed from HBM once per query group.  No reference counterpart
+ *      (the reference answers one query per generator); each reply is identical to
+ *      xpg_reply_dev of the same query. ---- */
+typedef struct {
+  const xpg_db *db;
+  const uint64_t *query_dev;
+  uint64_t *reply_dev;
+} xpg_request;
+int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs);
+
+/* ---- kernel timing: HIP events around every MAC launch on the context stream ---- */
+int xpg_timing_enable(xpg_ctx *ctx, int on);
+/* total milliseconds and launch count of the MAC kernels since the last reset */
+int xpg_timing_read(xpg_ctx *ctx, double *mac_ms, uint64_t *mac_launches, int reset);
+
+/* ---- client-side helpers (test and load-generation support; the reference's
+ *      NFLLWE::enc/encrypt, crypto/NFLLWE.cpp:276-372,481-498, and
+ *      PIRQueryGenerator_internal::generateQuery, :37-70) ---- */
+/* secret key (m*n NTT-domain words) from a seed, uniform per modulus */
+int xpg_keygen(const xpg_params *prm, uint64_t seed, uint64_t *sk);
+/* query for element `index` over dims[0..d-1], noise stream `noise_seed`, on the GPU */
+int xpg_query_gen_dev(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *sk_dev, uint32_t d,
+                      const uint32_t *dims, uint64_t index, uint64_t noise_seed,
+                      uint64_t *query_dev);
+/* bytes [offset, offset+nbytes) of the seeded byte stream `seed`, written on the GPU */
+int xpg_random_bytes_dev(xpg_ctx *ctx, uint64_t seed, uint64_t offset, uint8_t *dst_dev,
+                         uint64_t nbytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,35 @@
+"""Workload definitions (seeds and sizes), shared by tests and bench.py.
+
+Conventions fixed here (the task's configurations leave them open):
+  * "10 KB" / "4 KB" / "1 KB" element sizes are KiB (10240 / 4096 / 1024 bytes);
+    element counts such as "10k" are decimal.
+  * DB bytes: byte i of the database is byte (i mod 8) of word i//8 of the seeded
+    SplitMix64 stream `db_seed` (xpir_amd/csrc/arith.hpp: rand_word).
+  * The query index is word 0 of stream `index_seed`, modulo the element count.
+  * Secret key: stream `key_seed`; noise and uniform `a`: stream `noise_seed`
+    (and noise_seed ^ 0xA0A0A0A0A0A0A0A0 for `a`).
+"""
+
+
+def _index(seed, n):
+    z = (seed + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    z ^= z >> 31
+    return z % n
+
+
+C1_PLAIN = dict(name="plain-LWE:80:1024:60-1000x10KiB", crypto="LWE:80:1024:60", n_elems=1000,
+                elem_bytes=10240, dims=[1000], db_seed=1, index=_index(2, 1000), noise_seed=3, key_seed=4)
+
+C2_SINGLE = dict(name="single-LWE:80:2048:120-65536x4KiB", crypto="LWE:80:2048:120", n_elems=65536,
+                 elem_bytes=4096, dims=[65536], db_seed=21, index=_index(22, 65536), noise_seed=23,
+                 key_seed=24)
+
+C3_RECURSIVE = dict(name="recursive-LWE:100:4096:180-2^20x1KiB-d2", crypto="LWE:100:4096:180",
+                    n_elems=1 << 20, elem_bytes=1024, dims=[1024, 1024], db_seed=31,
+                    index=_index(32, 1 << 20), noise_seed=33, key_seed=34)
+
+# scaled-down grid of the recursive configuration: same cryptosystem and element size
+C3_SCALED = dict(C3_RECURSIVE, name="recursive-scaled-64x64", n_elems=4096, dims=[64, 64],
+                 index=_index(32, 4096))

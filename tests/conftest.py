@@ -1,0 +1,22 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libxpir_amd.so)")
+    config.addinivalue_line("markers", "slow: full-size configuration")
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx():
+    import xpir_amd
+
+    ctx = xpir_amd.Context(0)
+    yield ctx
+    ctx.close()

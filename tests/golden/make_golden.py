@@ -1,0 +1,113 @@
+"""Write the golden fixtures of tests/golden/ from the REFERENCE's own arithmetic.
+
+Server-side words (import = bitsplitter + nttAndPowPhi, reply = mulandaddShoup chains,
+recursion = deserializeDataNFL over the level's reply buffers) come from
+oracle/_ref/libxpir_ref.so, the reference's NFLlib compiled from its sources by
+oracle/Makefile.  The reference's query generator (NFLLWE::enc) needs Boost, which the
+image lacks, so queries are produced by the oracle's encryption; they are inputs here,
+not checked outputs.
+
+    python tests/golden/make_golden.py            # C1, C3-scaled, absorb vectors
+    python tests/golden/make_golden.py --c2       # also the 65536 x 4 KiB single query
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle.oracle import Oracle, RefNFL, build  # noqa: E402
+from tests.configs import C1_PLAIN, C2_SINGLE, C3_SCALED  # noqa: E402
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def ref_generate(cfg, threads=8):
+    """The reference's arithmetic for a whole d-level reply on a seeded workload."""
+    desc, dims = cfg["crypto"], cfg["dims"]
+    orc = Oracle(desc, dims[0])
+    ref = RefNFL(orc.n, orc.m * 60)
+    n, L = cfg["n_elems"], cfg["elem_bytes"]
+    data = orc.rand_bytes(cfg["db_seed"], 0, n * L)
+    P0 = orc.polys(L)
+    units = int(np.prod(dims))
+    words = np.zeros(units * P0 * orc.mn, dtype=np.uint64)
+    for e in range(n):
+        words[e * P0 * orc.mn:(e + 1) * P0 * orc.mn] = ref.import_buffers(data[e * L:(e + 1) * L], 1, orc.A)
+    sk = orc.keygen(cfg["key_seed"])
+    q = orc.make_query(sk, dims, cfg["index"], cfg["noise_seed"])
+    level_words, P = words, P0
+    samples = {"ntt_unit0_head": [int(x) for x in words[:8]],
+               "ntt_last_head": [int(x) for x in words[(n - 1) * P0 * orc.mn:(n - 1) * P0 * orc.mn + 8]],
+               "ntt_sha256": sha(words)}
+    for lvl, nl in enumerate(dims):
+        groups = int(np.prod(dims[lvl + 1:], dtype=np.int64)) if lvl + 1 < len(dims) else 1
+        rep = ref.reply_level(level_words, q[lvl], nl, groups, P, threads)
+        if lvl + 1 < len(dims):
+            ct_bytes = P * 2 * orc.mn * 8
+            P2 = orc.next_polys(P)
+            nxt = np.zeros(groups * P2 * orc.mn, dtype=np.uint64)
+            rb = rep.view(np.uint8)
+            for g in range(groups):
+                nxt[g * P2 * orc.mn:(g + 1) * P2 * orc.mn] = ref.import_buffers(
+                    rb[g * ct_bytes:(g + 1) * ct_bytes], P, orc.A)
+            samples[f"level{lvl}_reply_sha256"] = sha(rep)
+            level_words, P = nxt, P2
+        else:
+            final = rep
+    entry = dict(cfg, A=orc.A, P0=P0, Pd=P, reply_sha256=sha(final),
+                 reply_head=[int(x) for x in final[:16]], reply_tail=[int(x) for x in final[-16:]],
+                 noncanonical_words=int(np.sum(final >= np.tile(np.repeat(np.array(orc.p, dtype=np.uint64), orc.n),
+                                                                2 * P))),
+                 **samples)
+    return entry, final
+
+
+def absorb_vectors():
+    """Bitsplitter vectors: A = 15 / 42 / 74, single buffers and buffers whose chunks
+    straddle buffer boundaries (the recursion layout)."""
+    out = []
+    for desc, n0, nbufs, per in [("LWE:80:1024:60", 1000, 1, 10240), ("LWE:80:1024:60", 1000, 3, 16384),
+                                 ("LWE:80:2048:120", 65536, 1, 4096), ("LWE:80:2048:120", 65536, 2, 65536),
+                                 ("LWE:100:4096:180", 1024, 1, 1024), ("LWE:100:4096:180", 1024, 3, 196608),
+                                 ("LWE:80:2048:120", 7, 1, 4101)]:
+        orc = Oracle(desc, n0)
+        ref = RefNFL(orc.n, orc.m * 60)
+        data = orc.rand_bytes(99, 0, nbufs * per)
+        words = ref.import_buffers(data, nbufs, orc.A)
+        out.append(dict(crypto=desc, n0=n0, nbufs=nbufs, bytes_per_buf=per, seed=99, A=orc.A,
+                        polys=int(words.size // orc.mn), sha256=sha(words), head=[int(x) for x in words[:8]]))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--c2", action="store_true")
+    args = ap.parse_args()
+    build()
+    assert RefNFL.available(), "oracle/_ref/libxpir_ref.so missing (needs the reference tree)"
+    path = os.path.join(HERE, "golden.json")
+    gold = json.load(open(path)) if os.path.exists(path) else {}
+    e, rep = ref_generate(C1_PLAIN)
+    gold["c1_plain"] = e
+    np.save(os.path.join(HERE, "c1_plain_reply.npy"), rep)
+    gold["c3_scaled"], _ = ref_generate(C3_SCALED)
+    gold["absorb"] = absorb_vectors()
+    if args.c2:
+        gold["c2_single"], _ = ref_generate(C2_SINGLE)
+    json.dump(gold, open(path, "w"), indent=1)
+    print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+"""Parity of the HIP path (libxpir_amd.so through its C ABI) with the oracle.
+
+Bar: bit-exact words for import (absorb + NTT), query encryption, every reply level and
+the recursion step, on the same seeded inputs; plus decrypt(reply) == chosen element.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, extract_reply
+
+pytestmark = pytest.mark.gpu
+
+C1 = "LWE:80:1024:60"
+C2 = "LWE:80:2048:120"
+C3 = "LWE:100:4096:180"
+
+
+def _db_bytes(orc, seed, n_elems, elem_bytes):
+    return orc.rand_bytes(seed, 0, n_elems * elem_bytes)
+
+
+def _oracle_import(orc, data, n_elems, elem_bytes, alpha, units_total):
+    unit = elem_bytes * alpha
+    P = orc.polys(unit)
+    units = -(-n_elems // alpha)
+    padded = np.zeros(units * unit, dtype=np.uint8)
+    padded[: data.size] = data
+    words = [orc.import_element(padded[u * unit:(u + 1) * unit]) for u in range(units)]
+    out = np.zeros(units_total * P * orc.mn, dtype=np.uint64)
+    if words:
+        out[: units * P * orc.mn] = np.concatenate(words)
+    return out, P
+
+
+@pytest.mark.parametrize("desc,n_elems,elem_bytes,alpha,dims", [
+    (C1, 20, 10240, 1, [20]),
+    (C1, 13, 1000, 1, [16]),          # ragged element size, padding units
+    (C2, 9, 4096, 2, [5]),            # alpha aggregation with a short last unit
+    (C2, 7, 4101, 1, [7]),
+    (C3, 5, 1024, 1, [6]),            # A = 74: two sub-chunks per coefficient
+    (C3, 3, 40000, 1, [3]),
+])
+def test_import_matches_oracle(gpu_ctx, desc, n_elems, elem_bytes, alpha, dims):
+    import xpir_amd
+    orc = Oracle(desc, dims[0])
+    prm = xpir_amd.params(desc, dims[0])
+    assert prm.A == orc.A
+    data = _db_bytes(orc, 11, n_elems, elem_bytes)
+    db = gpu_ctx.import_db(prm, data, n_elems, elem_bytes, dims, alpha=alpha)
+    want, P = _oracle_import(orc, data, n_elems, elem_bytes, alpha, int(np.prod(dims)))
+    assert db.polys == P
+    got = db.export_words()
+    assert np.array_equal(got, want)
+    db.free()
+
+
+@pytest.mark.parametrize("desc,dims", [(C1, [7]), (C2, [3, 4]), (C3, [2, 3])])
+def test_query_generation_matches_oracle(gpu_ctx, desc, dims):
+    import xpir_amd
+    orc = Oracle(desc, dims[0])
+    prm = xpir_amd.params(desc, dims[0])
+    sk = orc.keygen(4)
+    assert np.array_equal(sk, xpir_amd.keygen(prm, 4))
+    index = int(np.prod(dims)) - 2
+    want = np.concatenate(orc.make_query(sk, dims, index, 3))
+    skd = gpu_ctx.alloc(sk.nbytes).upload(sk)
+    qd = gpu_ctx.alloc(want.nbytes)
+    gpu_ctx.query_gen_dev(prm, skd, dims, index, 3, qd)
+    got = qd.download()
+    assert np.array_equal(got, want)
+    skd.free()
+    qd.free()
+
+
+@pytest.mark.parametrize("desc,n_elems,elem_bytes,dims", [
+    (C1, 40, 10240, [40]),
+    (C1, 37, 3000, [10, 4]),
+    (C2, 64, 4096, [64]),
+    (C2, 30, 4096, [6, 5]),
+    (C3, 16, 1024, [4, 4]),
+    (C2, 25, 700, [5, 5, 1]),
+])
+def test_reply_matches_oracle(gpu_ctx, desc, n_elems, elem_bytes, dims):
+    import xpir_amd
+    orc = Oracle(desc, dims[0])
+    prm = xpir_amd.params(desc, dims[0])
+    data = _db_bytes(orc, 1, n_elems, elem_bytes)
+    units = int(np.prod(dims))
+    words, P = _oracle_import(orc, data, n_elems, elem_bytes, 1, units)
+    sk = orc.keygen(4)
+    index = n_elems - 1
+    qlevels = orc.make_query(sk, dims, index, 3)
+    want, Pd = orc.generate_reply(words, P, dims, qlevels)
+    db = gpu_ctx.import_db(prm, data, n_elems, elem_bytes, dims)
+    got = gpu_ctx.reply(db, np.concatenate(qlevels))
+    assert db.reply_polys == Pd
+    assert np.array_equal(got, want)
+    assert extract_reply(orc, sk, got, dims, elem_bytes) == data[index * elem_bytes:(index + 1) * elem_bytes].tobytes()
+    db.free()
+
+
+def test_c1_plain_full(gpu_ctx):
+    """LWE:80:1024:60, 1000 x 10 KiB from seed 1, index from seed 2, noise seed 3."""
+    import xpir_amd
+    from tests.configs import C1_PLAIN as cfg
+    orc = Oracle(cfg["crypto"], cfg["dims"][0])
+    prm = xpir_amd.params(cfg["crypto"], cfg["dims"][0])
+    n, L = cfg["n_elems"], cfg["elem_bytes"]
+    data = _db_bytes(orc, cfg["db_seed"], n, L)
+    index = cfg["index"]
+    sk = orc.keygen(cfg["key_seed"])
+    q = orc.make_query(sk, cfg["dims"], index, cfg["noise_seed"])
+    db = gpu_ctx.import_db(prm, data, n, L, cfg["dims"])
+    words, P = _oracle_import(orc, data, n, L, 1, n)
+    assert np.array_equal(db.export_words(), words)
+    want, _ = orc.generate_reply(words, P, cfg["dims"], q)
+    got = gpu_ctx.reply(db, np.concatenate(q))
+    assert np.array_equal(got, want)
+    assert extract_reply(orc, sk, got, cfg["dims"], L) == data[index * L:(index + 1) * L].tobytes()
+    db.free()

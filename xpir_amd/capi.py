@@ -1,0 +1,269 @@
+"""ctypes binding of libxpir_amd.so (the C ABI in include/xpir_amd.h).
+
+Used by the tests and bench.py to drive the HIP path.  Loading fails loudly when the
+library has not been built: there is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxpir_amd.so")
+
+XPG_OK, XPG_EINVAL, XPG_ENOMEM, XPG_EDEVICE, XPG_ESIZE, XPG_EBUSY = 0, -1, -2, -3, -4, -5
+
+# every entry point include/xpir_amd.h declares
+EXPORTS = [
+    "xpg_params_from_string", "xpg_polys_for_bytes", "xpg_reply_polys", "xpg_ctx_create",
+    "xpg_ctx_destroy", "xpg_last_error", "xpg_ctx_stream", "xpg_sync", "xpg_dev_alloc",
+    "xpg_dev_free", "xpg_h2d", "xpg_d2h", "xpg_db_import", "xpg_db_import_dev",
+    "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
+    "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev",
+    "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_keygen",
+    "xpg_query_gen_dev", "xpg_random_bytes_dev",
+]
+
+
+class XpgParams(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("n", C.c_uint32), ("logn", C.c_uint32), ("m", C.c_uint32),
+                ("bits", C.c_uint32), ("Berr", C.c_uint32), ("A", C.c_int32), ("n0", C.c_uint64),
+                ("moduli", C.c_uint64 * 4), ("abs_bytes", C.c_uint64), ("ct_bytes", C.c_uint64)]
+
+
+class XpgRequest(C.Structure):
+    _fields_ = [("db", C.c_void_p), ("query_dev", C.c_void_p), ("reply_dev", C.c_void_p)]
+
+
+class XpgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run `python -m xpir_amd.build` "
+                          "(the HIP path has no fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, u64, u32 = C.c_void_p, C.c_uint64, C.c_uint32
+    P = C.POINTER(XpgParams)
+    sig = {
+        "xpg_params_from_string": (C.c_int, [C.c_char_p, u64, P]),
+        "xpg_polys_for_bytes": (u32, [P, u64]),
+        "xpg_reply_polys": (u32, [P, u32, u32]),
+        "xpg_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "xpg_ctx_destroy": (None, [vp]),
+        "xpg_last_error": (C.c_char_p, [vp]),
+        "xpg_ctx_stream": (vp, [vp]),
+        "xpg_sync": (C.c_int, [vp]),
+        "xpg_dev_alloc": (C.c_int, [vp, u64, C.POINTER(vp)]),
+        "xpg_dev_free": (C.c_int, [vp, vp]),
+        "xpg_h2d": (C.c_int, [vp, vp, vp, u64]),
+        "xpg_d2h": (C.c_int, [vp, vp, vp, u64]),
+        "xpg_db_import": (C.c_int, [vp, P, vp, u64, u64, u32, u32, C.POINTER(u32), C.POINTER(vp)]),
+        "xpg_db_import_dev": (C.c_int, [vp, P, vp, u64, u64, u32, u32, C.POINTER(u32), C.POINTER(vp)]),
+        "xpg_db_import_ntt_words": (C.c_int, [vp, P, vp, u64, u32, u64, u32, C.POINTER(u32), C.POINTER(vp)]),
+        "xpg_db_export_ntt_words": (C.c_int, [vp, vp, vp, u64]),
+        "xpg_db_polys": (u32, [vp]),
+        "xpg_db_reply_polys": (u32, [vp]),
+        "xpg_db_query_words": (u64, [vp]),
+        "xpg_db_raw_bytes": (u64, [vp]),
+        "xpg_db_free": (None, [vp]),
+        "xpg_reply": (C.c_int, [vp, vp, vp, u64, vp, u64]),
+        "xpg_reply_dev": (C.c_int, [vp, vp, vp, vp]),
+        "xpg_reply_batch_dev": (C.c_int, [vp, C.POINTER(XpgRequest), u32]),
+        "xpg_timing_enable": (C.c_int, [vp, C.c_int]),
+        "xpg_timing_read": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
+        "xpg_keygen": (C.c_int, [P, u64, vp]),
+        "xpg_query_gen_dev": (C.c_int, [vp, P, vp, u32, C.POINTER(u32), u64, u64, vp]),
+        "xpg_random_bytes_dev": (C.c_int, [vp, u64, u64, vp, u64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def params(desc: str, n0: int) -> XpgParams:
+    prm = XpgParams()
+    rc = lib().xpg_params_from_string(desc.encode(), n0, C.byref(prm))
+    if rc != XPG_OK:
+        raise XpgError(f"unsupported cryptosystem {desc!r} (n0={n0}): status {rc}")
+    return prm
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class DeviceBuffer:
+    """Device allocation owned by a Context (freed explicitly or with the context)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx, self.nbytes = ctx, nbytes
+        p = C.c_void_p()
+        ctx._check(lib().xpg_dev_alloc(ctx.h, nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx._check(lib().xpg_h2d(self.ctx.h, self.ptr, _ptr(arr), arr.nbytes))
+        return self
+
+    def download(self, dtype=np.uint64, count=None) -> np.ndarray:
+        itemsize = np.dtype(dtype).itemsize
+        count = self.nbytes // itemsize if count is None else count
+        out = np.empty(count, dtype=dtype)
+        self.ctx._check(lib().xpg_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().xpg_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Database:
+    def __init__(self, ctx: "Context", handle: int, prm: XpgParams, dims):
+        self.ctx, self.h, self.prm, self.dims = ctx, handle, prm, list(dims)
+
+    @property
+    def polys(self) -> int:
+        return lib().xpg_db_polys(self.h)
+
+    @property
+    def reply_polys(self) -> int:
+        return lib().xpg_db_reply_polys(self.h)
+
+    @property
+    def query_words(self) -> int:
+        return lib().xpg_db_query_words(self.h)
+
+    @property
+    def reply_words(self) -> int:
+        return self.reply_polys * 2 * self.prm.m * self.prm.n
+
+    @property
+    def raw_bytes(self) -> int:
+        return lib().xpg_db_raw_bytes(self.h)
+
+    def export_words(self) -> np.ndarray:
+        units = int(np.prod(self.dims, dtype=np.int64))
+        out = np.empty(units * self.polys * self.prm.m * self.prm.n, dtype=np.uint64)
+        self.ctx._check(lib().xpg_db_export_ntt_words(self.ctx.h, self.h, _ptr(out), out.size))
+        return out
+
+    def free(self):
+        if self.h:
+            lib().xpg_db_free(self.h)
+            self.h = None
+
+
+class Context:
+    """One GPU, one HIP stream (xpg_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().xpg_ctx_create(device, C.byref(h))
+        if rc != XPG_OK:
+            raise XpgError(f"xpg_ctx_create({device}) failed: {rc}")
+        self.h = h.value
+
+    def _check(self, rc: int):
+        if rc != XPG_OK:
+            raise XpgError(f"status {rc}: {lib().xpg_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            lib().xpg_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync(self):
+        self._check(lib().xpg_sync(self.h))
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    @staticmethod
+    def _dims(dims):
+        arr = (C.c_uint32 * len(dims))(*dims)
+        return arr
+
+    def import_db(self, prm: XpgParams, elems: np.ndarray, n_elems: int, elem_bytes: int, dims,
+                  alpha: int = 1) -> Database:
+        elems = np.ascontiguousarray(elems, dtype=np.uint8)
+        assert elems.size == n_elems * elem_bytes
+        h = C.c_void_p()
+        self._check(lib().xpg_db_import(self.h, C.byref(prm), _ptr(elems), n_elems, elem_bytes, alpha,
+                                        len(dims), self._dims(dims), C.byref(h)))
+        return Database(self, h.value, prm, dims)
+
+    def import_db_dev(self, prm: XpgParams, elems_dev: DeviceBuffer, n_elems: int, elem_bytes: int, dims,
+                      alpha: int = 1) -> Database:
+        h = C.c_void_p()
+        self._check(lib().xpg_db_import_dev(self.h, C.byref(prm), elems_dev.ptr, n_elems, elem_bytes, alpha,
+                                            len(dims), self._dims(dims), C.byref(h)))
+        return Database(self, h.value, prm, dims)
+
+    def import_ntt_words(self, prm: XpgParams, words: np.ndarray, units: int, P: int, raw_elem_bytes: int,
+                         dims) -> Database:
+        words = np.ascontiguousarray(words, dtype=np.uint64)
+        h = C.c_void_p()
+        self._check(lib().xpg_db_import_ntt_words(self.h, C.byref(prm), _ptr(words), units, P, raw_elem_bytes,
+                                                  len(dims), self._dims(dims), C.byref(h)))
+        return Database(self, h.value, prm, dims)
+
+    def reply(self, db: Database, query: np.ndarray) -> np.ndarray:
+        query = np.ascontiguousarray(query, dtype=np.uint64)
+        out = np.empty(db.reply_words, dtype=np.uint64)
+        self._check(lib().xpg_reply(self.h, db.h, _ptr(query), query.size, _ptr(out), out.size))
+        return out
+
+    def reply_dev(self, db: Database, query_dev: DeviceBuffer, reply_dev: DeviceBuffer):
+        self._check(lib().xpg_reply_dev(self.h, db.h, query_dev.ptr, reply_dev.ptr))
+
+    def reply_batch_dev(self, reqs):
+        arr = (XpgRequest * len(reqs))()
+        for i, (db, q, r) in enumerate(reqs):
+            arr[i].db, arr[i].query_dev, arr[i].reply_dev = db.h, q.ptr, r.ptr
+        self._check(lib().xpg_reply_batch_dev(self.h, arr, len(reqs)))
+
+    def timing(self, on: bool = True):
+        self._check(lib().xpg_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self, reset: bool = True):
+        ms, n = C.c_double(), C.c_uint64()
+        self._check(lib().xpg_timing_read(self.h, C.byref(ms), C.byref(n), 1 if reset else 0))
+        return ms.value, n.value
+
+    def query_gen_dev(self, prm: XpgParams, sk_dev: DeviceBuffer, dims, index: int, noise_seed: int,
+                      out: DeviceBuffer):
+        self._check(lib().xpg_query_gen_dev(self.h, C.byref(prm), sk_dev.ptr, len(dims), self._dims(dims),
+                                            index, noise_seed, out.ptr))
+
+    def random_bytes_dev(self, seed: int, offset: int, dst: DeviceBuffer, nbytes: int):
+        self._check(lib().xpg_random_bytes_dev(self.h, seed, offset, dst.ptr, nbytes))
+
+
+def keygen(prm: XpgParams, seed: int) -> np.ndarray:
+    sk = np.empty(prm.m * prm.n, dtype=np.uint64)
+    rc = lib().xpg_keygen(C.byref(prm), seed, _ptr(sk))
+    if rc != XPG_OK:
+        raise XpgError("keygen")
+    return sk

@@ -1,0 +1,68 @@
+// arith.hpp -- exact 64-bit modular arithmetic for the NFLLWE reply path on CDNA4.
+//
+// All words are uint64; moduli are the reference's p_i = 2^61 - delta_i with
+// delta_i = c_i*2^14 - 1 < 2^21 (crypto/NFLParams.cpp:34).  The lazy conventions
+// follow crypto/NFLlib.hpp:255-321 exactly, because the reply bytes depend on them.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define XPG_MAX_M 4
+
+struct ModSet {
+  uint64_t p[XPG_MAX_M];
+  uint64_t delta[XPG_MAX_M];  // 2^61 - p
+};
+
+__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
+
+// x*y - floor(x*yq / 2^64)*p modulo 2^64; in [0, 2p) for p < 2^62 and any x < 2^64
+// (crypto/NFLlib.hpp:283-285 before its final conditional subtraction).
+__device__ __forceinline__ uint64_t shoup_lazy(uint64_t x, uint64_t y, uint64_t yq, uint64_t p) {
+  return x * y - mulhi64(x, yq) * p;
+}
+
+// mulmodShoup: the lazy product with one conditional subtraction (canonical for x < 2^64).
+__device__ __forceinline__ uint64_t shoup_canon(uint64_t x, uint64_t y, uint64_t yq, uint64_t p) {
+  uint64_t r = shoup_lazy(x, y, yq, p);
+  return r >= p ? r - p : r;
+}
+
+// y*delta as a 128-bit value for delta < 2^32: two 32x32->64 products.
+__device__ __forceinline__ void mul64x32(uint64_t y, uint32_t d, uint64_t &lo, uint64_t &hi) {
+  uint64_t a = (uint64_t)(uint32_t)y * d;          // < 2^53
+  uint64_t b = (uint64_t)(uint32_t)(y >> 32) * d;  // < 2^53
+  lo = a + (b << 32);
+  hi = (b >> 32) + (lo < a ? 1 : 0);
+}
+
+// floor(y * 2^64 / p) mod 2^64 -- the Shoup quotient the reference precomputes per query
+// word on pushQuery (PIRReplyGeneratorNFL_internal.cpp:760-767) -- derived without a
+// division from p = 2^61 - delta:  y*2^64/p = 8y + 8*y*delta/p, and N = 8*y*delta
+// < 2^90 splits as N = t*2^61 + r0 = t*p + (r0 + t*delta) with r0 + t*delta < 2p.
+// Holds for every y < 2^64 (checked exhaustively at the edges and on random words).
+__device__ __forceinline__ uint64_t shoup_quot(uint64_t y, uint64_t p, uint64_t delta) {
+  uint64_t lo, hi;
+  mul64x32(y, (uint32_t)delta, lo, hi);
+  uint64_t nlo = lo << 3;
+  uint64_t nhi = (hi << 3) | (lo >> 61);
+  uint64_t t = (nhi << 3) | (nlo >> 61);
+  uint64_t r = (nlo & ((1ULL << 61) - 1)) + t * delta;
+  return 8 * y + t + (r >= p ? 1ULL : 0ULL);
+}
+
+// One reply-chain step (crypto/NFLlib.hpp:303-321): acc += lazy product; -p if >= p.
+__device__ __forceinline__ uint64_t mac_step(uint64_t acc, uint64_t x, uint64_t y, uint64_t yq,
+                                             uint64_t p) {
+  acc += shoup_lazy(x, y, yq, p);
+  return acc >= p ? acc - p : acc;
+}
+
+// counter-based SplitMix64: word k of stream `seed` (the seeded generator of this
+// project; stands in for the reference's /dev/urandom-keyed Salsa20 stream).
+__host__ __device__ __forceinline__ uint64_t rand_word(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}

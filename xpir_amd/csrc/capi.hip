@@ -1,0 +1,518 @@
+// capi.hip -- host side of libxpir_amd.so: parameters, NTT tables, contexts, resident
+// databases and the d-level reply orchestration behind the C ABI (include/xpir_amd.h).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "internal.hpp"
+
+namespace {
+
+using u128 = unsigned __int128;
+
+// Moduli and primitive 2^14-th roots of unity fixed by the reference
+// (crypto/NFLParams.cpp:34,41).  The roots decide every NTT-domain word.
+const uint64_t kModuli[4] = {2305843009213317121ULL, 2305843009213120513ULL, 2305843009212694529ULL,
+                             2305843009212399617ULL};
+const uint64_t kRoots[4] = {1187132827279672845ULL, 753478288701480417ULL, 717492273700781398ULL,
+                            1965831349000139179ULL};
+
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t p) { return (uint64_t)(((u128)a * b) % p); }
+uint64_t quot(uint64_t y, uint64_t p) { return (uint64_t)(((u128)y << 64) / p); }
+
+#define HIP_OK(expr, msg)                                                      \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return set_err(ctx, XPG_EDEVICE, std::string(msg) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+uint64_t prod_dims(uint32_t d, const uint32_t *dims, uint32_t from) {
+  uint64_t r = 1;
+  for (uint32_t i = from; i < d; i++) r *= dims[i];
+  return r;
+}
+
+}  // namespace
+
+int set_err(xpg_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+ModSet modset_of(const xpg_params *prm) {
+  ModSet ms{};
+  for (uint32_t c = 0; c < prm->m && c < XPG_MAX_M; c++) {
+    ms.p[c] = prm->moduli[c];
+    ms.delta[c] = (1ULL << 61) - prm->moduli[c];
+  }
+  return ms;
+}
+
+// configureNTT + prep_wtab restated (crypto/NFLlib.cpp:44-194, NFLlib.hpp:571-586)
+const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m) {
+  const uint64_t key = ((uint64_t)n << 8) | m;
+  auto it = ctx->tables.find(key);
+  if (it != ctx->tables.end()) return &it->second;
+  uint32_t logn = 0;
+  while ((1u << logn) < n) logn++;
+  std::vector<uint64_t> phi(n * m), phiq(n * m), w(n * m), wq(n * m);
+  for (uint32_t c = 0; c < m; c++) {
+    const uint64_t p = kModuli[c];
+    uint64_t f = kRoots[c];
+    for (uint32_t i = 0; i < 13 - logn; i++) f = mulmod(f, f, p);  // primitive 2n-th root
+    uint64_t v = 1;
+    for (uint32_t j = 0; j < n; j++) {
+      phi[c * n + j] = v;
+      phiq[c * n + j] = quot(v, p);
+      v = mulmod(v, f, p);
+    }
+    uint64_t om = mulmod(f, f, p);
+    size_t o = c * n;
+    for (uint32_t size = n; size >= 2; size >>= 1) {
+      uint64_t wi = 1;
+      for (uint32_t i = 0; i < size / 2; i++) {
+        w[o] = wi;
+        wq[o] = quot(wi, p);
+        o++;
+        wi = mulmod(wi, om, p);
+      }
+      om = mulmod(om, om, p);
+    }
+  }
+  DevTables t;
+  t.n = n;
+  t.m = m;
+  t.logn = logn;
+  const size_t bytes = sizeof(uint64_t) * n * m;
+  uint64_t **dst[4] = {&t.phi, &t.phiq, &t.w, &t.wq};
+  const std::vector<uint64_t> *src[4] = {&phi, &phiq, &w, &wq};
+  for (int i = 0; i < 4; i++) {
+    if (hipMalloc((void **)dst[i], bytes) != hipSuccess ||
+        hipMemcpy(*dst[i], src[i]->data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      set_err(ctx, XPG_ENOMEM, "NTT table allocation");
+      return nullptr;
+    }
+  }
+  return &(ctx->tables[key] = t);
+}
+
+int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes) {
+  if (ctx->level_bytes[which] >= bytes) return XPG_OK;
+  if (ctx->level_buf[which]) (void)hipFree(ctx->level_buf[which]);
+  ctx->level_buf[which] = nullptr;
+  ctx->level_bytes[which] = 0;
+  HIP_OK(hipMalloc((void **)&ctx->level_buf[which], bytes), "level buffer");
+  ctx->level_bytes[which] = bytes;
+  return XPG_OK;
+}
+
+int ensure_scratch(xpg_ctx *ctx, size_t bytes) {
+  if (ctx->scratch_bytes >= bytes) return XPG_OK;
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  ctx->scratch = nullptr;
+  ctx->scratch_bytes = 0;
+  HIP_OK(hipMalloc(&ctx->scratch, bytes), "scratch");
+  ctx->scratch_bytes = bytes;
+  return XPG_OK;
+}
+
+void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b) {
+  if (!ctx->timing) return;
+  for (hipEvent_t *e : {a, b}) {
+    if (!ctx->ev_pool.empty()) {
+      *e = ctx->ev_pool.back();
+      ctx->ev_pool.pop_back();
+    } else {
+      (void)hipEventCreate(e);
+    }
+  }
+  (void)hipEventRecord(*a, ctx->stream);
+}
+
+void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b) {
+  if (!ctx->timing || !a) return;
+  (void)hipEventRecord(b, ctx->stream);
+  ctx->pending.emplace_back(a, b);
+}
+
+static void timing_collect(xpg_ctx *ctx) {
+  for (auto &pr : ctx->pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(pr.second) == hipSuccess && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+      ctx->mac_ms += ms;
+      ctx->mac_launches++;
+    }
+    ctx->ev_pool.push_back(pr.first);
+    ctx->ev_pool.push_back(pr.second);
+  }
+  ctx->pending.clear();
+}
+
+// ------------------------------------------------------------------------------------
+extern "C" {
+
+int xpg_params_from_string(const char *desc, uint64_t n0, xpg_params *out) {
+  if (!desc || !out) return XPG_EINVAL;
+  unsigned k = 0, n = 0, bits = 0;
+  char name[8] = {0};
+  if (sscanf(desc, "%3[A-Z]:%u:%u:%u", name, &k, &n, &bits) != 4) return XPG_EINVAL;
+  if (strcmp(name, "LWE") != 0) return XPG_EINVAL;
+  if (bits == 0 || bits % 60 != 0 || bits / 60 > 4) return XPG_EINVAL;  // NFLlib.cpp:219-224
+  if (n < 512 || n > 8192 || (n & (n - 1))) return XPG_EINVAL;
+  memset(out, 0, sizeof(*out));
+  out->k = k;
+  out->n = n;
+  while ((1u << out->logn) < n) out->logn++;
+  out->m = bits / 60;
+  out->bits = bits;
+  out->Berr = 5 * k / 2;  // NFLLWE.cpp:84
+  out->n0 = n0 ? n0 : 1;
+  for (uint32_t c = 0; c < out->m; c++) out->moduli[c] = kModuli[c];
+  // NFLLWE.cpp:656-668, in double precision as the reference evaluates it
+  double a = floor(((double)(out->m * 60) - 1.0 - log2((double)out->n0) - log2((double)out->Berr) -
+                    log2((double)n)) / 2.0);
+  if (!(a >= 1.0) || a > 224.0) return XPG_EINVAL;
+  out->A = (int32_t)a;
+  out->abs_bytes = (uint64_t)out->A * n / 8;
+  out->ct_bytes = 16ULL * n * out->m;
+  return XPG_OK;
+}
+
+uint32_t xpg_polys_for_bytes(const xpg_params *prm, uint64_t unit_bytes) {
+  const uint64_t per = (uint64_t)prm->A * prm->n;
+  return (uint32_t)((unit_bytes * 8 + per - 1) / per);
+}
+
+uint32_t xpg_reply_polys(const xpg_params *prm, uint32_t P0, uint32_t d) {
+  uint64_t P = P0;
+  for (uint32_t l = 1; l < d; l++) P = (P * prm->ct_bytes + prm->abs_bytes - 1) / prm->abs_bytes;
+  return (uint32_t)P;
+}
+
+int xpg_ctx_create(int device, xpg_ctx **out) {
+  if (!out) return XPG_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return XPG_EDEVICE;
+  if (hipSetDevice(device) != hipSuccess) return XPG_EDEVICE;
+  xpg_ctx *ctx = new xpg_ctx();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return XPG_EDEVICE;
+  }
+  *out = ctx;
+  return XPG_OK;
+}
+
+void xpg_ctx_destroy(xpg_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  timing_collect(ctx);
+  for (auto &kv : ctx->tables) {
+    (void)hipFree(kv.second.phi);
+    (void)hipFree(kv.second.phiq);
+    (void)hipFree(kv.second.w);
+    (void)hipFree(kv.second.wq);
+  }
+  for (int i = 0; i < 2; i++)
+    if (ctx->level_buf[i]) (void)hipFree(ctx->level_buf[i]);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *xpg_last_error(const xpg_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+void *xpg_ctx_stream(xpg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int xpg_sync(xpg_ctx *ctx) {
+  if (!ctx) return XPG_EINVAL;
+  HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");
+  return XPG_OK;
+}
+
+int xpg_dev_alloc(xpg_ctx *ctx, uint64_t bytes, void **ptr) {
+  if (!ctx || !ptr) return XPG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  if (hipMalloc(ptr, bytes ? bytes : 8) != hipSuccess) return set_err(ctx, XPG_ENOMEM, "device allocation");
+  return XPG_OK;
+}
+
+int xpg_dev_free(xpg_ctx *ctx, void *ptr) {
+  if (!ctx) return XPG_EINVAL;
+  HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");
+  HIP_OK(hipFree(ptr), "device free");
+  return XPG_OK;
+}
+
+int xpg_h2d(xpg_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx) return XPG_EINVAL;
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D copy");
+  HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");
+  return XPG_OK;
+}
+
+int xpg_d2h(xpg_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx) return XPG_EINVAL;
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "D2H copy");
+  HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");
+  return XPG_OK;
+}
+
+// ---- databases ----
+static int db_setup(xpg_ctx *ctx, const xpg_params *prm, uint64_t n_elems, uint64_t elem_bytes,
+                    uint32_t alpha, uint32_t d, const uint32_t *dims, uint32_t P0, uint64_t real_units,
+                    xpg_db **out) {
+  if (!ctx || !prm || !out || !dims || d == 0 || d > XPG_MAX_LEVELS || alpha == 0)
+    return set_err(ctx, XPG_EINVAL, "bad database arguments");
+  for (uint32_t l = 0; l < d; l++)
+    if (dims[l] == 0) return set_err(ctx, XPG_EINVAL, "zero dimension");
+  if (dims[0] != prm->n0) return set_err(ctx, XPG_EINVAL, "params.n0 must equal dims[0] (A depends on it)");
+  const uint64_t units = prod_dims(d, dims, 0);
+  if (units < real_units) return set_err(ctx, XPG_EINVAL, "grid smaller than the database");
+  xpg_db *db = new xpg_db();
+  db->ctx = ctx;
+  db->prm = *prm;
+  db->d = d;
+  for (uint32_t l = 0; l < d; l++) db->dims[l] = dims[l];
+  db->units = units;
+  db->real_units = real_units;
+  db->P0 = P0;
+  db->Pl[0] = P0;
+  for (uint32_t l = 1; l <= d; l++) {
+    db->Pl[l] = (l < d) ? (uint32_t)((db->Pl[l - 1] * prm->ct_bytes + prm->abs_bytes - 1) / prm->abs_bytes)
+                        : db->Pl[l - 1];
+  }
+  db->n_elems = n_elems;
+  db->elem_bytes = elem_bytes;
+  db->alpha = alpha;
+  const uint64_t words = units * P0 * (uint64_t)prm->m * prm->n;
+  (void)hipSetDevice(ctx->device);
+  if (hipMalloc((void **)&db->words, words * 8) != hipSuccess) {
+    delete db;
+    return set_err(ctx, XPG_ENOMEM, "NTT database allocation");
+  }
+  *out = db;
+  return XPG_OK;
+}
+
+int xpg_db_import_dev(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems_dev, uint64_t n_elems,
+                      uint64_t elem_bytes, uint32_t alpha, uint32_t d, const uint32_t *dims, xpg_db **out) {
+  if (!ctx || !prm || !out) return XPG_EINVAL;
+  if (((uintptr_t)elems_dev & 7) != 0) return set_err(ctx, XPG_EINVAL, "element bytes must be 8-byte aligned");
+  if (elem_bytes == 0 || n_elems == 0) return set_err(ctx, XPG_EINVAL, "empty database");
+  const uint64_t unit_bytes = elem_bytes * alpha;
+  const uint32_t P0 = xpg_polys_for_bytes(prm, unit_bytes);
+  const uint64_t real_units = (n_elems + alpha - 1) / alpha;
+  int rc = db_setup(ctx, prm, n_elems, elem_bytes, alpha, d, dims, P0, real_units, out);
+  if (rc) return rc;
+  ByteSrc src{elems_dev, unit_bytes, unit_bytes, n_elems * elem_bytes};
+  rc = launch_import_bytes(ctx, prm, src, real_units, (*out)->units, P0, (*out)->words);
+  if (rc) {
+    xpg_db_free(*out);
+    *out = nullptr;
+  }
+  return rc;
+}
+
+int xpg_db_import(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems, uint64_t n_elems,
+                  uint64_t elem_bytes, uint32_t alpha, uint32_t d, const uint32_t *dims, xpg_db **out) {
+  if (!ctx || !elems) return XPG_EINVAL;
+  const uint64_t bytes = n_elems * elem_bytes;
+  uint8_t *tmp = nullptr;
+  (void)hipSetDevice(ctx->device);
+  HIP_OK(hipMalloc((void **)&tmp, bytes + 16), "staging allocation");
+  int rc = XPG_OK;
+  if (hipMemcpyAsync(tmp, elems, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = set_err(ctx, XPG_EDEVICE, "H2D copy");
+  if (!rc) rc = xpg_db_import_dev(ctx, prm, tmp, n_elems, elem_bytes, alpha, d, dims, out);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int xpg_db_import_ntt_words(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *words, uint64_t units,
+                            uint32_t P, uint64_t raw_elem_bytes, uint32_t d, const uint32_t *dims, xpg_db **out) {
+  if (!ctx || !words) return XPG_EINVAL;
+  int rc = db_setup(ctx, prm, units, raw_elem_bytes, 1, d, dims, P, units, out);
+  if (rc) return rc;
+  xpg_db *db = *out;
+  const uint64_t per = (uint64_t)P * prm->m * prm->n;
+  if (hipMemcpyAsync(db->words, words, units * per * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = set_err(ctx, XPG_EDEVICE, "H2D copy");
+  if (!rc) rc = launch_fill_zero(ctx, db->words + units * per, (db->units - units) * per);
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = set_err(ctx, XPG_EDEVICE, "sync");
+  if (rc) {
+    xpg_db_free(db);
+    *out = nullptr;
+  }
+  return rc;
+}
+
+int xpg_db_export_ntt_words(xpg_ctx *ctx, const xpg_db *db, uint64_t *out, uint64_t cap_words) {
+  if (!ctx || !db || !out) return XPG_EINVAL;
+  const uint64_t words = db->units * db->P0 * (uint64_t)db->prm.m * db->prm.n;
+  if (cap_words < words) return set_err(ctx, XPG_ESIZE, "export buffer too small");
+  HIP_OK(hipMemcpyAsync(out, db->words, words * 8, hipMemcpyDeviceToHost, ctx->stream), "D2H copy");
+  HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");
+  return XPG_OK;
+}
+
+uint32_t xpg_db_polys(const xpg_db *db) { return db ? db->P0 : 0; }
+uint32_t xpg_db_reply_polys(const xpg_db *db) { return db ? db->Pl[db->d] : 0; }
+uint64_t xpg_db_raw_bytes(const xpg_db *db) { return db ? db->n_elems * db->elem_bytes : 0; }
+uint64_t xpg_db_query_words(const xpg_db *db) {
+  if (!db) return 0;
+  uint64_t s = 0;
+  for (uint32_t l = 0; l < db->d; l++) s += db->dims[l];
+  return s * 2 * db->prm.m * db->prm.n;
+}
+
+void xpg_db_free(xpg_db *db) {
+  if (!db) return;
+  if (db->ctx) {
+    (void)hipSetDevice(db->ctx->device);
+    (void)hipStreamSynchronize(db->ctx->stream);
+  }
+  if (db->words) (void)hipFree(db->words);
+  delete db;
+}
+
+// ---- replies (PIRReplyGeneratorNFL_internal.cpp:340-445) ----
+int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev) {
+  if (!ctx || !db || !query_dev || !reply_dev) return set_err(ctx, XPG_EINVAL, "null argument");
+  if (db->ctx != ctx) return set_err(ctx, XPG_EINVAL, "database belongs to another context");
+  const xpg_params *prm = &db->prm;
+  const uint64_t mn = (uint64_t)prm->m * prm->n;
+  // intermediate levels: level_buf[0] holds a level's reply, level_buf[1] the next
+  // level's NTT database (sized once, before any launch reads them)
+  size_t need_reply = 0, need_db = 0;
+  for (uint32_t l = 0; l + 1 < db->d; l++) {
+    const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
+    need_reply = std::max<size_t>(need_reply, groups * db->Pl[l] * 2 * mn * 8);
+    need_db = std::max<size_t>(need_db, groups * db->Pl[l + 1] * mn * 8);
+  }
+  if (need_reply) {
+    int rc = ensure_level_buf(ctx, 0, need_reply);
+    if (!rc) rc = ensure_level_buf(ctx, 1, need_db);
+    if (rc) return rc;
+  }
+  const uint64_t *data = db->words;
+  const uint64_t *q = query_dev;
+  for (uint32_t l = 0; l < db->d; l++) {
+    const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
+    const uint32_t P = db->Pl[l];
+    const bool last = (l + 1 == db->d);
+    uint64_t *out = last ? reply_dev : ctx->level_buf[0];
+    int rc = launch_mac_level(ctx, prm, data, q, db->dims[l], groups, P, out);
+    if (rc) return rc;
+    q += (uint64_t)db->dims[l] * 2 * mn;
+    if (!last) {
+      // fromResulttoInData: each group's P ciphertexts, as bytes, become one unit
+      const uint64_t unit = (uint64_t)P * 2 * mn * 8;
+      ByteSrc src{(const uint8_t *)ctx->level_buf[0], unit, unit, groups * unit};
+      rc = launch_import_bytes(ctx, prm, src, groups, groups, db->Pl[l + 1], ctx->level_buf[1]);
+      if (rc) return rc;
+      data = ctx->level_buf[1];
+    }
+  }
+  return XPG_OK;
+}
+
+int xpg_reply(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query, uint64_t query_words, uint64_t *reply,
+              uint64_t reply_cap_words) {
+  if (!ctx || !db || !query || !reply) return set_err(ctx, XPG_EINVAL, "null argument");
+  const uint64_t qw = xpg_db_query_words(db);
+  const uint64_t rw = (uint64_t)db->Pl[db->d] * 2 * db->prm.m * db->prm.n;
+  if (query_words != qw) return set_err(ctx, XPG_ESIZE, "query size does not match the database grid");
+  if (reply_cap_words < rw) return set_err(ctx, XPG_ESIZE, "reply buffer too small");
+  uint64_t *qd = nullptr, *rd = nullptr;
+  (void)hipSetDevice(ctx->device);
+  HIP_OK(hipMalloc((void **)&qd, qw * 8), "query allocation");
+  if (hipMalloc((void **)&rd, rw * 8) != hipSuccess) {
+    (void)hipFree(qd);
+    return set_err(ctx, XPG_ENOMEM, "reply allocation");
+  }
+  int rc = XPG_OK;
+  if (hipMemcpyAsync(qd, query, qw * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = set_err(ctx, XPG_EDEVICE, "H2D copy");
+  if (!rc) rc = xpg_reply_dev(ctx, db, qd, rd);
+  if (!rc && hipMemcpyAsync(reply, rd, rw * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+    rc = set_err(ctx, XPG_EDEVICE, "D2H copy");
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = set_err(ctx, XPG_EDEVICE, "sync");
+  (void)hipFree(qd);
+  (void)hipFree(rd);
+  return rc;
+}
+
+int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs) {
+  if (!ctx || (!reqs && n_reqs)) return XPG_EINVAL;
+  for (uint32_t i = 0; i < n_reqs; i++) {
+    int rc = xpg_reply_dev(ctx, reqs[i].db, reqs[i].query_dev, reqs[i].reply_dev);
+    if (rc) return rc;
+  }
+  return XPG_OK;
+}
+
+// ---- timing ----
+int xpg_timing_enable(xpg_ctx *ctx, int on) {
+  if (!ctx) return XPG_EINVAL;
+  ctx->timing = on != 0;
+  return XPG_OK;
+}
+
+int xpg_timing_read(xpg_ctx *ctx, double *mac_ms, uint64_t *launches, int reset) {
+  if (!ctx) return XPG_EINVAL;
+  timing_collect(ctx);
+  if (mac_ms) *mac_ms = ctx->mac_ms;
+  if (launches) *launches = ctx->mac_launches;
+  if (reset) {
+    ctx->mac_ms = 0;
+    ctx->mac_launches = 0;
+  }
+  return XPG_OK;
+}
+
+// ---- client side ----
+int xpg_keygen(const xpg_params *prm, uint64_t seed, uint64_t *sk) {
+  if (!prm || !sk) return XPG_EINVAL;
+  for (uint64_t c = 0; c < prm->m; c++) {
+    const uint64_t p = prm->moduli[c];
+    for (uint64_t i = 0; i < prm->n; i++) {
+      uint64_t r = rand_word(seed, c * prm->n + i) & ((1ULL << 61) - 1);
+      sk[c * prm->n + i] = r >= p ? r - p : r;
+    }
+  }
+  return XPG_OK;
+}
+
+int xpg_query_gen_dev(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *sk_dev, uint32_t d,
+                      const uint32_t *dims, uint64_t index, uint64_t noise_seed, uint64_t *query_dev) {
+  if (!ctx || !prm || !sk_dev || !dims || !query_dev || d == 0 || d > XPG_MAX_LEVELS) return XPG_EINVAL;
+  uint64_t x = index, elem = 0;
+  const uint64_t mn = (uint64_t)prm->m * prm->n;
+  for (uint32_t l = 0; l < d; l++) {
+    EncSrc es{};
+    es.sk = sk_dev;
+    es.noise_seed = noise_seed;
+    es.elem0 = elem;
+    es.hot = x % dims[l];  // PIRQueryGenerator_internal.cpp:61-70
+    x /= dims[l];
+    int rc = launch_encrypt(ctx, prm, es, dims[l], query_dev + elem * 2 * mn);
+    if (rc) return rc;
+    elem += dims[l];
+  }
+  return XPG_OK;
+}
+
+int xpg_random_bytes_dev(xpg_ctx *ctx, uint64_t seed, uint64_t offset, uint8_t *dst_dev, uint64_t nbytes) {
+  if (!ctx || !dst_dev) return XPG_EINVAL;
+  return launch_random_bytes(ctx, seed, offset, dst_dev, nbytes);
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// internal.hpp -- host-side structures shared by the HIP translation units of
+// libxpir_amd.so.  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/xpir_amd.h"
+#include "arith.hpp"
+
+// NTT tables for one (n, m) on one device: per modulus n words each, the reference's
+// layouts (crypto/NFLlib.cpp:44-194, crypto/NFLlib.hpp:571-586).
+struct DevTables {
+  uint32_t n = 0, m = 0, logn = 0;
+  uint64_t *phi = nullptr, *phiq = nullptr;  // phi^j and Shoup quotients
+  uint64_t *w = nullptr, *wq = nullptr;      // DIF twiddles, layer after layer (n-1 used)
+};
+
+// Source of the coefficients an import kernel transforms.
+enum SrcKind : int { SRC_BYTES = 0, SRC_ENCRYPT = 1 };
+
+// Byte source: unit u covers bytes [u*stride, u*stride + len) of base, clipped at limit;
+// bytes past the clip read as zero.
+struct ByteSrc {
+  const uint8_t *base;
+  uint64_t stride, len, limit;
+};
+
+// Query-encryption source: element `elem0 + u` encrypts bit (u == hot) (NFLLWE.cpp:276-372).
+struct EncSrc {
+  const uint64_t *sk;  // m*n key words
+  uint64_t noise_seed;
+  uint64_t elem0;
+  uint64_t hot;        // unit index whose message bit is 1 (or ~0)
+  uint64_t emask, ebound;
+  uint64_t amp[XPG_MAX_M], ampq[XPG_MAX_M];  // 2^A mod p and quotient
+  uint64_t skq_unused;
+};
+
+struct xpg_db {
+  xpg_ctx *ctx = nullptr;
+  xpg_params prm{};
+  uint32_t d = 1;
+  uint32_t dims[XPG_MAX_LEVELS] = {0};
+  uint64_t units = 0;      // prod(dims)
+  uint64_t real_units = 0; // ceil(n_elems / alpha)
+  uint32_t P0 = 0;
+  uint32_t Pl[XPG_MAX_LEVELS + 1] = {0};  // polys per unit at each level; Pl[d] = reply polys
+  uint64_t n_elems = 0, elem_bytes = 0;
+  uint32_t alpha = 1;
+  uint64_t *words = nullptr;  // units x P0 x m x n
+};
+
+struct xpg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<uint64_t, DevTables> tables;  // key (n << 8) | m
+  // scratch reused across calls (grown on demand)
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  uint64_t *level_buf[2] = {nullptr, nullptr};
+  size_t level_bytes[2] = {0, 0};
+  // MAC timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double mac_ms = 0.0;
+  uint64_t mac_launches = 0;
+};
+
+// ---- host helpers (context.hip) ----
+int set_err(xpg_ctx *ctx, int code, const std::string &msg);
+const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m);
+int ensure_scratch(xpg_ctx *ctx, size_t bytes);
+int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
+ModSet modset_of(const xpg_params *prm);
+void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
+void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b);
+
+// ---- launchers (import.hip) ----
+// Transform `units` units of a byte source into NTT words (units x P x m x n); the
+// units in [units, units_total) are zero polys.
+int launch_import_bytes(xpg_ctx *ctx, const xpg_params *prm, const ByteSrc &src, uint64_t units,
+                        uint64_t units_total, uint32_t P, uint64_t *out);
+// Encrypt `count` query elements (each a||b, canonical) into out.
+int launch_encrypt(xpg_ctx *ctx, const xpg_params *prm, const EncSrc &src, uint64_t count,
+                   uint64_t *out);
+int launch_random_bytes(xpg_ctx *ctx, uint64_t seed, uint64_t offset, uint8_t *dst, uint64_t nbytes);
+int launch_fill_zero(xpg_ctx *ctx, uint64_t *dst, uint64_t words);
+
+// ---- launchers (mac.hip) ----
+// One reply level: out (groups x P x 2 x m x n) from db (groups*n_l x P x m x n) and
+// q (n_l x 2 x m x n).
+int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,
+                     uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out);
