@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from oracle.oracle import Oracle, RefNFL, build  # noqa: E402
-from tests.configs import C1_PLAIN, C2_SINGLE, C3_SCALED  # noqa: E402
+from xpir_amd.workloads import C1_PLAIN, C2_SINGLE, C3_SCALED  # noqa: E402
 
 
 def sha(a: np.ndarray) -> str:

@@ -102,7 +102,7 @@ def test_reply_matches_oracle(gpu_ctx, desc, n_elems, elem_bytes, dims):
 def test_c1_plain_full(gpu_ctx):
     """LWE:80:1024:60, 1000 x 10 KiB from seed 1, index from seed 2, noise seed 3."""
     import xpir_amd
-    from tests.configs import C1_PLAIN as cfg
+    from xpir_amd.workloads import C1_PLAIN as cfg
     orc = Oracle(cfg["crypto"], cfg["dims"][0])
     prm = xpir_amd.params(cfg["crypto"], cfg["dims"][0])
     n, L = cfg["n_elems"], cfg["elem_bytes"]
