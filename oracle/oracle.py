@@ -270,6 +270,8 @@ class RefNFL:
             L.xr_modulus.argtypes = [C.c_void_p, C.c_uint]
             L.xr_reply_level.argtypes = [C.c_void_p, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint,
                                          u64p, C.c_int]
+            L.xr_reply_level_timed.argtypes = [C.c_void_p, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint,
+                                               u64p, C.c_int, C.POINTER(C.c_double)]
             cls._lib = L
         return cls._lib
 
@@ -305,8 +307,13 @@ class RefNFL:
         return x
 
     def reply_level(self, db, q, n_l, groups, P, threads=1):
+        return self.reply_level_timed(db, q, n_l, groups, P, threads)[0]
+
+    def reply_level_timed(self, db, q, n_l, groups, P, threads=1):
+        """(reply words, seconds spent in the MAC loops)"""
         db = np.ascontiguousarray(db, dtype=np.uint64)
         q = np.ascontiguousarray(q, dtype=np.uint64)
         out = np.zeros(groups * P * 2 * self.mn, dtype=np.uint64)
-        self.lib().xr_reply_level(self.h, db, q, n_l, groups, P, out, threads)
-        return out
+        secs = C.c_double()
+        self.lib().xr_reply_level_timed(self.h, db, q, n_l, groups, P, out, threads, C.byref(secs))
+        return out, secs.value

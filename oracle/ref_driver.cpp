@@ -16,6 +16,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 struct RefHandle {
   NFLlib nfl;
   unsigned n;
@@ -77,8 +79,13 @@ uint64_t xr_modulus(void *hp, unsigned i) { return static_cast<RefHandle *>(hp)-
 
 // One reply level.  db: (groups*n_l) x P x m x n; q: n_l x {a,b} x m x n (canonical);
 // out: groups x P x {a,b} x m x n.  Shoup quotients as pushQuery computes them.
-void xr_reply_level(void *hp, const uint64_t *db, const uint64_t *q, uint64_t n_l,
-                    uint64_t groups, unsigned P, uint64_t *out, int threads) {
+// mac_seconds (optional) receives the time of the MAC loops alone, the span the
+// reference's own "Global reply generation" timer covers
+// (PIRReplyGeneratorNFL_internal.cpp:359,444); the query Shoup precompute happens on
+// pushQuery, before that span.
+void xr_reply_level_timed(void *hp, const uint64_t *db, const uint64_t *q, uint64_t n_l,
+                          uint64_t groups, unsigned P, uint64_t *out, int threads,
+                          double *mac_seconds) {
   RefHandle *h = static_cast<RefHandle *>(hp);
   const uint64_t n = h->n, m = h->m, mn = n * m;
   const uint64_t *moduli = h->nfl.getmoduli();
@@ -91,6 +98,7 @@ void xr_reply_level(void *hp, const uint64_t *db, const uint64_t *q, uint64_t n_
           qq[idx] = ((uint128_t)q[idx] << 64) / moduli[c];
         }
   if (threads <= 0) threads = 1;
+  double t0 = omp_get_wtime();
   for (uint64_t g = 0; g < groups; g++) {
 #pragma omp parallel for num_threads(threads)
     for (unsigned k = 0; k < P; k++) {
@@ -109,6 +117,12 @@ void xr_reply_level(void *hp, const uint64_t *db, const uint64_t *q, uint64_t n_
       }
     }
   }
+  if (mac_seconds) *mac_seconds = omp_get_wtime() - t0;
+}
+
+void xr_reply_level(void *hp, const uint64_t *db, const uint64_t *q, uint64_t n_l,
+                    uint64_t groups, unsigned P, uint64_t *out, int threads) {
+  xr_reply_level_timed(hp, db, q, n_l, groups, P, out, threads, nullptr);
 }
 
 }  // extern "C"

@@ -1,0 +1,45 @@
+"""The C++ facade (xpir_amd/cpp/libpir_amd.hpp, the reference's libpir.hpp server API)
+driven end to end by tests/cpp/simple_pir.cpp, compared with the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, extract_reply
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DEMO = os.path.join(ROOT, "xpir_amd", "_build", "simple_pir")
+
+
+@pytest.mark.parametrize("crypto,n,L,alpha,dims", [
+    ("LWE:80:1024:60", 50, 3000, 1, [50]),
+    ("LWE:80:2048:120", 30, 4096, 1, [6, 5]),
+    ("LWE:80:2048:120", 12, 1000, 3, [4]),
+])
+def test_simple_pir_through_the_facade(tmp_path, crypto, n, L, alpha, dims):
+    out = tmp_path / "reply.bin"
+    index = n // alpha - 1
+    args = [DEMO, crypto, str(n), str(L), str(alpha), str(len(dims)), *map(str, dims), "1", str(index), "3", "4",
+            str(out)]
+    print("run:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    got = np.fromfile(out, dtype=np.uint64)
+    orc = Oracle(crypto, dims[0])
+    data = orc.rand_bytes(1, 0, n * L)
+    unit = L * alpha
+    units = -(-n // alpha)
+    padded = np.zeros(units * unit, dtype=np.uint8)
+    padded[:data.size] = data
+    P = orc.polys(unit)
+    words = np.zeros(int(np.prod(dims)) * P * orc.mn, dtype=np.uint64)
+    words[:units * P * orc.mn] = np.concatenate([orc.import_element(padded[u * unit:(u + 1) * unit])
+                                                  for u in range(units)])
+    sk = orc.keygen(4)
+    q = orc.make_query(sk, dims, index, 3)
+    want, _ = orc.generate_reply(words, P, dims, q)
+    assert np.array_equal(got, want)
+    assert extract_reply(orc, sk, got, dims, unit) == padded[index * unit:(index + 1) * unit].tobytes()

@@ -50,9 +50,26 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    # the C++ facade (reference-shaped server API) and its end-to-end demo program
+    facade = os.path.join(HERE, "libpir_amd.so")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", facade + ".tmp",
+           os.path.join(HERE, "cpp", "libpir_amd.cpp"), "-L", HERE, "-lxpir_amd", "-Wl,-rpath,$ORIGIN"]
+    _run(cmd)
+    os.replace(facade + ".tmp", facade)
+    demo = os.path.join(OBJ, "simple_pir")
+    cmd = ["g++", "-O2", "-std=c++17", "-o", demo, os.path.join(ROOT, "tests", "cpp", "simple_pir.cpp"),
+           "-L", HERE, "-lpir_amd", "-lxpir_amd", "-Wl,-rpath,$ORIGIN/.."]
+    _run(cmd)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, facade, demo)
     return LIB
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd[:3]))
 
 
 if __name__ == "__main__":

@@ -18,7 +18,7 @@ XPG_OK, XPG_EINVAL, XPG_ENOMEM, XPG_EDEVICE, XPG_ESIZE, XPG_EBUSY = 0, -1, -2, -
 # every entry point include/xpir_amd.h declares
 EXPORTS = [
     "xpg_params_from_string", "xpg_polys_for_bytes", "xpg_reply_polys", "xpg_ctx_create",
-    "xpg_ctx_destroy", "xpg_last_error", "xpg_ctx_stream", "xpg_sync", "xpg_dev_alloc",
+    "xpg_ctx_destroy", "xpg_last_error", "xpg_ctx_stream", "xpg_ctx_set_option", "xpg_sync", "xpg_dev_alloc",
     "xpg_dev_free", "xpg_h2d", "xpg_d2h", "xpg_db_import", "xpg_db_import_dev",
     "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
     "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev",
@@ -62,6 +62,7 @@ def lib() -> C.CDLL:
         "xpg_ctx_destroy": (None, [vp]),
         "xpg_last_error": (C.c_char_p, [vp]),
         "xpg_ctx_stream": (vp, [vp]),
+        "xpg_ctx_set_option": (C.c_int, [vp, C.c_char_p, C.c_int64]),
         "xpg_sync": (C.c_int, [vp]),
         "xpg_dev_alloc": (C.c_int, [vp, u64, C.POINTER(vp)]),
         "xpg_dev_free": (C.c_int, [vp, vp]),
@@ -196,6 +197,9 @@ class Context:
 
     def sync(self):
         self._check(lib().xpg_sync(self.h))
+
+    def set_option(self, key: str, value: int):
+        self._check(lib().xpg_ctx_set_option(self.h, key.encode(), value))
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)

@@ -1,0 +1,176 @@
+// libpir_amd.cpp -- the C++ facade of libpir_amd.hpp over the C ABI.
+#include "libpir_amd.hpp"
+
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+
+namespace xpir_amd {
+
+HomomorphicCrypto::HomomorphicCrypto(const std::string &desc) : desc_(desc) {
+  memset(&prm_, 0, sizeof(prm_));
+  ok_ = xpg_params_from_string(desc.c_str(), 1, &prm_) == XPG_OK;
+}
+
+long HomomorphicCrypto::setandgetAbsBitPerCiphertext(unsigned int elt_nbr) {
+  xpg_params p;
+  if (xpg_params_from_string(desc_.c_str(), elt_nbr, &p) != XPG_OK) return -1;
+  prm_ = p;
+  return prm_.A;
+}
+
+unsigned int HomomorphicCrypto::getAbsorptionBitsize() const { return (unsigned)(prm_.A * prm_.n); }
+unsigned int HomomorphicCrypto::getCiphertextBitsize() const { return (unsigned)(prm_.ct_bytes * 8); }
+
+HomomorphicCrypto *HomomorphicCryptoFactory::getCryptoMethod(std::string cryptoParams) {
+  HomomorphicCrypto *c = new HomomorphicCrypto(cryptoParams);
+  if (!c->valid()) {
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+DBVectorProcessor::DBVectorProcessor(std::vector<element_t> &elements) : elements_(elements) {
+  for (auto &e : elements_) max_bytes_ = std::max<uint64_t>(max_bytes_, e.data_size);
+}
+
+bool DBVectorProcessor::openStream(uint64_t streamNb, uint64_t requested_offset) {
+  if (streamNb >= elements_.size()) return false;
+  offsets_[streamNb] = std::min<uint64_t>(requested_offset, elements_[streamNb].data_size);
+  return true;
+}
+
+uint64_t DBVectorProcessor::readStream(uint64_t streamNb, char *buf, uint64_t size) {
+  const element_t &e = elements_[streamNb];
+  uint64_t off = offsets_[streamNb];
+  uint64_t avail = e.data_size - off;
+  uint64_t take = std::min(avail, size);
+  memcpy(buf, e.data + off, take);
+  memset(buf + take, 0, size - take);
+  offsets_[streamNb] = off + take;
+  return size;
+}
+
+void DBVectorProcessor::closeStream(uint64_t streamNb) { offsets_.erase(streamNb); }
+
+DBGenerator::DBGenerator(uint64_t nbStreams, uint64_t streamBytesize, bool) : nb_(nbStreams), bytes_(streamBytesize) {}
+
+uint64_t DBGenerator::readStream(uint64_t, char *buf, uint64_t size) {
+  memset(buf, 0xaa, size);
+  return size;
+}
+
+imported_database::~imported_database() { xpg_db_free(imported_database_ptr); }
+
+PIRReplyGenerator::PIRReplyGenerator(PIRParameters &param, HomomorphicCrypto &cryptoMethod, DBHandler *db,
+                                     int device)
+    : param_(param), crypto_(cryptoMethod), db_(db) {
+  // setPirParams: A depends on n[0] (PIRReplyGeneratorNFL_internal.cpp:809-816)
+  crypto_.setandgetAbsBitPerCiphertext(param_.n[0]);
+  prm_ = crypto_.params();
+  int rc = xpg_ctx_create(device, &ctx_);
+  if (rc != XPG_OK) fail(rc, "no GPU context");
+}
+
+PIRReplyGenerator::~PIRReplyGenerator() {
+  freeQueries();
+  for (uint64_t i = next_reply_; i < replies_.size(); i++) free(replies_[i]);
+  xpg_ctx_destroy(ctx_);
+}
+
+int PIRReplyGenerator::fail(int code, const std::string &msg) {
+  status_ = code;
+  error_ = msg;
+  if (ctx_ && code != XPG_OK) {
+    const char *e = xpg_last_error(ctx_);
+    if (e && *e) error_ += std::string(": ") + e;
+  }
+  return code;
+}
+
+void PIRReplyGenerator::pushQuery(char *rawQuery) { queries_.push_back(rawQuery); }
+
+void PIRReplyGenerator::freeQueries() {
+  for (char *q : queries_) free(q);
+  queries_.clear();
+}
+
+imported_database *PIRReplyGenerator::importData(uint64_t offset, uint64_t bytes_per_db_element) {
+  status_ = XPG_OK;
+  if (!ctx_) {
+    fail(XPG_EDEVICE, "no GPU context");
+    return nullptr;
+  }
+  const uint64_t maxb = db_->getmaxFileBytesize();
+  const uint64_t file_bytes = offset >= maxb ? 0 : std::min(bytes_per_db_element, maxb - offset);
+  const uint64_t nb = db_->getNbStream();
+  if (file_bytes == 0 || nb == 0) {
+    fail(XPG_EINVAL, "empty database");
+    return nullptr;
+  }
+  std::vector<uint8_t> staging(nb * file_bytes);
+  for (uint64_t i = 0; i < nb; i++) {
+    db_->openStream(i, offset);
+    db_->readStream(i, reinterpret_cast<char *>(staging.data() + i * file_bytes), file_bytes);
+    db_->closeStream(i);
+  }
+  xpg_db *h = nullptr;
+  int rc = xpg_db_import(ctx_, &prm_, staging.data(), nb, file_bytes, param_.alpha, param_.d, param_.n, &h);
+  if (rc != XPG_OK) {
+    fail(rc, "importData");
+    return nullptr;
+  }
+  imported_database *out = new imported_database();
+  out->imported_database_ptr = h;
+  out->nbElements = (nb + param_.alpha - 1) / param_.alpha;
+  out->polysPerElement = xpg_db_polys(h);
+  out->beforeImportElementBytesize = bytes_per_db_element;
+  return out;
+}
+
+void PIRReplyGenerator::generateReply(const imported_database *database) {
+  status_ = XPG_OK;
+  for (uint64_t i = next_reply_; i < replies_.size(); i++) free(replies_[i]);
+  replies_.clear();
+  next_reply_ = generated_ = 0;
+  if (!database || !database->imported_database_ptr) {
+    fail(XPG_EINVAL, "no imported database");
+    return;
+  }
+  const xpg_db *h = database->imported_database_ptr;
+  const uint64_t qwords = xpg_db_query_words(h);
+  const uint64_t ct_words = 2ULL * prm_.m * prm_.n;
+  if (queries_.size() * ct_words != qwords) {
+    fail(XPG_ESIZE, "pushed query elements do not match the database grid");
+    return;
+  }
+  std::vector<uint64_t> q(qwords);
+  for (size_t i = 0; i < queries_.size(); i++) memcpy(q.data() + i * ct_words, queries_[i], ct_words * 8);
+  const uint64_t nrep = xpg_db_reply_polys(h);
+  std::vector<uint64_t> r(nrep * ct_words);
+  int rc = xpg_reply(ctx_, h, q.data(), qwords, r.data(), r.size());
+  if (rc != XPG_OK) {
+    fail(rc, "generateReply");
+    return;
+  }
+  for (uint64_t i = 0; i < nrep; i++) {
+    char *e = static_cast<char *>(malloc(ct_words * 8));
+    memcpy(e, r.data() + i * ct_words, ct_words * 8);
+    replies_.push_back(e);
+  }
+  generated_ = nrep;
+}
+
+bool PIRReplyGenerator::popReply(char **reply) {
+  if (next_reply_ >= replies_.size()) return false;
+  *reply = replies_[next_reply_];
+  replies_[next_reply_++] = nullptr;
+  return true;
+}
+
+uint64_t PIRReplyGenerator::getnbRepliesGenerated() { return generated_; }
+uint64_t PIRReplyGenerator::getReplyElementBytesize() { return prm_.ct_bytes; }
+
+}  // namespace xpir_amd

@@ -1,0 +1,155 @@
+// libpir_amd.hpp -- C++ facade with the reference library's server-side API, running
+// the reply on an MI355X through the C ABI of libxpir_amd.so.
+//
+// Mirrors libpir.hpp (reference root) for the NFLLWE reply path:
+//   HomomorphicCryptoFactory::getCryptoMethod   libpir.hpp:52-66, pir/libpir.cpp:20-23
+//   HomomorphicCrypto (the NFLLWE parameters)   crypto/HomomorphicCrypto.hpp:41-63
+//   PIRParameters                               pir/PIRParameters.hpp:22-29
+//   DBHandler / DBVectorProcessor / DBGenerator pir/dbhandlers/DBHandler.hpp:26-46,
+//                                               DBVectorProcessor.hpp:28-60, DBGenerator.hpp
+//   imported_database                           libpir.hpp:23-33, GenericPIRReplyGenerator.hpp:34-41
+//   PIRReplyGenerator                           libpir.hpp:101-169, pir/libpir.cpp:77-163
+// Names, argument meaning and ownership rules are the reference's: pushQuery takes
+// ownership of the query element (freed by freeQueries), popReply hands over one
+// malloc'd reply element per call and returns false when none is left.  Where the
+// reference exit()s or prints, this facade keeps the object usable and reports the
+// status through lastStatus()/lastError().
+#pragma once
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/xpir_amd.h"
+
+namespace xpir_amd {
+
+#ifndef MAX_REC_LVL
+#define MAX_REC_LVL 10
+#endif
+
+struct PIRParameters {
+  unsigned int alpha;
+  unsigned int d;
+  unsigned int n[MAX_REC_LVL];
+  std::string crypto_params;
+};
+
+// The NFLLWE cryptosystem's public parameters ("LWE:k:n:bits").
+class HomomorphicCrypto {
+ public:
+  explicit HomomorphicCrypto(const std::string &desc);
+  // NFLLWE::setandgetAbsBitPerCiphertext (crypto/NFLLWE.cpp:656-668)
+  long setandgetAbsBitPerCiphertext(unsigned int elt_nbr);
+  unsigned int getAbsorptionBitsize() const;  // A * n   (NFLLWEPublicParameters.cpp:138-141)
+  unsigned int getCiphertextBitsize() const;  // 128*n*m (NFLLWEPublicParameters.cpp:148-151)
+  unsigned int getpolyDegree() const { return prm_.n; }
+  unsigned short getnbModuli() const { return (unsigned short)prm_.m; }
+  std::string toString() const { return "LWE"; }
+  const std::string &descriptor() const { return desc_; }
+  const xpg_params &params() const { return prm_; }
+  bool valid() const { return ok_; }
+
+ private:
+  std::string desc_;
+  xpg_params prm_;
+  bool ok_;
+};
+
+class HomomorphicCryptoFactory {
+ public:
+  // NULL when the string names no supported NFLLWE parameter set
+  // (HomomorphicCryptoFactory_internal.cpp:46-65)
+  static HomomorphicCrypto *getCryptoMethod(std::string cryptoParams);
+};
+
+struct Element {
+  Element(std::string name_, uint64_t size_, char *data_) : name(name_), data_size(size_), data(data_) {}
+  std::string name;
+  size_t data_size;
+  char *data;
+};
+typedef Element element_t;
+
+class DBHandler {
+ public:
+  virtual uint64_t getNbStream() = 0;
+  virtual uint64_t getmaxFileBytesize() = 0;
+  virtual bool openStream(uint64_t streamNb, uint64_t requested_offset) = 0;
+  virtual uint64_t readStream(uint64_t streamNb, char *buf, uint64_t size) = 0;
+  virtual void closeStream(uint64_t streamNb) = 0;
+  virtual ~DBHandler() {}
+};
+
+// In-memory elements; short elements read as zero-padded (DBVectorProcessor.cpp:88-106).
+class DBVectorProcessor : public DBHandler {
+ public:
+  explicit DBVectorProcessor(std::vector<element_t> &elements);
+  uint64_t getNbStream() override { return elements_.size(); }
+  uint64_t getmaxFileBytesize() override { return max_bytes_; }
+  bool openStream(uint64_t streamNb, uint64_t requested_offset) override;
+  uint64_t readStream(uint64_t streamNb, char *buf, uint64_t size) override;
+  void closeStream(uint64_t streamNb) override;
+
+ private:
+  std::vector<element_t> &elements_;
+  std::map<uint64_t, uint64_t> offsets_;
+  uint64_t max_bytes_ = 0;
+};
+
+// Synthetic database of constant 0xaa bytes (DBGenerator.cpp:91-92,105).
+class DBGenerator : public DBHandler {
+ public:
+  DBGenerator(uint64_t nbStreams, uint64_t streamBytesize, bool random = false);
+  uint64_t getNbStream() override { return nb_; }
+  uint64_t getmaxFileBytesize() override { return bytes_; }
+  bool openStream(uint64_t, uint64_t) override { return true; }
+  uint64_t readStream(uint64_t, char *buf, uint64_t size) override;
+  void closeStream(uint64_t) override {}
+
+ private:
+  uint64_t nb_, bytes_;
+};
+
+// The pre-processed (NTT-domain) database, resident in HBM until destroyed.
+class imported_database {
+ public:
+  ~imported_database();
+  xpg_db *imported_database_ptr = nullptr;
+  uint64_t nbElements = 0;
+  uint64_t polysPerElement = 0;
+  uint64_t beforeImportElementBytesize = 0;
+};
+
+class PIRReplyGenerator {
+ public:
+  PIRReplyGenerator(PIRParameters &param, HomomorphicCrypto &cryptoMethod, DBHandler *db, int device = 0);
+  ~PIRReplyGenerator();
+  // one query element (16*n*m bytes, a||b), level after level; ownership passes here
+  void pushQuery(char *rawQuery);
+  imported_database *importData(uint64_t offset, uint64_t bytes_per_db_element);
+  void generateReply(const imported_database *database);
+  void freeQueries();
+  bool popReply(char **reply);
+  uint64_t getnbRepliesGenerated();
+  uint64_t getReplyElementBytesize();
+  // status of the last call (XPG_OK or an XPG_E* code) and its message
+  int lastStatus() const { return status_; }
+  const std::string &lastError() const { return error_; }
+
+ private:
+  int fail(int code, const std::string &msg);
+  PIRParameters param_;
+  HomomorphicCrypto &crypto_;
+  xpg_params prm_;
+  DBHandler *db_;
+  xpg_ctx *ctx_ = nullptr;
+  std::vector<char *> queries_;
+  std::vector<char *> replies_;
+  uint64_t next_reply_ = 0, generated_ = 0;
+  int status_ = XPG_OK;
+  std::string error_;
+};
+
+}  // namespace xpir_amd

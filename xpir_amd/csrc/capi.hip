@@ -230,6 +230,15 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
 const char *xpg_last_error(const xpg_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 void *xpg_ctx_stream(xpg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
+int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value) {
+  if (!ctx || !key) return XPG_EINVAL;
+  if (strcmp(key, "mac_kernel") == 0 && value >= 0 && value <= 2) {
+    ctx->mac_mode = (int)value;
+    return XPG_OK;
+  }
+  return set_err(ctx, XPG_EINVAL, std::string("unknown option or value: ") + key);
+}
+
 int xpg_sync(xpg_ctx *ctx) {
   if (!ctx) return XPG_EINVAL;
   HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");

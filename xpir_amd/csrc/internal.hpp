@@ -65,6 +65,8 @@ struct xpg_ctx {
   size_t scratch_bytes = 0;
   uint64_t *level_buf[2] = {nullptr, nullptr};
   size_t level_bytes[2] = {0, 0};
+  // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 2 = segmented chains
+  int mac_mode = 0;
   // MAC timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;

@@ -68,7 +68,278 @@ __global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
   o[mn] = acc_b;
 }
 
+
+// ------------------------------------------------------------------------------------
+// Segmented exact chains.  With few chains (a single query: 8192 chains of 65536 steps)
+// the element axis has to be split.  Write acc = rho + j*p with rho = acc mod p.  A step
+// with lazy product r = rc + b*p (rc < p, b in {0,1}) gives
+//     rho' = (rho + rc) mod p,   carry c = [rho + rc >= p],
+//     j'   = max(j + b + c - 1, 0)
+// (acc + r >= p  <=>  j + b + c >= 1).  The rho chain is a plain modular sum; the j chain
+// is a max-plus map j -> max(j + D, M) whose composition is associative:
+//     (D1,M1) then (D2,M2) = (D1 + D2, max(M1 + D2, M2)).
+// Its per-step coefficients need the incoming rho, which a decoupled look-back over the
+// tiles' residue sums provides while the tile's products wait in registers, so every DB
+// and query word is read from HBM exactly once.  A fixup kernel folds the tiles' (D,M)
+// maps in element order: final word = rho_final + j_final * p, identical to the
+// sequential chain for every input.
+//
+// Tile = W wavefronts x S elements of one column (64 consecutive slots of one
+// (group, poly, modulus)); lane = slot; each lane carries the a and the b chain.
+// Status words (one per tile, lane, chain): state in bits 62-63 (1 = tile aggregate,
+// 2 = inclusive prefix), residue in the low 61 bits; zero = not yet published.  The
+// fixup kernel returns every status word and the tile ticket to zero.
+
+constexpr int32_t kNegInf = -(1 << 30);
+
+struct SegArgs {
+  ModSet ms;
+  const uint64_t *db;
+  const uint64_t *q;
+  uint64_t *out;
+  uint64_t n_l;
+  uint64_t columns;   // groups * P * m * (n / 64)
+  uint64_t tiles;     // per column
+  uint32_t P, n, m, sblocks;
+  uint64_t *status;   // tiles * columns * 128
+  int2 *dm;           // tiles * columns * 128
+  uint32_t *ticket;
+};
+
+__device__ __forceinline__ uint64_t addmod_c(uint64_t a, uint64_t b, uint64_t p) {
+  uint64_t s = a + b;
+  return s >= p ? s - p : s;
+}
+
+__device__ __forceinline__ uint64_t ld_status(const uint64_t *ptr) {
+  return __hip_atomic_load(ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(uint64_t *ptr, uint64_t v) {
+  __hip_atomic_store(ptr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int S, int W>
+__global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
+  __shared__ uint32_t tile_sh;
+  __shared__ uint64_t rsum[W][2][64];
+  __shared__ int2 dmsh[W][2][64];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) tile_sh = atomicAdd(a.ticket, 1u);
+  __syncthreads();
+  const uint64_t ticket = tile_sh;
+  const uint64_t tile = ticket / a.columns, column = ticket % a.columns;
+  const uint32_t sb = (uint32_t)(column % a.sblocks);
+  uint64_t r = column / a.sblocks;
+  const uint32_t c = (uint32_t)(r % a.m);
+  r /= a.m;
+  const uint32_t k = (uint32_t)(r % a.P);
+  const uint64_t g = r / a.P;
+  const uint64_t mn = (uint64_t)a.m * a.n;
+  const uint64_t cs = (uint64_t)c * a.n + sb * 64 + lane;
+  const uint64_t p = a.ms.p[c], delta = a.ms.delta[c];
+
+  // phase A: lazy products of this lane's S elements, kept in registers
+  const uint64_t e0 = (tile * W + w) * S;
+  const int valid = e0 >= a.n_l ? 0 : (int)min<uint64_t>(S, a.n_l - e0);
+  const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
+  const uint64_t *xp = a.db + ((g * a.n_l + e0) * a.P + k) * mn + cs;
+  const uint64_t *qp = a.q + e0 * qstride + cs;
+  uint64_t ra[S], rb[S];
+  uint64_t sa = 0, sbm = 0;
+  if (valid == S) {
+#pragma unroll
+    for (int j0 = 0; j0 < S; j0 += 4) {
+      uint64_t xv[4], av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        xv[i] = __builtin_nontemporal_load(xp + (j0 + i) * xstride);
+        av[i] = __builtin_nontemporal_load(qp + (j0 + i) * qstride);
+        bv[i] = __builtin_nontemporal_load(qp + (j0 + i) * qstride + mn);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        ra[j0 + i] = shoup_lazy(xv[i], av[i], shoup_quot(av[i], p, delta), p);
+        rb[j0 + i] = shoup_lazy(xv[i], bv[i], shoup_quot(bv[i], p, delta), p);
+        sa = addmod_c(sa, ra[j0 + i] >= p ? ra[j0 + i] - p : ra[j0 + i], p);
+        sbm = addmod_c(sbm, rb[j0 + i] >= p ? rb[j0 + i] - p : rb[j0 + i], p);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+      ra[j] = rb[j] = 0;
+      if (j < valid) {
+        const uint64_t xv = xp[j * xstride], av = qp[j * qstride], bv = qp[j * qstride + mn];
+        ra[j] = shoup_lazy(xv, av, shoup_quot(av, p, delta), p);
+        rb[j] = shoup_lazy(xv, bv, shoup_quot(bv, p, delta), p);
+        sa = addmod_c(sa, ra[j] >= p ? ra[j] - p : ra[j], p);
+        sbm = addmod_c(sbm, rb[j] >= p ? rb[j] - p : rb[j], p);
+      }
+    }
+  }
+  rsum[w][0][lane] = sa;
+  rsum[w][1][lane] = sbm;
+  __syncthreads();
+
+  // phase B: tile aggregate, decoupled look-back over earlier tiles of this column
+  if (w == 0) {
+    uint64_t ta = 0, tb = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+      ta = addmod_c(ta, rsum[i][0][lane], p);
+      tb = addmod_c(tb, rsum[i][1][lane], p);
+    }
+    uint64_t *my = a.status + ticket * 128;
+    uint64_t exa = 0, exb = 0;
+    if (tile > 0) {
+      st_status(my + lane, (1ULL << 62) | ta);
+      st_status(my + 64 + lane, (1ULL << 62) | tb);
+      bool done_a = false, done_b = false;
+      uint64_t pa = tile - 1, pb = tile - 1;
+      while (true) {
+        if (!done_a) {
+          const uint64_t v = ld_status(a.status + (pa * a.columns + column) * 128 + lane);
+          const uint64_t st = v >> 62;
+          if (st != 0) {
+            exa = addmod_c(exa, v & ((1ULL << 61) - 1), p);
+            if (st == 2) done_a = true; else pa--;
+          }
+        }
+        if (!done_b) {
+          const uint64_t v = ld_status(a.status + (pb * a.columns + column) * 128 + 64 + lane);
+          const uint64_t st = v >> 62;
+          if (st != 0) {
+            exb = addmod_c(exb, v & ((1ULL << 61) - 1), p);
+            if (st == 2) done_b = true; else pb--;
+          }
+        }
+        if (__all(done_a && done_b)) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    st_status(my + lane, (2ULL << 62) | addmod_c(exa, ta, p));
+    st_status(my + 64 + lane, (2ULL << 62) | addmod_c(exb, tb, p));
+    // exclusive residue at each wave's first element
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+      const uint64_t va = rsum[i][0][lane], vb = rsum[i][1][lane];
+      rsum[i][0][lane] = exa;
+      rsum[i][1][lane] = exb;
+      exa = addmod_c(exa, va, p);
+      exb = addmod_c(exb, vb, p);
+    }
+  }
+  __syncthreads();
+
+  // phase C: carries from the now-known residues -> this lane's max-plus maps
+  {
+    uint64_t rho_a = rsum[w][0][lane], rho_b = rsum[w][1][lane];
+    int32_t Da = 0, Ma = kNegInf, Db = 0, Mb = kNegInf;
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+      if (j < valid) {
+        {
+          const uint64_t rv = ra[j];
+          const int32_t b = rv >= p;
+          const uint64_t t = rho_a + (b ? rv - p : rv);
+          const int32_t cy = t >= p;
+          rho_a = cy ? t - p : t;
+          const int32_t d = b + cy - 1;
+          Da += d;
+          Ma = max(Ma + d, 0);
+        }
+        {
+          const uint64_t rv = rb[j];
+          const int32_t b = rv >= p;
+          const uint64_t t = rho_b + (b ? rv - p : rv);
+          const int32_t cy = t >= p;
+          rho_b = cy ? t - p : t;
+          const int32_t d = b + cy - 1;
+          Db += d;
+          Mb = max(Mb + d, 0);
+        }
+      }
+    }
+    dmsh[w][0][lane] = make_int2(Da, Ma);
+    dmsh[w][1][lane] = make_int2(Db, Mb);
+  }
+  __syncthreads();
+
+  // phase D: fold the W wave maps in element order; one map per tile, lane, chain
+  if (w == 0) {
+    int2 fa = make_int2(0, kNegInf), fb = make_int2(0, kNegInf);
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+      const int2 x = dmsh[i][0][lane], y = dmsh[i][1][lane];
+      fa = make_int2(fa.x + x.x, max(fa.y + x.x, x.y));
+      fb = make_int2(fb.x + y.x, max(fb.y + y.x, y.y));
+    }
+    int2 *o = a.dm + ticket * 128;
+    o[lane] = fa;
+    o[64 + lane] = fb;
+  }
+}
+
+// One thread per (column, lane, chain): fold the tile maps, emit the reply word, and
+// leave the status words and the ticket at zero for the next launch.
+__global__ void __launch_bounds__(256) mac_fixup_kernel(SegArgs a) {
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (t == 0) *a.ticket = 0;
+  const uint64_t total = a.columns * 128;
+  if (t >= total) return;
+  const uint64_t column = t >> 7;
+  const uint32_t cl = (uint32_t)(t & 127), chain = cl >> 6, lane = cl & 63;
+  int32_t j = 0;
+  for (uint64_t tile = 0; tile < a.tiles; tile++) {
+    const int2 f = a.dm[(tile * a.columns + column) * 128 + cl];
+    j = max(j + f.x, f.y);
+  }
+  const uint64_t last = ((a.tiles - 1) * a.columns + column) * 128 + cl;
+  const uint64_t rho = a.status[last] & ((1ULL << 61) - 1);
+  for (uint64_t tile = 0; tile < a.tiles; tile++) a.status[(tile * a.columns + column) * 128 + cl] = 0;
+  // column -> (g, k, c, sblock)
+  const uint32_t sb = (uint32_t)(column % a.sblocks);
+  uint64_t r = column / a.sblocks;
+  const uint32_t c = (uint32_t)(r % a.m);
+  const uint64_t gk = r / a.m;
+  const uint64_t mn = (uint64_t)a.m * a.n;
+  a.out[gk * 2 * mn + chain * mn + (uint64_t)c * a.n + sb * 64 + lane] = rho + (uint64_t)j * a.ms.p[c];
+}
+
 }  // namespace
+
+namespace {
+
+template <int S, int W>
+int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
+  const uint64_t tile_elems = (uint64_t)S * W;
+  sa.tiles = (sa.n_l + tile_elems - 1) / tile_elems;
+  const uint64_t words = sa.tiles * sa.columns * 128;
+  // scratch: status words | dm maps | ticket; zero on first use, kept zero by the fixup
+  const size_t bytes = words * 8 + words * 8 + 256;
+  if (ctx->scratch_bytes < bytes) {
+    int rc = ensure_scratch(ctx, bytes);
+    if (rc) return rc;
+    rc = launch_fill_zero(ctx, (uint64_t *)ctx->scratch, bytes / 8);
+    if (rc) return rc;
+  }
+  sa.status = (uint64_t *)ctx->scratch;
+  sa.dm = (int2 *)((uint64_t *)ctx->scratch + words);
+  sa.ticket = (uint32_t *)((uint64_t *)ctx->scratch + 2 * words);
+  const uint64_t blocks = sa.tiles * sa.columns;
+  if (blocks > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
+  hipLaunchKernelGGL((mac_seg_kernel<S, W>), dim3((uint32_t)blocks), dim3(64 * W), 0, ctx->stream, sa);
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
+  const uint64_t fix = sa.columns * 128;
+  hipLaunchKernelGGL(mac_fixup_kernel, dim3((uint32_t)((fix + 255) / 256)), dim3(256), 0, ctx->stream, sa);
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC fixup launch");
+  return XPG_OK;
+}
+
+}  // namespace
+
+// Lanes at which one chain per lane already fills the GPU (256 CUs x 32 wavefronts).
+static const uint64_t kDirectLanes = 256ull * 32 * 64;
 
 int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,
                      uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out) {
@@ -84,12 +355,30 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   a.m = prm->m;
   a.logn = prm->logn;
   const uint64_t lanes = groups * P * (uint64_t)prm->m * prm->n;
-  const uint64_t blocks = (lanes + 255) / 256;
-  if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
-  hipLaunchKernelGGL(mac_direct_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
-  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
+  int rc = XPG_OK;
+  if (ctx->mac_mode == 1 || (ctx->mac_mode == 0 && (lanes >= kDirectLanes || n_l <= 64 || prm->n < 64))) {
+    const uint64_t blocks = (lanes + 255) / 256;
+    if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
+    hipLaunchKernelGGL(mac_direct_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
+    if (hipGetLastError() != hipSuccess) rc = set_err(ctx, XPG_EDEVICE, "MAC launch");
+  } else {
+    SegArgs sa{};
+    sa.ms = a.ms;
+    sa.db = db;
+    sa.q = q;
+    sa.out = out;
+    sa.n_l = n_l;
+    sa.P = P;
+    sa.n = prm->n;
+    sa.m = prm->m;
+    sa.sblocks = prm->n / 64;
+    sa.columns = groups * P * prm->m * sa.sblocks;
+    // enough tiles that columns x tiles x W wavefronts fill the GPU several times
+    if (n_l >= 16 * 8 * 16) rc = launch_seg<16, 8>(ctx, sa);
+    else rc = launch_seg<8, 4>(ctx, sa);
+  }
   timing_end(ctx, e0, e1);
-  return XPG_OK;
+  return rc;
 }
