@@ -101,7 +101,10 @@ const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m) {
 
 int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes) {
   if (ctx->level_bytes[which] >= bytes) return XPG_OK;
-  if (ctx->level_buf[which]) (void)hipFree(ctx->level_buf[which]);
+  if (ctx->level_buf[which]) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->level_buf[which]);
+  }
   ctx->level_buf[which] = nullptr;
   ctx->level_bytes[which] = 0;
   HIP_OK(hipMalloc((void **)&ctx->level_buf[which], bytes), "level buffer");
@@ -109,11 +112,41 @@ int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes) {
   return XPG_OK;
 }
 
+int ensure_batch_buf(xpg_ctx *ctx, size_t bytes) {
+  if (ctx->batch_bytes >= bytes) return XPG_OK;
+  if (ctx->batch_buf) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->batch_buf);
+  }
+  ctx->batch_buf = nullptr;
+  ctx->batch_bytes = 0;
+  HIP_OK(hipMalloc((void **)&ctx->batch_buf, bytes), "batch buffer");
+  ctx->batch_bytes = bytes;
+  return XPG_OK;
+}
+
+int ensure_ptr_buf(xpg_ctx *ctx, size_t count) {
+  if (ctx->ptr_count >= count) return XPG_OK;
+  if (ctx->ptr_buf) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->ptr_buf);
+  }
+  ctx->ptr_buf = nullptr;
+  ctx->ptr_count = 0;
+  HIP_OK(hipMalloc((void **)&ctx->ptr_buf, count * sizeof(void *)), "pointer buffer");
+  ctx->ptr_count = count;
+  return XPG_OK;
+}
+
 int ensure_scratch(xpg_ctx *ctx, size_t bytes) {
   if (ctx->scratch_bytes >= bytes) return XPG_OK;
-  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->scratch) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->scratch);
+  }
   ctx->scratch = nullptr;
   ctx->scratch_bytes = 0;
+  ctx->seg_capacity = 0;
   HIP_OK(hipMalloc(&ctx->scratch, bytes), "scratch");
   ctx->scratch_bytes = bytes;
   return XPG_OK;
@@ -222,6 +255,8 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
   for (int i = 0; i < 2; i++)
     if (ctx->level_buf[i]) (void)hipFree(ctx->level_buf[i]);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->batch_buf) (void)hipFree(ctx->batch_buf);
+  if (ctx->ptr_buf) (void)hipFree(ctx->ptr_buf);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -393,27 +428,37 @@ void xpg_db_free(xpg_db *db) {
 }
 
 // ---- replies (PIRReplyGeneratorNFL_internal.cpp:340-445) ----
-int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev) {
-  if (!ctx || !db || !query_dev || !reply_dev) return set_err(ctx, XPG_EINVAL, "null argument");
-  if (db->ctx != ctx) return set_err(ctx, XPG_EINVAL, "database belongs to another context");
+// Levels [first, d) of one query.  `data` holds level `first`'s NTT database (P polys per
+// unit); `q` points at level `first`'s query part.  When first > 0 the caller passes the
+// previous level's reply in `prev_reply` and this routine starts with the recursion.
+static int run_levels(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const uint64_t *data, const uint64_t *q,
+                      const uint64_t *prev_reply, uint64_t *reply_dev) {
   const xpg_params *prm = &db->prm;
   const uint64_t mn = (uint64_t)prm->m * prm->n;
   // intermediate levels: level_buf[0] holds a level's reply, level_buf[1] the next
-  // level's NTT database (sized once, before any launch reads them)
+  // level's NTT database (sized before any launch reads them)
   size_t need_reply = 0, need_db = 0;
-  for (uint32_t l = 0; l + 1 < db->d; l++) {
+  for (uint32_t l = (first ? first - 1 : 0); l + 1 < db->d; l++) {
     const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
     need_reply = std::max<size_t>(need_reply, groups * db->Pl[l] * 2 * mn * 8);
     need_db = std::max<size_t>(need_db, groups * db->Pl[l + 1] * mn * 8);
   }
-  if (need_reply) {
+  if (need_db) {
     int rc = ensure_level_buf(ctx, 0, need_reply);
     if (!rc) rc = ensure_level_buf(ctx, 1, need_db);
     if (rc) return rc;
   }
-  const uint64_t *data = db->words;
-  const uint64_t *q = query_dev;
-  for (uint32_t l = 0; l < db->d; l++) {
+  if (first > 0) {
+    // fromResulttoInData on the previous level's reply
+    const uint32_t l = first - 1;
+    const uint64_t groups = prod_dims(db->d, db->dims, first);
+    const uint64_t unit = (uint64_t)db->Pl[l] * 2 * mn * 8;
+    ByteSrc src{(const uint8_t *)prev_reply, unit, unit, groups * unit};
+    int rc = launch_import_bytes(ctx, prm, src, groups, groups, db->Pl[first], ctx->level_buf[1]);
+    if (rc) return rc;
+    data = ctx->level_buf[1];
+  }
+  for (uint32_t l = first; l < db->d; l++) {
     const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
     const uint32_t P = db->Pl[l];
     const bool last = (l + 1 == db->d);
@@ -429,6 +474,79 @@ int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uin
       if (rc) return rc;
       data = ctx->level_buf[1];
     }
+  }
+  return XPG_OK;
+}
+
+int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev) {
+  if (!ctx || !db || !query_dev || !reply_dev) return set_err(ctx, XPG_EINVAL, "null argument");
+  if (db->ctx != ctx) return set_err(ctx, XPG_EINVAL, "database belongs to another context");
+  return run_levels(ctx, db, 0, db->words, query_dev, nullptr, reply_dev);
+}
+
+// Level 0 of every request on one database in one batched launch, then each request's
+// remaining levels.  Worth it when level 0 has enough chains to fill the GPU per query
+// group (the batched kernel runs one chain per lane).
+static int batch_one_db(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs) {
+  const xpg_params *prm = &db->prm;
+  const uint64_t mn = (uint64_t)prm->m * prm->n;
+  const uint64_t groups0 = prod_dims(db->d, db->dims, 1);
+  const uint64_t lanes = groups0 * db->P0 * mn;
+  const uint32_t qn = (uint32_t)reqs.size();
+  if (qn < 2 || lanes * ((qn + 7) / 8) < (1ull << 16)) {
+    for (const xpg_request *r : reqs) {
+      int rc = xpg_reply_dev(ctx, db, r->query_dev, r->reply_dev);
+      if (rc) return rc;
+    }
+    return XPG_OK;
+  }
+  const uint64_t lvl0_words = groups0 * db->P0 * 2 * mn;
+  // level-0 outputs: the final replies when d == 1, else per-request intermediates
+  if (db->d > 1) {
+    int rc = ensure_batch_buf(ctx, qn * lvl0_words * 8);
+    if (rc) return rc;
+  }
+  std::vector<const uint64_t *> qp(qn);
+  std::vector<uint64_t *> op(qn);
+  for (uint32_t i = 0; i < qn; i++) {
+    qp[i] = reqs[i]->query_dev;
+    op[i] = db->d == 1 ? reqs[i]->reply_dev : ctx->batch_buf + i * lvl0_words;
+  }
+  int rc = ensure_ptr_buf(ctx, 2 * qn);
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(ctx->ptr_buf, qp.data(), qn * sizeof(void *), hipMemcpyHostToDevice, ctx->stream),
+         "pointer upload");
+  HIP_OK(hipMemcpyAsync(ctx->ptr_buf + qn, op.data(), qn * sizeof(void *), hipMemcpyHostToDevice, ctx->stream),
+         "pointer upload");
+  rc = launch_mac_batch(ctx, prm, db->words, (const uint64_t *const *)ctx->ptr_buf,
+                        (uint64_t *const *)(ctx->ptr_buf + qn), qn, db->dims[0], groups0, db->P0);
+  if (rc) return rc;
+  // the pointer arrays are read by the launch above: keep the host copies alive until it ran
+  HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");
+  if (db->d > 1) {
+    for (uint32_t i = 0; i < qn; i++) {
+      rc = run_levels(ctx, db, 1, nullptr, reqs[i]->query_dev + (uint64_t)db->dims[0] * 2 * mn, op[i],
+                      reqs[i]->reply_dev);
+      if (rc) return rc;
+    }
+  }
+  return XPG_OK;
+}
+
+int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs) {
+  if (!ctx || (!reqs && n_reqs)) return XPG_EINVAL;
+  // group requests by database, in order of first appearance
+  std::vector<const xpg_db *> order;
+  std::map<const xpg_db *, std::vector<const xpg_request *>> by_db;
+  for (uint32_t i = 0; i < n_reqs; i++) {
+    if (!reqs[i].db || !reqs[i].query_dev || !reqs[i].reply_dev) return set_err(ctx, XPG_EINVAL, "null request field");
+    if (reqs[i].db->ctx != ctx) return set_err(ctx, XPG_EINVAL, "database belongs to another context");
+    if (!by_db.count(reqs[i].db)) order.push_back(reqs[i].db);
+    by_db[reqs[i].db].push_back(&reqs[i]);
+  }
+  for (const xpg_db *db : order) {
+    int rc = batch_one_db(ctx, db, by_db[db]);
+    if (rc) return rc;
   }
   return XPG_OK;
 }
@@ -457,15 +575,6 @@ int xpg_reply(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query, uint64_t qu
   (void)hipFree(qd);
   (void)hipFree(rd);
   return rc;
-}
-
-int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs) {
-  if (!ctx || (!reqs && n_reqs)) return XPG_EINVAL;
-  for (uint32_t i = 0; i < n_reqs; i++) {
-    int rc = xpg_reply_dev(ctx, reqs[i].db, reqs[i].query_dev, reqs[i].reply_dev);
-    if (rc) return rc;
-  }
-  return XPG_OK;
 }
 
 // ---- timing ----

@@ -63,6 +63,11 @@ struct xpg_ctx {
   // scratch reused across calls (grown on demand)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  uint64_t seg_capacity = 0;
+  uint64_t *batch_buf = nullptr;  // level-0 replies of a batch (d > 1)
+  size_t batch_bytes = 0;
+  void **ptr_buf = nullptr;       // device arrays of query / output pointers
+  size_t ptr_count = 0;  // status words of the segmented MAC kept zero in scratch
   uint64_t *level_buf[2] = {nullptr, nullptr};
   size_t level_bytes[2] = {0, 0};
   // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 2 = segmented chains
@@ -80,6 +85,8 @@ int set_err(xpg_ctx *ctx, int code, const std::string &msg);
 const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m);
 int ensure_scratch(xpg_ctx *ctx, size_t bytes);
 int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
+int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
+int ensure_ptr_buf(xpg_ctx *ctx, size_t count);
 ModSet modset_of(const xpg_params *prm);
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
 void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b);
@@ -100,3 +107,6 @@ int launch_fill_zero(xpg_ctx *ctx, uint64_t *dst, uint64_t words);
 // q (n_l x 2 x m x n).
 int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,
                      uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out);
+// QN queries over one database level: q_dev / out_dev are device arrays of QN pointers
+int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q_dev,
+                     uint64_t *const *out_dev, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P);

@@ -70,6 +70,71 @@ __global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
 
 
 // ------------------------------------------------------------------------------------
+// Batched level: QN queries over one resident database level.  A lane carries the a and
+// b chains of QG queries for one (g, k, c, s), so each DB word it loads feeds 2*QG
+// products.  The QN/QG query groups of one column run as separate workgroups placed on
+// the same XCD (workgroup b runs on XCD b % 8), adjacent in dispatch order, so they read
+// the same DB rows at nearly the same time: each DB tile comes from HBM once and from
+// that XCD's L2 for the other query groups.  Chains stay sequential per lane, so each
+// reply is bit-identical to the single-query kernels.
+struct BatchArgs {
+  ModSet ms;
+  const uint64_t *db;
+  const uint64_t *const *q;  // QN query-level pointers (device array)
+  uint64_t *const *out;      // QN output pointers (device array)
+  uint64_t n_l, groups, columns;  // columns = groups * P * m * (n / 64)
+  uint32_t P, n, m, sblocks, qn, qgroups;
+};
+
+template <int QG>
+__global__ void __launch_bounds__(64) mac_batch_kernel(BatchArgs a) {
+  // workgroup -> (column, query group), query groups of a column on one XCD
+  const uint64_t b = blockIdx.x;
+  const uint64_t xcd = b & 7, j = b >> 3;
+  const uint64_t column = (j / a.qgroups) * 8 + xcd;
+  const uint32_t qg = (uint32_t)(j % a.qgroups);
+  if (column >= a.columns) return;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t sb = (uint32_t)(column % a.sblocks);
+  uint64_t r = column / a.sblocks;
+  const uint32_t c = (uint32_t)(r % a.m);
+  const uint64_t gk = r / a.m;
+  const uint32_t k = (uint32_t)(gk % a.P);
+  const uint64_t g = gk / a.P;
+  const uint64_t mn = (uint64_t)a.m * a.n;
+  const uint64_t cs = (uint64_t)c * a.n + sb * 64 + lane;
+  const uint64_t p = a.ms.p[c], delta = a.ms.delta[c];
+  const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
+  const uint64_t *x = a.db + (g * a.n_l * a.P + k) * mn + cs;
+  const uint64_t *qa[QG];
+  uint64_t acc_a[QG], acc_b[QG];
+  const uint32_t q0 = qg * QG;
+#pragma unroll
+  for (int i = 0; i < QG; i++) {
+    const uint32_t qi = min(q0 + i, a.qn - 1);
+    qa[i] = a.q[qi] + cs;
+    acc_a[i] = acc_b[i] = 0;
+  }
+  for (uint64_t e = 0; e < a.n_l; e++) {
+    const uint64_t xv = x[e * xstride];
+#pragma unroll
+    for (int i = 0; i < QG; i++) {
+      const uint64_t av = qa[i][e * qstride], bv = qa[i][e * qstride + mn];
+      acc_a[i] = mac_step(acc_a[i], xv, av, shoup_quot(av, p, delta), p);
+      acc_b[i] = mac_step(acc_b[i], xv, bv, shoup_quot(bv, p, delta), p);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < QG; i++) {
+    if (q0 + i < a.qn) {
+      uint64_t *o = a.out[q0 + i] + gk * 2 * mn + cs;
+      o[0] = acc_a[i];
+      o[mn] = acc_b[i];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // Segmented exact chains.  With few chains (a single query: 8192 chains of 65536 steps)
 // the element axis has to be split.  Write acc = rho + j*p with rho = acc mod p.  A step
 // with lazy product r = rc + b*p (rc < p, b in {0,1}) gives
@@ -315,17 +380,20 @@ int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
   const uint64_t tile_elems = (uint64_t)S * W;
   sa.tiles = (sa.n_l + tile_elems - 1) / tile_elems;
   const uint64_t words = sa.tiles * sa.columns * 128;
-  // scratch: status words | dm maps | ticket; zero on first use, kept zero by the fixup
-  const size_t bytes = words * 8 + words * 8 + 256;
-  if (ctx->scratch_bytes < bytes) {
+  // scratch layout: [ticket, 256 B][status words, capacity][dm maps, capacity].  The
+  // ticket and the whole status region are zero between launches: zeroed when the
+  // region is (re)allocated, and every launch's fixup zeroes exactly the words it used.
+  if (ctx->seg_capacity < words) {
+    const size_t bytes = 256 + 2 * words * 8;
     int rc = ensure_scratch(ctx, bytes);
     if (rc) return rc;
-    rc = launch_fill_zero(ctx, (uint64_t *)ctx->scratch, bytes / 8);
+    rc = launch_fill_zero(ctx, (uint64_t *)ctx->scratch, (256 + words * 8) / 8);
     if (rc) return rc;
+    ctx->seg_capacity = words;
   }
-  sa.status = (uint64_t *)ctx->scratch;
-  sa.dm = (int2 *)((uint64_t *)ctx->scratch + words);
-  sa.ticket = (uint32_t *)((uint64_t *)ctx->scratch + 2 * words);
+  sa.ticket = (uint32_t *)ctx->scratch;
+  sa.status = (uint64_t *)((char *)ctx->scratch + 256);
+  sa.dm = (int2 *)(sa.status + ctx->seg_capacity);
   const uint64_t blocks = sa.tiles * sa.columns;
   if (blocks > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   hipLaunchKernelGGL((mac_seg_kernel<S, W>), dim3((uint32_t)blocks), dim3(64 * W), 0, ctx->stream, sa);
@@ -381,4 +449,37 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   }
   timing_end(ctx, e0, e1);
   return rc;
+}
+
+int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q_dev,
+                     uint64_t *const *out_dev, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P) {
+  BatchArgs a{};
+  a.ms = modset_of(prm);
+  a.db = db;
+  a.q = q_dev;
+  a.out = out_dev;
+  a.n_l = n_l;
+  a.groups = groups;
+  a.P = P;
+  a.n = prm->n;
+  a.m = prm->m;
+  a.sblocks = prm->n / 64;
+  a.columns = groups * P * prm->m * a.sblocks;
+  a.qn = qn;
+  int qg = qn >= 8 ? 8 : (qn >= 4 ? 4 : (qn >= 2 ? 2 : 1));
+  a.qgroups = (qn + qg - 1) / qg;
+  const uint64_t col8 = (a.columns + 7) / 8;
+  const uint64_t blocks = col8 * 8 * a.qgroups;
+  if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "batched MAC grid too large");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  timing_begin(ctx, &e0, &e1);
+  switch (qg) {
+    case 8: hipLaunchKernelGGL(mac_batch_kernel<8>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
+    case 4: hipLaunchKernelGGL(mac_batch_kernel<4>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
+    case 2: hipLaunchKernelGGL(mac_batch_kernel<2>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
+    default: hipLaunchKernelGGL(mac_batch_kernel<1>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
+  }
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "batched MAC launch");
+  timing_end(ctx, e0, e1);
+  return XPG_OK;
 }
