@@ -126,6 +126,37 @@ typedef struct {
 } xpg_request;
 int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs);
 
+/* ---- serving loop: continuous batching over resident databases.  No reference
+ *      counterpart (the reference serves one query per session thread,
+ *      apps/server/PIRSession.cpp:371-493).  A dispatcher thread owns `workers` streams;
+ *      pending requests are grouped per database and launched as one batch on the first
+ *      idle worker, so arrivals during a batch overlap with it on the GPU.  Admission is
+ *      by reply memory: each request needs its reply slot (carved from a pool of
+ *      reply_pool_bytes) plus, for d > 1, its level-0 intermediates, all counted
+ *      against the same budget; a request that does not fit now gets XPG_EBUSY. ---- */
+typedef struct xpg_server xpg_server;
+typedef struct {
+  uint64_t reply_pool_bytes;  /* admission budget (device bytes)       */
+  uint32_t workers;           /* batches in flight (one stream each)   */
+  uint32_t max_batch;         /* queries per batch                     */
+} xpg_server_config;
+typedef struct {
+  uint64_t ticket;
+  uint64_t *reply_dev;        /* valid until xpg_server_release(ticket) */
+  uint64_t reply_words;
+  double submit_us, start_us, done_us;  /* server clock (xpg_server_now_us) */
+  int status;
+} xpg_completion;
+int xpg_server_create(int device, const xpg_server_config *cfg, xpg_server **out);
+void xpg_server_destroy(xpg_server *srv);
+/* query_dev stays owned by the caller and must stay valid until its completion */
+int xpg_server_submit(xpg_server *srv, const xpg_db *db, const uint64_t *query_dev, uint64_t *ticket);
+/* up to max completions; waits up to timeout_us when none is ready */
+int xpg_server_poll(xpg_server *srv, xpg_completion *out, uint32_t max, uint32_t *n_out, int64_t timeout_us);
+int xpg_server_release(xpg_server *srv, uint64_t ticket);
+double xpg_server_now_us(xpg_server *srv);
+int xpg_server_stats(xpg_server *srv, uint64_t *batches, uint64_t *batched_requests, uint64_t *budget_used);
+
 /* ---- kernel timing: HIP events around every MAC launch on the context stream ---- */
 int xpg_timing_enable(xpg_ctx *ctx, int on);
 /* total milliseconds and launch count of the MAC kernels since the last reset */

@@ -5,6 +5,6 @@ The compute path is libxpir_amd.so: hand-written HIP kernels for database import
 include/xpir_amd.h.  ``xpir_amd.capi`` is its ctypes binding; ``xpir_amd.cpp`` holds the
 C++ facade that mirrors the reference's libpir.hpp classes.
 """
-from .capi import Context, Database, DeviceBuffer, XpgError, keygen, lib, params  # noqa: F401
+from .capi import Context, Database, DeviceBuffer, Server, XpgError, keygen, lib, params  # noqa: F401
 
-__all__ = ["Context", "Database", "DeviceBuffer", "XpgError", "keygen", "lib", "params"]
+__all__ = ["Context", "Database", "DeviceBuffer", "Server", "XpgError", "keygen", "lib", "params"]

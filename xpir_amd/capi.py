@@ -23,7 +23,8 @@ EXPORTS = [
     "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
     "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev",
     "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_keygen",
-    "xpg_query_gen_dev", "xpg_random_bytes_dev",
+    "xpg_query_gen_dev", "xpg_random_bytes_dev", "xpg_server_create", "xpg_server_destroy",
+    "xpg_server_submit", "xpg_server_poll", "xpg_server_release", "xpg_server_now_us", "xpg_server_stats",
 ]
 
 
@@ -35,6 +36,16 @@ class XpgParams(C.Structure):
 
 class XpgRequest(C.Structure):
     _fields_ = [("db", C.c_void_p), ("query_dev", C.c_void_p), ("reply_dev", C.c_void_p)]
+
+
+class XpgServerConfig(C.Structure):
+    _fields_ = [("reply_pool_bytes", C.c_uint64), ("workers", C.c_uint32), ("max_batch", C.c_uint32)]
+
+
+class XpgCompletion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("reply_dev", C.c_void_p), ("reply_words", C.c_uint64),
+                ("submit_us", C.c_double), ("start_us", C.c_double), ("done_us", C.c_double),
+                ("status", C.c_int)]
 
 
 class XpgError(RuntimeError):
@@ -85,6 +96,13 @@ def lib() -> C.CDLL:
         "xpg_keygen": (C.c_int, [P, u64, vp]),
         "xpg_query_gen_dev": (C.c_int, [vp, P, vp, u32, C.POINTER(u32), u64, u64, vp]),
         "xpg_random_bytes_dev": (C.c_int, [vp, u64, u64, vp, u64]),
+        "xpg_server_create": (C.c_int, [C.c_int, C.POINTER(XpgServerConfig), C.POINTER(vp)]),
+        "xpg_server_destroy": (None, [vp]),
+        "xpg_server_submit": (C.c_int, [vp, vp, vp, C.POINTER(u64)]),
+        "xpg_server_poll": (C.c_int, [vp, C.POINTER(XpgCompletion), u32, C.POINTER(u32), C.c_int64]),
+        "xpg_server_release": (C.c_int, [vp, u64]),
+        "xpg_server_now_us": (C.c_double, [vp]),
+        "xpg_server_stats": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -204,6 +222,12 @@ class Context:
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
 
+    def download_ptr(self, ptr: int, count: int, dtype=np.uint64) -> np.ndarray:
+        """copy `count` items from a device pointer the library handed out"""
+        out = np.empty(count, dtype=dtype)
+        self._check(lib().xpg_d2h(self.h, _ptr(out), ptr, out.nbytes))
+        return out
+
     @staticmethod
     def _dims(dims):
         arr = (C.c_uint32 * len(dims))(*dims)
@@ -271,3 +295,50 @@ def keygen(prm: XpgParams, seed: int) -> np.ndarray:
     if rc != XPG_OK:
         raise XpgError("keygen")
     return sk
+
+
+class Server:
+    """Continuous-batching reply server (xpg_server_*)."""
+
+    def __init__(self, device: int = 0, reply_pool_bytes: int = 1 << 30, workers: int = 2, max_batch: int = 64):
+        cfg = XpgServerConfig(reply_pool_bytes, workers, max_batch)
+        h = C.c_void_p()
+        rc = lib().xpg_server_create(device, C.byref(cfg), C.byref(h))
+        if rc != XPG_OK:
+            raise XpgError(f"xpg_server_create failed: {rc}")
+        self.h = h.value
+        self._buf = (XpgCompletion * 256)()
+
+    def submit(self, db: "Database", query_dev: "DeviceBuffer"):
+        """ticket, or None when admission is refused (reply memory full)"""
+        t = C.c_uint64()
+        rc = lib().xpg_server_submit(self.h, db.h, query_dev.ptr, C.byref(t))
+        if rc == XPG_EBUSY:
+            return None
+        if rc != XPG_OK:
+            raise XpgError(f"submit failed: {rc}")
+        return t.value
+
+    def poll(self, timeout_us: int = 0):
+        n = C.c_uint32()
+        rc = lib().xpg_server_poll(self.h, self._buf, 256, C.byref(n), timeout_us)
+        if rc != XPG_OK:
+            raise XpgError(f"poll failed: {rc}")
+        return [XpgCompletion.from_buffer_copy(self._buf[i]) for i in range(n.value)]
+
+    def release(self, ticket: int):
+        if lib().xpg_server_release(self.h, ticket) != XPG_OK:
+            raise XpgError("release")
+
+    def now_us(self) -> float:
+        return lib().xpg_server_now_us(self.h)
+
+    def stats(self):
+        b, r, u = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        lib().xpg_server_stats(self.h, C.byref(b), C.byref(r), C.byref(u))
+        return {"batches": b.value, "batched_requests": r.value, "budget_used": u.value}
+
+    def close(self):
+        if self.h:
+            lib().xpg_server_destroy(self.h)
+            self.h = None

@@ -125,19 +125,6 @@ int ensure_batch_buf(xpg_ctx *ctx, size_t bytes) {
   return XPG_OK;
 }
 
-int ensure_ptr_buf(xpg_ctx *ctx, size_t count) {
-  if (ctx->ptr_count >= count) return XPG_OK;
-  if (ctx->ptr_buf) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(ctx->ptr_buf);
-  }
-  ctx->ptr_buf = nullptr;
-  ctx->ptr_count = 0;
-  HIP_OK(hipMalloc((void **)&ctx->ptr_buf, count * sizeof(void *)), "pointer buffer");
-  ctx->ptr_count = count;
-  return XPG_OK;
-}
-
 int ensure_scratch(xpg_ctx *ctx, size_t bytes) {
   if (ctx->scratch_bytes >= bytes) return XPG_OK;
   if (ctx->scratch) {
@@ -256,7 +243,6 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
     if (ctx->level_buf[i]) (void)hipFree(ctx->level_buf[i]);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->batch_buf) (void)hipFree(ctx->batch_buf);
-  if (ctx->ptr_buf) (void)hipFree(ctx->ptr_buf);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -480,7 +466,7 @@ static int run_levels(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const uint
 
 int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev) {
   if (!ctx || !db || !query_dev || !reply_dev) return set_err(ctx, XPG_EINVAL, "null argument");
-  if (db->ctx != ctx) return set_err(ctx, XPG_EINVAL, "database belongs to another context");
+  if (db->ctx->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
   return run_levels(ctx, db, 0, db->words, query_dev, nullptr, reply_dev);
 }
 
@@ -503,8 +489,8 @@ static int batch_one_db(xpg_ctx *ctx, const xpg_db *db, const std::vector<const 
   const uint64_t lvl0_words = groups0 * db->P0 * 2 * mn;
   // level-0 outputs: the final replies when d == 1, else per-request intermediates
   if (db->d > 1) {
-    int rc = ensure_batch_buf(ctx, qn * lvl0_words * 8);
-    if (rc) return rc;
+    int rc0 = ensure_batch_buf(ctx, qn * lvl0_words * 8);
+    if (rc0) return rc0;
   }
   std::vector<const uint64_t *> qp(qn);
   std::vector<uint64_t *> op(qn);
@@ -512,17 +498,12 @@ static int batch_one_db(xpg_ctx *ctx, const xpg_db *db, const std::vector<const 
     qp[i] = reqs[i]->query_dev;
     op[i] = db->d == 1 ? reqs[i]->reply_dev : ctx->batch_buf + i * lvl0_words;
   }
-  int rc = ensure_ptr_buf(ctx, 2 * qn);
+  int rc = XPG_OK;
+  for (uint32_t s = 0; s < qn && !rc; s += 64) {
+    const uint32_t cnt = std::min<uint32_t>(64, qn - s);
+    rc = launch_mac_batch(ctx, prm, db->words, qp.data() + s, op.data() + s, cnt, db->dims[0], groups0, db->P0);
+  }
   if (rc) return rc;
-  HIP_OK(hipMemcpyAsync(ctx->ptr_buf, qp.data(), qn * sizeof(void *), hipMemcpyHostToDevice, ctx->stream),
-         "pointer upload");
-  HIP_OK(hipMemcpyAsync(ctx->ptr_buf + qn, op.data(), qn * sizeof(void *), hipMemcpyHostToDevice, ctx->stream),
-         "pointer upload");
-  rc = launch_mac_batch(ctx, prm, db->words, (const uint64_t *const *)ctx->ptr_buf,
-                        (uint64_t *const *)(ctx->ptr_buf + qn), qn, db->dims[0], groups0, db->P0);
-  if (rc) return rc;
-  // the pointer arrays are read by the launch above: keep the host copies alive until it ran
-  HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");
   if (db->d > 1) {
     for (uint32_t i = 0; i < qn; i++) {
       rc = run_levels(ctx, db, 1, nullptr, reqs[i]->query_dev + (uint64_t)db->dims[0] * 2 * mn, op[i],
@@ -540,7 +521,7 @@ int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs) 
   std::map<const xpg_db *, std::vector<const xpg_request *>> by_db;
   for (uint32_t i = 0; i < n_reqs; i++) {
     if (!reqs[i].db || !reqs[i].query_dev || !reqs[i].reply_dev) return set_err(ctx, XPG_EINVAL, "null request field");
-    if (reqs[i].db->ctx != ctx) return set_err(ctx, XPG_EINVAL, "database belongs to another context");
+    if (reqs[i].db->ctx->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
     if (!by_db.count(reqs[i].db)) order.push_back(reqs[i].db);
     by_db[reqs[i].db].push_back(&reqs[i]);
   }

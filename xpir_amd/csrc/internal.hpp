@@ -86,7 +86,6 @@ const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m);
 int ensure_scratch(xpg_ctx *ctx, size_t bytes);
 int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
-int ensure_ptr_buf(xpg_ctx *ctx, size_t count);
 ModSet modset_of(const xpg_params *prm);
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
 void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b);
@@ -107,6 +106,7 @@ int launch_fill_zero(xpg_ctx *ctx, uint64_t *dst, uint64_t words);
 // q (n_l x 2 x m x n).
 int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,
                      uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out);
-// QN queries over one database level: q_dev / out_dev are device arrays of QN pointers
-int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q_dev,
-                     uint64_t *const *out_dev, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P);
+// QN (1..64) queries over one database level; q / out are host arrays of QN device
+// pointers (they travel in the kernel arguments)
+int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
+                     uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P);

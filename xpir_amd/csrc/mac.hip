@@ -77,11 +77,13 @@ __global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
 // the same DB rows at nearly the same time: each DB tile comes from HBM once and from
 // that XCD's L2 for the other query groups.  Chains stay sequential per lane, so each
 // reply is bit-identical to the single-query kernels.
+constexpr int kMaxBatch = 64;  // queries per launch; pointers travel in the kernel arguments
+
 struct BatchArgs {
   ModSet ms;
   const uint64_t *db;
-  const uint64_t *const *q;  // QN query-level pointers (device array)
-  uint64_t *const *out;      // QN output pointers (device array)
+  const uint64_t *q[kMaxBatch];  // query-level pointers
+  uint64_t *out[kMaxBatch];      // output pointers
   uint64_t n_l, groups, columns;  // columns = groups * P * m * (n / 64)
   uint32_t P, n, m, sblocks, qn, qgroups;
 };
@@ -451,13 +453,16 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   return rc;
 }
 
-int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q_dev,
-                     uint64_t *const *out_dev, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P) {
+int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
+                     uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P) {
+  if (qn == 0 || qn > (uint32_t)kMaxBatch) return set_err(ctx, XPG_EINVAL, "batch of 1..64 queries per launch");
   BatchArgs a{};
   a.ms = modset_of(prm);
   a.db = db;
-  a.q = q_dev;
-  a.out = out_dev;
+  for (uint32_t i = 0; i < qn; i++) {
+    a.q[i] = q[i];
+    a.out[i] = out[i];
+  }
   a.n_l = n_l;
   a.groups = groups;
   a.P = P;
