@@ -51,6 +51,39 @@ __device__ __forceinline__ uint64_t shoup_quot(uint64_t y, uint64_t p, uint64_t 
   return 8 * y + t + (r >= p ? 1ULL : 0ULL);
 }
 
+__device__ __forceinline__ uint64_t mad32(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
+
+// The lazy Shoup product x*y - floor(x*y'/2^64)*p, y' = floor(y*2^64/p) mod 2^64 derived
+// on the fly (shoup_quot), written as 32x32->64 multiply-adds (v_mad_u64_u32): 3 for
+// y', 4 for the high word of x*y', 3 for the low word of x*y, 2 for q*p via
+// q*p = (q << 61) - q*delta.  Equal to shoup_lazy(x, y, shoup_quot(y, p, delta), p) for
+// every x, y < 2^64.
+__device__ __forceinline__ uint64_t shoup_lazy_qf(uint64_t x, uint64_t y, uint64_t p, uint64_t delta) {
+  constexpr uint64_t M32 = 0xffffffffULL;
+  const uint32_t d = (uint32_t)delta, yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
+  // y' = 8y + t0 + [r >= p], with y*delta = p1:p0 and t0 = (8*y*delta) >> 61 = p1 >> 26
+  const uint64_t p0 = mad32(yl, d, 0);
+  const uint64_t p1 = mad32(yh, d, p0 >> 32);
+  const uint64_t t0 = p1 >> 26;
+  const uint64_t nmod = (((p1 << 32) | (p0 & M32)) & ((1ULL << 58) - 1)) << 3;
+  const uint64_t yq = (y << 3) + t0 + ((nmod + mad32((uint32_t)t0, d, 0)) >= p ? 1 : 0);
+  const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), ql = (uint32_t)yq, qh = (uint32_t)(yq >> 32);
+  // q = high word of x*y'
+  const uint64_t ll = mad32(xl, ql, 0);
+  const uint64_t m1 = mad32(xl, qh, ll >> 32);
+  const uint64_t m2 = mad32(xh, ql, m1 & M32);
+  const uint64_t q = mad32(xh, qh, m1 >> 32) + (m2 >> 32);
+  // low word of x*y
+  const uint64_t xy0 = mad32(xl, yl, 0);
+  const uint32_t xyh = (uint32_t)(xy0 >> 32) + xl * yh + xh * yl;
+  const uint64_t xy_lo = ((uint64_t)xyh << 32) | (xy0 & M32);
+  // q*p mod 2^64 = (q << 61) - q*delta
+  const uint64_t b0 = mad32((uint32_t)q, d, 0);
+  const uint32_t b1 = (uint32_t)(q >> 32) * d + (uint32_t)(b0 >> 32);
+  const uint64_t qd = ((uint64_t)b1 << 32) | (b0 & M32);
+  return xy_lo - ((q << 61) - qd);
+}
+
 // One reply-chain step (crypto/NFLlib.hpp:303-321): acc += lazy product; -p if >= p.
 __device__ __forceinline__ uint64_t mac_step(uint64_t acc, uint64_t x, uint64_t y, uint64_t yq,
                                              uint64_t p) {
@@ -65,4 +98,10 @@ __host__ __device__ __forceinline__ uint64_t rand_word(uint64_t seed, uint64_t k
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
   return z ^ (z >> 31);
+}
+
+// reply-chain step with the query quotient derived on the fly
+__device__ __forceinline__ uint64_t mac_step_qf(uint64_t acc, uint64_t x, uint64_t y, uint64_t p, uint64_t delta) {
+  acc += shoup_lazy_qf(x, y, p, delta);
+  return acc >= p ? acc - p : acc;
 }

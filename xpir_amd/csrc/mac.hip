@@ -54,14 +54,14 @@ __global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      acc_a = mac_step(acc_a, xv[i], av[i], shoup_quot(av[i], p, delta), p);
-      acc_b = mac_step(acc_b, xv[i], bv[i], shoup_quot(bv[i], p, delta), p);
+      acc_a = mac_step_qf(acc_a, xv[i], av[i], p, delta);
+      acc_b = mac_step_qf(acc_b, xv[i], bv[i], p, delta);
     }
   }
   for (; e < a.n_l; e++) {
     const uint64_t xv = x[e * xstride], av = ya[e * qstride], bv = yb[e * qstride];
-    acc_a = mac_step(acc_a, xv, av, shoup_quot(av, p, delta), p);
-    acc_b = mac_step(acc_b, xv, bv, shoup_quot(bv, p, delta), p);
+    acc_a = mac_step_qf(acc_a, xv, av, p, delta);
+    acc_b = mac_step_qf(acc_b, xv, bv, p, delta);
   }
   uint64_t *o = a.out + gk * 2 * mn + cs;
   o[0] = acc_a;
@@ -122,8 +122,8 @@ __global__ void __launch_bounds__(64) mac_batch_kernel(BatchArgs a) {
 #pragma unroll
     for (int i = 0; i < QG; i++) {
       const uint64_t av = qa[i][e * qstride], bv = qa[i][e * qstride + mn];
-      acc_a[i] = mac_step(acc_a[i], xv, av, shoup_quot(av, p, delta), p);
-      acc_b[i] = mac_step(acc_b[i], xv, bv, shoup_quot(bv, p, delta), p);
+      acc_a[i] = mac_step_qf(acc_a[i], xv, av, p, delta);
+      acc_b[i] = mac_step_qf(acc_b[i], xv, bv, p, delta);
     }
   }
 #pragma unroll
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
   uint64_t sa = 0, sbm = 0;
   if (valid == S) {
 #pragma unroll
-    for (int j0 = 0; j0 < S; j0 += 4) {
+    for (int j0 = 0; j0 < S; j0 += 4) {  // S is a multiple of 4
       uint64_t xv[4], av[4], bv[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
@@ -225,8 +225,8 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
       }
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        ra[j0 + i] = shoup_lazy(xv[i], av[i], shoup_quot(av[i], p, delta), p);
-        rb[j0 + i] = shoup_lazy(xv[i], bv[i], shoup_quot(bv[i], p, delta), p);
+        ra[j0 + i] = shoup_lazy_qf(xv[i], av[i], p, delta);
+        rb[j0 + i] = shoup_lazy_qf(xv[i], bv[i], p, delta);
         sa = addmod_c(sa, ra[j0 + i] >= p ? ra[j0 + i] - p : ra[j0 + i], p);
         sbm = addmod_c(sbm, rb[j0 + i] >= p ? rb[j0 + i] - p : rb[j0 + i], p);
       }
@@ -237,8 +237,8 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
       ra[j] = rb[j] = 0;
       if (j < valid) {
         const uint64_t xv = xp[j * xstride], av = qp[j * qstride], bv = qp[j * qstride + mn];
-        ra[j] = shoup_lazy(xv, av, shoup_quot(av, p, delta), p);
-        rb[j] = shoup_lazy(xv, bv, shoup_quot(bv, p, delta), p);
+        ra[j] = shoup_lazy_qf(xv, av, p, delta);
+        rb[j] = shoup_lazy_qf(xv, bv, p, delta);
         sa = addmod_c(sa, ra[j] >= p ? ra[j] - p : ra[j], p);
         sbm = addmod_c(sbm, rb[j] >= p ? rb[j] - p : rb[j], p);
       }
@@ -446,7 +446,7 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
     sa.sblocks = prm->n / 64;
     sa.columns = groups * P * prm->m * sa.sblocks;
     // enough tiles that columns x tiles x W wavefronts fill the GPU several times
-    if (n_l >= 16 * 8 * 16) rc = launch_seg<16, 8>(ctx, sa);
+    if (n_l >= 8 * 8 * 16) rc = launch_seg<8, 8>(ctx, sa);
     else rc = launch_seg<8, 4>(ctx, sa);
   }
   timing_end(ctx, e0, e1);
