@@ -71,7 +71,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="single", choices=["single", "plain"])
+    ap.add_argument("--workload", default="single", choices=["single", "plain", "batched", "serving"])
+    ap.add_argument("--scale", type=float, default=1.0, help="element-count scale of the batched/serving databases")
+    ap.add_argument("--queries", type=int, default=2000, help="serving: queries in the Poisson stream")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -81,6 +84,11 @@ def main():
 
     import xpir_amd
     from xpir_amd.workloads import C1_PLAIN, C2_SINGLE
+
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     cfg = C2_SINGLE if args.workload == "single" else C1_PLAIN
     gold_path = os.path.join(ROOT, "tests", "golden", "golden.json")
@@ -180,6 +188,196 @@ def main():
     for b in (skd, qd, rd):
         b.free()
     db.free()
+    ctx.close()
+
+
+def _load_batch_dbs(ctx, scale, d_override=None):
+    """The eight uneven databases, imported on the GPU and resident (d = 2 grids)."""
+    import xpir_amd
+    from xpir_amd.workloads import batch_databases
+    dbs = []
+    for spec in batch_databases(scale):
+        dims = spec["dims"]
+        prm = xpir_amd.params(spec["crypto"], dims[0])
+        raw = spec["n_elems"] * spec["elem_bytes"]
+        buf = ctx.alloc(raw + 16)
+        ctx.random_bytes_dev(spec["db_seed"], 0, buf, raw)
+        t0 = time.perf_counter()
+        db = ctx.import_db_dev(prm, buf, spec["n_elems"], spec["elem_bytes"], dims)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        buf.free()
+        log(f"db {spec['n_elems']} x {spec['elem_bytes']} B dims {dims}: import {dt:.2f} s "
+            f"({raw / dt / 1e9:.2f} GB/s raw), NTT {db.polys * dims[0] * dims[1] * prm.m * prm.n * 8 / 2**30:.2f} GiB")
+        sk = xpir_amd.keygen(prm, spec["key_seed"])
+        skd = ctx.alloc(sk.nbytes).upload(sk)
+        dbs.append(dict(spec=spec, db=db, prm=prm, raw=raw, skd=skd, import_s=dt))
+    return dbs
+
+
+def bench_batched(args):
+    """Batched uneven workload: every database's queries in one xpg_reply_batch_dev call,
+    against the plain per-query path on the same databases."""
+    import xpir_amd
+    from xpir_amd.workloads import batch_query_counts
+    ctx = xpir_amd.Context(0)
+    dbs = _load_batch_dbs(ctx, args.scale)
+    counts = batch_query_counts()
+    rng = np.random.default_rng(42)
+    reqs = []
+    for i, (d, nq) in enumerate(zip(dbs, counts)):
+        d["reqs"] = []
+        for j in range(nq):
+            qd = ctx.alloc(d["db"].query_words * 8)
+            idx = int(rng.integers(0, d["spec"]["n_elems"]))
+            ctx.query_gen_dev(d["prm"], d["skd"], d["spec"]["dims"], idx, 9000 + 100 * i + j, qd)
+            rd = ctx.alloc(d["db"].reply_words * 8)
+            d["reqs"].append((d["db"], qd, rd))
+        reqs += d["reqs"]
+    ctx.sync()
+    log(f"{len(reqs)} queries over {len(dbs)} databases")
+    # plain: one query per database, alone (best of 2)
+    plain_t = []
+    for d in dbs:
+        db, qd, rd = d["reqs"][0]
+        best = 1e30
+        for _ in range(2):
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.reply_dev(db, qd, rd)
+            ctx.sync()
+            best = min(best, time.perf_counter() - t0)
+        d["plain_reply"] = rd.download()
+        plain_t.append(best)
+        log(f"plain reply {d['spec']['n_elems']} x {d['spec']['elem_bytes']} B: {best * 1e3:.1f} ms "
+            f"({d['raw'] / best / 1e9:.2f} GB/s)")
+    order = np.random.default_rng(43).permutation(len(reqs))
+    batch = [reqs[k] for k in order]
+    ctx.reply_batch_dev(batch)  # warm-up (allocates the batch buffers)
+    ctx.sync()
+    times = []
+    for _ in range(max(1, args.steps)):
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.reply_batch_dev(batch)
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+    makespan = min(times)
+    exact = all(np.array_equal(d["reqs"][0][2].download(), d["plain_reply"]) for d in dbs)
+    total_raw = sum(d["raw"] * nq for d, nq in zip(dbs, counts))
+    plain_total = sum(t * nq for t, nq in zip(plain_t, counts))
+    out = {
+        "metric": "DB GB/s per reply, batched vs plain, 1 MI355X, bit-exact; p50/p99 reply ms",
+        "workload": "batched", "value": total_raw / makespan / 1e9, "unit": "GB/s",
+        "plain_value": total_raw / plain_total / 1e9, "batched_over_plain": plain_total / makespan,
+        "makespan_ms": makespan * 1e3, "queries": int(sum(counts)), "n_gpus": 1,
+        "bit_exact_vs_plain": bool(exact), "dtype": "u64", "data": "synthetic (seeded SplitMix64 bytes)",
+        "config": {"workload": "8 uneven LWE:80:2048:120 databases, d=2, 1-64 queries each",
+                   "scale": args.scale, "counts": counts,
+                   "dbs": [[d["spec"]["n_elems"], d["spec"]["elem_bytes"], d["spec"]["dims"]] for d in dbs]},
+        "plain_ms_per_db": [t * 1e3 for t in plain_t],
+        "import_s": [d["import_s"] for d in dbs],
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_serving(args):
+    """Serving: the batched databases resident, a seeded Poisson stream with Zipf(1.1)
+    database choice through the continuous-batching server; replies checked against the
+    same query answered alone."""
+    import xpir_amd
+    ctx = xpir_amd.Context(0)
+    dbs = _load_batch_dbs(ctx, args.scale)
+    # d = 1 views of the two smallest databases (their d = 1 queries fit the budget; the
+    # absorption size depends on the first dimension, so each view is its own transform)
+    for base in dbs[:2]:
+        spec = dict(base["spec"])
+        spec["dims"] = [spec["n_elems"]]
+        prm = xpir_amd.params(spec["crypto"], spec["n_elems"])
+        raw = spec["n_elems"] * spec["elem_bytes"]
+        buf = ctx.alloc(raw + 16)
+        ctx.random_bytes_dev(spec["db_seed"], 0, buf, raw)
+        db = ctx.import_db_dev(prm, buf, spec["n_elems"], spec["elem_bytes"], spec["dims"])
+        buf.free()
+        sk = xpir_amd.keygen(prm, spec["key_seed"])
+        base["d1"] = dict(spec=spec, db=db, prm=prm, raw=raw, skd=ctx.alloc(sk.nbytes).upload(sk))
+    rng = np.random.default_rng(51)
+    pool_n = 4
+    for d in dbs[:2]:
+        v = d["d1"]
+        v["pool"] = []
+        for j in range(2):
+            qd = ctx.alloc(v["db"].query_words * 8)
+            idx = int(rng.integers(0, v["spec"]["n_elems"]))
+            ctx.query_gen_dev(v["prm"], v["skd"], v["spec"]["dims"], idx, 6000 + j, qd)
+            rd = ctx.alloc(v["db"].reply_words * 8)
+            ctx.reply_dev(v["db"], qd, rd)
+            v["pool"].append((qd, hashlib.sha256(rd.download().tobytes()).hexdigest()))
+            rd.free()
+    for i, d in enumerate(dbs):
+        d["pool"] = []
+        for j in range(pool_n):
+            qd = ctx.alloc(d["db"].query_words * 8)
+            idx = int(rng.integers(0, d["spec"]["n_elems"]))
+            ctx.query_gen_dev(d["prm"], d["skd"], d["spec"]["dims"], idx, 5000 + 100 * i + j, qd)
+            rd = ctx.alloc(d["db"].reply_words * 8)
+            ctx.reply_dev(d["db"], qd, rd)
+            d["pool"].append((qd, hashlib.sha256(rd.download().tobytes()).hexdigest()))
+            rd.free()
+    ctx.sync()
+    w = np.array([1.0 / (k + 1) ** 1.1 for k in range(len(dbs))])
+    w /= w.sum()
+    n = args.queries
+    choice = rng.choice(len(dbs), size=n, p=w)
+    pick = rng.integers(0, pool_n, size=n)
+    use_d1 = rng.integers(0, 2, size=n)  # d in {1, 2} where a d = 1 view exists
+    gaps = rng.exponential(1.0 / args.rate, size=n)
+    srv = xpir_amd.Server(0, reply_pool_bytes=8 << 30, workers=2, max_batch=64)
+    expected, lat, refused, served_raw, mismatches = {}, [], 0, 0, 0
+    i = 0
+    t0 = time.perf_counter()
+    t_next = t0
+    last_log = t0
+    while i < n or expected:
+        now = time.perf_counter()
+        while i < n and now >= t_next:
+            d = dbs[choice[i]]
+            if "d1" in d and use_d1[i]:
+                d = d["d1"]
+            qd, digest = d["pool"][pick[i] % len(d["pool"])]
+            t = srv.submit(d["db"], qd)
+            if t is None:
+                refused += 1
+                break
+            expected[t] = (digest, d["raw"])
+            t_next += gaps[i]
+            i += 1
+        for c in srv.poll(timeout_us=500):
+            digest, raw = expected.pop(c.ticket)
+            got = ctx.download_ptr(c.reply_dev, c.reply_words)
+            mismatches += hashlib.sha256(got.tobytes()).hexdigest() != digest
+            lat.append((c.done_us - c.submit_us) / 1e3)
+            served_raw += raw
+            srv.release(c.ticket)
+        if now - last_log > 10:
+            log(f"served {len(lat)}/{n}, in flight {len(expected)}, refused {refused}")
+            last_log = now
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    out = {
+        "metric": "DB GB/s per reply, batched vs plain, 1 MI355X, bit-exact; p50/p99 reply ms",
+        "workload": "serving", "value": served_raw / makespan / 1e9, "unit": "GB/s",
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "queries": n, "rate_per_s": args.rate, "makespan_s": makespan, "refused_submits": refused,
+        "batches": st["batches"], "bit_exact": mismatches == 0, "mismatches": int(mismatches), "n_gpus": 1,
+        "config": {"workload": "8 resident LWE:80:2048:120 databases, Zipf(1.1) choice, Poisson arrivals, "
+                               "d=2 (d in {1,2} on the two smallest)",
+                   "scale": args.scale, "workers": 2, "max_batch": 64, "reply_pool_bytes": 8 << 30},
+    }
+    print(json.dumps(out), flush=True)
     ctx.close()
 
 

@@ -118,3 +118,66 @@ def test_c1_plain_full(gpu_ctx):
     assert np.array_equal(got, want)
     assert extract_reply(orc, sk, got, cfg["dims"], L) == data[index * L:(index + 1) * L].tobytes()
     db.free()
+
+
+def test_recursive_scaled_matches_golden(gpu_ctx):
+    """LWE:100:4096:180, d=2 over a 64x64 grid of 1 KiB elements: the level-1 reply and
+    the final reply equal the reference's (golden hashes); the reply decrypts."""
+    import hashlib
+    import json
+    import os
+    import xpir_amd
+    from xpir_amd.workloads import C3_SCALED as cfg
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))["c3_scaled"]
+    orc = Oracle(cfg["crypto"], cfg["dims"][0])
+    prm = xpir_amd.params(cfg["crypto"], cfg["dims"][0])
+    n, L = cfg["n_elems"], cfg["elem_bytes"]
+    raw = gpu_ctx.alloc(n * L + 16)
+    gpu_ctx.random_bytes_dev(cfg["db_seed"], 0, raw, n * L)
+    db = gpu_ctx.import_db_dev(prm, raw, n, L, cfg["dims"])
+    raw.free()
+    assert hashlib.sha256(db.export_words().tobytes()).hexdigest() == gold["ntt_sha256"]
+    sk = orc.keygen(cfg["key_seed"])
+    q = np.concatenate(orc.make_query(sk, cfg["dims"], cfg["index"], cfg["noise_seed"]))
+    rep = gpu_ctx.reply(db, q)
+    assert hashlib.sha256(rep.tobytes()).hexdigest() == gold["reply_sha256"]
+    data = orc.rand_bytes(cfg["db_seed"], cfg["index"] * L, L)
+    assert extract_reply(orc, sk, rep, cfg["dims"], L) == data.tobytes()
+    db.free()
+
+
+@pytest.mark.skipif(not __import__("os").environ.get("XPIR_AMD_FULL"),
+                    reason="full-size recursive configuration (96 GiB NTT database): set XPIR_AMD_FULL=1")
+def test_recursive_full_size_decrypts(gpu_ctx):
+    """LWE:100:4096:180, d=2 over 1024x1024, 2^20 x 1 KiB: the reply (6 ciphertexts)
+    decrypts to the chosen element.  The oracle cannot replay this size on the CPU; the
+    scaled grid above pins the bytes."""
+    import time
+    import xpir_amd
+    from xpir_amd.workloads import C3_RECURSIVE as cfg
+    orc = Oracle(cfg["crypto"], cfg["dims"][0])
+    prm = xpir_amd.params(cfg["crypto"], cfg["dims"][0])
+    n, L = cfg["n_elems"], cfg["elem_bytes"]
+    raw = gpu_ctx.alloc(n * L + 16)
+    gpu_ctx.random_bytes_dev(cfg["db_seed"], 0, raw, n * L)
+    t0 = time.perf_counter()
+    db = gpu_ctx.import_db_dev(prm, raw, n, L, cfg["dims"])
+    gpu_ctx.sync()
+    print(f"import of 2^20 x 1 KiB: {time.perf_counter() - t0:.2f} s", flush=True)
+    raw.free()
+    sk = orc.keygen(cfg["key_seed"])
+    skd = gpu_ctx.alloc(sk.nbytes).upload(sk)
+    qd = gpu_ctx.alloc(db.query_words * 8)
+    gpu_ctx.query_gen_dev(prm, skd, cfg["dims"], cfg["index"], cfg["noise_seed"], qd)
+    rd = gpu_ctx.alloc(db.reply_words * 8)
+    t0 = time.perf_counter()
+    gpu_ctx.reply_dev(db, qd, rd)
+    gpu_ctx.sync()
+    print(f"reply: {time.perf_counter() - t0:.2f} s", flush=True)
+    rep = rd.download()
+    assert db.reply_polys == 6
+    data = orc.rand_bytes(cfg["db_seed"], cfg["index"] * L, L)
+    assert extract_reply(orc, sk, rep, cfg["dims"], L) == data.tobytes()
+    for b in (skd, qd, rd):
+        b.free()
+    db.free()

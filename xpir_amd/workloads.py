@@ -33,3 +33,42 @@ C3_RECURSIVE = dict(name="recursive-LWE:100:4096:180-2^20x1KiB-d2", crypto="LWE:
 # scaled-down grid of the recursive configuration: same cryptosystem and element size
 C3_SCALED = dict(C3_RECURSIVE, name="recursive-scaled-64x64", n_elems=4096, dims=[64, 64],
                  index=_index(32, 4096))
+
+
+def dim_sizes(n: int, alpha: int, d: int):
+    """The d-dimensional grid for n elements: factors floor((n/alpha)^(1/d)), then each
+    incremented in turn until their product covers n/alpha
+    (PIROptimizer::getDimSize, apps/optim/PIROptimizer.cpp:338-366)."""
+    import math
+    new_n = math.ceil(n / alpha)
+    f = [math.floor(new_n ** (1.0 / d))] * d
+    if float(f[0]) != new_n ** (1.0 / d):
+        prod, j = 1, 0
+        while prod < new_n and j < d:
+            prod = 1
+            f[j] += 1
+            j += 1
+            for x in f:
+                prod *= x
+    return f
+
+
+# Batched uneven: 8 resident LWE:80:2048:120 databases; element sizes cycle through
+# {256 B, 1 KiB, 4 KiB, 16 KiB}; d = 2 grids (d = 1 queries on these sizes would be
+# 0.6-61 GiB each); 1..64 queries per database, counts from seed 41.
+BATCH_COUNTS = [10_000, 25_000, 50_000, 100_000, 200_000, 400_000, 700_000, 1_000_000]
+BATCH_SIZES = [256, 1024, 4096, 16384]
+
+
+def batch_databases(scale: float = 1.0):
+    out = []
+    for i, n in enumerate(BATCH_COUNTS):
+        n = max(4, int(n * scale))
+        out.append(dict(crypto="LWE:80:2048:120", n_elems=n, elem_bytes=BATCH_SIZES[i % 4],
+                        dims=dim_sizes(n, 1, 2), db_seed=4100 + i, key_seed=4200 + i))
+    return out
+
+
+def batch_query_counts(seed: int = 41, n_db: int = 8, lo: int = 1, hi: int = 64):
+    import numpy as np
+    return [int(x) for x in np.random.default_rng(seed).integers(lo, hi + 1, size=n_db)]
