@@ -65,8 +65,10 @@ const char *xpg_last_error(const xpg_ctx *ctx);
 void *xpg_ctx_stream(xpg_ctx *ctx);
 int xpg_sync(xpg_ctx *ctx);
 
-/* tuning knobs: "mac_kernel" = 0 (by shape), 1 (one chain per lane), 2 (segmented chains).
- * No reference counterpart; every setting yields the same reply bytes. */
+/* tuning knobs: "mac_kernel" = 0 (by shape), 1 (one chain per lane), 2 (segmented chains);
+ * "seg_s" / "seg_w" = segmented tile shape (elements per lane 4|8|16, wavefronts per
+ * workgroup 4|8|16; 0 = by shape).  No reference counterpart; every setting yields the
+ * same reply bytes. */
 int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value);
 
 /* device memory plumbing for callers without a GPU runtime of their own */

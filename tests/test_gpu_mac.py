@@ -88,3 +88,27 @@ def test_single_query_full_size_matches_golden(gpu_ctx):
         for b in (skd, qd, rd):
             b.free()
         db.free()
+
+
+@pytest.mark.parametrize("shape", [(4, 8), (4, 16), (8, 4), (8, 8), (8, 16), (16, 4), (16, 8)])
+def test_segmented_tile_shapes_match_oracle(gpu_ctx, shape):
+    """Every tile shape of the segmented kernel gives the oracle's words (partial last
+    tile, look-back over many tiles)."""
+    import xpir_amd
+    desc, n_l = "LWE:80:2048:120", 3001
+    orc = Oracle(desc, n_l)
+    prm = xpir_amd.params(desc, n_l)
+    rng = np.random.default_rng(99)
+    db_words, q = _random_level(orc, rng, n_l, 1, 1)
+    want = orc.reply_level(db_words, q, n_l, 1, 1)
+    db = gpu_ctx.import_ntt_words(prm, db_words, n_l, 1, 8, [n_l])
+    gpu_ctx.set_option("mac_kernel", 2)
+    gpu_ctx.set_option("seg_s", shape[0])
+    gpu_ctx.set_option("seg_w", shape[1])
+    try:
+        assert np.array_equal(gpu_ctx.reply(db, q), want)
+    finally:
+        gpu_ctx.set_option("mac_kernel", 0)
+        gpu_ctx.set_option("seg_s", 0)
+        gpu_ctx.set_option("seg_w", 0)
+        db.free()

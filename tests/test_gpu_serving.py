@@ -42,7 +42,10 @@ def test_serving_mix_equals_alone(gpu_ctx):
     n_req = 300
     choice = rng.choice(len(dbs), size=n_req, p=w)
     qpick = rng.integers(0, 6, size=n_req)
-    gaps = rng.exponential(1.0 / 4000.0, size=n_req)  # 4000 queries/s
+    # bursts: Poisson arrivals at 4000/s, with every 25th gap stretched so requests
+    # pile up behind a running batch and must share the next one
+    gaps = rng.exponential(1.0 / 4000.0, size=n_req)
+    gaps[np.arange(n_req) % 25 != 24] = 0.0
     # budget: room for a handful of replies at once, so admission refuses some
     srv = xpir_amd.Server(0, reply_pool_bytes=6 << 20, workers=2, max_batch=16)
     try:
@@ -52,17 +55,17 @@ def test_serving_mix_equals_alone(gpu_ctx):
         i = 0
         while i < n_req or len(done) < n_req:
             now = time.perf_counter()
-            if i < n_req and now >= t_next:
+            while i < n_req and now >= t_next:
                 db, pool = dbs[choice[i]]
                 qd, alone = pool[qpick[i]]
                 t = srv.submit(db, qd)
                 if t is None:
                     refused += 1
-                else:
-                    expected[t] = alone
-                    i += 1
-                    t_next += gaps[i - 1]
-            for c in srv.poll(timeout_us=200):
+                    break
+                expected[t] = alone
+                t_next += gaps[i]
+                i += 1
+            for c in srv.poll(timeout_us=200 if i >= n_req or time.perf_counter() < t_next else 0):
                 assert c.status == 0
                 got = gpu_ctx.download_ptr(c.reply_dev, c.reply_words)
                 assert np.array_equal(got, expected[c.ticket]), f"ticket {c.ticket}"

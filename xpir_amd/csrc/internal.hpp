@@ -72,6 +72,7 @@ struct xpg_ctx {
   size_t level_bytes[2] = {0, 0};
   // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 2 = segmented chains
   int mac_mode = 0;
+  int seg_s = 0, seg_w = 0;  // segmented tile shape (0 = by shape)
   // MAC timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;

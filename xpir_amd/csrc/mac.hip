@@ -214,22 +214,20 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
   uint64_t ra[S], rb[S];
   uint64_t sa = 0, sbm = 0;
   if (valid == S) {
+    // all 3*S loads in flight before any product: the tile's HBM requests overlap
+    uint64_t xv[S], av[S], bv[S];
 #pragma unroll
-    for (int j0 = 0; j0 < S; j0 += 4) {  // S is a multiple of 4
-      uint64_t xv[4], av[4], bv[4];
+    for (int j = 0; j < S; j++) {
+      xv[j] = __builtin_nontemporal_load(xp + j * xstride);
+      av[j] = __builtin_nontemporal_load(qp + j * qstride);
+      bv[j] = __builtin_nontemporal_load(qp + j * qstride + mn);
+    }
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        xv[i] = __builtin_nontemporal_load(xp + (j0 + i) * xstride);
-        av[i] = __builtin_nontemporal_load(qp + (j0 + i) * qstride);
-        bv[i] = __builtin_nontemporal_load(qp + (j0 + i) * qstride + mn);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        ra[j0 + i] = shoup_lazy_qf(xv[i], av[i], p, delta);
-        rb[j0 + i] = shoup_lazy_qf(xv[i], bv[i], p, delta);
-        sa = addmod_c(sa, ra[j0 + i] >= p ? ra[j0 + i] - p : ra[j0 + i], p);
-        sbm = addmod_c(sbm, rb[j0 + i] >= p ? rb[j0 + i] - p : rb[j0 + i], p);
-      }
+    for (int j = 0; j < S; j++) {
+      ra[j] = shoup_lazy_qf(xv[j], av[j], p, delta);
+      rb[j] = shoup_lazy_qf(xv[j], bv[j], p, delta);
+      sa = addmod_c(sa, ra[j] >= p ? ra[j] - p : ra[j], p);
+      sbm = addmod_c(sbm, rb[j] >= p ? rb[j] - p : rb[j], p);
     }
   } else {
 #pragma unroll
@@ -261,22 +259,34 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
     if (tile > 0) {
       st_status(my + lane, (1ULL << 62) | ta);
       st_status(my + 64 + lane, (1ULL << 62) | tb);
+      // look back over up to LB predecessor tiles per round trip: the loads are issued
+      // together, then folded from the nearest tile until an inclusive prefix is found
+      constexpr int LB = 4;
       bool done_a = false, done_b = false;
-      uint64_t pa = tile - 1, pb = tile - 1;
+      int64_t pa = (int64_t)tile - 1, pb = (int64_t)tile - 1;
       while (true) {
+        uint64_t va[LB], vb[LB];
+#pragma unroll
+        for (int i = 0; i < LB; i++) {
+          va[i] = (!done_a && pa - i >= 0) ? ld_status(a.status + ((uint64_t)(pa - i) * a.columns + column) * 128 + lane) : 0;
+          vb[i] = (!done_b && pb - i >= 0) ? ld_status(a.status + ((uint64_t)(pb - i) * a.columns + column) * 128 + 64 + lane) : 0;
+        }
+        // fold the contiguous published run from the nearest tile
         if (!done_a) {
-          const uint64_t v = ld_status(a.status + (pa * a.columns + column) * 128 + lane);
-          const uint64_t st = v >> 62;
-          if (st != 0) {
-            exa = addmod_c(exa, v & ((1ULL << 61) - 1), p);
+#pragma unroll
+          for (int i = 0; i < LB; i++) {
+            const uint64_t st = va[i] >> 62;
+            if (st == 0 || done_a || pa < 0) break;
+            exa = addmod_c(exa, va[i] & ((1ULL << 61) - 1), p);
             if (st == 2) done_a = true; else pa--;
           }
         }
         if (!done_b) {
-          const uint64_t v = ld_status(a.status + (pb * a.columns + column) * 128 + 64 + lane);
-          const uint64_t st = v >> 62;
-          if (st != 0) {
-            exb = addmod_c(exb, v & ((1ULL << 61) - 1), p);
+#pragma unroll
+          for (int i = 0; i < LB; i++) {
+            const uint64_t st = vb[i] >> 62;
+            if (st == 0 || done_b || pb < 0) break;
+            exb = addmod_c(exb, vb[i] & ((1ULL << 61) - 1), p);
             if (st == 2) done_b = true; else pb--;
           }
         }
@@ -347,24 +357,44 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
   }
 }
 
-// One thread per (column, lane, chain): fold the tile maps, emit the reply word, and
-// leave the status words and the ticket at zero for the next launch.
-__global__ void __launch_bounds__(256) mac_fixup_kernel(SegArgs a) {
-  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (t == 0) *a.ticket = 0;
-  const uint64_t total = a.columns * 128;
-  if (t >= total) return;
-  const uint64_t column = t >> 7;
-  const uint32_t cl = (uint32_t)(t & 127), chain = cl >> 6, lane = cl & 63;
-  int32_t j = 0;
-  for (uint64_t tile = 0; tile < a.tiles; tile++) {
-    const int2 f = a.dm[(tile * a.columns + column) * 128 + cl];
-    j = max(j + f.x, f.y);
+// Fold the tile maps of every chain in element order, emit the reply words, and leave
+// the status words and the ticket at zero for the next launch.  Block = one column:
+// 128 chains x FIX_PARTS threads; each thread folds a contiguous run of tiles (the map
+// loads are independent of the fold, so they stream), then the runs are folded in order.
+constexpr int FIX_PARTS = 8;
+
+__global__ void __launch_bounds__(128 * FIX_PARTS) mac_fixup_kernel(SegArgs a) {
+  __shared__ int2 part[FIX_PARTS][128];
+  const uint64_t column = blockIdx.x;
+  const uint32_t cl = threadIdx.x & 127, pi = threadIdx.x >> 7;
+  if (column == 0 && threadIdx.x == 0) *a.ticket = 0;
+  const uint64_t per = (a.tiles + FIX_PARTS - 1) / FIX_PARTS;
+  const uint64_t t0 = pi * per, t1 = min<uint64_t>(a.tiles, t0 + per);
+  int2 f = make_int2(0, kNegInf);
+  uint64_t t = t0;
+  for (; t + 8 <= t1; t += 8) {
+    int2 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = a.dm[((t + i) * a.columns + column) * 128 + cl];
+#pragma unroll
+    for (int i = 0; i < 8; i++) f = make_int2(f.x + v[i].x, max(f.y + v[i].x, v[i].y));
   }
-  const uint64_t last = ((a.tiles - 1) * a.columns + column) * 128 + cl;
-  const uint64_t rho = a.status[last] & ((1ULL << 61) - 1);
-  for (uint64_t tile = 0; tile < a.tiles; tile++) a.status[(tile * a.columns + column) * 128 + cl] = 0;
-  // column -> (g, k, c, sblock)
+  for (; t < t1; t++) {
+    const int2 v = a.dm[(t * a.columns + column) * 128 + cl];
+    f = make_int2(f.x + v.x, max(f.y + v.x, v.y));
+  }
+  part[pi][cl] = f;
+  // the last tile's inclusive residue is the chain's final residue; read it before the
+  // status words are cleared
+  uint64_t rho = 0;
+  if (pi == FIX_PARTS - 1) rho = a.status[((a.tiles - 1) * a.columns + column) * 128 + cl] & ((1ULL << 61) - 1);
+  __syncthreads();
+  for (uint64_t tt = t0; tt < t1; tt++) a.status[(tt * a.columns + column) * 128 + cl] = 0;
+  if (pi != FIX_PARTS - 1) return;
+  int32_t j = 0;
+#pragma unroll
+  for (int i = 0; i < FIX_PARTS; i++) j = max(j + part[i][cl].x, part[i][cl].y);
+  const uint32_t chain = cl >> 6, lane = cl & 63;
   const uint32_t sb = (uint32_t)(column % a.sblocks);
   uint64_t r = column / a.sblocks;
   const uint32_t c = (uint32_t)(r % a.m);
@@ -400,8 +430,7 @@ int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
   if (blocks > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   hipLaunchKernelGGL((mac_seg_kernel<S, W>), dim3((uint32_t)blocks), dim3(64 * W), 0, ctx->stream, sa);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
-  const uint64_t fix = sa.columns * 128;
-  hipLaunchKernelGGL(mac_fixup_kernel, dim3((uint32_t)((fix + 255) / 256)), dim3(256), 0, ctx->stream, sa);
+  hipLaunchKernelGGL(mac_fixup_kernel, dim3((uint32_t)sa.columns), dim3(128 * FIX_PARTS), 0, ctx->stream, sa);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC fixup launch");
   return XPG_OK;
 }
@@ -445,9 +474,21 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
     sa.m = prm->m;
     sa.sblocks = prm->n / 64;
     sa.columns = groups * P * prm->m * sa.sblocks;
-    // enough tiles that columns x tiles x W wavefronts fill the GPU several times
-    if (n_l >= 8 * 8 * 16) rc = launch_seg<8, 8>(ctx, sa);
-    else rc = launch_seg<8, 4>(ctx, sa);
+    // tile shape: S elements per lane, W wavefronts per workgroup (tunable; by default
+    // 8 x 8 for long chains, 8 x 4 for short ones)
+    int S = ctx->seg_s, W = ctx->seg_w;
+    if (!S) S = 8;
+    if (!W) W = n_l >= 8 * 8 * 16 ? 8 : 4;
+    const int key = S * 100 + W;
+    switch (key) {
+      case 408: rc = launch_seg<4, 8>(ctx, sa); break;
+      case 416: rc = launch_seg<4, 16>(ctx, sa); break;
+      case 804: rc = launch_seg<8, 4>(ctx, sa); break;
+      case 816: rc = launch_seg<8, 16>(ctx, sa); break;
+      case 1604: rc = launch_seg<16, 4>(ctx, sa); break;
+      case 1608: rc = launch_seg<16, 8>(ctx, sa); break;
+      default: rc = launch_seg<8, 8>(ctx, sa); break;
+    }
   }
   timing_end(ctx, e0, e1);
   return rc;
