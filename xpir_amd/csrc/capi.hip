@@ -257,11 +257,11 @@ int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value) {
     ctx->mac_mode = (int)value;
     return XPG_OK;
   }
-  if (strcmp(key, "seg_s") == 0 && (value == 0 || value == 4 || value == 8 || value == 16)) {
+  if (strcmp(key, "seg_s") == 0 && (value == 0 || value == 4 || value == 8 || value == 16 || value == 24 || value == 32)) {
     ctx->seg_s = (int)value;
     return XPG_OK;
   }
-  if (strcmp(key, "seg_w") == 0 && (value == 0 || value == 4 || value == 8 || value == 16)) {
+  if (strcmp(key, "seg_w") == 0 && (value == 0 || value == 2 || value == 4 || value == 8 || value == 16)) {
     ctx->seg_w = (int)value;
     return XPG_OK;
   }
