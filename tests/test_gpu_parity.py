@@ -79,6 +79,8 @@ def test_query_generation_matches_oracle(gpu_ctx, desc, dims):
     (C2, 30, 4096, [6, 5]),
     (C3, 16, 1024, [4, 4]),
     (C2, 25, 700, [5, 5, 1]),
+    (C2, 50, 512, [5, 11]),           # 11 groups: a full and a partial block of 8
+    (C1, 70, 900, [3, 9, 3]),
 ])
 def test_reply_matches_oracle(gpu_ctx, desc, n_elems, elem_bytes, dims):
     import xpir_amd

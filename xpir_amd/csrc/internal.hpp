@@ -111,3 +111,4 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
 // pointers (they travel in the kernel arguments)
 int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                      uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P);
+

@@ -27,7 +27,7 @@ struct MacArgs {
 
 // One lane = both chains (a and b) of one (g, k, c, s): x is loaded once per element.
 // Lanes with consecutive s read consecutive words, so every load is a coalesced 512 B
-// row per wavefront.
+// row per wavefront.  The query quotients are derived in registers.
 __global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
   const uint64_t mn = (uint64_t)a.m * a.n;
   const uint64_t lane = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -68,70 +68,83 @@ __global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
   o[mn] = acc_b;
 }
 
-
 // ------------------------------------------------------------------------------------
-// Batched level: QN queries over one resident database level.  A lane carries the a and
-// b chains of QG queries for one (g, k, c, s), so each DB word it loads feeds 2*QG
-// products.  The QN/QG query groups of one column run as separate workgroups placed on
-// the same XCD (workgroup b runs on XCD b % 8), adjacent in dispatch order, so they read
-// the same DB rows at nearly the same time: each DB tile comes from HBM once and from
-// that XCD's L2 for the other query groups.  Chains stay sequential per lane, so each
-// reply is bit-identical to the single-query kernels.
+// Levels with several groups, one or many queries.
 constexpr int kMaxBatch = 64;  // queries per launch; pointers travel in the kernel arguments
 
-struct BatchArgs {
+// Register-blocked level: a lane carries GR groups x QR queries (x a|b) of one (k, c, s).
+// Per element it loads GR DB words and QR query word pairs, derives the pairs' Shoup
+// quotients once, then does 2*GR*QR Shoup steps, reusing each DB word QR times and each
+// query word (and its quotient) GR times from registers.  Levels with many groups are
+// bound by how often query words are re-read from L2/MALL across groups, which this
+// blocking divides by GR.  The query blocks of one column group run on one XCD, so
+// its DB rows come from HBM once and from L2 for the other query blocks.
+struct Batch2Args {
   ModSet ms;
   const uint64_t *db;
-  const uint64_t *q[kMaxBatch];  // query-level pointers
-  uint64_t *out[kMaxBatch];      // output pointers
-  uint64_t n_l, groups, columns;  // columns = groups * P * m * (n / 64)
-  uint32_t P, n, m, sblocks, qn, qgroups;
+  const uint64_t *q[kMaxBatch];
+  uint64_t *out[kMaxBatch];
+  uint64_t n_l, groups, colgroups;  // colgroups = ceil(groups / GR) * P * m * sblocks
+  uint32_t P, n, m, sblocks, qn, qblocks;
 };
 
-template <int QG>
-__global__ void __launch_bounds__(64) mac_batch_kernel(BatchArgs a) {
-  // workgroup -> (column, query group), query groups of a column on one XCD
+template <int GR, int QR>
+__global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
   const uint64_t b = blockIdx.x;
   const uint64_t xcd = b & 7, j = b >> 3;
-  const uint64_t column = (j / a.qgroups) * 8 + xcd;
-  const uint32_t qg = (uint32_t)(j % a.qgroups);
-  if (column >= a.columns) return;
+  const uint64_t cg = (j / a.qblocks) * 8 + xcd;
+  const uint32_t qb = (uint32_t)(j % a.qblocks);
+  if (cg >= a.colgroups) return;
   const uint32_t lane = threadIdx.x;
-  const uint32_t sb = (uint32_t)(column % a.sblocks);
-  uint64_t r = column / a.sblocks;
+  const uint32_t sb = (uint32_t)(cg % a.sblocks);
+  uint64_t r = cg / a.sblocks;
   const uint32_t c = (uint32_t)(r % a.m);
-  const uint64_t gk = r / a.m;
-  const uint32_t k = (uint32_t)(gk % a.P);
-  const uint64_t g = gk / a.P;
+  r /= a.m;
+  const uint32_t k = (uint32_t)(r % a.P);
+  const uint64_t g0 = (r / a.P) * GR;
   const uint64_t mn = (uint64_t)a.m * a.n;
   const uint64_t cs = (uint64_t)c * a.n + sb * 64 + lane;
-  const uint64_t p = a.ms.p[c], delta = a.ms.delta[c];
-  const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
-  const uint64_t *x = a.db + (g * a.n_l * a.P + k) * mn + cs;
-  const uint64_t *qa[QG];
-  uint64_t acc_a[QG], acc_b[QG];
-  const uint32_t q0 = qg * QG;
+  const uint64_t p = a.ms.p[c];
+  const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn, gstride = a.n_l * xstride;
+  const uint64_t *x = a.db + (g0 * a.n_l * a.P + k) * mn + cs;
+  const uint64_t *qa[QR];
+  const uint32_t q0 = qb * QR;
 #pragma unroll
-  for (int i = 0; i < QG; i++) {
-    const uint32_t qi = min(q0 + i, a.qn - 1);
-    qa[i] = a.q[qi] + cs;
-    acc_a[i] = acc_b[i] = 0;
-  }
+  for (int i = 0; i < QR; i++) qa[i] = a.q[min(q0 + i, a.qn - 1)] + cs;
+  const uint64_t delta = a.ms.delta[c];
+  int gv = (int)min<uint64_t>(GR, a.groups - g0);  // groups of this block that exist
+  uint64_t acc_a[GR][QR], acc_b[GR][QR];
+#pragma unroll
+  for (int g = 0; g < GR; g++)
+#pragma unroll
+    for (int i = 0; i < QR; i++) acc_a[g][i] = acc_b[g][i] = 0;
   for (uint64_t e = 0; e < a.n_l; e++) {
-    const uint64_t xv = x[e * xstride];
+    uint64_t xv[GR];
 #pragma unroll
-    for (int i = 0; i < QG; i++) {
-      const uint64_t av = qa[i][e * qstride], bv = qa[i][e * qstride + mn];
-      acc_a[i] = mac_step_qf(acc_a[i], xv, av, p, delta);
-      acc_b[i] = mac_step_qf(acc_b[i], xv, bv, p, delta);
+    for (int g = 0; g < GR; g++) xv[g] = g < gv ? __builtin_nontemporal_load(x + g * gstride + e * xstride) : 0;
+    const uint64_t o = e * qstride;
+#pragma unroll
+    for (int i = 0; i < QR; i++) {
+      const uint64_t ya = qa[i][o], yb = qa[i][o + mn];
+      const uint64_t yaq = shoup_quot(ya, p, delta), ybq = shoup_quot(yb, p, delta);
+#pragma unroll
+      for (int g = 0; g < GR; g++) {
+        acc_a[g][i] = mac_step(acc_a[g][i], xv[g], ya, yaq, p);
+        acc_b[g][i] = mac_step(acc_b[g][i], xv[g], yb, ybq, p);
+      }
     }
   }
 #pragma unroll
-  for (int i = 0; i < QG; i++) {
-    if (q0 + i < a.qn) {
-      uint64_t *o = a.out[q0 + i] + gk * 2 * mn + cs;
-      o[0] = acc_a[i];
-      o[mn] = acc_b[i];
+  for (int g = 0; g < GR; g++) {
+    if (g >= gv) continue;
+    const uint64_t gk = (g0 + g) * a.P + k;
+#pragma unroll
+    for (int i = 0; i < QR; i++) {
+      if (q0 + i < a.qn) {
+        uint64_t *o = a.out[q0 + i] + gk * 2 * mn + cs;
+        o[0] = acc_a[g][i];
+        o[mn] = acc_b[g][i];
+      }
     }
   }
 }
@@ -454,10 +467,17 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   a.m = prm->m;
   a.logn = prm->logn;
   const uint64_t lanes = groups * P * (uint64_t)prm->m * prm->n;
+  const bool one_chain_per_lane =
+      ctx->mac_mode == 1 || (ctx->mac_mode == 0 && (lanes >= kDirectLanes || n_l <= 64 || prm->n < 64));
+  if (one_chain_per_lane && groups >= 4 && ctx->mac_mode != 1) {
+    // many groups share each query word: register-blocked over groups
+    uint64_t *outp = out;
+    return launch_mac_batch(ctx, prm, db, &q, &outp, 1, n_l, groups, P);
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
   int rc = XPG_OK;
-  if (ctx->mac_mode == 1 || (ctx->mac_mode == 0 && (lanes >= kDirectLanes || n_l <= 64 || prm->n < 64))) {
+  if (one_chain_per_lane) {
     const uint64_t blocks = (lanes + 255) / 256;
     if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
     hipLaunchKernelGGL(mac_direct_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
@@ -503,33 +523,39 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
 int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                      uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P) {
   if (qn == 0 || qn > (uint32_t)kMaxBatch) return set_err(ctx, XPG_EINVAL, "batch of 1..64 queries per launch");
-  BatchArgs a{};
-  a.ms = modset_of(prm);
-  a.db = db;
+  // register blocks: GR groups x QR queries per lane
+  Batch2Args b2{};
+  b2.ms = modset_of(prm);
+  b2.db = db;
   for (uint32_t i = 0; i < qn; i++) {
-    a.q[i] = q[i];
-    a.out[i] = out[i];
+    b2.q[i] = q[i];
+    b2.out[i] = out[i];
   }
-  a.n_l = n_l;
-  a.groups = groups;
-  a.P = P;
-  a.n = prm->n;
-  a.m = prm->m;
-  a.sblocks = prm->n / 64;
-  a.columns = groups * P * prm->m * a.sblocks;
-  a.qn = qn;
-  int qg = qn >= 8 ? 8 : (qn >= 4 ? 4 : (qn >= 2 ? 2 : 1));
-  a.qgroups = (qn + qg - 1) / qg;
-  const uint64_t col8 = (a.columns + 7) / 8;
-  const uint64_t blocks = col8 * 8 * a.qgroups;
+  b2.n_l = n_l;
+  b2.groups = groups;
+  b2.P = P;
+  b2.n = prm->n;
+  b2.m = prm->m;
+  b2.sblocks = prm->n / 64;
+  b2.qn = qn;
+  const int QR = qn >= 4 ? 4 : (qn >= 2 ? 2 : 1);
+  const int GR = groups >= 8 && QR <= 2 ? 8 : (groups >= 4 ? 4 : 1);
+  b2.qblocks = (qn + QR - 1) / QR;
+  b2.colgroups = ((groups + GR - 1) / GR) * P * prm->m * b2.sblocks;
+  const uint64_t blocks = ((b2.colgroups + 7) / 8) * 8 * b2.qblocks;
   if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "batched MAC grid too large");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
-  switch (qg) {
-    case 8: hipLaunchKernelGGL(mac_batch_kernel<8>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
-    case 4: hipLaunchKernelGGL(mac_batch_kernel<4>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
-    case 2: hipLaunchKernelGGL(mac_batch_kernel<2>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
-    default: hipLaunchKernelGGL(mac_batch_kernel<1>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
+  const int key = GR * 10 + QR;
+  switch (key) {
+    case 82: hipLaunchKernelGGL((mac_batch2_kernel<8, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
+    case 81: hipLaunchKernelGGL((mac_batch2_kernel<8, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
+    case 44: hipLaunchKernelGGL((mac_batch2_kernel<4, 4>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
+    case 42: hipLaunchKernelGGL((mac_batch2_kernel<4, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
+    case 41: hipLaunchKernelGGL((mac_batch2_kernel<4, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
+    case 14: hipLaunchKernelGGL((mac_batch2_kernel<1, 4>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
+    case 12: hipLaunchKernelGGL((mac_batch2_kernel<1, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
+    default: hipLaunchKernelGGL((mac_batch2_kernel<1, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
   }
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "batched MAC launch");
   timing_end(ctx, e0, e1);
