@@ -90,7 +90,7 @@ def test_single_query_full_size_matches_golden(gpu_ctx):
         db.free()
 
 
-@pytest.mark.parametrize("shape", [(4, 8), (4, 16), (8, 4), (8, 8), (8, 16), (16, 4), (16, 8)])
+@pytest.mark.parametrize("shape", [(4, 8), (4, 16), (8, 4), (8, 8), (8, 16), (12, 4), (16, 4), (16, 8), (24, 4), (32, 2), (32, 4)])
 def test_segmented_tile_shapes_match_oracle(gpu_ctx, shape):
     """Every tile shape of the segmented kernel gives the oracle's words (partial last
     tile, look-back over many tiles)."""

@@ -11,7 +11,7 @@
 
 struct ModSet {
   uint64_t p[XPG_MAX_M];
-  uint64_t delta[XPG_MAX_M];  // 2^61 - p
+  uint32_t delta[XPG_MAX_M];  // 2^61 - p (below 2^21)
 };
 
 __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
@@ -41,47 +41,39 @@ __device__ __forceinline__ void mul64x32(uint64_t y, uint32_t d, uint64_t &lo, u
 // division from p = 2^61 - delta:  y*2^64/p = 8y + 8*y*delta/p, and N = 8*y*delta
 // < 2^90 splits as N = t*2^61 + r0 = t*p + (r0 + t*delta) with r0 + t*delta < 2p.
 // Holds for every y < 2^64 (checked exhaustively at the edges and on random words).
-__device__ __forceinline__ uint64_t shoup_quot(uint64_t y, uint64_t p, uint64_t delta) {
+__device__ __forceinline__ uint64_t shoup_quot(uint64_t y, uint64_t p, uint32_t delta) {
   uint64_t lo, hi;
-  mul64x32(y, (uint32_t)delta, lo, hi);
+  mul64x32(y, delta, lo, hi);
   uint64_t nlo = lo << 3;
   uint64_t nhi = (hi << 3) | (lo >> 61);
   uint64_t t = (nhi << 3) | (nlo >> 61);
-  uint64_t r = (nlo & ((1ULL << 61) - 1)) + t * delta;
+  uint64_t r = (nlo & ((1ULL << 61) - 1)) + t * (uint64_t)delta;
   return 8 * y + t + (r >= p ? 1ULL : 0ULL);
 }
 
-__device__ __forceinline__ uint64_t mad32(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
-
 // The lazy Shoup product x*y - floor(x*y'/2^64)*p, y' = floor(y*2^64/p) mod 2^64 derived
-// on the fly (shoup_quot), written as 32x32->64 multiply-adds (v_mad_u64_u32): 3 for
-// y', 4 for the high word of x*y', 3 for the low word of x*y, 2 for q*p via
-// q*p = (q << 61) - q*delta.  Equal to shoup_lazy(x, y, shoup_quot(y, p, delta), p) for
-// every x, y < 2^64.
-__device__ __forceinline__ uint64_t shoup_lazy_qf(uint64_t x, uint64_t y, uint64_t p, uint64_t delta) {
-  constexpr uint64_t M32 = 0xffffffffULL;
-  const uint32_t d = (uint32_t)delta, yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
-  // y' = 8y + t0 + [r >= p], with y*delta = p1:p0 and t0 = (8*y*delta) >> 61 = p1 >> 26
-  const uint64_t p0 = mad32(yl, d, 0);
-  const uint64_t p1 = mad32(yh, d, p0 >> 32);
-  const uint64_t t0 = p1 >> 26;
-  const uint64_t nmod = (((p1 << 32) | (p0 & M32)) & ((1ULL << 58) - 1)) << 3;
-  const uint64_t yq = (y << 3) + t0 + ((nmod + mad32((uint32_t)t0, d, 0)) >= p ? 1 : 0);
+// on the fly (shoup_quot), written as 32x32->64 multiply-adds (v_mad_u64_u32) and 32-bit
+// low multiplies:
+//   y'  = 8y + t0 + [((y*delta mod 2^58) << 3) + t0*delta >= p],  t0 = (y*delta) >> 58
+//   q   = high word of x*y'
+//   r   = x*y - q*p  (mod 2^64)  =  x*y + q*delta - (q << 61), where only the low three
+//         bits of q reach the high word through q << 61.
+// Equal to shoup_lazy(x, y, shoup_quot(y, p, delta), p) for every x, y < 2^64.
+__host__ __device__ __forceinline__ uint64_t shoup_lazy_qf(uint64_t x, uint64_t y, uint64_t p, uint32_t delta) {
+  const uint32_t d = delta, yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
+  const uint64_t a0 = (uint64_t)yl * d;
+  const uint64_t a1 = (uint64_t)yh * d + (a0 >> 32);  // y*delta = a1 * 2^32 + low word of a0
+  const uint32_t t0 = (uint32_t)(a1 >> 26);
+  const uint64_t nm = (((a1 & 0x3ffffffULL) << 32) | (uint32_t)a0) << 3;
+  const uint32_t t = t0 + ((uint64_t)t0 * d + nm >= p ? 1u : 0u);  // floor(8*y*delta / p)
+  const uint64_t yq = (y << 3) + t;
   const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), ql = (uint32_t)yq, qh = (uint32_t)(yq >> 32);
-  // q = high word of x*y'
-  const uint64_t ll = mad32(xl, ql, 0);
-  const uint64_t m1 = mad32(xl, qh, ll >> 32);
-  const uint64_t m2 = mad32(xh, ql, m1 & M32);
-  const uint64_t q = mad32(xh, qh, m1 >> 32) + (m2 >> 32);
-  // low word of x*y
-  const uint64_t xy0 = mad32(xl, yl, 0);
-  const uint32_t xyh = (uint32_t)(xy0 >> 32) + xl * yh + xh * yl;
-  const uint64_t xy_lo = ((uint64_t)xyh << 32) | (xy0 & M32);
-  // q*p mod 2^64 = (q << 61) - q*delta
-  const uint64_t b0 = mad32((uint32_t)q, d, 0);
-  const uint32_t b1 = (uint32_t)(q >> 32) * d + (uint32_t)(b0 >> 32);
-  const uint64_t qd = ((uint64_t)b1 << 32) | (b0 & M32);
-  return xy_lo - ((q << 61) - qd);
+  const uint64_t m1 = (uint64_t)xh * ql + (((uint64_t)xl * ql) >> 32);
+  const uint64_t m2 = (uint64_t)xl * qh + (uint32_t)m1;
+  const uint64_t q = (uint64_t)xh * qh + (m1 >> 32) + (m2 >> 32);
+  const uint64_t lo = (uint64_t)xl * yl + (uint64_t)(uint32_t)q * d;
+  const uint32_t hi = (uint32_t)(lo >> 32) + xl * yh + xh * yl + (uint32_t)(q >> 32) * d - ((uint32_t)q << 29);
+  return ((uint64_t)hi << 32) | (uint32_t)lo;
 }
 
 // One reply-chain step (crypto/NFLlib.hpp:303-321): acc += lazy product; -p if >= p.
@@ -101,7 +93,7 @@ __host__ __device__ __forceinline__ uint64_t rand_word(uint64_t seed, uint64_t k
 }
 
 // reply-chain step with the query quotient derived on the fly
-__device__ __forceinline__ uint64_t mac_step_qf(uint64_t acc, uint64_t x, uint64_t y, uint64_t p, uint64_t delta) {
+__device__ __forceinline__ uint64_t mac_step_qf(uint64_t acc, uint64_t x, uint64_t y, uint64_t p, uint32_t delta) {
   acc += shoup_lazy_qf(x, y, p, delta);
   return acc >= p ? acc - p : acc;
 }

@@ -46,7 +46,7 @@ ModSet modset_of(const xpg_params *prm) {
   ModSet ms{};
   for (uint32_t c = 0; c < prm->m && c < XPG_MAX_M; c++) {
     ms.p[c] = prm->moduli[c];
-    ms.delta[c] = (1ULL << 61) - prm->moduli[c];
+    ms.delta[c] = (uint32_t)((1ULL << 61) - prm->moduli[c]);
   }
   return ms;
 }
@@ -257,7 +257,7 @@ int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value) {
     ctx->mac_mode = (int)value;
     return XPG_OK;
   }
-  if (strcmp(key, "seg_s") == 0 && (value == 0 || value == 4 || value == 8 || value == 16 || value == 24 || value == 32)) {
+  if (strcmp(key, "seg_s") == 0 && (value == 0 || value == 4 || value == 8 || value == 12 || value == 16 || value == 24 || value == 32)) {
     ctx->seg_s = (int)value;
     return XPG_OK;
   }

@@ -38,7 +38,8 @@ __global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
   const uint64_t gk = lane / mn;
   const uint32_t k = (uint32_t)(gk % a.P);
   const uint64_t g = gk / a.P;
-  const uint64_t p = a.ms.p[c], delta = a.ms.delta[c];
+  const uint64_t p = a.ms.p[c];
+  const uint32_t delta = a.ms.delta[c];
   const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
   const uint64_t *x = a.db + (g * a.n_l * a.P + k) * mn + cs;
   const uint64_t *ya = a.q + cs, *yb = a.q + mn + cs;
@@ -111,7 +112,7 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
   const uint32_t q0 = qb * QR;
 #pragma unroll
   for (int i = 0; i < QR; i++) qa[i] = a.q[min(q0 + i, a.qn - 1)] + cs;
-  const uint64_t delta = a.ms.delta[c];
+  const uint32_t delta = a.ms.delta[c];
   int gv = (int)min<uint64_t>(GR, a.groups - g0);  // groups of this block that exist
   uint64_t acc_a[GR][QR], acc_b[GR][QR];
 #pragma unroll
@@ -178,9 +179,10 @@ struct SegArgs {
   const uint64_t *q;
   uint64_t *out;
   uint64_t n_l;
-  uint64_t columns;   // groups * P * m * (n / 64)
+  uint32_t columns;   // groups * P * m * (n / 64)
+  uint32_t cpg;       // columns per group: P * m * (n / 64)
   uint64_t tiles;     // per column
-  uint32_t P, n, m, sblocks;
+  uint32_t P, n, m, sblocks, log_sb;  // sblocks = n / 64, a power of two
   uint64_t *status;   // tiles * columns * 128
   int2 *dm;           // tiles * columns * 128
   uint32_t *ticket;
@@ -189,6 +191,36 @@ struct SegArgs {
 __device__ __forceinline__ uint64_t addmod_c(uint64_t a, uint64_t b, uint64_t p) {
   uint64_t s = a + b;
   return s >= p ? s - p : s;
+}
+
+// Residue sum of lazy products (each below 2^62) without a reduction per term: a 64-bit
+// sum and a count of its wraps past 2^64.
+__device__ __forceinline__ void lazy_add(uint64_t &s, uint32_t &wraps, uint64_t r) {
+  s += r;
+  wraps += s < r ? 1u : 0u;
+}
+
+// (s + wraps * 2^64) mod p, from 2^61 = delta and 2^64 = 8 * delta (mod p); wraps < 2^20
+// keeps the folded value below 2p.
+__device__ __forceinline__ uint64_t lazy_sum_mod(uint64_t s, uint32_t wraps, uint64_t p, uint32_t delta) {
+  const uint64_t v = (s & ((1ULL << 61) - 1)) + (uint64_t)((uint32_t)(s >> 61) + 8 * wraps) * delta;
+  return v >= p ? v - p : v;
+}
+
+// One element of a chain in (rho, D, M) form: rho in [0, p) is the running residue and r
+// in [0, 2p) the lazy product.  rho + r = h * 2^61 + l = h * p + v with v = l + h * delta
+// below 2p, so rho + r = k * p + rho' with k = h + [v >= p]; the element changes the
+// chain's count of subtracted-late p's by d = k - 1 (max-plus map (d, 0)).
+__device__ __forceinline__ void chain_step(uint64_t &rho, int32_t &D, int32_t &M, uint64_t r, uint64_t p,
+                                           uint32_t delta) {
+  const uint64_t u = rho + r;
+  const uint32_t h = (uint32_t)(u >> 61);
+  const uint64_t v = (u & ((1ULL << 61) - 1)) + (uint64_t)h * delta;
+  const bool b = v >= p;
+  rho = b ? v - p : v;
+  const int32_t d = (int32_t)h - (b ? 0 : 1);
+  D += d;
+  M = max(M + d, 0);
 }
 
 __device__ __forceinline__ uint64_t ld_status(const uint64_t *ptr) {
@@ -206,26 +238,26 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (threadIdx.x == 0) tile_sh = atomicAdd(a.ticket, 1u);
   __syncthreads();
-  const uint64_t ticket = tile_sh;
-  const uint64_t tile = ticket / a.columns, column = ticket % a.columns;
-  const uint32_t sb = (uint32_t)(column % a.sblocks);
-  uint64_t r = column / a.sblocks;
-  const uint32_t c = (uint32_t)(r % a.m);
-  r /= a.m;
-  const uint32_t k = (uint32_t)(r % a.P);
-  const uint64_t g = r / a.P;
+  // the ticket is workgroup-uniform: its decomposition runs once, in 32-bit scalar math.
+  // ticket = tile * columns + column;  column = g * cpg + cw;  cw = (k * m + c) * sblocks + sb
+  const uint32_t ticket = __builtin_amdgcn_readfirstlane(tile_sh);
+  const uint32_t tile = ticket / a.columns, column = ticket - tile * a.columns;
+  const uint32_t g = column / a.cpg, cw = column - g * a.cpg;
+  const uint32_t c = (cw >> a.log_sb) % a.m, sb = cw & (a.sblocks - 1);
   const uint64_t mn = (uint64_t)a.m * a.n;
-  const uint64_t cs = (uint64_t)c * a.n + sb * 64 + lane;
-  const uint64_t p = a.ms.p[c], delta = a.ms.delta[c];
+  const uint64_t p = a.ms.p[c];
+  const uint32_t delta = a.ms.delta[c];
 
   // phase A: lazy products of this lane's S elements, kept in registers
-  const uint64_t e0 = (tile * W + w) * S;
+  const uint64_t e0 = ((uint64_t)tile * W + w) * S;
   const int valid = e0 >= a.n_l ? 0 : (int)min<uint64_t>(S, a.n_l - e0);
   const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
-  const uint64_t *xp = a.db + ((g * a.n_l + e0) * a.P + k) * mn + cs;
-  const uint64_t *qp = a.q + e0 * qstride + cs;
+  // (k * m + c) * n + sb * 64 = cw * 64: the column's slots inside its element row
+  const uint64_t *xp = a.db + ((uint64_t)g * a.n_l + e0) * xstride + (uint64_t)cw * 64 + lane;
+  const uint64_t *qp = a.q + e0 * qstride + (uint64_t)c * a.n + sb * 64 + lane;
   uint64_t ra[S], rb[S];
   uint64_t sa = 0, sbm = 0;
+  uint32_t wa = 0, wb = 0;
   if (valid == S) {
     // all 3*S loads in flight before any product: the tile's HBM requests overlap
     uint64_t xv[S], av[S], bv[S];
@@ -239,8 +271,8 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
     for (int j = 0; j < S; j++) {
       ra[j] = shoup_lazy_qf(xv[j], av[j], p, delta);
       rb[j] = shoup_lazy_qf(xv[j], bv[j], p, delta);
-      sa = addmod_c(sa, ra[j] >= p ? ra[j] - p : ra[j], p);
-      sbm = addmod_c(sbm, rb[j] >= p ? rb[j] - p : rb[j], p);
+      lazy_add(sa, wa, ra[j]);
+      lazy_add(sbm, wb, rb[j]);
     }
   } else {
 #pragma unroll
@@ -250,13 +282,13 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
         const uint64_t xv = xp[j * xstride], av = qp[j * qstride], bv = qp[j * qstride + mn];
         ra[j] = shoup_lazy_qf(xv, av, p, delta);
         rb[j] = shoup_lazy_qf(xv, bv, p, delta);
-        sa = addmod_c(sa, ra[j] >= p ? ra[j] - p : ra[j], p);
-        sbm = addmod_c(sbm, rb[j] >= p ? rb[j] - p : rb[j], p);
+        lazy_add(sa, wa, ra[j]);
+        lazy_add(sbm, wb, rb[j]);
       }
     }
   }
-  rsum[w][0][lane] = sa;
-  rsum[w][1][lane] = sbm;
+  rsum[w][0][lane] = lazy_sum_mod(sa, wa, p, delta);
+  rsum[w][1][lane] = lazy_sum_mod(sbm, wb, p, delta);
   __syncthreads();
 
   // phase B: tile aggregate, decoupled look-back over earlier tiles of this column
@@ -267,7 +299,7 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
       ta = addmod_c(ta, rsum[i][0][lane], p);
       tb = addmod_c(tb, rsum[i][1][lane], p);
     }
-    uint64_t *my = a.status + ticket * 128;
+    uint64_t *my = a.status + (uint64_t)ticket * 128;
     uint64_t exa = 0, exb = 0;
     if (tile > 0) {
       st_status(my + lane, (1ULL << 62) | ta);
@@ -325,28 +357,18 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
   {
     uint64_t rho_a = rsum[w][0][lane], rho_b = rsum[w][1][lane];
     int32_t Da = 0, Ma = kNegInf, Db = 0, Mb = kNegInf;
+    if (valid == S) {
 #pragma unroll
-    for (int j = 0; j < S; j++) {
-      if (j < valid) {
-        {
-          const uint64_t rv = ra[j];
-          const int32_t b = rv >= p;
-          const uint64_t t = rho_a + (b ? rv - p : rv);
-          const int32_t cy = t >= p;
-          rho_a = cy ? t - p : t;
-          const int32_t d = b + cy - 1;
-          Da += d;
-          Ma = max(Ma + d, 0);
-        }
-        {
-          const uint64_t rv = rb[j];
-          const int32_t b = rv >= p;
-          const uint64_t t = rho_b + (b ? rv - p : rv);
-          const int32_t cy = t >= p;
-          rho_b = cy ? t - p : t;
-          const int32_t d = b + cy - 1;
-          Db += d;
-          Mb = max(Mb + d, 0);
+      for (int j = 0; j < S; j++) {
+        chain_step(rho_a, Da, Ma, ra[j], p, delta);
+        chain_step(rho_b, Db, Mb, rb[j], p, delta);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < S; j++) {
+        if (j < valid) {
+          chain_step(rho_a, Da, Ma, ra[j], p, delta);
+          chain_step(rho_b, Db, Mb, rb[j], p, delta);
         }
       }
     }
@@ -364,7 +386,7 @@ __global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
       fa = make_int2(fa.x + x.x, max(fa.y + x.x, x.y));
       fb = make_int2(fb.x + y.x, max(fb.y + y.x, y.y));
     }
-    int2 *o = a.dm + ticket * 128;
+    int2 *o = a.dm + (uint64_t)ticket * 128;
     o[lane] = fa;
     o[64 + lane] = fb;
   }
@@ -420,11 +442,11 @@ __global__ void __launch_bounds__(128 * FIX_PARTS) mac_fixup_kernel(SegArgs a) {
 
 namespace {
 
-template <int S, int W>
-int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
-  const uint64_t tile_elems = (uint64_t)S * W;
+// Tiles per column and the scratch layout for a segmented launch of tile_elems elements per tile.
+int seg_prepare(xpg_ctx *ctx, SegArgs &sa, uint64_t tile_elems) {
   sa.tiles = (sa.n_l + tile_elems - 1) / tile_elems;
   const uint64_t words = sa.tiles * sa.columns * 128;
+  if (sa.tiles * sa.columns > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   // scratch layout: [ticket, 256 B][status words, capacity][dm maps, capacity].  The
   // ticket and the whole status region are zero between launches: zeroed when the
   // region is (re)allocated, and every launch's fixup zeroes exactly the words it used.
@@ -439,13 +461,22 @@ int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
   sa.ticket = (uint32_t *)ctx->scratch;
   sa.status = (uint64_t *)((char *)ctx->scratch + 256);
   sa.dm = (int2 *)(sa.status + ctx->seg_capacity);
-  const uint64_t blocks = sa.tiles * sa.columns;
-  if (blocks > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
-  hipLaunchKernelGGL((mac_seg_kernel<S, W>), dim3((uint32_t)blocks), dim3(64 * W), 0, ctx->stream, sa);
-  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
+  return XPG_OK;
+}
+
+int launch_fixup(xpg_ctx *ctx, SegArgs &sa) {
   hipLaunchKernelGGL(mac_fixup_kernel, dim3((uint32_t)sa.columns), dim3(128 * FIX_PARTS), 0, ctx->stream, sa);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC fixup launch");
   return XPG_OK;
+}
+
+template <int S, int W>
+int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
+  int rc = seg_prepare(ctx, sa, (uint64_t)S * W);
+  if (rc) return rc;
+  hipLaunchKernelGGL((mac_seg_kernel<S, W>), dim3((uint32_t)(sa.tiles * sa.columns)), dim3(64 * W), 0, ctx->stream, sa);
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
+  return launch_fixup(ctx, sa);
 }
 
 }  // namespace
@@ -467,8 +498,9 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   a.m = prm->m;
   a.logn = prm->logn;
   const uint64_t lanes = groups * P * (uint64_t)prm->m * prm->n;
+  // the segmented kernel's columns are 64 slots wide: degrees below 64 always take one chain per lane
   const bool one_chain_per_lane =
-      ctx->mac_mode == 1 || (ctx->mac_mode == 0 && (lanes >= kDirectLanes || n_l <= 64 || prm->n < 64));
+      prm->n < 64 || ctx->mac_mode == 1 || (ctx->mac_mode == 0 && (lanes >= kDirectLanes || n_l <= 64));
   if (one_chain_per_lane && groups >= 4 && ctx->mac_mode != 1) {
     // many groups share each query word: register-blocked over groups
     uint64_t *outp = out;
@@ -493,12 +525,15 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
     sa.n = prm->n;
     sa.m = prm->m;
     sa.sblocks = prm->n / 64;
-    sa.columns = groups * P * prm->m * sa.sblocks;
+    sa.log_sb = (uint32_t)__builtin_ctz(sa.sblocks);
+    sa.cpg = P * prm->m * sa.sblocks;
+    if ((uint64_t)groups * sa.cpg > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
+    sa.columns = (uint32_t)(groups * sa.cpg);
     // tile shape: S elements per lane, W wavefronts per workgroup (tunable).  Measured on
-    // the 65536-element single query: 24 x 4 streams at 5.1 TB/s (more loads in flight
-    // per lane, fewer tiles to look back over) against 4.0 TB/s for 8 x 8.
+    // the 65536-element single query: 32 x 4 streams at 5.8 TB/s (more loads in flight
+    // per lane, fewer tiles to look back over) against 5.6 for 24 x 4 and 5.3 for 16 x 4.
     int S = ctx->seg_s, W = ctx->seg_w;
-    if (!S) S = n_l >= 4096 ? 24 : (n_l >= 512 ? 16 : 8);
+    if (!S) S = n_l >= 4096 ? 32 : (n_l >= 512 ? 16 : 8);
     if (!W) W = 4;
     const int key = S * 100 + W;
     switch (key) {
@@ -506,6 +541,9 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
       case 416: rc = launch_seg<4, 16>(ctx, sa); break;
       case 804: rc = launch_seg<8, 4>(ctx, sa); break;
       case 816: rc = launch_seg<8, 16>(ctx, sa); break;
+      case 1202: rc = launch_seg<12, 2>(ctx, sa); break;
+      case 1204: rc = launch_seg<12, 4>(ctx, sa); break;
+      case 1208: rc = launch_seg<12, 8>(ctx, sa); break;
       case 1602: rc = launch_seg<16, 2>(ctx, sa); break;
       case 1604: rc = launch_seg<16, 4>(ctx, sa); break;
       case 1608: rc = launch_seg<16, 8>(ctx, sa); break;
