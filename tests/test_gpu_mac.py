@@ -24,7 +24,7 @@ def _random_level(orc, rng, n_l, groups, P):
     return db, q
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 @pytest.mark.parametrize("desc,n_l,groups,P", [
     ("LWE:80:2048:120", 1, 1, 1),
     ("LWE:80:2048:120", 63, 1, 1),
@@ -78,7 +78,7 @@ def test_single_query_full_size_matches_golden(gpu_ctx):
     gpu_ctx.query_gen_dev(prm, skd, cfg["dims"], cfg["index"], cfg["noise_seed"], qd)
     rd = gpu_ctx.alloc(db.reply_words * 8)
     try:
-        for mode in (2, 1, 0):
+        for mode in (3, 2, 1, 0):
             gpu_ctx.set_option("mac_kernel", mode)
             gpu_ctx.reply_dev(db, qd, rd)
             rep = rd.download()
@@ -111,4 +111,28 @@ def test_segmented_tile_shapes_match_oracle(gpu_ctx, shape):
         gpu_ctx.set_option("mac_kernel", 0)
         gpu_ctx.set_option("seg_s", 0)
         gpu_ctx.set_option("seg_w", 0)
+        db.free()
+
+
+@pytest.mark.parametrize("u,e", [(4, 16), (8, 64), (8, 256), (16, 512), (8, 4096), (16, 48)])
+def test_streaming_block_shapes_match_oracle(gpu_ctx, u, e):
+    """The streaming MAC (block summaries + scan) at several block and chunk sizes: partial
+    last blocks and chunks, one block per chain, many blocks per chain."""
+    import xpir_amd
+    desc, n_l = "LWE:80:2048:120", 3001
+    orc = Oracle(desc, n_l)
+    prm = xpir_amd.params(desc, n_l)
+    rng = np.random.default_rng(123)
+    db_words, q = _random_level(orc, rng, n_l, 1, 1)
+    db_words.reshape(-1, orc.m, orc.n)[::7, 1, :] = np.uint64(2 * orc.p[1] - 1)
+    want = orc.reply_level(db_words, q, n_l, 1, 1)
+    db = gpu_ctx.import_ntt_words(prm, db_words, n_l, 1, 8, [n_l])
+    gpu_ctx.set_option("mac_kernel", 3)
+    gpu_ctx.set_option("stream_u", u)
+    gpu_ctx.set_option("stream_e", e)
+    try:
+        assert np.array_equal(gpu_ctx.reply(db, q), want)
+    finally:
+        for k in ("mac_kernel", "stream_u", "stream_e"):
+            gpu_ctx.set_option(k, 0)
         db.free()

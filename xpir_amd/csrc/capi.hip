@@ -253,8 +253,16 @@ void *xpg_ctx_stream(xpg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr;
 
 int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value) {
   if (!ctx || !key) return XPG_EINVAL;
-  if (strcmp(key, "mac_kernel") == 0 && value >= 0 && value <= 2) {
+  if (strcmp(key, "mac_kernel") == 0 && value >= 0 && value <= 3) {
     ctx->mac_mode = (int)value;
+    return XPG_OK;
+  }
+  if (strcmp(key, "stream_u") == 0 && (value == 0 || value == 4 || value == 8 || value == 16)) {
+    ctx->stream_u = (int)value;
+    return XPG_OK;
+  }
+  if (strcmp(key, "stream_e") == 0 && (value == 0 || (value >= 16 && value <= (1 << 20) && value % 16 == 0))) {
+    ctx->stream_e = (int)value;
     return XPG_OK;
   }
   if (strcmp(key, "seg_s") == 0 && (value == 0 || value == 4 || value == 8 || value == 12 || value == 16 || value == 24 || value == 32)) {

@@ -70,9 +70,11 @@ struct xpg_ctx {
   size_t ptr_count = 0;  // status words of the segmented MAC kept zero in scratch
   uint64_t *level_buf[2] = {nullptr, nullptr};
   size_t level_bytes[2] = {0, 0};
-  // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 2 = segmented chains
+  // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 2 = segmented chains with
+  // look-back, 3 = streaming block summaries
   int mac_mode = 0;
   int seg_s = 0, seg_w = 0;  // segmented tile shape (0 = by shape)
+  int stream_e = 0, stream_u = 0;  // streaming MAC: elements per block, per load chunk (0 = default)
   // MAC timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;

@@ -438,6 +438,150 @@ __global__ void __launch_bounds__(128 * FIX_PARTS) mac_fixup_kernel(SegArgs a) {
   a.out[gk * 2 * mn + chain * mn + (uint64_t)c * a.n + sb * 64 + lane] = rho + (uint64_t)j * a.ms.p[c];
 }
 
+// ---- streaming MAC: per-block chain summaries, no inter-workgroup communication ----
+//
+// Every wave runs one (column, block of E consecutive elements) and keeps, per chain, the
+// summary of its block's lazy products r_1..r_S read as a chain started from zero:
+//   rho0 = A_S mod p,  J_t = t - floor(A_t / p),  Jend = J_S,  Jmax = max_t J_t,
+//   Rstar = min { A_t mod p : J_t = Jmax },        A_t = r_1 + ... + r_t.
+// Entering the block with residue rho_in (the chain's residue before it), the number of
+// p's subtracted by step t is W_t = floor((rho_in + A_t) / p) = t - J_t + [rho_in + (A_t
+// mod p) >= p], so the block's max-plus map is
+//   D = [rho_in + rho0 >= p] - Jend,   M = D + Jmax - [rho_in + Rstar >= p]
+// (the largest J_t - [rho_in + (A_t mod p) >= p] is Jmax unless every t reaching Jmax
+// wraps, i.e. unless the smallest such residue does).  A second kernel scans the blocks
+// of each chain in element order: residues by a sum mod p, maps by max-plus composition.
+// Each DB and query word is read once; blocks never wait for one another.
+
+struct StreamArgs {
+  ModSet ms;
+  const uint64_t *db;
+  const uint64_t *q;
+  uint64_t *out;
+  uint64_t n_l;
+  uint32_t columns, cpg, P, n, m, sblocks, log_sb;
+  uint32_t E, nblk;     // elements per block (a multiple of the chunk), blocks per column
+  uint64_t *rho0;       // [nblk][columns][2][64]
+  uint64_t *rstar;      // same layout
+  int2 *jj;             // (Jend, Jmax), same layout
+};
+
+struct ChainSum {
+  uint64_t rho, rs;
+  int32_t J, Jmax;
+};
+
+// one element of a block summary (chain from zero): rho + r = k * p + rho', J += 1 - k
+__device__ __forceinline__ void sum_step(ChainSum &s, uint64_t r, uint64_t p, uint32_t delta) {
+  const uint64_t u = s.rho + r;  // < 3p
+  const uint32_t h = (uint32_t)(u >> 61);
+  const uint64_t v = (u & ((1ULL << 61) - 1)) + (uint64_t)h * delta;  // < 2p
+  const bool b = v >= p;
+  s.rho = b ? v - p : v;
+  s.J += 1 - (int32_t)h - (b ? 1 : 0);
+  const bool take = s.J > s.Jmax || (s.J == s.Jmax && s.rho < s.rs);
+  s.Jmax = take ? s.J : s.Jmax;
+  s.rs = take ? s.rho : s.rs;
+}
+
+template <int U>
+__global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
+  const uint32_t lane = threadIdx.x & 63;
+  // wave-uniform: global wave -> (block, column); adjacent waves take adjacent columns
+  const uint32_t gw = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const uint32_t blk = gw / a.columns, column = gw - blk * a.columns;
+  if (blk >= a.nblk) return;
+  const uint32_t g = column / a.cpg, cw = column - g * a.cpg;
+  const uint32_t c = (cw >> a.log_sb) % a.m, sb = cw & (a.sblocks - 1);
+  const uint64_t p = a.ms.p[c];
+  const uint32_t delta = a.ms.delta[c];
+  const uint64_t mn = (uint64_t)a.m * a.n;
+  const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
+  const uint64_t e0 = (uint64_t)blk * a.E;
+  const uint64_t e1 = min<uint64_t>(a.n_l, e0 + a.E);
+  const uint64_t *xp = a.db + ((uint64_t)g * a.n_l + e0) * xstride + (uint64_t)cw * 64 + lane;
+  const uint64_t *qp = a.q + e0 * qstride + (uint64_t)c * a.n + sb * 64 + lane;
+  ChainSum sa{0, 0, 0, kNegInf}, sbs{0, 0, 0, kNegInf};
+  for (uint64_t e = e0; e < e1; e += U) {
+    const int valid = (int)min<uint64_t>(U, e1 - e);
+    uint64_t xv[U], av[U], bv[U];
+    if (valid == U) {
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        xv[j] = __builtin_nontemporal_load(xp + j * xstride);
+        av[j] = __builtin_nontemporal_load(qp + j * qstride);
+        bv[j] = __builtin_nontemporal_load(qp + j * qstride + mn);
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        sum_step(sa, shoup_lazy_qf(xv[j], av[j], p, delta), p, delta);
+        sum_step(sbs, shoup_lazy_qf(xv[j], bv[j], p, delta), p, delta);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        if (j < valid) {
+          const uint64_t x = xp[j * xstride], ya = qp[j * qstride], yb = qp[j * qstride + mn];
+          sum_step(sa, shoup_lazy_qf(x, ya, p, delta), p, delta);
+          sum_step(sbs, shoup_lazy_qf(x, yb, p, delta), p, delta);
+        }
+      }
+    }
+    xp += U * xstride;
+    qp += U * qstride;
+  }
+  const uint64_t i = ((uint64_t)gw * 2) * 64 + lane;  // gw = blk * columns + column
+  a.rho0[i] = sa.rho;
+  a.rho0[i + 64] = sbs.rho;
+  a.rstar[i] = sa.rs;
+  a.rstar[i + 64] = sbs.rs;
+  a.jj[i] = make_int2(sa.J, sa.Jmax);
+  a.jj[i + 64] = make_int2(sbs.J, sbs.Jmax);
+}
+
+// Scan the block summaries of every chain in element order and emit the reply words.
+// Workgroup = one column: 128 chains x FIX_PARTS threads, each over a contiguous run of
+// blocks.  Pass 1 sums each run's residues; pass 2 evaluates the run's maps from the now
+// known incoming residues; the runs' maps are then composed in order.
+__global__ void __launch_bounds__(128 * FIX_PARTS) mac_stream_fixup_kernel(StreamArgs a) {
+  __shared__ uint64_t part_rho[FIX_PARTS][128];
+  __shared__ int2 part_map[FIX_PARTS][128];
+  const uint32_t column = blockIdx.x;
+  const uint32_t cl = threadIdx.x & 127, pi = threadIdx.x >> 7;
+  const uint32_t r = column >> a.log_sb;
+  const uint32_t c = r % a.m;
+  const uint64_t p = a.ms.p[c];
+  const uint32_t per = (a.nblk + FIX_PARTS - 1) / FIX_PARTS;
+  const uint32_t t0 = min(a.nblk, pi * per), t1 = min(a.nblk, t0 + per);
+  auto idx = [&](uint32_t t) { return ((uint64_t)t * a.columns + column) * 128 + cl; };
+  uint64_t s = 0;
+  for (uint32_t t = t0; t < t1; t++) s = addmod_c(s, a.rho0[idx(t)], p);
+  part_rho[pi][cl] = s;
+  __syncthreads();
+  uint64_t rho_in = 0;
+  for (uint32_t i = 0; i < pi; i++) rho_in = addmod_c(rho_in, part_rho[i][cl], p);
+  int2 f = make_int2(0, kNegInf);
+  for (uint32_t t = t0; t < t1; t++) {
+    const uint64_t r0 = a.rho0[idx(t)], rs = a.rstar[idx(t)];
+    const int2 j = a.jj[idx(t)];
+    const int32_t D = (rho_in + r0 >= p ? 1 : 0) - j.x;
+    const int32_t M = D + j.y - (rho_in + rs >= p ? 1 : 0);
+    f = make_int2(f.x + D, max(f.y + D, M));
+    rho_in = addmod_c(rho_in, r0, p);
+  }
+  part_map[pi][cl] = f;
+  __syncthreads();
+  if (pi != FIX_PARTS - 1) return;
+  int32_t jfin = 0;
+#pragma unroll
+  for (int i = 0; i < FIX_PARTS; i++) jfin = max(jfin + part_map[i][cl].x, part_map[i][cl].y);
+  const uint32_t chain = cl >> 6, lane = cl & 63;
+  const uint32_t sb = column & (a.sblocks - 1);
+  const uint64_t gk = r / a.m;
+  const uint64_t mn = (uint64_t)a.m * a.n;
+  a.out[gk * 2 * mn + chain * mn + (uint64_t)c * a.n + sb * 64 + lane] = rho_in + (uint64_t)jfin * p;
+}
+
 }  // namespace
 
 namespace {
@@ -479,6 +623,54 @@ int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
   return launch_fixup(ctx, sa);
 }
 
+// Streaming launch: block summaries into scratch, then the per-column scan.
+template <int U>
+int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
+  const uint64_t recs = (uint64_t)st.nblk * st.columns * 128;
+  const uint64_t waves = (uint64_t)st.nblk * st.columns;
+  if (waves > 0xffffffffULL - 3) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
+  int rc = ensure_scratch(ctx, recs * 24);
+  if (rc) return rc;
+  ctx->seg_capacity = 0;  // the segmented kernel's zeroed status region is overwritten
+  st.rho0 = (uint64_t *)ctx->scratch;
+  st.rstar = st.rho0 + recs;
+  st.jj = (int2 *)(st.rstar + recs);
+  hipLaunchKernelGGL((mac_stream_kernel<U>), dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, ctx->stream, st);
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
+  hipLaunchKernelGGL(mac_stream_fixup_kernel, dim3(st.columns), dim3(128 * FIX_PARTS), 0, ctx->stream, st);
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC fixup launch");
+  return XPG_OK;
+}
+
+// The streaming MAC for one level: block summaries, then the per-column scan.
+int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t groups) {
+  StreamArgs st{};
+  st.ms = a.ms;
+  st.db = a.db;
+  st.q = a.q;
+  st.out = a.out;
+  st.n_l = a.n_l;
+  st.P = a.P;
+  st.n = n;
+  st.m = a.m;
+  st.sblocks = n / 64;
+  st.log_sb = (uint32_t)__builtin_ctz(st.sblocks);
+  st.cpg = a.P * a.m * st.sblocks;
+  if (groups * st.cpg > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
+  st.columns = (uint32_t)(groups * st.cpg);
+  const int U = ctx->stream_u ? ctx->stream_u : 8;
+  uint64_t E = ctx->stream_e ? (uint64_t)ctx->stream_e : 256;
+  E = (E + U - 1) / U * U;
+  st.E = (uint32_t)E;
+  if ((a.n_l + E - 1) / E > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
+  st.nblk = (uint32_t)((a.n_l + E - 1) / E);
+  switch (U) {
+    case 4: return launch_stream<4>(ctx, st);
+    case 16: return launch_stream<16>(ctx, st);
+    default: return launch_stream<8>(ctx, st);
+  }
+}
+
 }  // namespace
 
 // Lanes at which one chain per lane already fills the GPU (256 CUs x 32 wavefronts).
@@ -509,7 +701,9 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
   int rc = XPG_OK;
-  if (one_chain_per_lane) {
+  if (!one_chain_per_lane && ctx->mac_mode == 3) {
+    rc = launch_stream_level(ctx, a, prm->n, groups);
+  } else if (one_chain_per_lane) {
     const uint64_t blocks = (lanes + 255) / 256;
     if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
     hipLaunchKernelGGL(mac_direct_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
