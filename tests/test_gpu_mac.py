@@ -114,8 +114,9 @@ def test_segmented_tile_shapes_match_oracle(gpu_ctx, shape):
         db.free()
 
 
-@pytest.mark.parametrize("u,e", [(4, 16), (8, 64), (8, 256), (16, 512), (8, 4096), (16, 48)])
-def test_streaming_block_shapes_match_oracle(gpu_ctx, u, e):
+@pytest.mark.parametrize("pf", [0, 1])
+@pytest.mark.parametrize("u,e", [(4, 16), (8, 64), (8, 256), (16, 512), (8, 4096), (16, 48), (4, 1000)])
+def test_streaming_block_shapes_match_oracle(gpu_ctx, u, e, pf):
     """The streaming MAC (block summaries + scan) at several block and chunk sizes: partial
     last blocks and chunks, one block per chain, many blocks per chain."""
     import xpir_amd
@@ -130,9 +131,10 @@ def test_streaming_block_shapes_match_oracle(gpu_ctx, u, e):
     gpu_ctx.set_option("mac_kernel", 3)
     gpu_ctx.set_option("stream_u", u)
     gpu_ctx.set_option("stream_e", e)
+    gpu_ctx.set_option("stream_pf", pf)
     try:
         assert np.array_equal(gpu_ctx.reply(db, q), want)
     finally:
-        for k in ("mac_kernel", "stream_u", "stream_e"):
+        for k in ("mac_kernel", "stream_u", "stream_e", "stream_pf"):
             gpu_ctx.set_option(k, 0)
         db.free()

@@ -220,6 +220,9 @@ int xpg_ctx_create(int device, xpg_ctx **out) {
   if (hipSetDevice(device) != hipSuccess) return XPG_EDEVICE;
   xpg_ctx *ctx = new xpg_ctx();
   ctx->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    ctx->cu_count = cus;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return XPG_EDEVICE;
@@ -257,11 +260,15 @@ int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value) {
     ctx->mac_mode = (int)value;
     return XPG_OK;
   }
+  if (strcmp(key, "stream_pf") == 0 && (value == 0 || value == 1)) {
+    ctx->stream_pf = (int)value;
+    return XPG_OK;
+  }
   if (strcmp(key, "stream_u") == 0 && (value == 0 || value == 4 || value == 8 || value == 16)) {
     ctx->stream_u = (int)value;
     return XPG_OK;
   }
-  if (strcmp(key, "stream_e") == 0 && (value == 0 || (value >= 16 && value <= (1 << 20) && value % 16 == 0))) {
+  if (strcmp(key, "stream_e") == 0 && value >= 0 && value <= (1 << 20)) {
     ctx->stream_e = (int)value;
     return XPG_OK;
   }

@@ -75,6 +75,8 @@ struct xpg_ctx {
   int mac_mode = 0;
   int seg_s = 0, seg_w = 0;  // segmented tile shape (0 = by shape)
   int stream_e = 0, stream_u = 0;  // streaming MAC: elements per block, per load chunk (0 = default)
+  int stream_pf = 0;               // streaming MAC: double-buffered chunk loads
+  int cu_count = 256;              // compute units of the device
   // MAC timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;

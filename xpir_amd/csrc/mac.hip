@@ -12,6 +12,8 @@
 // The final words are not canonical (about 0.4% lie in [p,2p)), so the chain order is
 // part of the result.  y' is derived in registers from the special form of p (never
 // stored: for a single query it would double the query bytes).
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace {
@@ -471,11 +473,11 @@ struct ChainSum {
   int32_t J, Jmax;
 };
 
-// one element of a block summary (chain from zero): rho + r = k * p + rho', J += 1 - k
-__device__ __forceinline__ void sum_step(ChainSum &s, uint64_t r, uint64_t p, uint32_t delta) {
-  const uint64_t u = s.rho + r;  // < 3p
+// one element of a block summary (chain from zero), given u = rho + r < 3p:
+// u = k * p + rho' with k = h + [v >= p], u = h * 2^61 + l, v = l + h * delta < 2p; J += 1 - k
+__device__ __forceinline__ void sum_step_u(ChainSum &s, uint64_t u, uint64_t p, uint32_t delta) {
   const uint32_t h = (uint32_t)(u >> 61);
-  const uint64_t v = (u & ((1ULL << 61) - 1)) + (uint64_t)h * delta;  // < 2p
+  const uint64_t v = (u & ((1ULL << 61) - 1)) + (uint64_t)h * delta;
   const bool b = v >= p;
   s.rho = b ? v - p : v;
   s.J += 1 - (int32_t)h - (b ? 1 : 0);
@@ -484,7 +486,54 @@ __device__ __forceinline__ void sum_step(ChainSum &s, uint64_t r, uint64_t p, ui
   s.rs = take ? s.rho : s.rs;
 }
 
+// Loads of one chunk of U elements (x, a, b words of this lane); the elements at or past
+// `valid` read as zero and are not summed.
 template <int U>
+__device__ __forceinline__ void stream_load(const uint64_t *xp, const uint64_t *qp, uint64_t xstride, uint64_t qstride,
+                                            uint64_t mn, int valid, uint64_t (&xv)[U], uint64_t (&av)[U],
+                                            uint64_t (&bv)[U]) {
+  if (valid >= U) {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      xv[j] = __builtin_nontemporal_load(xp + j * xstride);
+      av[j] = __builtin_nontemporal_load(qp + j * qstride);
+      bv[j] = __builtin_nontemporal_load(qp + j * qstride + mn);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      xv[j] = j < valid ? xp[j * xstride] : 0;
+      av[j] = j < valid ? qp[j * qstride] : 0;
+      bv[j] = j < valid ? qp[j * qstride + mn] : 0;
+    }
+  }
+}
+
+template <int U>
+__device__ __forceinline__ void stream_sum(ChainSum &sa, ChainSum &sb, int valid, const uint64_t (&xv)[U],
+                                           const uint64_t (&av)[U], const uint64_t (&bv)[U], uint64_t p,
+                                           uint32_t delta) {
+  if (valid >= U) {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      sum_step_u(sa, add_shoup_lazy_qf(sa.rho, xv[j], av[j], p, delta), p, delta);
+      sum_step_u(sb, add_shoup_lazy_qf(sb.rho, xv[j], bv[j], p, delta), p, delta);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      if (j < valid) {
+        sum_step_u(sa, add_shoup_lazy_qf(sa.rho, xv[j], av[j], p, delta), p, delta);
+        sum_step_u(sb, add_shoup_lazy_qf(sb.rho, xv[j], bv[j], p, delta), p, delta);
+      }
+    }
+  }
+}
+
+// One wave = one (column, block), read in chunks of U elements.  PF: two register sets in
+// turn, so the next chunk's loads are in flight while the current chunk's products and
+// chain steps run.
+template <int U, bool PF>
 __global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
   const uint32_t lane = threadIdx.x & 63;
   // wave-uniform: global wave -> (block, column); adjacent waves take adjacent columns
@@ -498,37 +547,32 @@ __global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
   const uint64_t mn = (uint64_t)a.m * a.n;
   const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
   const uint64_t e0 = (uint64_t)blk * a.E;
-  const uint64_t e1 = min<uint64_t>(a.n_l, e0 + a.E);
+  const int64_t n = (int64_t)(min<uint64_t>(a.n_l, e0 + a.E) - e0);  // elements in this block
   const uint64_t *xp = a.db + ((uint64_t)g * a.n_l + e0) * xstride + (uint64_t)cw * 64 + lane;
   const uint64_t *qp = a.q + e0 * qstride + (uint64_t)c * a.n + sb * 64 + lane;
   ChainSum sa{0, 0, 0, kNegInf}, sbs{0, 0, 0, kNegInf};
-  for (uint64_t e = e0; e < e1; e += U) {
-    const int valid = (int)min<uint64_t>(U, e1 - e);
-    uint64_t xv[U], av[U], bv[U];
-    if (valid == U) {
-#pragma unroll
-      for (int j = 0; j < U; j++) {
-        xv[j] = __builtin_nontemporal_load(xp + j * xstride);
-        av[j] = __builtin_nontemporal_load(qp + j * qstride);
-        bv[j] = __builtin_nontemporal_load(qp + j * qstride + mn);
-      }
-#pragma unroll
-      for (int j = 0; j < U; j++) {
-        sum_step(sa, shoup_lazy_qf(xv[j], av[j], p, delta), p, delta);
-        sum_step(sbs, shoup_lazy_qf(xv[j], bv[j], p, delta), p, delta);
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < U; j++) {
-        if (j < valid) {
-          const uint64_t x = xp[j * xstride], ya = qp[j * qstride], yb = qp[j * qstride + mn];
-          sum_step(sa, shoup_lazy_qf(x, ya, p, delta), p, delta);
-          sum_step(sbs, shoup_lazy_qf(x, yb, p, delta), p, delta);
-        }
-      }
+  if (!PF) {
+    // one register set: the chunk's loads, then its arithmetic (other waves fill the gap)
+    for (int64_t e = 0; e < n; e += U) {
+      uint64_t xv[U], av[U], bv[U];
+      const int valid = (int)min<int64_t>(n - e, U);
+      stream_load<U>(xp + e * xstride, qp + e * qstride, xstride, qstride, mn, valid, xv, av, bv);
+      stream_sum<U>(sa, sbs, valid, xv, av, bv, p, delta);
     }
-    xp += U * xstride;
-    qp += U * qstride;
+  } else {
+  uint64_t x0[U], a0[U], b0[U], x1[U], a1[U], b1[U];
+  stream_load<U>(xp, qp, xstride, qstride, mn, (int)min<int64_t>(n, U), x0, a0, b0);
+  for (int64_t e = 0; e < n; e += 2 * U) {
+    if (e + U < n)
+      stream_load<U>(xp + (e + U) * xstride, qp + (e + U) * qstride, xstride, qstride, mn,
+                     (int)min<int64_t>(n - e - U, U), x1, a1, b1);
+    stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e, U), x0, a0, b0, p, delta);
+    if (e + U >= n) break;
+    if (e + 2 * U < n)
+      stream_load<U>(xp + (e + 2 * U) * xstride, qp + (e + 2 * U) * qstride, xstride, qstride, mn,
+                     (int)min<int64_t>(n - e - 2 * U, U), x0, a0, b0);
+    stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e - U, U), x1, a1, b1, p, delta);
+  }
   }
   const uint64_t i = ((uint64_t)gw * 2) * 64 + lane;  // gw = blk * columns + column
   a.rho0[i] = sa.rho;
@@ -624,7 +668,7 @@ int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
 }
 
 // Streaming launch: block summaries into scratch, then the per-column scan.
-template <int U>
+template <int U, bool PF>
 int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
   const uint64_t recs = (uint64_t)st.nblk * st.columns * 128;
   const uint64_t waves = (uint64_t)st.nblk * st.columns;
@@ -635,7 +679,7 @@ int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
   st.rho0 = (uint64_t *)ctx->scratch;
   st.rstar = st.rho0 + recs;
   st.jj = (int2 *)(st.rstar + recs);
-  hipLaunchKernelGGL((mac_stream_kernel<U>), dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, ctx->stream, st);
+  hipLaunchKernelGGL((mac_stream_kernel<U, PF>), dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, ctx->stream, st);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
   hipLaunchKernelGGL(mac_stream_fixup_kernel, dim3(st.columns), dim3(128 * FIX_PARTS), 0, ctx->stream, st);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC fixup launch");
@@ -659,15 +703,23 @@ int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t gro
   if (groups * st.cpg > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.columns = (uint32_t)(groups * st.cpg);
   const int U = ctx->stream_u ? ctx->stream_u : 8;
-  uint64_t E = ctx->stream_e ? (uint64_t)ctx->stream_e : 256;
+  uint64_t E = (uint64_t)ctx->stream_e;
+  if (!E) {
+    // about four waves per SIMD, all resident at once: one pass over the level, and few
+    // block summaries (their writes interleave with the level's reads)
+    const uint64_t target = 16ull * (uint64_t)ctx->cu_count;
+    const uint64_t nb = std::max<uint64_t>(1, target / st.columns);
+    E = (a.n_l + nb - 1) / nb;
+  }
   E = (E + U - 1) / U * U;
   st.E = (uint32_t)E;
   if ((a.n_l + E - 1) / E > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.nblk = (uint32_t)((a.n_l + E - 1) / E);
+  const bool pf = ctx->stream_pf != 0;
   switch (U) {
-    case 4: return launch_stream<4>(ctx, st);
-    case 16: return launch_stream<16>(ctx, st);
-    default: return launch_stream<8>(ctx, st);
+    case 4: return pf ? launch_stream<4, true>(ctx, st) : launch_stream<4, false>(ctx, st);
+    case 16: return pf ? launch_stream<16, true>(ctx, st) : launch_stream<16, false>(ctx, st);
+    default: return pf ? launch_stream<8, true>(ctx, st) : launch_stream<8, false>(ctx, st);
   }
 }
 
@@ -701,7 +753,7 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
   int rc = XPG_OK;
-  if (!one_chain_per_lane && ctx->mac_mode == 3) {
+  if (!one_chain_per_lane && (ctx->mac_mode == 3 || ctx->mac_mode == 0)) {
     rc = launch_stream_level(ctx, a, prm->n, groups);
   } else if (one_chain_per_lane) {
     const uint64_t blocks = (lanes + 255) / 256;
