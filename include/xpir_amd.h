@@ -163,6 +163,9 @@ int xpg_server_stats(xpg_server *srv, uint64_t *batches, uint64_t *batched_reque
 int xpg_timing_enable(xpg_ctx *ctx, int on);
 /* total milliseconds and launch count of the MAC kernels since the last reset */
 int xpg_timing_read(xpg_ctx *ctx, double *mac_ms, uint64_t *mac_launches, int reset);
+/* the same for the dominant kernel of each MAC alone (the streaming block-summary,
+ * segmented, one-chain-per-lane or batched kernel, without its scan or fixup) */
+int xpg_timing_read_main(xpg_ctx *ctx, double *ms, uint64_t *launches, int reset);
 
 /* ---- client-side helpers (test and load-generation support; the reference's
  *      NFLLWE::enc/encrypt, crypto/NFLLWE.cpp:276-372,481-498, and

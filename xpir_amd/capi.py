@@ -22,7 +22,7 @@ EXPORTS = [
     "xpg_dev_free", "xpg_h2d", "xpg_d2h", "xpg_db_import", "xpg_db_import_dev",
     "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
     "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev",
-    "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_keygen",
+    "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_timing_read_main", "xpg_keygen",
     "xpg_query_gen_dev", "xpg_random_bytes_dev", "xpg_server_create", "xpg_server_destroy",
     "xpg_server_submit", "xpg_server_poll", "xpg_server_release", "xpg_server_now_us", "xpg_server_stats",
 ]
@@ -93,6 +93,7 @@ def lib() -> C.CDLL:
         "xpg_reply_batch_dev": (C.c_int, [vp, C.POINTER(XpgRequest), u32]),
         "xpg_timing_enable": (C.c_int, [vp, C.c_int]),
         "xpg_timing_read": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
+        "xpg_timing_read_main": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
         "xpg_keygen": (C.c_int, [P, u64, vp]),
         "xpg_query_gen_dev": (C.c_int, [vp, P, vp, u32, C.POINTER(u32), u64, u64, vp]),
         "xpg_random_bytes_dev": (C.c_int, [vp, u64, u64, vp, u64]),
@@ -278,6 +279,12 @@ class Context:
     def timing_read(self, reset: bool = True):
         ms, n = C.c_double(), C.c_uint64()
         self._check(lib().xpg_timing_read(self.h, C.byref(ms), C.byref(n), 1 if reset else 0))
+        return ms.value, n.value
+
+    def timing_read_main(self, reset: bool = True):
+        """(ms, launches) of the dominant MAC kernel alone, without its scan or fixup."""
+        ms, n = C.c_double(), C.c_uint64()
+        self._check(lib().xpg_timing_read_main(self.h, C.byref(ms), C.byref(n), 1 if reset else 0))
         return ms.value, n.value
 
     def query_gen_dev(self, prm: XpgParams, sk_dev: DeviceBuffer, dims, index: int, noise_seed: int,

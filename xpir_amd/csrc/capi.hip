@@ -152,21 +152,27 @@ void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b) {
   (void)hipEventRecord(*a, ctx->stream);
 }
 
-void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b) {
+void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b, int tag) {
   if (!ctx->timing || !a) return;
   (void)hipEventRecord(b, ctx->stream);
-  ctx->pending.emplace_back(a, b);
+  ctx->pending.emplace_back(a, b, tag);
 }
 
 static void timing_collect(xpg_ctx *ctx) {
   for (auto &pr : ctx->pending) {
     float ms = 0.f;
-    if (hipEventSynchronize(pr.second) == hipSuccess && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
-      ctx->mac_ms += ms;
-      ctx->mac_launches++;
+    hipEvent_t a = std::get<0>(pr), b = std::get<1>(pr);
+    if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+      if (std::get<2>(pr) == 1) {
+        ctx->main_ms += ms;
+        ctx->main_launches++;
+      } else {
+        ctx->mac_ms += ms;
+        ctx->mac_launches++;
+      }
     }
-    ctx->ev_pool.push_back(pr.first);
-    ctx->ev_pool.push_back(pr.second);
+    ctx->ev_pool.push_back(a);
+    ctx->ev_pool.push_back(b);
   }
   ctx->pending.clear();
 }
@@ -596,6 +602,18 @@ int xpg_timing_read(xpg_ctx *ctx, double *mac_ms, uint64_t *launches, int reset)
   if (reset) {
     ctx->mac_ms = 0;
     ctx->mac_launches = 0;
+  }
+  return XPG_OK;
+}
+
+int xpg_timing_read_main(xpg_ctx *ctx, double *ms, uint64_t *launches, int reset) {
+  if (!ctx) return XPG_EINVAL;
+  timing_collect(ctx);
+  if (ms) *ms = ctx->main_ms;
+  if (launches) *launches = ctx->main_launches;
+  if (reset) {
+    ctx->main_ms = 0;
+    ctx->main_launches = 0;
   }
   return XPG_OK;
 }

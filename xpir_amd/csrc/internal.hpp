@@ -7,6 +7,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/xpir_amd.h"
@@ -80,9 +81,11 @@ struct xpg_ctx {
   // MAC timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<std::tuple<hipEvent_t, hipEvent_t, int>> pending;  // (start, end, 0 = whole MAC | 1 = main kernel)
   double mac_ms = 0.0;
   uint64_t mac_launches = 0;
+  double main_ms = 0.0;        // the dominant kernel of each MAC (without its scan or fixup)
+  uint64_t main_launches = 0;
 };
 
 // ---- host helpers (context.hip) ----
@@ -93,7 +96,7 @@ int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
 ModSet modset_of(const xpg_params *prm);
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
-void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b);
+void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b, int tag = 0);
 
 // ---- launchers (import.hip) ----
 // Transform `units` units of a byte source into NTT words (units x P x m x n); the

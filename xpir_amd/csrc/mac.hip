@@ -662,7 +662,10 @@ template <int S, int W>
 int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
   int rc = seg_prepare(ctx, sa, (uint64_t)S * W);
   if (rc) return rc;
+  hipEvent_t k0 = nullptr, k1 = nullptr;
+  timing_begin(ctx, &k0, &k1);
   hipLaunchKernelGGL((mac_seg_kernel<S, W>), dim3((uint32_t)(sa.tiles * sa.columns)), dim3(64 * W), 0, ctx->stream, sa);
+  timing_end(ctx, k0, k1, 1);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
   return launch_fixup(ctx, sa);
 }
@@ -679,7 +682,10 @@ int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
   st.rho0 = (uint64_t *)ctx->scratch;
   st.rstar = st.rho0 + recs;
   st.jj = (int2 *)(st.rstar + recs);
+  hipEvent_t k0 = nullptr, k1 = nullptr;
+  timing_begin(ctx, &k0, &k1);
   hipLaunchKernelGGL((mac_stream_kernel<U, PF>), dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, ctx->stream, st);
+  timing_end(ctx, k0, k1, 1);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
   hipLaunchKernelGGL(mac_stream_fixup_kernel, dim3(st.columns), dim3(128 * FIX_PARTS), 0, ctx->stream, st);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC fixup launch");
@@ -758,7 +764,10 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   } else if (one_chain_per_lane) {
     const uint64_t blocks = (lanes + 255) / 256;
     if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
+    hipEvent_t k0 = nullptr, k1 = nullptr;
+    timing_begin(ctx, &k0, &k1);
     hipLaunchKernelGGL(mac_direct_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
+    timing_end(ctx, k0, k1, 1);
     if (hipGetLastError() != hipSuccess) rc = set_err(ctx, XPG_EDEVICE, "MAC launch");
   } else {
     SegArgs sa{};
@@ -830,6 +839,8 @@ int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "batched MAC grid too large");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
+  hipEvent_t k0 = nullptr, k1 = nullptr;
+  timing_begin(ctx, &k0, &k1);
   const int key = GR * 10 + QR;
   switch (key) {
     case 82: hipLaunchKernelGGL((mac_batch2_kernel<8, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
@@ -842,6 +853,7 @@ int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
     default: hipLaunchKernelGGL((mac_batch2_kernel<1, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
   }
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "batched MAC launch");
+  timing_end(ctx, k0, k1, 1);
   timing_end(ctx, e0, e1);
   return XPG_OK;
 }
