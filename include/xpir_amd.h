@@ -65,10 +65,13 @@ const char *xpg_last_error(const xpg_ctx *ctx);
 void *xpg_ctx_stream(xpg_ctx *ctx);
 int xpg_sync(xpg_ctx *ctx);
 
-/* tuning knobs: "mac_kernel" = 0 (by shape), 1 (one chain per lane), 2 (segmented chains);
- * "seg_s" / "seg_w" = segmented tile shape (elements per lane 4|8|16, wavefronts per
- * workgroup 4|8|16; 0 = by shape).  No reference counterpart; every setting yields the
- * same reply bytes. */
+/* tuning knobs: "mac_kernel" = 0 (by shape), 1 (one chain per lane), 2 (segmented chains
+ * with look-back), 3 (streaming block summaries; what 0 picks for few, long chains);
+ * "seg_s" / "seg_w" = segmented tile shape (elements per lane 4|8|12|16|24|32,
+ * wavefronts per workgroup 2|4|8|16; 0 = by shape); "stream_e" = elements per streaming
+ * block (0 = sized for one pass), "stream_u" = elements per load chunk 4|8|16 (0 = 8),
+ * "stream_pf" = 1 to double-buffer the chunks.  No reference counterpart; every setting
+ * yields the same reply bytes. */
 int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value);
 
 /* device memory plumbing for callers without a GPU runtime of their own */
