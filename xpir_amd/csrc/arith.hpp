@@ -28,45 +28,29 @@ __device__ __forceinline__ uint64_t shoup_canon(uint64_t x, uint64_t y, uint64_t
   return r >= p ? r - p : r;
 }
 
-// y*delta as a 128-bit value for delta < 2^32: two 32x32->64 products.
-__device__ __forceinline__ void mul64x32(uint64_t y, uint32_t d, uint64_t &lo, uint64_t &hi) {
-  uint64_t a = (uint64_t)(uint32_t)y * d;          // < 2^53
-  uint64_t b = (uint64_t)(uint32_t)(y >> 32) * d;  // < 2^53
-  lo = a + (b << 32);
-  hi = (b >> 32) + (lo < a ? 1 : 0);
-}
-
 // floor(y * 2^64 / p) mod 2^64 -- the Shoup quotient the reference precomputes per query
 // word on pushQuery (PIRReplyGeneratorNFL_internal.cpp:760-767) -- derived without a
-// division from p = 2^61 - delta:  y*2^64/p = 8y + 8*y*delta/p, and N = 8*y*delta
-// < 2^90 splits as N = t*2^61 + r0 = t*p + (r0 + t*delta) with r0 + t*delta < 2p.
+// division from p = 2^61 - delta:  y * 2^64 / p = 8y + 8*y*delta/p, and with
+// t0 = (y*delta) >> 58 and N = 8*y*delta = t0 * 2^61 + nm = t0 * p + (nm + t0*delta),
+// where nm + t0*delta < 2p:  y' = 8y + t0 + [nm + t0*delta >= p]  (mod 2^64).
 // Holds for every y < 2^64 (checked exhaustively at the edges and on random words).
-__device__ __forceinline__ uint64_t shoup_quot(uint64_t y, uint64_t p, uint32_t delta) {
-  uint64_t lo, hi;
-  mul64x32(y, delta, lo, hi);
-  uint64_t nlo = lo << 3;
-  uint64_t nhi = (hi << 3) | (lo >> 61);
-  uint64_t t = (nhi << 3) | (nlo >> 61);
-  uint64_t r = (nlo & ((1ULL << 61) - 1)) + t * (uint64_t)delta;
-  return 8 * y + t + (r >= p ? 1ULL : 0ULL);
-}
-
-// The lazy Shoup product x*y - floor(x*y'/2^64)*p, y' = floor(y*2^64/p) mod 2^64 derived
-// on the fly (shoup_quot), written as 32x32->64 multiply-adds (v_mad_u64_u32) and 32-bit
-// low multiplies:
-//   y'  = 8y + t0 + [((y*delta mod 2^58) << 3) + t0*delta >= p],  t0 = (y*delta) >> 58
-//   q   = high word of x*y'
-//   r   = x*y - q*p  (mod 2^64)  =  x*y + q*delta - (q << 61), where only the low three
-//         bits of q reach the high word through q << 61.
-// Equal to shoup_lazy(x, y, shoup_quot(y, p, delta), p) for every x, y < 2^64.
-__host__ __device__ __forceinline__ uint64_t shoup_lazy_qf(uint64_t x, uint64_t y, uint64_t p, uint32_t delta) {
+__host__ __device__ __forceinline__ uint64_t shoup_quot(uint64_t y, uint64_t p, uint32_t delta) {
   const uint32_t d = delta, yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
   const uint64_t a0 = (uint64_t)yl * d;
   const uint64_t a1 = (uint64_t)yh * d + (a0 >> 32);  // y*delta = a1 * 2^32 + low word of a0
   const uint32_t t0 = (uint32_t)(a1 >> 26);
   const uint64_t nm = (((a1 & 0x3ffffffULL) << 32) | (uint32_t)a0) << 3;
   const uint32_t t = t0 + ((uint64_t)t0 * d + nm >= p ? 1u : 0u);  // floor(8*y*delta / p)
-  const uint64_t yq = (y << 3) + t;
+  return (y << 3) + t;
+}
+
+// The lazy Shoup product x*y - floor(x*yq/2^64)*p (mod 2^64) for p = 2^61 - delta,
+// written as 32x32->64 multiply-adds (v_mad_u64_u32) and 32-bit low multiplies:
+//   q = high word of x*yq,   r = x*y + q*delta - (q << 61)  (mod 2^64),
+// where only the low three bits of q reach the high word through q << 61.
+// Equal to shoup_lazy(x, y, yq, p) for every x, y, yq < 2^64.
+__host__ __device__ __forceinline__ uint64_t shoup_lazy_sf(uint64_t x, uint64_t y, uint64_t yq, uint32_t delta) {
+  const uint32_t d = delta, yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
   const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), ql = (uint32_t)yq, qh = (uint32_t)(yq >> 32);
   const uint64_t m1 = (uint64_t)xh * ql + (((uint64_t)xl * ql) >> 32);
   const uint64_t m2 = (uint64_t)xl * qh + (uint32_t)m1;
@@ -74,6 +58,12 @@ __host__ __device__ __forceinline__ uint64_t shoup_lazy_qf(uint64_t x, uint64_t 
   const uint64_t lo = (uint64_t)xl * yl + (uint64_t)(uint32_t)q * d;
   const uint32_t hi = (uint32_t)(lo >> 32) + xl * yh + xh * yl + (uint32_t)(q >> 32) * d - ((uint32_t)q << 29);
   return ((uint64_t)hi << 32) | (uint32_t)lo;
+}
+
+// The lazy Shoup product with the query quotient derived on the fly (shoup_quot):
+// equal to shoup_lazy(x, y, shoup_quot(y, p, delta), p) for every x, y < 2^64.
+__host__ __device__ __forceinline__ uint64_t shoup_lazy_qf(uint64_t x, uint64_t y, uint64_t p, uint32_t delta) {
+  return shoup_lazy_sf(x, y, shoup_quot(y, p, delta), delta);
 }
 
 // acc + shoup_lazy_qf(x, y, p, delta) (mod 2^64), the addition written on 32-bit halves
@@ -89,8 +79,8 @@ __device__ __forceinline__ uint64_t add_shoup_lazy_qf(uint64_t acc, uint64_t x, 
 
 // One reply-chain step (crypto/NFLlib.hpp:303-321): acc += lazy product; -p if >= p.
 __device__ __forceinline__ uint64_t mac_step(uint64_t acc, uint64_t x, uint64_t y, uint64_t yq,
-                                             uint64_t p) {
-  acc += shoup_lazy(x, y, yq, p);
+                                             uint64_t p, uint32_t delta) {
+  acc += shoup_lazy_sf(x, y, yq, delta);
   return acc >= p ? acc - p : acc;
 }
 

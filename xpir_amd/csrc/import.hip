@@ -96,6 +96,7 @@ __global__ void __launch_bounds__(1024) ntt_kernel(NttArgs a, ByteSrc bs, Absorb
   const uint32_t k = (uint32_t)(uk % a.P);
   const uint64_t u = uk / a.P;
   const uint64_t p = a.ms.p[c], p2 = 2 * p;
+  const uint32_t delta = a.ms.delta[c];
   const uint64_t *phi = a.phi + (size_t)c * n, *phiq = a.phiq + (size_t)c * n;
   const uint64_t *w = a.w + (size_t)c * n, *wq = a.wq + (size_t)c * n;
 
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(1024) ntt_kernel(NttArgs a, ByteSrc bs, Absorb
       s = s >= p2 ? s - p2 : s;
       const uint64_t d = u0 - u1 + p2;
       sh[i0] = s;
-      sh[i1] = shoup_lazy(d, tw[j], twq[j], p);
+      sh[i1] = shoup_lazy_sf(d, tw[j], twq[j], delta);
     }
     __syncthreads();
   }
@@ -137,7 +138,7 @@ __global__ void __launch_bounds__(1024) ntt_kernel(NttArgs a, ByteSrc bs, Absorb
     uint64_t v0 = u0 + u2; v0 = v0 >= p2 ? v0 - p2 : v0;
     uint64_t v2 = u0 - u2; v2 = (int64_t)v2 < 0 ? v2 + p2 : v2;
     uint64_t v1 = u1 + u3; v1 = v1 >= p2 ? v1 - p2 : v1;
-    const uint64_t v3 = shoup_lazy(u1 - u3 + p2, W, Wq, p);
+    const uint64_t v3 = shoup_lazy_sf(u1 - u3 + p2, W, Wq, delta);
     uint64_t z[4];
     z[0] = v0 + v1; z[0] = z[0] >= p2 ? z[0] - p2 : z[0];
     z[1] = v0 - v1; z[1] = (int64_t)z[1] < 0 ? z[1] + p2 : z[1];
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(1024) ntt_kernel(NttArgs a, ByteSrc bs, Absorb
         const uint32_t pos = base + i;
         const uint64_t av = uniform_mod(rand_word(es.noise_seed ^ kAStreamXor, elem * mn + (uint64_t)c * n + pos), p);
         const uint64_t s = es.sk[(uint64_t)c * n + pos];
-        uint64_t bv = mac_step(z[i], av, s, shoup_quot(s, p, a.ms.delta[c]), p);
+        uint64_t bv = mac_step(z[i], av, s, shoup_quot(s, p, a.ms.delta[c]), p, a.ms.delta[c]);
         bv = bv >= p ? bv - p : bv;
         bv = bv >= p ? bv - p : bv;
         oa[i] = av;

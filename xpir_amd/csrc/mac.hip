@@ -121,19 +121,36 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
   for (int g = 0; g < GR; g++)
 #pragma unroll
     for (int i = 0; i < QR; i++) acc_a[g][i] = acc_b[g][i] = 0;
-  for (uint64_t e = 0; e < a.n_l; e++) {
-    uint64_t xv[GR];
+  // UE elements per iteration: every load of the chunk is issued before its arithmetic,
+  // which then runs in element order (the chains' order)
+  constexpr int UE = GR * QR >= 16 ? 2 : 4;
+  for (uint64_t e = 0; e < a.n_l; e += UE) {
+    const int ve = (int)min<uint64_t>(UE, a.n_l - e);
+    uint64_t xv[UE][GR], ya[UE][QR], yb[UE][QR];
 #pragma unroll
-    for (int g = 0; g < GR; g++) xv[g] = g < gv ? __builtin_nontemporal_load(x + g * gstride + e * xstride) : 0;
-    const uint64_t o = e * qstride;
+    for (int u = 0; u < UE; u++) {
+      const bool in = u < ve;
+      const uint64_t eo = (e + u) * xstride, o = (e + u) * qstride;
 #pragma unroll
-    for (int i = 0; i < QR; i++) {
-      const uint64_t ya = qa[i][o], yb = qa[i][o + mn];
-      const uint64_t yaq = shoup_quot(ya, p, delta), ybq = shoup_quot(yb, p, delta);
+      for (int g = 0; g < GR; g++) xv[u][g] = in && g < gv ? __builtin_nontemporal_load(x + g * gstride + eo) : 0;
 #pragma unroll
-      for (int g = 0; g < GR; g++) {
-        acc_a[g][i] = mac_step(acc_a[g][i], xv[g], ya, yaq, p);
-        acc_b[g][i] = mac_step(acc_b[g][i], xv[g], yb, ybq, p);
+      for (int i = 0; i < QR; i++) {
+        ya[u][i] = in ? qa[i][o] : 0;
+        yb[u][i] = in ? qa[i][o + mn] : 0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UE; u++) {
+      if (u < ve) {
+#pragma unroll
+        for (int i = 0; i < QR; i++) {
+          const uint64_t yaq = shoup_quot(ya[u][i], p, delta), ybq = shoup_quot(yb[u][i], p, delta);
+#pragma unroll
+          for (int g = 0; g < GR; g++) {
+            acc_a[g][i] = mac_step(acc_a[g][i], xv[u][g], ya[u][i], yaq, p, delta);
+            acc_b[g][i] = mac_step(acc_b[g][i], xv[u][g], yb[u][i], ybq, p, delta);
+          }
+        }
       }
     }
   }
@@ -852,8 +869,8 @@ int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
     case 12: hipLaunchKernelGGL((mac_batch2_kernel<1, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
     default: hipLaunchKernelGGL((mac_batch2_kernel<1, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
   }
-  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "batched MAC launch");
+  const int rc = hipGetLastError() != hipSuccess ? set_err(ctx, XPG_EDEVICE, "batched MAC launch") : XPG_OK;
   timing_end(ctx, k0, k1, 1);
   timing_end(ctx, e0, e1);
-  return XPG_OK;
+  return rc;
 }
