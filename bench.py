@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--queries", type=int, default=2000, help="serving: queries in the Poisson stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-io", action="store_true",
+                    help="single/plain: also time replies from a host query to a host reply (PCIe included)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path; run with --gpus 1",
@@ -206,6 +208,23 @@ def main():
                 "reply_mac_ms": mac_ms / args.steps,
                 "reply_mac_gbs": alg_bytes / (mac_ms / args.steps / 1e3) / 1e9}
 
+    host_io = None
+    if args.host_io:
+        # the boundary's host-buffer entry point (xpg_reply): query host -> HBM, reply,
+        # reply HBM -> host; reported beside value, never as value
+        q_host = qd.download()
+        ctx.reply(db, q_host)  # staging buffer allocated outside the timed loop
+        t0 = time.perf_counter()
+        k = max(1, min(args.steps, 5))
+        for _ in range(k):
+            rep_h = ctx.reply(db, q_host)
+        dt = (time.perf_counter() - t0) / k
+        host_io = {"ms_per_reply": dt * 1e3, "gbs_raw_db": raw / dt / 1e9,
+                   "query_gbs_over_pcie": q_host.nbytes / dt / 1e9,
+                   "bit_exact": bool(np.array_equal(rep_h, reply)), "replies": k}
+        log(f"host-buffer replies: {dt * 1e3:.1f} ms/reply -> {raw / dt / 1e9:.2f} GB/s raw (PCIe included)")
+        del q_host
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: the reference's MAC on this host over the same inputs")
@@ -230,6 +249,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if host_io is not None:
+        out["host_io"] = host_io
     print(json.dumps(out), flush=True)
     for b in (skd, qd, rd):
         b.free()

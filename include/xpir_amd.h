@@ -114,7 +114,9 @@ void xpg_db_free(xpg_db *db);
  *      PIRReplyGeneratorNFL_internal::generateReply, :340-445, with pushQuery's Shoup
  *      precompute (:738-781) folded into the kernels and popReply (libpir.cpp:144-154)
  *      replaced by one contiguous output buffer) ----
- * query: xpg_db_query_words(db) words, levels back to back; reply: P_d*2*m*n words. */
+ * query: xpg_db_query_words(db) words, levels back to back; reply: P_d*2*m*n words.
+ * Host buffers: the context keeps one device buffer of query + reply size for these
+ * copies (grown on demand, released by xpg_ctx_destroy). */
 int xpg_reply(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query, uint64_t query_words,
               uint64_t *reply, uint64_t reply_cap_words);
 /* device-resident query and reply, asynchronous on the context stream */

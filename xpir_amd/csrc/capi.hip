@@ -112,6 +112,19 @@ int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes) {
   return XPG_OK;
 }
 
+int ensure_io_buf(xpg_ctx *ctx, size_t bytes) {
+  if (ctx->io_bytes >= bytes) return XPG_OK;
+  if (ctx->io_buf) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->io_buf);
+  }
+  ctx->io_buf = nullptr;
+  ctx->io_bytes = 0;
+  HIP_OK(hipMalloc((void **)&ctx->io_buf, bytes), "query staging buffer");
+  ctx->io_bytes = bytes;
+  return XPG_OK;
+}
+
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes) {
   if (ctx->batch_bytes >= bytes) return XPG_OK;
   if (ctx->batch_buf) {
@@ -252,6 +265,7 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
     if (ctx->level_buf[i]) (void)hipFree(ctx->level_buf[i]);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->batch_buf) (void)hipFree(ctx->batch_buf);
+  if (ctx->io_buf) (void)hipFree(ctx->io_buf);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -568,22 +582,17 @@ int xpg_reply(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query, uint64_t qu
   const uint64_t rw = (uint64_t)db->Pl[db->d] * 2 * db->prm.m * db->prm.n;
   if (query_words != qw) return set_err(ctx, XPG_ESIZE, "query size does not match the database grid");
   if (reply_cap_words < rw) return set_err(ctx, XPG_ESIZE, "reply buffer too small");
-  uint64_t *qd = nullptr, *rd = nullptr;
   (void)hipSetDevice(ctx->device);
-  HIP_OK(hipMalloc((void **)&qd, qw * 8), "query allocation");
-  if (hipMalloc((void **)&rd, rw * 8) != hipSuccess) {
-    (void)hipFree(qd);
-    return set_err(ctx, XPG_ENOMEM, "reply allocation");
-  }
-  int rc = XPG_OK;
+  // device copies of the query and the reply in one context-owned buffer, kept across calls
+  int rc = ensure_io_buf(ctx, (qw + rw) * 8);
+  if (rc) return rc;
+  uint64_t *qd = ctx->io_buf, *rd = ctx->io_buf + qw;
   if (hipMemcpyAsync(qd, query, qw * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
     rc = set_err(ctx, XPG_EDEVICE, "H2D copy");
   if (!rc) rc = xpg_reply_dev(ctx, db, qd, rd);
   if (!rc && hipMemcpyAsync(reply, rd, rw * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
     rc = set_err(ctx, XPG_EDEVICE, "D2H copy");
   if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = set_err(ctx, XPG_EDEVICE, "sync");
-  (void)hipFree(qd);
-  (void)hipFree(rd);
   return rc;
 }
 

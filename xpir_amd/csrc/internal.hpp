@@ -71,6 +71,8 @@ struct xpg_ctx {
   size_t ptr_count = 0;  // status words of the segmented MAC kept zero in scratch
   uint64_t *level_buf[2] = {nullptr, nullptr};
   size_t level_bytes[2] = {0, 0};
+  uint64_t *io_buf = nullptr;     // device copies of a host query and its reply (xpg_reply)
+  size_t io_bytes = 0;
   // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 2 = segmented chains with
   // look-back, 3 = streaming block summaries
   int mac_mode = 0;
@@ -94,6 +96,7 @@ const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m);
 int ensure_scratch(xpg_ctx *ctx, size_t bytes);
 int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
+int ensure_io_buf(xpg_ctx *ctx, size_t bytes);
 ModSet modset_of(const xpg_params *prm);
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
 void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b, int tag = 0);
