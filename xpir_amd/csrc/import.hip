@@ -112,9 +112,46 @@ __global__ void __launch_bounds__(1024) ntt_kernel(NttArgs a, ByteSrc bs, Absorb
   }
   __syncthreads();
 
-  // radix-2 DIF layers (NFLlib.hpp:478-499)
-  uint32_t lg_half = a.logn - 1;
-  for (uint32_t L = 0; L + 2 < a.logn; L++, lg_half--) {
+  // radix-2 DIF layers (NFLlib.hpp:478-499), two per pass: a thread takes the four words
+  // {i, i + h/2, i + h, i + 3h/2} of a 2h-block, which the butterflies of layer L
+  // (pairs at distance h) and of layer L + 1 (distance h/2) exchange only among
+  // themselves.  Every butterfly is the reference's, on the same inputs and twiddles.
+  const uint32_t radix2_layers = a.logn - 2;
+  uint32_t L = 0, lg_half = a.logn - 1;
+  for (; L + 1 < radix2_layers; L += 2, lg_half -= 2) {
+    const uint32_t half = 1u << lg_half, q = half >> 1;
+    const uint64_t *tw = w + (n - 2 * half), *twq = wq + (n - 2 * half);  // layer L
+    const uint64_t *tv = w + (n - half), *tvq = wq + (n - half);          // layer L + 1
+    for (uint32_t b = threadIdx.x; b < n / 4; b += blockDim.x) {
+      const uint32_t i = b & (q - 1);
+      const uint32_t e0 = ((b >> (lg_half - 1)) << (lg_half + 1)) + i;
+      uint64_t x0 = sh[e0], x1 = sh[e0 + q], x2 = sh[e0 + half], x3 = sh[e0 + half + q];
+      // layer L: (x0, x2) with twiddle i, (x1, x3) with twiddle i + q
+      uint64_t s = x0 + x2;
+      s = s >= p2 ? s - p2 : s;
+      x2 = shoup_lazy_sf(x0 - x2 + p2, tw[i], twq[i], delta);
+      x0 = s;
+      s = x1 + x3;
+      s = s >= p2 ? s - p2 : s;
+      x3 = shoup_lazy_sf(x1 - x3 + p2, tw[i + q], twq[i + q], delta);
+      x1 = s;
+      // layer L + 1: (x0, x1) and (x2, x3), both with twiddle i
+      s = x0 + x1;
+      s = s >= p2 ? s - p2 : s;
+      x1 = shoup_lazy_sf(x0 - x1 + p2, tv[i], tvq[i], delta);
+      x0 = s;
+      s = x2 + x3;
+      s = s >= p2 ? s - p2 : s;
+      x3 = shoup_lazy_sf(x2 - x3 + p2, tv[i], tvq[i], delta);
+      x2 = s;
+      sh[e0] = x0;
+      sh[e0 + q] = x1;
+      sh[e0 + half] = x2;
+      sh[e0 + half + q] = x3;
+    }
+    __syncthreads();
+  }
+  if (L < radix2_layers) {  // an odd layer count leaves one single layer
     const uint32_t half = 1u << lg_half;
     const uint64_t *tw = w + (n - 2 * half), *twq = wq + (n - 2 * half);
     for (uint32_t b = threadIdx.x; b < n / 2; b += blockDim.x) {
