@@ -177,6 +177,16 @@ int xpg_timing_read_main(xpg_ctx *ctx, double *ms, uint64_t *launches, int reset
  *      PIRQueryGenerator_internal::generateQuery, :37-70) ---- */
 /* secret key (m*n NTT-domain words) from a seed, uniform per modulus */
 int xpg_keygen(const xpg_params *prm, uint64_t seed, uint64_t *sk);
+/* decrypt n_ct ciphertexts (each a[m][n] || b[m][n], lazy words as replies hold them)
+ * with the NTT-domain secret key sk_dev (m*n words) into n_ct * A*n/8 plaintext bytes
+ * (plain_dev 8-byte aligned): NFLLWE::dec (crypto/NFLLWE.cpp:412-458) -> b - a*s,
+ * invnttAndPowInvPhi (crypto/NFLlib.hpp:613-626), the centred CRT lift of poly2mpz
+ * (crypto/NFLlib.cpp:504-532) and the low A bits of each coefficient packed
+ * little-endian (serializeData64/32, NFLlib.cpp:412-487), as
+ * PIRReplyExtraction_internal (pir/replyExtraction/PIRReplyExtraction_internal.cpp:41-106)
+ * decrypts each level.  A <= 128.  Asynchronous on the context stream. */
+int xpg_decrypt_dev(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *sk_dev, const uint64_t *ct_dev,
+                    uint64_t n_ct, uint8_t *plain_dev);
 /* query for element `index` over dims[0..d-1], noise stream `noise_seed`, on the GPU */
 int xpg_query_gen_dev(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *sk_dev, uint32_t d,
                       const uint32_t *dims, uint64_t index, uint64_t noise_seed,

@@ -23,7 +23,7 @@ EXPORTS = [
     "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
     "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev",
     "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_timing_read_main", "xpg_keygen",
-    "xpg_query_gen_dev", "xpg_random_bytes_dev", "xpg_server_create", "xpg_server_destroy",
+    "xpg_query_gen_dev", "xpg_random_bytes_dev", "xpg_decrypt_dev", "xpg_server_create", "xpg_server_destroy",
     "xpg_server_submit", "xpg_server_poll", "xpg_server_release", "xpg_server_now_us", "xpg_server_stats",
 ]
 
@@ -96,6 +96,7 @@ def lib() -> C.CDLL:
         "xpg_timing_read_main": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
         "xpg_keygen": (C.c_int, [P, u64, vp]),
         "xpg_query_gen_dev": (C.c_int, [vp, P, vp, u32, C.POINTER(u32), u64, u64, vp]),
+        "xpg_decrypt_dev": (C.c_int, [vp, P, vp, vp, u64, vp]),
         "xpg_random_bytes_dev": (C.c_int, [vp, u64, u64, vp, u64]),
         "xpg_server_create": (C.c_int, [C.c_int, C.POINTER(XpgServerConfig), C.POINTER(vp)]),
         "xpg_server_destroy": (None, [vp]),
@@ -294,6 +295,34 @@ class Context:
 
     def random_bytes_dev(self, seed: int, offset: int, dst: DeviceBuffer, nbytes: int):
         self._check(lib().xpg_random_bytes_dev(self.h, seed, offset, dst.ptr, nbytes))
+
+    def decrypt_dev(self, prm: XpgParams, sk_dev: DeviceBuffer, ct_ptr: int, n_ct: int, out: DeviceBuffer):
+        """Decrypt n_ct ciphertexts at device address ct_ptr into out (n_ct * A*n/8 bytes)."""
+        assert out.nbytes >= n_ct * prm.A * prm.n // 8
+        self._check(lib().xpg_decrypt_dev(self.h, C.byref(prm), sk_dev.ptr, ct_ptr, n_ct, out.ptr))
+
+    def extract_reply_dev(self, prm: XpgParams, sk_dev: DeviceBuffer, reply_ptr: int, dims, elem_bytes: int) -> bytes:
+        """The element bytes of a d-level reply at device address reply_ptr, decrypted on the
+        GPU level by level: level-l plaintexts are the bytes of the level-(l-1) ciphertexts
+        (PIRReplyExtraction_internal.cpp:41-106)."""
+        ct_bytes = 16 * prm.m * prm.n
+        abs_bytes = prm.A * prm.n // 8
+        counts = [-(-elem_bytes // abs_bytes)]
+        for _ in range(1, len(dims)):
+            counts.append(-(-counts[-1] * ct_bytes // abs_bytes))
+        cur, held = reply_ptr, None
+        try:
+            for lvl in range(len(dims) - 1, -1, -1):
+                out = self.alloc(counts[lvl] * abs_bytes)
+                self.decrypt_dev(prm, sk_dev, cur, counts[lvl], out)
+                if held is not None:
+                    self.sync()
+                    held.free()
+                held, cur = out, out.ptr
+            return held.download(np.uint8)[:elem_bytes].tobytes()
+        finally:
+            if held is not None:
+                held.free()
 
 
 def keygen(prm: XpgParams, seed: int) -> np.ndarray:

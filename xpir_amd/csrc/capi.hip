@@ -58,7 +58,21 @@ const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m) {
   if (it != ctx->tables.end()) return &it->second;
   uint32_t logn = 0;
   while ((1u << logn) < n) logn++;
-  std::vector<uint64_t> phi(n * m), phiq(n * m), w(n * m), wq(n * m);
+  std::vector<uint64_t> phi(n * m), phiq(n * m), w(n * m), wq(n * m), iw(n * m), iwq(n * m), ipt(n * m),
+      iptq(n * m);
+  // twiddles layer after layer: sizes n/2, n/4, ..., 1 holding om^(i * 2^L)
+  auto layers = [&](std::vector<uint64_t> &tab, std::vector<uint64_t> &tabq, size_t o, uint64_t om, uint64_t p) {
+    for (uint32_t size = n; size >= 2; size >>= 1) {
+      uint64_t wi = 1;
+      for (uint32_t i = 0; i < size / 2; i++) {
+        tab[o] = wi;
+        tabq[o] = quot(wi, p);
+        o++;
+        wi = mulmod(wi, om, p);
+      }
+      om = mulmod(om, om, p);
+    }
+  };
   for (uint32_t c = 0; c < m; c++) {
     const uint64_t p = kModuli[c];
     uint64_t f = kRoots[c];
@@ -69,27 +83,25 @@ const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m) {
       phiq[c * n + j] = quot(v, p);
       v = mulmod(v, f, p);
     }
-    uint64_t om = mulmod(f, f, p);
-    size_t o = c * n;
-    for (uint32_t size = n; size >= 2; size >>= 1) {
-      uint64_t wi = 1;
-      for (uint32_t i = 0; i < size / 2; i++) {
-        w[o] = wi;
-        wq[o] = quot(wi, p);
-        o++;
-        wi = mulmod(wi, om, p);
-      }
-      om = mulmod(om, om, p);
+    // v = phi^n = -1, so phi^-1 = -phi^(n-1); n^-1 = p - (p-1)/n since 2n divides p - 1
+    const uint64_t fi = mulmod(v, phi[c * n + n - 1], p);
+    uint64_t u = p - (p - 1) / n;
+    for (uint32_t j = 0; j < n; j++) {
+      ipt[c * n + j] = u;
+      iptq[c * n + j] = quot(u, p);
+      u = mulmod(u, fi, p);
     }
+    layers(w, wq, c * n, mulmod(f, f, p), p);
+    layers(iw, iwq, c * n, mulmod(fi, fi, p), p);
   }
   DevTables t;
   t.n = n;
   t.m = m;
   t.logn = logn;
   const size_t bytes = sizeof(uint64_t) * n * m;
-  uint64_t **dst[4] = {&t.phi, &t.phiq, &t.w, &t.wq};
-  const std::vector<uint64_t> *src[4] = {&phi, &phiq, &w, &wq};
-  for (int i = 0; i < 4; i++) {
+  uint64_t **dst[8] = {&t.phi, &t.phiq, &t.w, &t.wq, &t.iw, &t.iwq, &t.ipt, &t.iptq};
+  const std::vector<uint64_t> *src[8] = {&phi, &phiq, &w, &wq, &iw, &iwq, &ipt, &iptq};
+  for (int i = 0; i < 8; i++) {
     if (hipMalloc((void **)dst[i], bytes) != hipSuccess ||
         hipMemcpy(*dst[i], src[i]->data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
       set_err(ctx, XPG_ENOMEM, "NTT table allocation");
@@ -122,6 +134,19 @@ int ensure_io_buf(xpg_ctx *ctx, size_t bytes) {
   ctx->io_bytes = 0;
   HIP_OK(hipMalloc((void **)&ctx->io_buf, bytes), "query staging buffer");
   ctx->io_bytes = bytes;
+  return XPG_OK;
+}
+
+int ensure_dec_buf(xpg_ctx *ctx, size_t bytes) {
+  if (ctx->dec_bytes >= bytes) return XPG_OK;
+  if (ctx->dec_buf) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->dec_buf);
+  }
+  ctx->dec_buf = nullptr;
+  ctx->dec_bytes = 0;
+  HIP_OK(hipMalloc((void **)&ctx->dec_buf, bytes), "decryption buffer");
+  ctx->dec_bytes = bytes;
   return XPG_OK;
 }
 
@@ -256,16 +281,16 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   timing_collect(ctx);
   for (auto &kv : ctx->tables) {
-    (void)hipFree(kv.second.phi);
-    (void)hipFree(kv.second.phiq);
-    (void)hipFree(kv.second.w);
-    (void)hipFree(kv.second.wq);
+    for (uint64_t *b : {kv.second.phi, kv.second.phiq, kv.second.w, kv.second.wq, kv.second.iw, kv.second.iwq,
+                        kv.second.ipt, kv.second.iptq})
+      (void)hipFree(b);
   }
   for (int i = 0; i < 2; i++)
     if (ctx->level_buf[i]) (void)hipFree(ctx->level_buf[i]);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->batch_buf) (void)hipFree(ctx->batch_buf);
   if (ctx->io_buf) (void)hipFree(ctx->io_buf);
+  if (ctx->dec_buf) (void)hipFree(ctx->dec_buf);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -10,9 +10,10 @@
 //
 // Exactness: the NTT output words are lazy in [0,2p) and depend on the exact butterfly
 // sequence, so every layer is the reference's radix-2 DIF butterfly (no re-associated
-// twiddles); layers run out of LDS, one barrier per layer, the last two layers fused in
-// registers as the reference fuses them.
+// twiddles); layers run out of LDS, two per barrier (ntt_dif.hpp), the last two layers
+// fused in registers as the reference fuses them.
 #include "internal.hpp"
+#include "ntt_dif.hpp"
 
 namespace {
 
@@ -95,7 +96,7 @@ __global__ void __launch_bounds__(1024) ntt_kernel(NttArgs a, ByteSrc bs, Absorb
   const uint64_t uk = blk / m;
   const uint32_t k = (uint32_t)(uk % a.P);
   const uint64_t u = uk / a.P;
-  const uint64_t p = a.ms.p[c], p2 = 2 * p;
+  const uint64_t p = a.ms.p[c];
   const uint32_t delta = a.ms.delta[c];
   const uint64_t *phi = a.phi + (size_t)c * n, *phiq = a.phiq + (size_t)c * n;
   const uint64_t *w = a.w + (size_t)c * n, *wq = a.wq + (size_t)c * n;
@@ -112,75 +113,14 @@ __global__ void __launch_bounds__(1024) ntt_kernel(NttArgs a, ByteSrc bs, Absorb
   }
   __syncthreads();
 
-  // radix-2 DIF layers (NFLlib.hpp:478-499), two per pass: a thread takes the four words
-  // {i, i + h/2, i + h, i + 3h/2} of a 2h-block, which the butterflies of layer L
-  // (pairs at distance h) and of layer L + 1 (distance h/2) exchange only among
-  // themselves.  Every butterfly is the reference's, on the same inputs and twiddles.
-  const uint32_t radix2_layers = a.logn - 2;
-  uint32_t L = 0, lg_half = a.logn - 1;
-  for (; L + 1 < radix2_layers; L += 2, lg_half -= 2) {
-    const uint32_t half = 1u << lg_half, q = half >> 1;
-    const uint64_t *tw = w + (n - 2 * half), *twq = wq + (n - 2 * half);  // layer L
-    const uint64_t *tv = w + (n - half), *tvq = wq + (n - half);          // layer L + 1
-    for (uint32_t b = threadIdx.x; b < n / 4; b += blockDim.x) {
-      const uint32_t i = b & (q - 1);
-      const uint32_t e0 = ((b >> (lg_half - 1)) << (lg_half + 1)) + i;
-      uint64_t x0 = sh[e0], x1 = sh[e0 + q], x2 = sh[e0 + half], x3 = sh[e0 + half + q];
-      // layer L: (x0, x2) with twiddle i, (x1, x3) with twiddle i + q
-      uint64_t s = x0 + x2;
-      s = s >= p2 ? s - p2 : s;
-      x2 = shoup_lazy_sf(x0 - x2 + p2, tw[i], twq[i], delta);
-      x0 = s;
-      s = x1 + x3;
-      s = s >= p2 ? s - p2 : s;
-      x3 = shoup_lazy_sf(x1 - x3 + p2, tw[i + q], twq[i + q], delta);
-      x1 = s;
-      // layer L + 1: (x0, x1) and (x2, x3), both with twiddle i
-      s = x0 + x1;
-      s = s >= p2 ? s - p2 : s;
-      x1 = shoup_lazy_sf(x0 - x1 + p2, tv[i], tvq[i], delta);
-      x0 = s;
-      s = x2 + x3;
-      s = s >= p2 ? s - p2 : s;
-      x3 = shoup_lazy_sf(x2 - x3 + p2, tv[i], tvq[i], delta);
-      x2 = s;
-      sh[e0] = x0;
-      sh[e0 + q] = x1;
-      sh[e0 + half] = x2;
-      sh[e0 + half + q] = x3;
-    }
-    __syncthreads();
-  }
-  if (L < radix2_layers) {  // an odd layer count leaves one single layer
-    const uint32_t half = 1u << lg_half;
-    const uint64_t *tw = w + (n - 2 * half), *twq = wq + (n - 2 * half);
-    for (uint32_t b = threadIdx.x; b < n / 2; b += blockDim.x) {
-      const uint32_t j = b & (half - 1);
-      const uint32_t i0 = ((b >> lg_half) << (lg_half + 1)) + j, i1 = i0 + half;
-      const uint64_t u0 = sh[i0], u1 = sh[i1];
-      uint64_t s = u0 + u1;
-      s = s >= p2 ? s - p2 : s;
-      const uint64_t d = u0 - u1 + p2;
-      sh[i0] = s;
-      sh[i1] = shoup_lazy_sf(d, tw[j], twq[j], delta);
-    }
-    __syncthreads();
-  }
+  dif_radix2_layers(sh, n, a.logn, w, wq, p, delta);
 
   // last two layers fused (NFLlib.hpp:502-535), straight to HBM
   const uint64_t W = w[n - 4 + 1], Wq = wq[n - 4 + 1];
   for (uint32_t b = threadIdx.x; b < n / 4; b += blockDim.x) {
     const uint32_t base = 4 * b;
-    const uint64_t u0 = sh[base], u1 = sh[base + 1], u2 = sh[base + 2], u3 = sh[base + 3];
-    uint64_t v0 = u0 + u2; v0 = v0 >= p2 ? v0 - p2 : v0;
-    uint64_t v2 = u0 - u2; v2 = (int64_t)v2 < 0 ? v2 + p2 : v2;
-    uint64_t v1 = u1 + u3; v1 = v1 >= p2 ? v1 - p2 : v1;
-    const uint64_t v3 = shoup_lazy_sf(u1 - u3 + p2, W, Wq, delta);
     uint64_t z[4];
-    z[0] = v0 + v1; z[0] = z[0] >= p2 ? z[0] - p2 : z[0];
-    z[1] = v0 - v1; z[1] = (int64_t)z[1] < 0 ? z[1] + p2 : z[1];
-    z[2] = v2 + v3; z[2] = z[2] >= p2 ? z[2] - p2 : z[2];
-    z[3] = v2 - v3; z[3] = (int64_t)z[3] < 0 ? z[3] + p2 : z[3];
+    dif_last_two(z, sh[base], sh[base + 1], sh[base + 2], sh[base + 3], W, Wq, p, delta);
     if (KIND == SRC_BYTES) {
       uint64_t *o = out + (((u * a.P) + k) * m + c) * (uint64_t)n + base;
 #pragma unroll

@@ -19,6 +19,8 @@ struct DevTables {
   uint32_t n = 0, m = 0, logn = 0;
   uint64_t *phi = nullptr, *phiq = nullptr;  // phi^j and Shoup quotients
   uint64_t *w = nullptr, *wq = nullptr;      // DIF twiddles, layer after layer (n-1 used)
+  uint64_t *iw = nullptr, *iwq = nullptr;    // the same for the inverse root (decryption)
+  uint64_t *ipt = nullptr, *iptq = nullptr;  // phi^-j * n^-1 and Shoup quotients
 };
 
 // Source of the coefficients an import kernel transforms.
@@ -73,6 +75,8 @@ struct xpg_ctx {
   size_t level_bytes[2] = {0, 0};
   uint64_t *io_buf = nullptr;     // device copies of a host query and its reply (xpg_reply)
   size_t io_bytes = 0;
+  uint64_t *dec_buf = nullptr;    // residues and lifted coefficients of xpg_decrypt_dev
+  size_t dec_bytes = 0;
   // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 2 = segmented chains with
   // look-back, 3 = streaming block summaries
   int mac_mode = 0;
@@ -97,6 +101,7 @@ int ensure_scratch(xpg_ctx *ctx, size_t bytes);
 int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
 int ensure_io_buf(xpg_ctx *ctx, size_t bytes);
+int ensure_dec_buf(xpg_ctx *ctx, size_t bytes);
 ModSet modset_of(const xpg_params *prm);
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
 void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b, int tag = 0);
