@@ -1,0 +1,48 @@
+"""Error paths of the C ABI on a real context: malformed arguments come back as status
+codes with a message (the reference prints or exit()s; this ABI never aborts), and the
+context stays usable afterwards."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C2 = "LWE:80:2048:120"
+
+
+def _raises(fn, code):
+    import xpir_amd
+    with pytest.raises(xpir_amd.capi.XpgError) as e:
+        fn()
+    assert f"status {code}" in str(e.value)
+
+
+def test_import_argument_errors(gpu_ctx):
+    import xpir_amd
+    prm = xpir_amd.params(C2, 8)
+    data = np.zeros(8 * 64, dtype=np.uint8)
+    _raises(lambda: gpu_ctx.import_db(prm, data, 8, 64, [8], alpha=0), -1)        # alpha 0
+    _raises(lambda: gpu_ctx.import_db(prm, data, 8, 64, [0]), -1)                 # zero dimension
+    _raises(lambda: gpu_ctx.import_db(prm, data, 8, 64, [4]), -1)                 # dims[0] != n0
+    _raises(lambda: gpu_ctx.import_db(prm, data, 8, 64, [8, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1]), -1)  # d > 10
+    prm7 = xpir_amd.params(C2, 7)
+    _raises(lambda: gpu_ctx.import_db(prm7, data, 8, 64, [7]), -1)               # grid too small
+    # the context still works
+    db = gpu_ctx.import_db(prm, data, 8, 64, [8])
+    assert db.polys == 1
+    db.free()
+
+
+def test_reply_argument_errors(gpu_ctx):
+    import xpir_amd
+    prm = xpir_amd.params(C2, 8)
+    data = np.arange(8 * 64, dtype=np.uint8)
+    db = gpu_ctx.import_db(prm, data, 8, 64, [8])
+    try:
+        q = np.zeros(db.query_words - 1, dtype=np.uint64)
+        _raises(lambda: gpu_ctx.reply(db, q), -4)                                   # wrong query size
+        _raises(lambda: gpu_ctx.set_option("mac_kernel", 9), -1)                   # unknown value
+        _raises(lambda: gpu_ctx.set_option("no_such_knob", 1), -1)
+        rep = gpu_ctx.reply(db, np.zeros(db.query_words, dtype=np.uint64))          # still usable
+        assert rep.size == db.reply_words
+    finally:
+        db.free()

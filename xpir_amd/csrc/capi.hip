@@ -104,6 +104,8 @@ const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m) {
   for (int i = 0; i < 8; i++) {
     if (hipMalloc((void **)dst[i], bytes) != hipSuccess ||
         hipMemcpy(*dst[i], src[i]->data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      for (int j = 0; j <= i; j++)
+        if (*dst[j]) (void)hipFree(*dst[j]);
       set_err(ctx, XPG_ENOMEM, "NTT table allocation");
       return nullptr;
     }
@@ -404,6 +406,7 @@ int xpg_db_import_dev(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems_
   if (!ctx || !prm || !out) return XPG_EINVAL;
   if (((uintptr_t)elems_dev & 7) != 0) return set_err(ctx, XPG_EINVAL, "element bytes must be 8-byte aligned");
   if (elem_bytes == 0 || n_elems == 0) return set_err(ctx, XPG_EINVAL, "empty database");
+  if (alpha == 0) return set_err(ctx, XPG_EINVAL, "alpha must be at least 1");
   const uint64_t unit_bytes = elem_bytes * alpha;
   const uint32_t P0 = xpg_polys_for_bytes(prm, unit_bytes);
   const uint64_t real_units = (n_elems + alpha - 1) / alpha;
