@@ -124,7 +124,41 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
   // UE elements per iteration: every load of the chunk is issued before its arithmetic,
   // which then runs in element order (the chains' order)
   constexpr int UE = GR * QR >= 16 ? 2 : 4;
-  for (uint64_t e = 0; e < a.n_l; e += UE) {
+  uint64_t e = 0;
+  if (gv == GR) {
+    // every group of the block exists: whole chunks without per-load guards
+    for (; e + UE <= a.n_l; e += UE) {
+      uint64_t xv[UE][GR], ya[UE][QR], yb[UE][QR];
+#pragma unroll
+      for (int u = 0; u < UE; u++) {
+        const uint64_t eo = (e + u) * xstride, o = (e + u) * qstride;
+#pragma unroll
+        for (int g = 0; g < GR; g++) xv[u][g] = __builtin_nontemporal_load(x + g * gstride + eo);
+#pragma unroll
+        for (int i = 0; i < QR; i++) {
+          ya[u][i] = qa[i][o];
+          yb[u][i] = qa[i][o + mn];
+        }
+      }
+      // keep the whole chunk's loads ahead of its arithmetic (the scheduler would
+      // otherwise sink each load to its first use and wait for it there)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < UE; u++) {
+#pragma unroll
+        for (int i = 0; i < QR; i++) {
+          const uint64_t yaq = shoup_quot(ya[u][i], p, delta), ybq = shoup_quot(yb[u][i], p, delta);
+#pragma unroll
+          for (int g = 0; g < GR; g++) {
+            acc_a[g][i] = mac_step(acc_a[g][i], xv[u][g], ya[u][i], yaq, p, delta);
+            acc_b[g][i] = mac_step(acc_b[g][i], xv[u][g], yb[u][i], ybq, p, delta);
+          }
+        }
+      }
+    }
+  }
+  // the tail chunk, and blocks whose last groups lie past the level
+  for (; e < a.n_l; e += UE) {
     const int ve = (int)min<uint64_t>(UE, a.n_l - e);
     uint64_t xv[UE][GR], ya[UE][QR], yb[UE][QR];
 #pragma unroll
