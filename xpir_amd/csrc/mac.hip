@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
     for (int i = 0; i < QR; i++) acc_a[g][i] = acc_b[g][i] = 0;
   // UE elements per iteration: every load of the chunk is issued before its arithmetic,
   // which then runs in element order (the chains' order)
-  constexpr int UE = GR * QR >= 16 ? 2 : 4;
+  constexpr int UE = GR * QR >= 16 ? 1 : 4;
   uint64_t e = 0;
   if (gv == GR) {
     // every group of the block exists: whole chunks without per-load guards
