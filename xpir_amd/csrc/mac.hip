@@ -55,6 +55,7 @@ __global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
       av[i] = ya[(e + i) * qstride];
       bv[i] = yb[(e + i) * qstride];
     }
+    __builtin_amdgcn_sched_barrier(0);  // the chunk's loads stay ahead of its arithmetic
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       acc_a = mac_step_qf(acc_a, xv[i], av[i], p, delta);
