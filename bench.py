@@ -66,6 +66,24 @@ def cpu_baseline(db_words: np.ndarray, query: np.ndarray, cfg, prm, gpu_reply: n
                       f"reference parallelises over the {P} poly(s) of an element"}
 
 
+def cpu_all_cores(db_words: np.ndarray, query: np.ndarray, cfg, prm, gpu_reply: np.ndarray, raw_bytes: int):
+    """The clean-room restatement (oracle/xpir_oracle.c) with every host core busy: threads
+    split over slot blocks, each chain still in element order, so the words are the
+    reference's.  Timed: the MAC loops, the span of the reference's reply timer."""
+    from oracle.oracle import Oracle
+    n_l = cfg["dims"][0]
+    P = db_words.size // (n_l * prm.m * prm.n)
+    # the host share of one GPU: OMP_NUM_THREADS when set (16 per GPU), else every core
+    threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+                         or (os.cpu_count() or 1)))
+    orc = Oracle(cfg["crypto"], n_l)
+    rep, secs = orc.reply_level_chains_timed(db_words, query, n_l, 1, P, threads)
+    return {"value": raw_bytes / secs / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+            "seconds": secs, "matches_gpu_reply": bool(np.array_equal(rep, gpu_reply)),
+            "sample": f"the full workload, one reply ({n_l} elements x {cfg['elem_bytes']} B), MAC loops "
+                      f"only, {threads} threads over slot blocks"}
+
+
 def measured_traffic(workload_name):
     """HBM bytes per reply of the MAC kernels, from the rocprofv3 counter runs committed
     under profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor the
@@ -228,7 +246,7 @@ def main():
         log(f"host-buffer replies: {dt * 1e3:.1f} ms/reply -> {raw / dt / 1e9:.2f} GB/s raw (PCIe included)")
         del q_host
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu_baseline:
         log("cpu baseline: the reference's MAC on this host over the same inputs")
         words = db.export_words()
@@ -236,6 +254,9 @@ def main():
         cpu = cpu_baseline(words, query, cfg, prm, reply, raw)
         log(f"cpu baseline {cpu.get('value', 0):.3f} GB/s ({cpu.get('seconds', 0):.2f} s), "
             f"matches GPU reply: {cpu.get('matches_gpu_reply')}")
+        cpu_all = cpu_all_cores(words, query, cfg, prm, reply, raw)
+        log(f"cpu all cores {cpu_all['value']:.3f} GB/s on {cpu_all['cores']} threads "
+            f"({cpu_all['seconds']:.2f} s), matches GPU reply: {cpu_all['matches_gpu_reply']}")
 
     lat_ms = np.array(lat) * 1e3
     out = {
@@ -251,6 +272,7 @@ def main():
         "import_gbs": raw / import_s / 1e9,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_all_cores": cpu_all,
     }
     if host_io is not None:
         out["host_io"] = host_io

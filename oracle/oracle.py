@@ -82,6 +82,7 @@ class Oracle:
             L.xo_import.argtypes = [C.POINTER(XoTables), C.POINTER(XoParams), u8p, C.c_uint64, u64p]
             L.xo_reply_level.argtypes = [C.POINTER(XoParams), u64p, u64p, C.c_uint64, C.c_uint64,
                                          C.c_uint32, u64p, C.c_int]
+            L.xo_reply_level_chains.argtypes = L.xo_reply_level.argtypes + [C.POINTER(C.c_double)]
             L.xo_recursion.restype = C.c_uint32
             L.xo_recursion.argtypes = [C.POINTER(XoTables), C.POINTER(XoParams), u64p, C.c_uint64,
                                        C.c_uint32, u64p]
@@ -159,6 +160,22 @@ class Oracle:
         out = np.zeros(groups * P * 2 * self.mn, dtype=np.uint64)
         self.lib().xo_reply_level(C.byref(self.prm), db, q, n_l, groups, P, out, threads)
         return out
+
+    def reply_level_chains(self, db: np.ndarray, q: np.ndarray, n_l: int, groups: int, P: int,
+                           threads: int = 0) -> np.ndarray:
+        """reply_level with the host threads split over slot blocks (all cores busy at P = 1)."""
+        return self.reply_level_chains_timed(db, q, n_l, groups, P, threads)[0]
+
+    def reply_level_chains_timed(self, db, q, n_l, groups, P, threads=0):
+        """(reply words, seconds spent in the MAC loops)"""
+        db = np.ascontiguousarray(db, dtype=np.uint64)
+        q = np.ascontiguousarray(q, dtype=np.uint64)
+        assert db.size == groups * n_l * P * self.mn and q.size == n_l * 2 * self.mn
+        out = np.zeros(groups * P * 2 * self.mn, dtype=np.uint64)
+        secs = C.c_double()
+        self.lib().xo_reply_level_chains(C.byref(self.prm), db, q, n_l, groups, P, out, threads,
+                                         C.byref(secs))
+        return out, secs.value
 
     def recursion(self, reply: np.ndarray, groups: int, P: int) -> tuple[np.ndarray, int]:
         reply = np.ascontiguousarray(reply, dtype=np.uint64)

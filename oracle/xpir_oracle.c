@@ -329,6 +329,58 @@ void xo_reply_level(const xo_params *prm, const uint64_t *db, const uint64_t *q,
   free(qq);
 }
 
+/* The same chains as xo_reply_level, with the threads split over slot blocks instead of
+ * over (group, poly): each chain still runs its elements in ascending order on one
+ * thread, so the words are the same; only the work split differs (all host cores busy
+ * even when P = 1). */
+void xo_reply_level_chains(const xo_params *prm, const uint64_t *db, const uint64_t *q,
+                           uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out,
+                           int threads, double *mac_seconds) {
+  const uint64_t n = prm->n, m = prm->m, mn = m * n;
+  const uint64_t SB = n < 256 ? n : 256; /* slots per work item (n is a power of two) */
+  const uint64_t nsb = n / SB;
+  uint64_t *qq = (uint64_t *)malloc(sizeof(uint64_t) * n_l * 2 * mn);
+  (void)threads;
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (uint64_t e = 0; e < n_l; e++)
+    for (uint64_t h = 0; h < 2; h++)
+      for (uint64_t i = 0; i < mn; i++) {
+        uint64_t idx = (e * 2 + h) * mn + i;
+        qq[idx] = xo_shoup_quot(q[idx], prm->p[i / n]);
+      }
+#ifdef _OPENMP
+  double t0 = omp_get_wtime();
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) collapse(4)
+#endif
+  for (uint64_t g = 0; g < groups; g++) {
+    for (uint64_t k = 0; k < P; k++) {
+      for (uint64_t c = 0; c < m; c++) {
+        for (uint64_t b = 0; b < nsb; b++) {
+          const uint64_t p = prm->p[c], lo = c * n + b * SB;
+          uint64_t *ra = out + (g * P + k) * 2 * mn + lo, *rb = ra + mn;
+          for (uint64_t s = 0; s < SB; s++) ra[s] = rb[s] = 0;
+          for (uint64_t e = 0; e < n_l; e++) {
+            const uint64_t *x = db + ((g * n_l + e) * P + k) * mn + lo;
+            const uint64_t *ya = q + (e * 2) * mn + lo, *yb = ya + mn;
+            const uint64_t *yaq = qq + (e * 2) * mn + lo, *ybq = yaq + mn;
+            for (uint64_t s = 0; s < SB; s++) ra[s] = xo_mac_step(ra[s], x[s], ya[s], yaq[s], p);
+            for (uint64_t s = 0; s < SB; s++) rb[s] = xo_mac_step(rb[s], x[s], yb[s], ybq[s], p);
+          }
+        }
+      }
+    }
+  }
+#ifdef _OPENMP
+  if (mac_seconds) *mac_seconds = omp_get_wtime() - t0;
+#else
+  if (mac_seconds) *mac_seconds = 0;
+#endif
+  free(qq);
+}
+
 uint32_t xo_next_polys(const xo_params *prm, uint32_t P_l) {
   uint64_t ct_bytes = 16ULL * prm->n * prm->m;
   uint64_t abs_bytes = (uint64_t)prm->A * prm->n / 8;

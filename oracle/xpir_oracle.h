@@ -93,6 +93,13 @@ void xo_import(const xo_tables *t, const xo_params *prm, const uint8_t *data, ui
 void xo_reply_level(const xo_params *prm, const uint64_t *db, const uint64_t *q,
                     uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out, int threads);
 
+/* xo_reply_level with the host threads split over slot blocks (all cores busy at P = 1);
+ * identical words, since every chain still runs in element order on one thread.
+ * mac_seconds (may be NULL) receives the time of the MAC loops alone. */
+void xo_reply_level_chains(const xo_params *prm, const uint64_t *db, const uint64_t *q,
+                           uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out,
+                           int threads, double *mac_seconds);
+
 /* fromResulttoInData (PIRReplyGeneratorNFL_internal.cpp:599-626): the P_l ciphertexts of
  * each group, as bytes, absorbed with the same A and transformed.  Returns P_{l+1}. */
 uint32_t xo_recursion(const xo_tables *t, const xo_params *prm, const uint64_t *reply,

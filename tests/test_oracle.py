@@ -156,6 +156,27 @@ def test_mac_residues_and_bounds():
                 assert got % p == want and got < 3 * p
 
 
+def test_chain_parallel_reply_matches_golden():
+    """The all-core split (threads over slot blocks) gives the golden C1 reply and the same
+    words as the per-poly split on a grid with several groups, polys and moduli."""
+    g = GOLD["c1_plain"]
+    o = Oracle(g["crypto"], g["dims"][0])
+    n, L = g["n_elems"], g["elem_bytes"]
+    data = o.rand_bytes(g["db_seed"], 0, n * L)
+    words = np.concatenate([o.import_element(data[e * L:(e + 1) * L]) for e in range(n)])
+    sk = o.keygen(g["key_seed"])
+    q = o.make_query(sk, g["dims"], g["index"], g["noise_seed"])
+    rep = o.reply_level_chains(words, q[0], n, 1, g["P0"], 4)
+    assert np.array_equal(rep, np.load(os.path.join(HERE, "golden", "c1_plain_reply.npy")))
+    o3 = Oracle(C3, 16)
+    rng = np.random.default_rng(12)
+    nl, groups, P = 16, 3, 2
+    db = np.concatenate([rng.integers(0, 2 * p, o3.n, dtype=np.uint64)
+                         for _ in range(groups * nl * P) for p in o3.p])
+    qq = np.concatenate([rng.integers(0, p, o3.n, dtype=np.uint64) for _ in range(nl) for _ in range(2) for p in o3.p])
+    assert np.array_equal(o3.reply_level_chains(db, qq, nl, groups, P, 3), o3.reply_level(db, qq, nl, groups, P))
+
+
 def test_golden_c1_plain_reply_bytes():
     g = GOLD["c1_plain"]
     o = Oracle(g["crypto"], g["dims"][0])
