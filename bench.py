@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--queries", type=int, default=2000, help="serving: queries in the Poisson stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="recursive/batched/serving: skip the MAC autotuner (fixed kernel rules)")
     ap.add_argument("--host-io", action="store_true",
                     help="single/plain: also time replies from a host query to a host reply (PCIe included)")
     args = ap.parse_args()
@@ -283,6 +285,23 @@ def main():
     ctx.close()
 
 
+def autotune_db(ctx, db, max_batch, args, label):
+    """The MAC autotuner on db's level shapes (and batches up to max_batch queries) before
+    any timed region; returns what it recorded and how long it took."""
+    if args.no_autotune:
+        return None
+    t0 = time.perf_counter()
+    entries = ctx.autotune(db, max_batch=max_batch)
+    dt = time.perf_counter() - t0
+    names = {1: "one-chain", 2: "segmented", 3: "streaming", 4: "blocked"}
+    for e in entries:
+        what = f"level n_l={e['n_l']} groups={e['groups']} P={e['P']}" if e["kind"] == 0 else \
+            f"batch of {e['qn']} on n_l={e['n_l']} groups={e['groups']} P={e['P']}"
+        log(f"autotune {label} {what}: {names.get(e['kernel'], e['kernel'])}({e['a']},{e['b']},{e['c']}) "
+            f"{e['ms']:.3f} ms vs {e['default_ms']:.3f} ms untuned")
+    return {"seconds": dt, "entries": entries}
+
+
 def _load_batch_dbs(ctx, scale, d_override=None):
     """The eight uneven databases, imported on the GPU and resident (d = 2 grids)."""
     import xpir_amd
@@ -327,6 +346,7 @@ def bench_recursive(args):
     ctx.sync()
     import_s = time.perf_counter() - t0
     log(f"import of {n} x {L} B: {import_s:.3f} s")
+    tuned = autotune_db(ctx, db, 0, args, "recursive")
     sk = xpir_amd.keygen(prm, cfg["key_seed"])
     skd = ctx.alloc(sk.nbytes).upload(sk)
     qd = ctx.alloc(db.query_words * 8)
@@ -367,6 +387,7 @@ def bench_recursive(args):
         "config": {"workload": cfg["name"], "crypto": cfg["crypto"], "n_elems": n, "elem_bytes": L,
                    "dims": dims, "ntt_db_bytes": n * P0 * prm.m * prm.n * 8},
         "import_s": import_s, "mac_ms_per_reply": mac_ms / steps,
+        "autotune": tuned,
         "shoup_steps_per_reply": steps_l0 + steps_l1,
         "shoup_steps_per_s": (steps_l0 + steps_l1) / (mac_ms / steps / 1e3),
     }
@@ -398,6 +419,12 @@ def bench_batched(args):
         reqs += d["reqs"]
     ctx.sync()
     log(f"{len(reqs)} queries over {len(dbs)} databases")
+    tune = {}
+    for i, (d, nq) in enumerate(zip(dbs, counts)):
+        bucket = 1
+        while bucket < min(nq, 64):
+            bucket *= 2
+        tune[i] = autotune_db(ctx, d["db"], bucket if bucket >= 2 else 0, args, f"db{i}")
     # plain: one query per database, alone (best of 2)
     plain_t = []
     for d in dbs:
@@ -438,6 +465,7 @@ def bench_batched(args):
                    "scale": args.scale, "counts": counts,
                    "dbs": [[d["spec"]["n_elems"], d["spec"]["elem_bytes"], d["spec"]["dims"]] for d in dbs]},
         "plain_ms_per_db": [t * 1e3 for t in plain_t],
+        "autotune_s": None if args.no_autotune else sum(v["seconds"] for v in tune.values()),
         "import_s": [d["import_s"] for d in dbs],
     }
     print(json.dumps(out), flush=True)
@@ -464,6 +492,12 @@ def bench_serving(args):
         buf.free()
         sk = xpir_amd.keygen(prm, spec["key_seed"])
         base["d1"] = dict(spec=spec, db=db, prm=prm, raw=raw, skd=ctx.alloc(sk.nbytes).upload(sk))
+    tune_s = 0.0
+    for i, d in enumerate(dbs):
+        for label, view in (("d2", d), ("d1", d.get("d1"))):
+            if view is not None:
+                r = autotune_db(ctx, view["db"], 8, args, f"db{i}-{label}")
+                tune_s += r["seconds"] if r else 0.0
     rng = np.random.default_rng(51)
     pool_n = 4
     for d in dbs[:2]:
@@ -535,6 +569,7 @@ def bench_serving(args):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "queries": n, "rate_per_s": args.rate, "makespan_s": makespan, "refused_submits": refused,
         "batches": st["batches"], "bit_exact": mismatches == 0, "mismatches": int(mismatches), "n_gpus": 1,
+        "autotune_s": None if args.no_autotune else tune_s,
         "config": {"workload": "8 resident LWE:80:2048:120 databases, Zipf(1.1) choice, Poisson arrivals, "
                                "d=2 (d in {1,2} on the two smallest)",
                    "scale": args.scale, "workers": 2, "max_batch": 64, "reply_pool_bytes": 8 << 30},

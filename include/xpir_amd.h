@@ -70,8 +70,12 @@ int xpg_sync(xpg_ctx *ctx);
  * "seg_s" / "seg_w" = segmented tile shape (elements per lane 4|8|12|16|24|32,
  * wavefronts per workgroup 2|4|8|16; 0 = by shape); "stream_e" = elements per streaming
  * block (0 = sized for one pass), "stream_u" = elements per load chunk 4|8|16 (0 = 8),
- * "stream_pf" = 1 to double-buffer the chunks.  No reference counterpart; every setting
- * yields the same reply bytes. */
+ * "stream_pf" = 1 to double-buffer the chunks; "graphs" = 1 to replay each reply's
+ * launches (every level, the recursion between them) from a HIP graph captured on the
+ * second call with the same database, query buffer and reply buffer (the first runs
+ * eagerly and sizes the context buffers; any option change or buffer growth re-captures;
+ * with timing on, replies run eagerly).  No reference counterpart; every setting yields
+ * the same reply bytes. */
 int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value);
 
 /* device memory plumbing for callers without a GPU runtime of their own */
@@ -171,6 +175,41 @@ int xpg_timing_read(xpg_ctx *ctx, double *mac_ms, uint64_t *mac_launches, int re
 /* the same for the dominant kernel of each MAC alone (the streaming block-summary,
  * segmented, one-chain-per-lane or batched kernel, without its scan or fixup) */
 int xpg_timing_read_main(xpg_ctx *ctx, double *ms, uint64_t *launches, int reset);
+
+/* ---- MAC autotuning (no reference counterpart: the reference runs one CPU loop nest
+ *      for every shape, pir/replyGenerator/PIRReplyGeneratorNFL_internal.cpp:511-595) ----
+ * A MAC launch shape (one query's reply level, or a batch of queries on level 0) can run
+ * on several kernels that all give the same words.  xpg_autotune times every candidate
+ * on the levels of `db` (and on batches of 2, 4, ... max_batch queries on level 0),
+ * checks each candidate's words against the untuned choice's (XPG_EDEVICE if any
+ * differs), and records the fastest per shape in a table shared by every context on the
+ * device; later launches of a recorded shape use it unless an option pins a kernel.
+ * kind: 0 = one query's level, 1 = a batch (qn = batch size rounded up to a power of
+ * two).  kernel: 1 = one chain per lane, 2 = segmented (a x b = elements per lane x
+ * wavefronts), 3 = streaming (a = chunk, b = double-buffered, c = elements per block,
+ * 0 = one pass), 4 = register blocks (a groups x b queries per lane); a kind-1 entry
+ * with kernel 0 records that the batch runs faster as single-query launches (the batch
+ * call then takes that path for batches of this size on this database shape).  A
+ * measured gain under 3 % keeps the fixed rule's choice. */
+typedef struct {
+  uint32_t kind, n, m, P;
+  uint64_t n_l, groups;
+  uint32_t qn;
+  int32_t kernel, a, b, c;
+  float ms;          /* measured ms per launch of this choice */
+  float default_ms;  /* measured ms per launch of the untuned choice */
+} xpg_tune_entry;
+int xpg_autotune(xpg_ctx *ctx, const xpg_db *db, uint32_t max_batch, xpg_tune_entry *out, uint32_t cap,
+                 uint32_t *count);
+/* install entries (a saved table); XPG_EINVAL names a kernel or shape that is not built */
+int xpg_tune_set(xpg_ctx *ctx, const xpg_tune_entry *entries, uint32_t count);
+/* the device's table: up to cap entries into out, the total into *count */
+int xpg_tune_get(xpg_ctx *ctx, xpg_tune_entry *out, uint32_t cap, uint32_t *count);
+int xpg_tune_clear(xpg_ctx *ctx);
+
+/* captured replies (option "graphs" = 1): replies captured into a HIP graph and replies
+ * replayed from one since the context was created.  No reference counterpart. */
+int xpg_graph_stats(xpg_ctx *ctx, uint64_t *captures, uint64_t *replays);
 
 /* ---- client-side helpers (test and load-generation support; the reference's
  *      NFLLWE::enc/encrypt, crypto/NFLLWE.cpp:276-372,481-498, and

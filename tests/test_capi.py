@@ -37,6 +37,30 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name)
 
 
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of the ABI's structs have the C compiler's sizes and offsets."""
+    src = tmp_path / "layout.c"
+    checks = {
+        "xpg_tune_entry": (capi.XpgTuneEntry, ["n_l", "qn", "kernel", "c", "ms", "default_ms"]),
+        "xpg_params": (capi.XpgParams, [f for f, _ in capi.XpgParams._fields_]),
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {']
+    for cname, (_, fields) in checks.items():
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in fields:
+            lines.append(f'  printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ['  return 0;', '}']
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for cname, (cls, fields) in checks.items():
+        assert int(got[cname]) == C.sizeof(cls), cname
+        for f in fields:
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, f"{cname}.{f}"
+
+
 @pytest.mark.parametrize("desc,n0", [("LWE:80:1024:60", 1000), ("LWE:80:2048:120", 65536),
                                      ("LWE:100:4096:180", 1024), ("LWE:80:2048:120", 7),
                                      ("LWE:128:8192:240", 100), ("LWE:80:512:60", 10)])

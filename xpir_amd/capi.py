@@ -22,7 +22,8 @@ EXPORTS = [
     "xpg_dev_free", "xpg_h2d", "xpg_d2h", "xpg_db_import", "xpg_db_import_dev",
     "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
     "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev",
-    "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_timing_read_main", "xpg_keygen",
+    "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_timing_read_main", "xpg_graph_stats", "xpg_autotune", "xpg_tune_set",
+    "xpg_tune_get", "xpg_tune_clear", "xpg_keygen",
     "xpg_query_gen_dev", "xpg_random_bytes_dev", "xpg_decrypt_dev", "xpg_server_create", "xpg_server_destroy",
     "xpg_server_submit", "xpg_server_poll", "xpg_server_release", "xpg_server_now_us", "xpg_server_stats",
 ]
@@ -94,6 +95,11 @@ def lib() -> C.CDLL:
         "xpg_timing_enable": (C.c_int, [vp, C.c_int]),
         "xpg_timing_read": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
         "xpg_timing_read_main": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
+        "xpg_graph_stats": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
+        "xpg_autotune": (C.c_int, [vp, vp, u32, vp, u32, C.POINTER(u32)]),
+        "xpg_tune_set": (C.c_int, [vp, vp, u32]),
+        "xpg_tune_get": (C.c_int, [vp, vp, u32, C.POINTER(u32)]),
+        "xpg_tune_clear": (C.c_int, [vp]),
         "xpg_keygen": (C.c_int, [P, u64, vp]),
         "xpg_query_gen_dev": (C.c_int, [vp, P, vp, u32, C.POINTER(u32), u64, u64, vp]),
         "xpg_decrypt_dev": (C.c_int, [vp, P, vp, vp, u64, vp]),
@@ -124,6 +130,24 @@ def params(desc: str, n0: int) -> XpgParams:
 
 def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
+
+
+class XpgTuneEntry(C.Structure):
+    """One tuned MAC launch shape (include/xpir_amd.h, xpg_tune_entry)."""
+    _fields_ = [("kind", C.c_uint32), ("n", C.c_uint32), ("m", C.c_uint32), ("P", C.c_uint32),
+                ("n_l", C.c_uint64), ("groups", C.c_uint64), ("qn", C.c_uint32),
+                ("kernel", C.c_int32), ("a", C.c_int32), ("b", C.c_int32), ("c", C.c_int32),
+                ("ms", C.c_float), ("default_ms", C.c_float)]
+
+    def to_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d):
+        e = cls()
+        for f, _ in cls._fields_:
+            setattr(e, f, d.get(f, 0))
+        return e
 
 
 class DeviceBuffer:
@@ -287,6 +311,36 @@ class Context:
         ms, n = C.c_double(), C.c_uint64()
         self._check(lib().xpg_timing_read_main(self.h, C.byref(ms), C.byref(n), 1 if reset else 0))
         return ms.value, n.value
+
+    def autotune(self, db: "Database", max_batch: int = 0):
+        """Time every MAC kernel candidate on db's level shapes (and batches of 2..max_batch
+        queries on level 0), check their words agree, record the fastest per shape in the
+        device's table; returns the recorded entries as dicts."""
+        cap = 16 + 8
+        arr = (XpgTuneEntry * cap)()
+        n = C.c_uint32(0)
+        self._check(lib().xpg_autotune(self.h, db.h, max_batch, arr, cap, C.byref(n)))
+        return [arr[i].to_dict() for i in range(min(n.value, cap))]
+
+    def tune_get(self):
+        n = C.c_uint32(0)
+        self._check(lib().xpg_tune_get(self.h, None, 0, C.byref(n)))
+        arr = (XpgTuneEntry * max(1, n.value))()
+        self._check(lib().xpg_tune_get(self.h, arr, n.value, C.byref(n)))
+        return [arr[i].to_dict() for i in range(n.value)]
+
+    def tune_set(self, entries):
+        arr = (XpgTuneEntry * max(1, len(entries)))(*[XpgTuneEntry.from_dict(e) for e in entries])
+        self._check(lib().xpg_tune_set(self.h, arr, len(entries)))
+
+    def tune_clear(self):
+        self._check(lib().xpg_tune_clear(self.h))
+
+    def graph_stats(self):
+        """(replies captured into a HIP graph, replies replayed from one)"""
+        c, r = C.c_uint64(), C.c_uint64()
+        self._check(lib().xpg_graph_stats(self.h, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     def query_gen_dev(self, prm: XpgParams, sk_dev: DeviceBuffer, dims, index: int, noise_seed: int,
                       out: DeviceBuffer):

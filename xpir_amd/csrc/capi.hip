@@ -5,8 +5,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "internal.hpp"
+
+static std::atomic<uint64_t> g_db_serial{0};
 
 namespace {
 
@@ -121,6 +124,7 @@ int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes) {
   }
   ctx->level_buf[which] = nullptr;
   ctx->level_bytes[which] = 0;
+  ctx->epoch++;
   HIP_OK(hipMalloc((void **)&ctx->level_buf[which], bytes), "level buffer");
   ctx->level_bytes[which] = bytes;
   return XPG_OK;
@@ -134,6 +138,7 @@ int ensure_io_buf(xpg_ctx *ctx, size_t bytes) {
   }
   ctx->io_buf = nullptr;
   ctx->io_bytes = 0;
+  ctx->epoch++;
   HIP_OK(hipMalloc((void **)&ctx->io_buf, bytes), "query staging buffer");
   ctx->io_bytes = bytes;
   return XPG_OK;
@@ -147,6 +152,7 @@ int ensure_dec_buf(xpg_ctx *ctx, size_t bytes) {
   }
   ctx->dec_buf = nullptr;
   ctx->dec_bytes = 0;
+  ctx->epoch++;
   HIP_OK(hipMalloc((void **)&ctx->dec_buf, bytes), "decryption buffer");
   ctx->dec_bytes = bytes;
   return XPG_OK;
@@ -160,8 +166,24 @@ int ensure_batch_buf(xpg_ctx *ctx, size_t bytes) {
   }
   ctx->batch_buf = nullptr;
   ctx->batch_bytes = 0;
+  ctx->epoch++;
   HIP_OK(hipMalloc((void **)&ctx->batch_buf, bytes), "batch buffer");
   ctx->batch_bytes = bytes;
+  return XPG_OK;
+}
+
+int ensure_seg_scratch(xpg_ctx *ctx, size_t bytes) {
+  if (ctx->seg_scratch_bytes >= bytes) return XPG_OK;
+  if (ctx->seg_scratch) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->seg_scratch);
+  }
+  ctx->seg_scratch = nullptr;
+  ctx->seg_scratch_bytes = 0;
+  ctx->seg_capacity = 0;
+  ctx->epoch++;
+  HIP_OK(hipMalloc(&ctx->seg_scratch, bytes), "segmented MAC status region");
+  ctx->seg_scratch_bytes = bytes;
   return XPG_OK;
 }
 
@@ -173,10 +195,23 @@ int ensure_scratch(xpg_ctx *ctx, size_t bytes) {
   }
   ctx->scratch = nullptr;
   ctx->scratch_bytes = 0;
-  ctx->seg_capacity = 0;
+  ctx->epoch++;
   HIP_OK(hipMalloc(&ctx->scratch, bytes), "scratch");
   ctx->scratch_bytes = bytes;
   return XPG_OK;
+}
+
+void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial) {
+  auto &c = ctx->graph_cache;
+  for (size_t i = 0; i < c.size();) {
+    if (db_serial == 0 || c[i].db_serial == db_serial) {
+      if (c[i].exec) (void)hipGraphExecDestroy(c[i].exec);
+      c[i] = c.back();
+      c.pop_back();
+    } else {
+      i++;
+    }
+  }
 }
 
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b) {
@@ -290,10 +325,12 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
   for (int i = 0; i < 2; i++)
     if (ctx->level_buf[i]) (void)hipFree(ctx->level_buf[i]);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->seg_scratch) (void)hipFree(ctx->seg_scratch);
   if (ctx->batch_buf) (void)hipFree(ctx->batch_buf);
   if (ctx->io_buf) (void)hipFree(ctx->io_buf);
   if (ctx->dec_buf) (void)hipFree(ctx->dec_buf);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  graph_cache_clear(ctx, 0);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -301,8 +338,12 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
 const char *xpg_last_error(const xpg_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 void *xpg_ctx_stream(xpg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
-int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value) {
-  if (!ctx || !key) return XPG_EINVAL;
+static int set_option(xpg_ctx *ctx, const char *key, int64_t value) {
+  if (strcmp(key, "graphs") == 0 && (value == 0 || value == 1)) {
+    ctx->graphs = value != 0;
+    if (!ctx->graphs) graph_cache_clear(ctx, 0);
+    return XPG_OK;
+  }
   if (strcmp(key, "mac_kernel") == 0 && value >= 0 && value <= 3) {
     ctx->mac_mode = (int)value;
     return XPG_OK;
@@ -328,6 +369,13 @@ int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value) {
     return XPG_OK;
   }
   return set_err(ctx, XPG_EINVAL, std::string("unknown option or value: ") + key);
+}
+
+int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value) {
+  if (!ctx || !key) return XPG_EINVAL;
+  const int rc = set_option(ctx, key, value);
+  if (!rc) ctx->epoch++;  // captured replies baked the old settings in
+  return rc;
 }
 
 int xpg_sync(xpg_ctx *ctx) {
@@ -377,6 +425,7 @@ static int db_setup(xpg_ctx *ctx, const xpg_params *prm, uint64_t n_elems, uint6
   if (units < real_units) return set_err(ctx, XPG_EINVAL, "grid smaller than the database");
   xpg_db *db = new xpg_db();
   db->ctx = ctx;
+  db->serial = ++g_db_serial;
   db->prm = *prm;
   db->d = d;
   for (uint32_t l = 0; l < d; l++) db->dims[l] = dims[l];
@@ -479,6 +528,7 @@ void xpg_db_free(xpg_db *db) {
   if (db->ctx) {
     (void)hipSetDevice(db->ctx->device);
     (void)hipStreamSynchronize(db->ctx->stream);
+    graph_cache_clear(db->ctx, db->serial);  // other contexts' entries never match again
   }
   if (db->words) (void)hipFree(db->words);
   delete db;
@@ -535,28 +585,121 @@ static int run_levels(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const uint
   return XPG_OK;
 }
 
+// A reply replayed from a captured graph.  The first call for a (database, query,
+// reply) triple runs eagerly, which also sizes every context buffer the levels use; the
+// second captures the same launches (nothing allocates then) and replays them; later
+// calls replay.  Entries die with the epoch: a moved buffer or a changed option.
+static constexpr size_t kGraphCacheMax = 32;
+
+static int reply_graph(xpg_ctx *ctx, const xpg_db *db, const uint64_t *q, uint64_t *r) {
+  auto &cache = ctx->graph_cache;
+  GraphEntry *hit = nullptr;
+  for (auto &g : cache)
+    if (g.db_serial == db->serial && g.q == q && g.r == r) hit = &g;
+  ctx->use_clock++;
+  const uint64_t tg = tune_generation();
+  const bool current = hit && hit->epoch == ctx->epoch && hit->tune_gen == tg;
+  if (current && hit->exec) {
+    hit->last_use = ctx->use_clock;
+    ctx->graph_launches++;
+    HIP_OK(hipGraphLaunch(hit->exec, ctx->stream), "graph launch");
+    return XPG_OK;
+  }
+  if (!current) {
+    // first sight (or stale): run eagerly, remember the triple
+    const int rc = run_levels(ctx, db, 0, db->words, q, nullptr, r);
+    if (rc) return rc;
+    if (!hit) {
+      if (cache.size() >= kGraphCacheMax) {
+        size_t lru = 0;
+        for (size_t i = 1; i < cache.size(); i++)
+          if (cache[i].last_use < cache[lru].last_use) lru = i;
+        if (cache[lru].exec) (void)hipGraphExecDestroy(cache[lru].exec);
+        cache[lru] = cache.back();
+        cache.pop_back();
+      }
+      cache.push_back(GraphEntry{});
+      hit = &cache.back();
+    } else if (hit->exec) {
+      (void)hipGraphExecDestroy(hit->exec);
+    }
+    *hit = GraphEntry{db->serial, q, r, ctx->epoch, tg, nullptr, ctx->use_clock};
+    return XPG_OK;
+  }
+  // second sight at this epoch: capture
+  const uint64_t epoch = ctx->epoch;
+  HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  const int rc = run_levels(ctx, db, 0, db->words, q, nullptr, r);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+  hipGraphExec_t exec = nullptr;
+  if (!rc && ec == hipSuccess && graph && ctx->epoch == epoch && tune_generation() == tg &&
+      hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+    hit->exec = exec;
+    hit->last_use = ctx->use_clock;
+    ctx->graph_captures++;
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!hit->exec) {
+    // capture failed (nothing ran): run eagerly, which also reports any real error
+    hit->epoch = 0;
+    (void)hipGetLastError();
+    return run_levels(ctx, db, 0, db->words, q, nullptr, r);
+  }
+  ctx->graph_launches++;
+  HIP_OK(hipGraphLaunch(hit->exec, ctx->stream), "graph launch");
+  return XPG_OK;
+}
+
 int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev) {
   if (!ctx || !db || !query_dev || !reply_dev) return set_err(ctx, XPG_EINVAL, "null argument");
   if (db->ctx->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
+  if (ctx->graphs && !ctx->timing) return reply_graph(ctx, db, query_dev, reply_dev);
   return run_levels(ctx, db, 0, db->words, query_dev, nullptr, reply_dev);
 }
 
 // Level 0 of every request on one database in one batched launch, then each request's
 // remaining levels.  Worth it when level 0 has enough chains to fill the GPU per query
 // group (the batched kernel runs one chain per lane).
+static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs);
+
+// Whether a batch of qn queries on the database's level 0 runs as one batched launch:
+// the tuning table's measurement when it has one for this shape, else a fixed rule
+// (enough lanes per query group to fill the GPU).
+static bool batch_pays(const xpg_ctx *ctx, const xpg_db *db, uint32_t qn) {
+  if (qn < 2) return false;
+  const xpg_params *prm = &db->prm;
+  const uint64_t groups0 = prod_dims(db->d, db->dims, 1);
+  LevelCfg c;
+  const TuneKey key{1, prm->n, prm->m, db->P0, db->dims[0], groups0, tune_qn_bucket(qn)};
+  if (!options_pinned(ctx) && tune_lookup(ctx->device, key, &c)) return c.kernel != 0;
+  const uint64_t lanes = groups0 * db->P0 * (uint64_t)prm->m * prm->n;
+  return lanes * ((qn + 7) / 8) >= (1ull << 16);
+}
+
 static int batch_one_db(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs) {
+  const uint32_t total = (uint32_t)reqs.size();
+  for (uint32_t s = 0; s < total; s += 64) {
+    const uint32_t cnt = std::min<uint32_t>(64, total - s);
+    std::vector<const xpg_request *> part(reqs.begin() + s, reqs.begin() + s + cnt);
+    int rc = XPG_OK;
+    if (batch_pays(ctx, db, cnt)) {
+      rc = batch_chunk(ctx, db, part);
+    } else {
+      for (const xpg_request *r : part)
+        if (!rc) rc = xpg_reply_dev(ctx, db, r->query_dev, r->reply_dev);
+    }
+    if (rc) return rc;
+  }
+  return XPG_OK;
+}
+
+// One batched launch for level 0 of up to 64 requests, then each request's other levels.
+static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs) {
   const xpg_params *prm = &db->prm;
   const uint64_t mn = (uint64_t)prm->m * prm->n;
   const uint64_t groups0 = prod_dims(db->d, db->dims, 1);
-  const uint64_t lanes = groups0 * db->P0 * mn;
   const uint32_t qn = (uint32_t)reqs.size();
-  if (qn < 2 || lanes * ((qn + 7) / 8) < (1ull << 16)) {
-    for (const xpg_request *r : reqs) {
-      int rc = xpg_reply_dev(ctx, db, r->query_dev, r->reply_dev);
-      if (rc) return rc;
-    }
-    return XPG_OK;
-  }
   const uint64_t lvl0_words = groups0 * db->P0 * 2 * mn;
   // level-0 outputs: the final replies when d == 1, else per-request intermediates
   if (db->d > 1) {
@@ -569,11 +712,7 @@ static int batch_one_db(xpg_ctx *ctx, const xpg_db *db, const std::vector<const 
     qp[i] = reqs[i]->query_dev;
     op[i] = db->d == 1 ? reqs[i]->reply_dev : ctx->batch_buf + i * lvl0_words;
   }
-  int rc = XPG_OK;
-  for (uint32_t s = 0; s < qn && !rc; s += 64) {
-    const uint32_t cnt = std::min<uint32_t>(64, qn - s);
-    rc = launch_mac_batch(ctx, prm, db->words, qp.data() + s, op.data() + s, cnt, db->dims[0], groups0, db->P0);
-  }
+  int rc = launch_mac_batch(ctx, prm, db->words, qp.data(), op.data(), qn, db->dims[0], groups0, db->P0);
   if (rc) return rc;
   if (db->d > 1) {
     for (uint32_t i = 0; i < qn; i++) {
@@ -640,6 +779,13 @@ int xpg_timing_read(xpg_ctx *ctx, double *mac_ms, uint64_t *launches, int reset)
     ctx->mac_ms = 0;
     ctx->mac_launches = 0;
   }
+  return XPG_OK;
+}
+
+int xpg_graph_stats(xpg_ctx *ctx, uint64_t *captures, uint64_t *replays) {
+  if (!ctx) return XPG_EINVAL;
+  if (captures) *captures = ctx->graph_captures;
+  if (replays) *replays = ctx->graph_launches;
   return XPG_OK;
 }
 

@@ -56,6 +56,20 @@ struct xpg_db {
   uint64_t n_elems = 0, elem_bytes = 0;
   uint32_t alpha = 1;
   uint64_t *words = nullptr;  // units x P0 x m x n
+  uint64_t serial = 0;        // unique per database ever created (graph cache key)
+};
+
+// One captured reply (every launch of run_levels for one database, query and reply
+// buffer), replayed with hipGraphLaunch.  Valid while the context's buffer epoch is
+// unchanged: growing a context buffer or changing an option bumps the epoch.
+struct GraphEntry {
+  uint64_t db_serial = 0;
+  const uint64_t *q = nullptr;
+  uint64_t *r = nullptr;
+  uint64_t epoch = 0;             // context epoch at capture
+  uint64_t tune_gen = 0;          // tuning table generation at capture
+  hipGraphExec_t exec = nullptr;  // null: seen once (ran eagerly); captured on the next call
+  uint64_t last_use = 0;
 };
 
 struct xpg_ctx {
@@ -63,9 +77,13 @@ struct xpg_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   std::map<uint64_t, DevTables> tables;  // key (n << 8) | m
-  // scratch reused across calls (grown on demand)
+  // scratch reused across calls (grown on demand): block summaries of the streaming MAC
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // the segmented MAC's ticket and status words (zero between launches) and its maps;
+  // a region of its own, so no other kernel disturbs the zeros
+  void *seg_scratch = nullptr;
+  size_t seg_scratch_bytes = 0;
   uint64_t seg_capacity = 0;
   uint64_t *batch_buf = nullptr;  // level-0 replies of a batch (d > 1)
   size_t batch_bytes = 0;
@@ -92,19 +110,50 @@ struct xpg_ctx {
   uint64_t mac_launches = 0;
   double main_ms = 0.0;        // the dominant kernel of each MAC (without its scan or fixup)
   uint64_t main_launches = 0;
+  // captured replies (option "graphs")
+  bool graphs = false;
+  uint64_t epoch = 1;          // bumped whenever a context buffer moves or an option changes
+  uint64_t use_clock = 0;
+  std::vector<GraphEntry> graph_cache;
+  uint64_t graph_launches = 0, graph_captures = 0;
 };
 
 // ---- host helpers (context.hip) ----
 int set_err(xpg_ctx *ctx, int code, const std::string &msg);
 const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m);
 int ensure_scratch(xpg_ctx *ctx, size_t bytes);
+int ensure_seg_scratch(xpg_ctx *ctx, size_t bytes);
 int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
 int ensure_io_buf(xpg_ctx *ctx, size_t bytes);
 int ensure_dec_buf(xpg_ctx *ctx, size_t bytes);
+void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial);  // 0 = every entry
 ModSet modset_of(const xpg_params *prm);
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
 void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b, int tag = 0);
+
+// ---- tuning (tune.hip) ----
+// Kernel configuration of one MAC launch.  kernel: 1 = one chain per lane, 2 = segmented
+// chains with look-back (a = elements per lane, b = wavefronts per workgroup), 3 =
+// streaming block summaries (a = elements per load chunk, b = double-buffered, c =
+// elements per block, 0 = sized for one pass), 4 = register-blocked (a = groups per lane,
+// b = queries per lane).
+struct LevelCfg {
+  int kernel = 0;
+  int a = 0, b = 0, c = 0;
+};
+struct TuneKey {
+  uint32_t kind, n, m, P;
+  uint64_t n_l, groups;
+  uint32_t qn;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(kind, n, m, P, n_l, groups, qn) < std::tie(o.kind, o.n, o.m, o.P, o.n_l, o.groups, o.qn);
+  }
+};
+// process-wide table per device (every context on the device, the server's included)
+bool tune_lookup(int device, const TuneKey &k, LevelCfg *cfg);
+uint64_t tune_generation();
+uint32_t tune_qn_bucket(uint32_t qn);  // batch sizes are tuned at powers of two
 
 // ---- launchers (import.hip) ----
 // Transform `units` units of a byte source into NTT words (units x P x m x n); the
@@ -126,4 +175,21 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
 // pointers (they travel in the kernel arguments)
 int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                      uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P);
+// The same with an explicit configuration (the tuner's candidates): launch_mac_level_cfg
+// takes kernels 1..4, launch_mac_batch_cfg kernel 4 (a x b register blocks, 0 = by shape).
+int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,
+                         uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out, const LevelCfg &cfg);
+int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
+                         uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P, int gr,
+                         int qr);
+// what launch_mac_level runs for this shape with the context's options and no tuning
+LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P);
+bool seg_shape_built(int S, int W);     // segmented tile shapes compiled in
+bool batch_shape_built(int GR, int QR);  // register-block shapes compiled in
+// explicit options (a kernel or a tile shape asked for) win over the tuning table
+bool options_pinned(const xpg_ctx *ctx);
+// register blocks the batched MAC picks by shape for the ones given as 0
+void batch_default_shape(uint32_t qn, uint64_t groups, int *gr, int *qr);
+// stream E that launch_stream_level picks when none is given
+uint64_t stream_auto_e(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, int u);
 

@@ -681,25 +681,23 @@ __global__ void __launch_bounds__(128 * FIX_PARTS) mac_stream_fixup_kernel(Strea
 }  // namespace
 
 namespace {
-
 // Tiles per column and the scratch layout for a segmented launch of tile_elems elements per tile.
 int seg_prepare(xpg_ctx *ctx, SegArgs &sa, uint64_t tile_elems) {
   sa.tiles = (sa.n_l + tile_elems - 1) / tile_elems;
   const uint64_t words = sa.tiles * sa.columns * 128;
   if (sa.tiles * sa.columns > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
-  // scratch layout: [ticket, 256 B][status words, capacity][dm maps, capacity].  The
+  // region layout: [ticket, 256 B][status words, capacity][dm maps, capacity].  The
   // ticket and the whole status region are zero between launches: zeroed when the
   // region is (re)allocated, and every launch's fixup zeroes exactly the words it used.
   if (ctx->seg_capacity < words) {
-    const size_t bytes = 256 + 2 * words * 8;
-    int rc = ensure_scratch(ctx, bytes);
+    int rc = ensure_seg_scratch(ctx, 256 + 2 * words * 8);
     if (rc) return rc;
-    rc = launch_fill_zero(ctx, (uint64_t *)ctx->scratch, (256 + words * 8) / 8);
+    rc = launch_fill_zero(ctx, (uint64_t *)ctx->seg_scratch, (256 + words * 8) / 8);
     if (rc) return rc;
     ctx->seg_capacity = words;
   }
-  sa.ticket = (uint32_t *)ctx->scratch;
-  sa.status = (uint64_t *)((char *)ctx->scratch + 256);
+  sa.ticket = (uint32_t *)ctx->seg_scratch;
+  sa.status = (uint64_t *)((char *)ctx->seg_scratch + 256);
   sa.dm = (int2 *)(sa.status + ctx->seg_capacity);
   return XPG_OK;
 }
@@ -722,6 +720,29 @@ int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
   return launch_fixup(ctx, sa);
 }
 
+// Segmented launch of tile shape S x W (the instantiated shapes only).
+int launch_seg_shape(xpg_ctx *ctx, SegArgs &sa, int S, int W) {
+  if (!seg_shape_built(S, W)) return set_err(ctx, XPG_EINVAL, "segmented MAC tile shape not built");
+  switch (S * 100 + W) {
+    case 408: return launch_seg<4, 8>(ctx, sa);
+    case 416: return launch_seg<4, 16>(ctx, sa);
+    case 804: return launch_seg<8, 4>(ctx, sa);
+    case 808: return launch_seg<8, 8>(ctx, sa);
+    case 816: return launch_seg<8, 16>(ctx, sa);
+    case 1202: return launch_seg<12, 2>(ctx, sa);
+    case 1204: return launch_seg<12, 4>(ctx, sa);
+    case 1208: return launch_seg<12, 8>(ctx, sa);
+    case 1602: return launch_seg<16, 2>(ctx, sa);
+    case 1604: return launch_seg<16, 4>(ctx, sa);
+    case 1608: return launch_seg<16, 8>(ctx, sa);
+    case 2402: return launch_seg<24, 2>(ctx, sa);
+    case 2404: return launch_seg<24, 4>(ctx, sa);
+    case 3202: return launch_seg<32, 2>(ctx, sa);
+    case 3204: return launch_seg<32, 4>(ctx, sa);
+    default: return set_err(ctx, XPG_EINVAL, "segmented MAC tile shape not built");
+  }
+}
+
 // Streaming launch: block summaries into scratch, then the per-column scan.
 template <int U, bool PF>
 int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
@@ -730,7 +751,6 @@ int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
   if (waves > 0xffffffffULL - 3) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   int rc = ensure_scratch(ctx, recs * 24);
   if (rc) return rc;
-  ctx->seg_capacity = 0;  // the segmented kernel's zeroed status region is overwritten
   st.rho0 = (uint64_t *)ctx->scratch;
   st.rstar = st.rho0 + recs;
   st.jj = (int2 *)(st.rstar + recs);
@@ -744,8 +764,23 @@ int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
   return XPG_OK;
 }
 
-// The streaming MAC for one level: block summaries, then the per-column scan.
-int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t groups) {
+uint64_t columns_of(const xpg_params *prm, uint64_t groups, uint32_t P) {
+  return groups * P * (uint64_t)prm->m * (prm->n / 64);
+}
+
+// Elements per block that give about four waves per SIMD, all resident at once: one
+// pass over the level, and few block summaries (their writes interleave with the
+// level's reads).
+uint64_t auto_e(int cu_count, uint64_t n_l, uint64_t columns, int U) {
+  const uint64_t target = 16ull * (uint64_t)cu_count;
+  const uint64_t nb = std::max<uint64_t>(1, target / std::max<uint64_t>(1, columns));
+  const uint64_t E = (n_l + nb - 1) / nb;
+  return (E + U - 1) / U * U;
+}
+
+// The streaming MAC for one level: U elements per load chunk, PF double-buffered chunks,
+// E elements per block (0 = auto_e).
+int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t groups, int U, bool pf, uint64_t E) {
   StreamArgs st{};
   st.ms = a.ms;
   st.db = a.db;
@@ -760,20 +795,12 @@ int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t gro
   st.cpg = a.P * a.m * st.sblocks;
   if (groups * st.cpg > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.columns = (uint32_t)(groups * st.cpg);
-  const int U = ctx->stream_u ? ctx->stream_u : 8;
-  uint64_t E = (uint64_t)ctx->stream_e;
-  if (!E) {
-    // about four waves per SIMD, all resident at once: one pass over the level, and few
-    // block summaries (their writes interleave with the level's reads)
-    const uint64_t target = 16ull * (uint64_t)ctx->cu_count;
-    const uint64_t nb = std::max<uint64_t>(1, target / st.columns);
-    E = (a.n_l + nb - 1) / nb;
-  }
+  if (U != 4 && U != 8 && U != 16) return set_err(ctx, XPG_EINVAL, "streaming MAC chunk not built");
+  if (!E) E = auto_e(ctx->cu_count, a.n_l, st.columns, U);
   E = (E + U - 1) / U * U;
+  if (E > 0xffffffffULL || (a.n_l + E - 1) / E > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.E = (uint32_t)E;
-  if ((a.n_l + E - 1) / E > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.nblk = (uint32_t)((a.n_l + E - 1) / E);
-  const bool pf = ctx->stream_pf != 0;
   switch (U) {
     case 4: return pf ? launch_stream<4, true>(ctx, st) : launch_stream<4, false>(ctx, st);
     case 16: return pf ? launch_stream<16, true>(ctx, st) : launch_stream<16, false>(ctx, st);
@@ -783,11 +810,71 @@ int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t gro
 
 }  // namespace
 
+bool seg_shape_built(int S, int W) {
+  switch (S * 100 + W) {
+    case 408: case 416: case 804: case 808: case 816: case 1202: case 1204: case 1208:
+    case 1602: case 1604: case 1608: case 2402: case 2404: case 3202: case 3204:
+      return true;
+    default:
+      return false;
+  }
+}
+
+void batch_default_shape(uint32_t qn, uint64_t groups, int *gr, int *qr) {
+  if (!*qr) *qr = qn >= 4 ? 4 : (qn >= 2 ? 2 : 1);
+  if (!*gr) *gr = groups >= 8 && *qr <= 2 ? 8 : (groups >= 4 ? 4 : 1);
+}
+
+bool batch_shape_built(int GR, int QR) {
+  const int key = GR * 10 + QR;
+  return key == 11 || key == 12 || key == 14 || key == 41 || key == 42 || key == 44 || key == 81 || key == 82;
+}
+
 // Lanes at which one chain per lane already fills the GPU (256 CUs x 32 wavefronts).
 static const uint64_t kDirectLanes = 256ull * 32 * 64;
 
-int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,
-                     uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out) {
+uint64_t stream_auto_e(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, int u) {
+  return auto_e(ctx->cu_count, n_l, columns_of(prm, groups, P), u ? u : 8);
+}
+
+LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P) {
+  const uint64_t lanes = groups * P * (uint64_t)prm->m * prm->n;
+  // the segmented and streaming kernels' columns are 64 slots wide: degrees below 64
+  // always take one chain per lane
+  const bool one_chain_per_lane =
+      prm->n < 64 || ctx->mac_mode == 1 || (ctx->mac_mode == 0 && (lanes >= kDirectLanes || n_l <= 64));
+  LevelCfg c;
+  if (one_chain_per_lane && groups >= 4 && ctx->mac_mode != 1) {
+    // many groups share each query word: register-blocked over groups
+    c.kernel = 4;
+    c.b = 1;
+  } else if (!one_chain_per_lane && (ctx->mac_mode == 3 || ctx->mac_mode == 0)) {
+    c.kernel = 3;
+    c.a = ctx->stream_u ? ctx->stream_u : 8;
+    c.b = ctx->stream_pf;
+    c.c = ctx->stream_e;
+  } else if (one_chain_per_lane) {
+    c.kernel = 1;
+  } else {
+    // tile shape: S elements per lane, W wavefronts per workgroup.  Measured on the
+    // 65536-element single query: 32 x 4 streams at 5.8 TB/s (more loads in flight per
+    // lane, fewer tiles to look back over) against 5.6 for 24 x 4 and 5.3 for 16 x 4.
+    c.kernel = 2;
+    c.a = ctx->seg_s ? ctx->seg_s : (n_l >= 4096 ? 32 : (n_l >= 512 ? 16 : 8));
+    c.b = ctx->seg_w ? ctx->seg_w : 4;
+    if (!seg_shape_built(c.a, c.b)) c.a = c.b = 8;
+  }
+  return c;
+}
+
+int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,
+                         uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out, const LevelCfg &cfg) {
+  if (cfg.kernel == 4) {
+    uint64_t *outp = out;
+    return launch_mac_batch_cfg(ctx, prm, db, &q, &outp, 1, n_l, groups, P, cfg.a, 1);
+  }
+  if (cfg.kernel < 1 || cfg.kernel > 3) return set_err(ctx, XPG_EINVAL, "unknown MAC kernel");
+  if (cfg.kernel != 1 && prm->n < 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n >= 64");
   MacArgs a;
   a.ms = modset_of(prm);
   a.db = db;
@@ -799,21 +886,13 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   a.n = prm->n;
   a.m = prm->m;
   a.logn = prm->logn;
-  const uint64_t lanes = groups * P * (uint64_t)prm->m * prm->n;
-  // the segmented kernel's columns are 64 slots wide: degrees below 64 always take one chain per lane
-  const bool one_chain_per_lane =
-      prm->n < 64 || ctx->mac_mode == 1 || (ctx->mac_mode == 0 && (lanes >= kDirectLanes || n_l <= 64));
-  if (one_chain_per_lane && groups >= 4 && ctx->mac_mode != 1) {
-    // many groups share each query word: register-blocked over groups
-    uint64_t *outp = out;
-    return launch_mac_batch(ctx, prm, db, &q, &outp, 1, n_l, groups, P);
-  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
   int rc = XPG_OK;
-  if (!one_chain_per_lane && (ctx->mac_mode == 3 || ctx->mac_mode == 0)) {
-    rc = launch_stream_level(ctx, a, prm->n, groups);
-  } else if (one_chain_per_lane) {
+  if (cfg.kernel == 3) {
+    rc = launch_stream_level(ctx, a, prm->n, groups, cfg.a ? cfg.a : 8, cfg.b != 0, (uint64_t)cfg.c);
+  } else if (cfg.kernel == 1) {
+    const uint64_t lanes = groups * P * (uint64_t)prm->m * prm->n;
     const uint64_t blocks = (lanes + 255) / 256;
     if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
     hipEvent_t k0 = nullptr, k1 = nullptr;
@@ -836,39 +915,43 @@ int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
     sa.cpg = P * prm->m * sa.sblocks;
     if ((uint64_t)groups * sa.cpg > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
     sa.columns = (uint32_t)(groups * sa.cpg);
-    // tile shape: S elements per lane, W wavefronts per workgroup (tunable).  Measured on
-    // the 65536-element single query: 32 x 4 streams at 5.8 TB/s (more loads in flight
-    // per lane, fewer tiles to look back over) against 5.6 for 24 x 4 and 5.3 for 16 x 4.
-    int S = ctx->seg_s, W = ctx->seg_w;
-    if (!S) S = n_l >= 4096 ? 32 : (n_l >= 512 ? 16 : 8);
-    if (!W) W = 4;
-    const int key = S * 100 + W;
-    switch (key) {
-      case 408: rc = launch_seg<4, 8>(ctx, sa); break;
-      case 416: rc = launch_seg<4, 16>(ctx, sa); break;
-      case 804: rc = launch_seg<8, 4>(ctx, sa); break;
-      case 816: rc = launch_seg<8, 16>(ctx, sa); break;
-      case 1202: rc = launch_seg<12, 2>(ctx, sa); break;
-      case 1204: rc = launch_seg<12, 4>(ctx, sa); break;
-      case 1208: rc = launch_seg<12, 8>(ctx, sa); break;
-      case 1602: rc = launch_seg<16, 2>(ctx, sa); break;
-      case 1604: rc = launch_seg<16, 4>(ctx, sa); break;
-      case 1608: rc = launch_seg<16, 8>(ctx, sa); break;
-      case 2402: rc = launch_seg<24, 2>(ctx, sa); break;
-      case 2404: rc = launch_seg<24, 4>(ctx, sa); break;
-      case 3202: rc = launch_seg<32, 2>(ctx, sa); break;
-      case 3204: rc = launch_seg<32, 4>(ctx, sa); break;
-      default: rc = launch_seg<8, 8>(ctx, sa); break;
-    }
+    rc = launch_seg_shape(ctx, sa, cfg.a, cfg.b);
   }
   timing_end(ctx, e0, e1);
   return rc;
 }
 
+// Explicit options (a kernel or a tile shape asked for) win over the tuning table.
+bool options_pinned(const xpg_ctx *ctx) {
+  return ctx->mac_mode || ctx->seg_s || ctx->seg_w || ctx->stream_e || ctx->stream_u || ctx->stream_pf;
+}
+
+int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,
+                     uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out) {
+  LevelCfg cfg;
+  const TuneKey key{0, prm->n, prm->m, P, n_l, groups, 0};
+  if (options_pinned(ctx) || !tune_lookup(ctx->device, key, &cfg)) cfg = default_level_cfg(ctx, prm, n_l, groups, P);
+  return launch_mac_level_cfg(ctx, prm, db, q, n_l, groups, P, out, cfg);
+}
+
 int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                      uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P) {
+  LevelCfg cfg;
+  const TuneKey key{1, prm->n, prm->m, P, n_l, groups, tune_qn_bucket(qn)};
+  if (options_pinned(ctx) || !tune_lookup(ctx->device, key, &cfg) || cfg.kernel != 4) cfg = LevelCfg{};
+  return launch_mac_batch_cfg(ctx, prm, db, q, out, qn, n_l, groups, P, cfg.a, cfg.b);
+}
+
+int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
+                         uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P, int gr,
+                         int qr) {
   if (qn == 0 || qn > (uint32_t)kMaxBatch) return set_err(ctx, XPG_EINVAL, "batch of 1..64 queries per launch");
-  // register blocks: GR groups x QR queries per lane
+  if (prm->n < 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n >= 64");
+  // register blocks: GR groups x QR queries per lane (0 = by shape)
+  int QR = qr, GR = gr;
+  batch_default_shape(qn, groups, &GR, &QR);
+  const int key = GR * 10 + QR;
+  if (!batch_shape_built(GR, QR)) return set_err(ctx, XPG_EINVAL, "register block shape not built");
   Batch2Args b2{};
   b2.ms = modset_of(prm);
   b2.db = db;
@@ -883,8 +966,6 @@ int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   b2.m = prm->m;
   b2.sblocks = prm->n / 64;
   b2.qn = qn;
-  const int QR = qn >= 4 ? 4 : (qn >= 2 ? 2 : 1);
-  const int GR = groups >= 8 && QR <= 2 ? 8 : (groups >= 4 ? 4 : 1);
   b2.qblocks = (qn + QR - 1) / QR;
   b2.colgroups = ((groups + GR - 1) / GR) * P * prm->m * b2.sblocks;
   const uint64_t blocks = ((b2.colgroups + 7) / 8) * 8 * b2.qblocks;
@@ -893,7 +974,6 @@ int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   timing_begin(ctx, &e0, &e1);
   hipEvent_t k0 = nullptr, k1 = nullptr;
   timing_begin(ctx, &k0, &k1);
-  const int key = GR * 10 + QR;
   switch (key) {
     case 82: hipLaunchKernelGGL((mac_batch2_kernel<8, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
     case 81: hipLaunchKernelGGL((mac_batch2_kernel<8, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
