@@ -20,7 +20,7 @@ C3 = "LWE:100:4096:180"
     (C3, 16, 1024, [4, 4]),
     (C2, 2000, 512, [40, 50]),
 ])
-def test_tuned_replies_match_untuned_and_oracle(desc, n_elems, elem_bytes, dims):
+def test_tuned_replies_match_untuned_and_oracle(desc, n_elems, elem_bytes, dims, tmp_path):
     import xpir_amd
     orc = Oracle(desc, dims[0])
     data = orc.rand_bytes(51, 0, n_elems * elem_bytes)
@@ -71,12 +71,14 @@ def test_tuned_replies_match_untuned_and_oracle(desc, n_elems, elem_bytes, dims)
         ctx.reply_batch_dev([(db, qd, o) for (qd, _), o in zip(qs, outs)])
         for o, want in zip(outs, untuned):
             assert np.array_equal(o.download(), want)
-        # save, clear, reinstall
+        # save, clear, reinstall (through a JSON file)
         saved = ctx.tune_get()
         assert len(saved) >= len(entries)
+        path = tmp_path / "tune.json"
+        ctx.tune_save(str(path))
         ctx.tune_clear()
         assert ctx.tune_get() == []
-        ctx.tune_set(saved)
+        ctx.tune_load(str(path))
         assert len(ctx.tune_get()) == len(saved)
         qd, rd = qs[1]
         ctx.reply_dev(db, qd, rd)

@@ -336,6 +336,18 @@ class Context:
     def tune_clear(self):
         self._check(lib().xpg_tune_clear(self.h))
 
+    def tune_save(self, path: str):
+        """Write the device's tuning table as JSON (a list of entries)."""
+        import json
+        with open(path, "w") as f:
+            json.dump(self.tune_get(), f, indent=1)
+
+    def tune_load(self, path: str):
+        """Install a table written by tune_save."""
+        import json
+        with open(path) as f:
+            self.tune_set(json.load(f))
+
     def graph_stats(self):
         """(replies captured into a HIP graph, replies replayed from one)"""
         c, r = C.c_uint64(), C.c_uint64()
