@@ -4,13 +4,19 @@
 // from the library's GPU query helper; the reply bytes go to a file that the test
 // compares with the oracle.
 //
-// usage: simple_pir CRYPTO N ELEM_BYTES ALPHA D n0 [n1 ...] DB_SEED INDEX NOISE_SEED KEY_SEED OUT
+// usage: simple_pir CRYPTO N ELEM_BYTES ALPHA D n0 [n1 ...] DB_SEED INDEX NOISE_SEED KEY_SEED OUT [DIR]
+// With DIR the database is read from files through DBDirectoryProcessor: element e is
+// written to DIR/e<e> with ELEM_BYTES - (e mod 3) bytes (shorter files read zero-padded),
+// the catalog order is the directory listing's, and the catalog (getCatalog(true)) is
+// written to OUT.catalog; INDEX is a catalog position.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "../../xpir_amd/cpp/libpir_amd.hpp"
 
@@ -58,9 +64,31 @@ int main(int argc, char **argv) {
     }
     elements.emplace_back("e" + std::to_string(e), L, store[e].data());
   }
-  DBVectorProcessor db(elements);
+  DBVectorProcessor vec(elements);
+  DBHandler *db = &vec;
+  DBDirectoryProcessor *dirdb = nullptr;
+  if (a < argc) {
+    const std::string dir = std::string(argv[a++]) + "/";
+    mkdir(dir.c_str(), 0755);
+    for (uint64_t e = 0; e < N; e++) {
+      FILE *f = fopen((dir + "e" + std::to_string(e)).c_str(), "wb");
+      if (!f) return 1;
+      fwrite(store[e].data(), 1, L - (e % 3 < L ? e % 3 : 0), f);
+      fclose(f);
+    }
+    dirdb = new DBDirectoryProcessor(dir);
+    if (dirdb->getErrorStatus() || dirdb->getNbStream() != N || dirdb->getmaxFileBytesize() != L) {
+      fprintf(stderr, "DBDirectoryProcessor: bad catalog\n");
+      return 1;
+    }
+    FILE *c = fopen((std::string(out_path) + ".catalog").c_str(), "wb");
+    const std::string cat = dirdb->getCatalog(true);
+    fwrite(cat.data(), 1, cat.size(), c);
+    fclose(c);
+    db = dirdb;
+  }
 
-  PIRReplyGenerator server(params, *crypto_method, &db);
+  PIRReplyGenerator server(params, *crypto_method, db);
   imported_database *imported = server.importData(0, L);
   if (!imported) {
     fprintf(stderr, "importData: %s\n", server.lastError().c_str());
@@ -109,6 +137,7 @@ int main(int argc, char **argv) {
   server.freeQueries();
   delete imported;
   delete crypto_method;
+  delete dirdb;
   printf("simple_pir: %llu reply elements of %llu bytes\n", (unsigned long long)count,
          (unsigned long long)server.getReplyElementBytesize());
   return count == server.getnbRepliesGenerated() ? 0 : 1;

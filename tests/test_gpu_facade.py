@@ -43,3 +43,34 @@ def test_simple_pir_through_the_facade(tmp_path, crypto, n, L, alpha, dims):
     want, _ = orc.generate_reply(words, P, dims, q)
     assert np.array_equal(got, want)
     assert extract_reply(orc, sk, got, dims, unit) == padded[index * unit:(index + 1) * unit].tobytes()
+
+
+def test_simple_pir_from_a_directory(tmp_path):
+    """The same through DBDirectoryProcessor: files of uneven length in directory-listing
+    order, shorter ones zero-padded to the largest."""
+    crypto, n, L, dims = "LWE:80:2048:120", 30, 4096, [6, 5]
+    out = tmp_path / "reply.bin"
+    index = 17
+    args = [DEMO, crypto, str(n), str(L), "1", str(len(dims)), *map(str, dims), "1", str(index), "3", "4",
+            str(out), str(tmp_path / "db")]
+    print("run:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    lines = (tmp_path / "reply.bin.catalog").read_text().split("\n")
+    assert lines[0] == "0" and int(lines[1]) == n
+    names, sizes = lines[2:2 + 2 * n:2], [int(s) for s in lines[3:3 + 2 * n:2]]
+    orc = Oracle(crypto, dims[0])
+    data = orc.rand_bytes(1, 0, n * L)
+    elems = []
+    for name, size in zip(names, sizes):
+        e = int(name[1:])
+        assert size == L - e % 3
+        elems.append(np.concatenate([data[e * L:e * L + size], np.zeros(L - size, dtype=np.uint8)]))
+    P = orc.polys(L)
+    words = np.concatenate([orc.import_element(x) for x in elems])
+    sk = orc.keygen(4)
+    q = orc.make_query(sk, dims, index, 3)
+    want, _ = orc.generate_reply(words, P, dims, q)
+    got = np.fromfile(out, dtype=np.uint64)
+    assert np.array_equal(got, want)
+    assert extract_reply(orc, sk, got, dims, L) == elems[index].tobytes()

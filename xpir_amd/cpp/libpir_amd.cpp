@@ -1,8 +1,10 @@
 // libpir_amd.cpp -- the C++ facade of libpir_amd.hpp over the C ABI.
 #include "libpir_amd.hpp"
 
+#include <dirent.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 
@@ -54,6 +56,119 @@ uint64_t DBVectorProcessor::readStream(uint64_t streamNb, char *buf, uint64_t si
 }
 
 void DBVectorProcessor::closeStream(uint64_t streamNb) { offsets_.erase(streamNb); }
+
+void DBHandler::readAggregatedStream(uint64_t streamNb, uint64_t alpha, uint64_t offset, uint64_t bytes_per_file,
+                                     char *rawBits) {
+  const uint64_t maxb = getmaxFileBytesize();
+  const uint64_t file_bytes = offset >= maxb ? 0 : std::min(bytes_per_file, maxb - offset);
+  const uint64_t first = streamNb * alpha, nb = getNbStream();
+  for (uint64_t s = 0; s < alpha; s++) {
+    char *dst = rawBits + s * file_bytes;
+    if (first + s < nb && openStream(first + s, offset)) {
+      readStream(first + s, dst, file_bytes);
+      closeStream(first + s);
+    } else {
+      memset(dst, 0, file_bytes);
+    }
+  }
+}
+
+static std::string with_slash(const std::string &d) {
+  return d.empty() || d.back() == '/' ? d : d + "/";
+}
+
+static bool is_regular(const std::string &path) {
+  struct stat st;
+  return stat(path.c_str(), &st) == 0 && S_ISREG(st.st_mode);
+}
+
+uint64_t DBDirectoryProcessor::getFileSize(const std::string &filePath) {
+  struct stat st;
+  return stat(filePath.c_str(), &st) == 0 ? (uint64_t)st.st_size : 0;
+}
+
+DBDirectoryProcessor::DBDirectoryProcessor(const std::string &directory) : directory_(with_slash(directory)) {
+  DIR *dir = opendir(directory_.c_str());
+  if (!dir) {
+    error_ = true;
+    return;
+  }
+  while (struct dirent *ent = readdir(dir)) {
+    // the reference's filter: strcmp(name, ".") > 0 && strcmp(name, "..") != 0
+    if (!(strcmp(ent->d_name, ".") > 0 && strcmp(ent->d_name, "..") != 0)) continue;
+    const std::string name(ent->d_name);
+    if (!is_regular(directory_ + name)) continue;
+    file_list_.push_back(name);
+    max_bytes_ = std::max(max_bytes_, getFileSize(directory_ + name));
+  }
+  closedir(dir);
+  if (file_list_.empty()) error_ = true;
+}
+
+DBDirectoryProcessor::DBDirectoryProcessor(uint64_t nbStreams, const std::string &directory)
+    : directory_(with_slash(directory)), splitting_(true) {
+  DIR *dir = opendir(directory_.c_str());
+  if (!dir) {
+    error_ = true;
+    return;
+  }
+  while (struct dirent *ent = readdir(dir)) {
+    const std::string name(ent->d_name);
+    if (name == "." || name == ".." || !is_regular(directory_ + name)) continue;
+    real_file_ = directory_ + name;  // the first regular file, as the reference splits
+    break;
+  }
+  closedir(dir);
+  const uint64_t size = real_file_.empty() ? 0 : getFileSize(real_file_);
+  max_bytes_ = nbStreams ? size / nbStreams : 0;
+  if (real_file_.empty() || max_bytes_ == 0) {
+    error_ = true;
+    return;
+  }
+  for (uint64_t i = 0; i < nbStreams; i++) file_list_.push_back(std::to_string(i));
+}
+
+DBDirectoryProcessor::~DBDirectoryProcessor() {
+  for (auto &kv : fd_pool_) fclose(kv.second);
+}
+
+std::string DBDirectoryProcessor::getCatalog(bool typeOfCatalog) {
+  std::string buf;
+  if (typeOfCatalog) {
+    buf = "0\n" + std::to_string(getNbStream()) + "\n";
+    for (const std::string &f : file_list_)
+      buf += f + "\n" + std::to_string(splitting_ ? max_bytes_ : getFileSize(directory_ + f)) + "\n";
+    return buf;
+  }
+  return "1\n" + std::to_string(getNbStream()) + "\n" + std::to_string(max_bytes_) + "\n";
+}
+
+bool DBDirectoryProcessor::openStream(uint64_t streamNb, uint64_t requested_offset) {
+  if (fd_pool_.count(streamNb) || streamNb >= file_list_.size()) return false;
+  FILE *f = fopen(splitting_ ? real_file_.c_str() : (directory_ + file_list_[streamNb]).c_str(), "rb");
+  if (!f) return false;
+  const uint64_t at = (splitting_ ? streamNb * max_bytes_ : 0) + requested_offset;
+  if (fseeko(f, (off_t)at, SEEK_SET) != 0) {
+    fclose(f);
+    return false;
+  }
+  fd_pool_[streamNb] = f;
+  return true;
+}
+
+uint64_t DBDirectoryProcessor::readStream(uint64_t streamNb, char *buf, uint64_t size) {
+  auto it = fd_pool_.find(streamNb);
+  const uint64_t got = it == fd_pool_.end() ? 0 : fread(buf, 1, size, it->second);
+  memset(buf + got, 0, size - got);  // short file: zero padding
+  return size;
+}
+
+void DBDirectoryProcessor::closeStream(uint64_t streamNb) {
+  auto it = fd_pool_.find(streamNb);
+  if (it == fd_pool_.end()) return;
+  fclose(it->second);
+  fd_pool_.erase(it);
+}
 
 DBGenerator::DBGenerator(uint64_t nbStreams, uint64_t streamBytesize, bool) : nb_(nbStreams), bytes_(streamBytesize) {}
 

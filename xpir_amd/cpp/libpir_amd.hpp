@@ -7,6 +7,7 @@
 //   PIRParameters                               pir/PIRParameters.hpp:22-29
 //   DBHandler / DBVectorProcessor / DBGenerator pir/dbhandlers/DBHandler.hpp:26-46,
 //                                               DBVectorProcessor.hpp:28-60, DBGenerator.hpp
+//   DBDirectoryProcessor                        pir/dbhandlers/DBDirectoryProcessor.hpp:33-66
 //   imported_database                           libpir.hpp:23-33, GenericPIRReplyGenerator.hpp:34-41
 //   PIRReplyGenerator                           libpir.hpp:101-169, pir/libpir.cpp:77-163
 // Names, argument meaning and ownership rules are the reference's: pushQuery takes
@@ -16,6 +17,8 @@
 // status through lastStatus()/lastError().
 #pragma once
 #include <stdint.h>
+
+#include <stdio.h>
 
 #include <map>
 #include <string>
@@ -80,6 +83,12 @@ class DBHandler {
   virtual uint64_t readStream(uint64_t streamNb, char *buf, uint64_t size) = 0;
   virtual void closeStream(uint64_t streamNb) = 0;
   virtual ~DBHandler() {}
+  // The alpha streams of aggregate `streamNb`, each read from `offset` and zero-padded to
+  // min(bytes_per_file, maxFileBytesize - offset) bytes, back to back in rawBits; slots
+  // past the last stream are zeroed (DBHandler.cpp:3-26, whose bzero starts at the last
+  // real stream's slot instead, erasing it when the last aggregate is partial).
+  void readAggregatedStream(uint64_t streamNb, uint64_t alpha, uint64_t offset, uint64_t bytes_per_file,
+                            char *rawBits);
 };
 
 // In-memory elements; short elements read as zero-padded (DBVectorProcessor.cpp:88-106).
@@ -110,6 +119,41 @@ class DBGenerator : public DBHandler {
 
  private:
   uint64_t nb_, bytes_;
+};
+
+// Elements read from files (DBDirectoryProcessor.cpp:22-121,161-212): one element per
+// file of `directory` in directory-listing order (the reference's catalog order), the
+// largest file setting maxFileBytesize, shorter files read as zero-padded; or, with
+// nbStreams, the first regular file split into nbStreams elements of
+// floor(size / nbStreams) bytes.  As in the reference, "." and ".." and every name that
+// sorts at or before "." are skipped; unlike it, only regular files are listed (the
+// reference would list a subdirectory as an element).  Errors (no directory, no files,
+// a file shorter than nbStreams bytes) set getErrorStatus() instead of printing.
+class DBDirectoryProcessor : public DBHandler {
+ public:
+  explicit DBDirectoryProcessor(const std::string &directory = "db/");
+  DBDirectoryProcessor(uint64_t nbStreams, const std::string &directory = "db/");
+  ~DBDirectoryProcessor() override;
+  // typeOfCatalog true: "0\n<count>\n" then "<name>\n<size>\n" per element; false:
+  // "1\n<count>\n<maxFileBytesize>\n" (DBDirectoryProcessor.cpp:127-151)
+  std::string getCatalog(bool typeOfCatalog);
+  uint64_t getDBSizeBits() { return max_bytes_ * file_list_.size() * 8; }
+  uint64_t getNbStream() override { return file_list_.size(); }
+  uint64_t getmaxFileBytesize() override { return max_bytes_; }
+  bool getErrorStatus() const { return error_; }
+  bool openStream(uint64_t streamNb, uint64_t requested_offset) override;
+  uint64_t readStream(uint64_t streamNb, char *buf, uint64_t size) override;
+  void closeStream(uint64_t streamNb) override;
+  static uint64_t getFileSize(const std::string &filePath);
+  const std::vector<std::string> &fileList() const { return file_list_; }
+
+ private:
+  std::string directory_;
+  std::vector<std::string> file_list_;
+  std::map<uint64_t, FILE *> fd_pool_;
+  bool splitting_ = false, error_ = false;
+  std::string real_file_;  // the split file
+  uint64_t max_bytes_ = 0;
 };
 
 // The pre-processed (NTT-domain) database, resident in HBM until destroyed.
