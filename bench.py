@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="recursive/batched/serving: skip the MAC autotuner (fixed kernel rules)")
+    ap.add_argument("--tune-file", help="install this saved tuning table instead of tuning when it exists; "
+                                        "else tune and save the table there")
     ap.add_argument("--host-io", action="store_true",
                     help="single/plain: also time replies from a host query to a host reply (PCIe included)")
     args = ap.parse_args()
@@ -290,9 +292,18 @@ def autotune_db(ctx, db, max_batch, args, label):
     any timed region; returns what it recorded and how long it took."""
     if args.no_autotune:
         return None
+    if args.tune_file and not hasattr(args, "_tune_load"):
+        args._tune_load = os.path.exists(args.tune_file)  # decided once, before any tuning
+        if args._tune_load:
+            ctx.tune_load(args.tune_file)
+            log(f"tuning table loaded from {args.tune_file}")
+    if getattr(args, "_tune_load", False):
+        return {"seconds": 0.0, "entries": [], "loaded": args.tune_file}
     t0 = time.perf_counter()
     entries = ctx.autotune(db, max_batch=max_batch)
     dt = time.perf_counter() - t0
+    if args.tune_file:
+        ctx.tune_save(args.tune_file)
     names = {1: "one-chain", 2: "segmented", 3: "streaming", 4: "blocked"}
     for e in entries:
         what = f"level n_l={e['n_l']} groups={e['groups']} P={e['P']}" if e["kind"] == 0 else \
