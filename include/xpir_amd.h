@@ -75,7 +75,8 @@ int xpg_sync(xpg_ctx *ctx);
  * second call with the same database, query buffer and reply buffer (the first runs
  * eagerly and sizes the context buffers; any option change or buffer growth re-captures;
  * with timing on, replies run eagerly).  No reference counterpart; every setting yields
- * the same reply bytes. */
+ * the same reply bytes.  "group_budget" = device bytes a batch's levels 1..d-1 may take
+ * at once (requests are grouped under it; 0 = 24 GiB). */
 int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value);
 
 /* device memory plumbing for callers without a GPU runtime of their own */

@@ -22,9 +22,12 @@ def _setup_db(ctx, orc, prm, n, L, dims, seed):
     [(60, 256, [10, 6], 5), (35, 1024, [7, 5], 3), (48, 4096, [8, 6], 9)],
     [(400, 300, [20, 20], 12)],
     [(90, 512, [90], 4), (30, 2000, [30], 2)],       # d = 1 batches
+    [(96, 200, [8, 4, 3], 20)],                       # d = 3: levels 1 and 2 grouped over requests
 ])
-def test_batch_equals_plain_and_oracle(gpu_ctx, specs):
+@pytest.mark.parametrize("budget", [0, 1])             # 1 byte: one request per level group
+def test_batch_equals_plain_and_oracle(gpu_ctx, specs, budget):
     import xpir_amd
+    gpu_ctx.set_option("group_budget", budget)
     rng = np.random.default_rng(len(specs))
     dbs, reqs, expect = [], [], []
     for si, (n, L, dims, nq) in enumerate(specs):
@@ -71,3 +74,4 @@ def test_batch_equals_plain_and_oracle(gpu_ctx, specs):
         rd.free()
     for db in dbs:
         db.free()
+    gpu_ctx.set_option("group_budget", 0)

@@ -339,6 +339,10 @@ const char *xpg_last_error(const xpg_ctx *ctx) { return ctx ? ctx->err.c_str() :
 void *xpg_ctx_stream(xpg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 static int set_option(xpg_ctx *ctx, const char *key, int64_t value) {
+  if (strcmp(key, "group_budget") == 0 && value >= 0) {
+    ctx->group_budget = (uint64_t)value;
+    return XPG_OK;
+  }
   if (strcmp(key, "graphs") == 0 && (value == 0 || value == 1)) {
     ctx->graphs = value != 0;
     if (!ctx->graphs) graph_cache_clear(ctx, 0);
@@ -662,6 +666,9 @@ int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uin
 // remaining levels.  Worth it when level 0 has enough chains to fill the GPU per query
 // group (the batched kernel runs one chain per lane).
 static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs);
+// device bytes the grouped levels of one batch may take (level databases + outputs);
+// option "group_budget" overrides it
+static const uint64_t kGroupedBudget = 24ull << 30;
 
 // Whether a batch of qn queries on the database's level 0 runs as one batched launch:
 // the tuning table's measurement when it has one for this shape, else a fixed rule
@@ -713,12 +720,45 @@ static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const x
     op[i] = db->d == 1 ? reqs[i]->reply_dev : ctx->batch_buf + i * lvl0_words;
   }
   int rc = launch_mac_batch(ctx, prm, db->words, qp.data(), op.data(), qn, db->dims[0], groups0, db->P0);
-  if (rc) return rc;
-  if (db->d > 1) {
-    for (uint32_t i = 0; i < qn; i++) {
-      rc = run_levels(ctx, db, 1, nullptr, reqs[i]->query_dev + (uint64_t)db->dims[0] * 2 * mn, op[i],
-                      reqs[i]->reply_dev);
+  if (rc || db->d == 1) return rc;
+  // Levels 1..d-1 of all the chunk's requests together: per level, one recursion import
+  // over every request's previous-level reply (they lie side by side) and one MAC launch
+  // over the requests' level databases side by side, each request reading its own query.
+  // Requests are taken in sub-chunks whose level buffers fit kGroupedBudget.
+  const uint32_t d = db->d;
+  size_t per_db = 0, per_out = 0;
+  for (uint32_t l = 1; l < d; l++) {
+    per_db = std::max<size_t>(per_db, prod_dims(d, db->dims, l) * db->Pl[l] * mn * 8);
+    if (l + 1 < d) per_out = std::max<size_t>(per_out, prod_dims(d, db->dims, l + 1) * db->Pl[l] * 2 * mn * 8);
+  }
+  const uint64_t budget = ctx->group_budget ? ctx->group_budget : kGroupedBudget;
+  const uint32_t sub = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(qn, budget / (per_db + per_out)));
+  for (uint32_t s0 = 0; s0 < qn; s0 += sub) {
+    const uint32_t cnt = std::min<uint32_t>(sub, qn - s0);
+    rc = ensure_level_buf(ctx, 1, cnt * per_db);
+    if (!rc && per_out) rc = ensure_level_buf(ctx, 0, cnt * per_out);
+    if (rc) return rc;
+    const uint64_t *prev = ctx->batch_buf + s0 * lvl0_words;
+    uint64_t qoff = (uint64_t)db->dims[0] * 2 * mn;
+    std::vector<const uint64_t *> qs(cnt);
+    std::vector<uint64_t *> outs(cnt);
+    for (uint32_t l = 1; l < d; l++) {
+      const uint64_t groups_prev = prod_dims(d, db->dims, l), groups_l = prod_dims(d, db->dims, l + 1);
+      const uint64_t unit = (uint64_t)db->Pl[l - 1] * 2 * mn * 8;
+      const uint64_t units = cnt * groups_prev;
+      ByteSrc src{(const uint8_t *)prev, unit, unit, units * unit};
+      rc = launch_import_bytes(ctx, prm, src, units, units, db->Pl[l], ctx->level_buf[1]);
       if (rc) return rc;
+      for (uint32_t i = 0; i < cnt; i++) {
+        qs[i] = reqs[s0 + i]->query_dev + qoff;
+        outs[i] = reqs[s0 + i]->reply_dev;
+      }
+      const bool last = l + 1 == d;
+      rc = launch_mac_level_multi(ctx, prm, ctx->level_buf[1], qs.data(), cnt, db->dims[l], groups_l, db->Pl[l],
+                                  last ? outs.data() : nullptr, last ? nullptr : ctx->level_buf[0]);
+      if (rc) return rc;
+      prev = ctx->level_buf[0];
+      qoff += (uint64_t)db->dims[l] * 2 * mn;
     }
   }
   return XPG_OK;

@@ -87,8 +87,6 @@ struct xpg_ctx {
   uint64_t seg_capacity = 0;
   uint64_t *batch_buf = nullptr;  // level-0 replies of a batch (d > 1)
   size_t batch_bytes = 0;
-  void **ptr_buf = nullptr;       // device arrays of query / output pointers
-  size_t ptr_count = 0;  // status words of the segmented MAC kept zero in scratch
   uint64_t *level_buf[2] = {nullptr, nullptr};
   size_t level_bytes[2] = {0, 0};
   uint64_t *io_buf = nullptr;     // device copies of a host query and its reply (xpg_reply)
@@ -102,6 +100,7 @@ struct xpg_ctx {
   int stream_e = 0, stream_u = 0;  // streaming MAC: elements per block, per load chunk (0 = default)
   int stream_pf = 0;               // streaming MAC: double-buffered chunk loads
   int cu_count = 256;              // compute units of the device
+  uint64_t group_budget = 0;       // bytes for a batch's grouped levels (0 = default)
   // MAC timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -182,6 +181,12 @@ int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
 int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                          uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P, int gr,
                          int qr);
+// One level of up to 64 requests at once: their level databases side by side (request i's
+// groups are groups [i*gpr, (i+1)*gpr) of db), request i reading query q[i]; outputs to
+// out_per_req[i] (each request's own buffer) when given, else contiguous in out.
+int launch_mac_level_multi(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
+                           uint32_t nreq, uint64_t n_l, uint64_t gpr, uint32_t P, uint64_t *const *out_per_req,
+                           uint64_t *out);
 // what launch_mac_level runs for this shape with the context's options and no tuning
 LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P);
 bool seg_shape_built(int S, int W);     // segmented tile shapes compiled in

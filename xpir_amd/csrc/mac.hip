@@ -27,27 +27,14 @@ struct MacArgs {
   uint32_t P, n, m, logn;
 };
 
-// One lane = both chains (a and b) of one (g, k, c, s): x is loaded once per element.
-// Lanes with consecutive s read consecutive words, so every load is a coalesced 512 B
-// row per wavefront.  The query quotients are derived in registers.
-__global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
-  const uint64_t mn = (uint64_t)a.m * a.n;
-  const uint64_t lane = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  const uint64_t total = a.groups * a.P * mn;
-  if (lane >= total) return;
-  const uint64_t cs = lane % mn;          // c*n + s
-  const uint32_t c = (uint32_t)(cs >> a.logn);
-  const uint64_t gk = lane / mn;
-  const uint32_t k = (uint32_t)(gk % a.P);
-  const uint64_t g = gk / a.P;
-  const uint64_t p = a.ms.p[c];
-  const uint32_t delta = a.ms.delta[c];
-  const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
-  const uint64_t *x = a.db + (g * a.n_l * a.P + k) * mn + cs;
-  const uint64_t *ya = a.q + cs, *yb = a.q + mn + cs;
-  uint64_t acc_a = 0, acc_b = 0;
+// Both chains (a and b) of one lane over n_l elements, in element order: x from the
+// database column, ya / yb from the query's a and b halves (strides in words).
+__device__ __forceinline__ void direct_chains(const uint64_t *x, const uint64_t *ya, const uint64_t *yb,
+                                              uint64_t n_l, uint64_t xstride, uint64_t qstride, uint64_t p,
+                                              uint32_t delta, uint64_t &acc_a, uint64_t &acc_b) {
+  acc_a = acc_b = 0;
   uint64_t e = 0;
-  for (; e + 4 <= a.n_l; e += 4) {
+  for (; e + 4 <= n_l; e += 4) {
     uint64_t xv[4], av[4], bv[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -62,12 +49,64 @@ __global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
       acc_b = mac_step_qf(acc_b, xv[i], bv[i], p, delta);
     }
   }
-  for (; e < a.n_l; e++) {
+  for (; e < n_l; e++) {
     const uint64_t xv = x[e * xstride], av = ya[e * qstride], bv = yb[e * qstride];
     acc_a = mac_step_qf(acc_a, xv, av, p, delta);
     acc_b = mac_step_qf(acc_b, xv, bv, p, delta);
   }
+}
+
+// One lane = both chains (a and b) of one (g, k, c, s): x is loaded once per element.
+// Lanes with consecutive s read consecutive words, so every load is a coalesced 512 B
+// row per wavefront.  The query quotients are derived in registers.
+__global__ void __launch_bounds__(256) mac_direct_kernel(MacArgs a) {
+  const uint64_t mn = (uint64_t)a.m * a.n;
+  const uint64_t lane = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t total = a.groups * a.P * mn;
+  if (lane >= total) return;
+  const uint64_t cs = lane % mn;          // c*n + s
+  const uint32_t c = (uint32_t)(cs >> a.logn);
+  const uint64_t gk = lane / mn;
+  const uint32_t k = (uint32_t)(gk % a.P);
+  const uint64_t g = gk / a.P;
+  const uint64_t xstride = (uint64_t)a.P * mn;
+  uint64_t acc_a, acc_b;
+  direct_chains(a.db + (g * a.n_l * a.P + k) * mn + cs, a.q + cs, a.q + mn + cs, a.n_l, xstride, 2 * mn,
+                a.ms.p[c], a.ms.delta[c], acc_a, acc_b);
   uint64_t *o = a.out + gk * 2 * mn + cs;
+  o[0] = acc_a;
+  o[mn] = acc_b;
+}
+
+// Several requests' levels in one launch: group G of the launch belongs to request
+// G / gpr and reads that request's query (q[req]); its output goes to out[req] (the
+// request's own buffer, group G mod gpr) when per_req_out, else to base.out + G.
+constexpr int kMaxReqs = 64;
+struct MultiArgs {
+  MacArgs base;
+  const uint64_t *q[kMaxReqs];
+  uint64_t *out[kMaxReqs];
+  uint64_t gpr;          // groups per request
+  uint32_t per_req_out;
+};
+
+__global__ void __launch_bounds__(256) mac_direct_multi_kernel(MultiArgs m) {
+  const MacArgs &a = m.base;
+  const uint64_t mn = (uint64_t)a.m * a.n;
+  const uint64_t lane = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t total = a.groups * a.P * mn;
+  if (lane >= total) return;
+  const uint64_t cs = lane % mn;
+  const uint32_t c = (uint32_t)(cs >> a.logn);
+  const uint64_t gk = lane / mn;
+  const uint32_t k = (uint32_t)(gk % a.P);
+  const uint64_t g = gk / a.P;
+  const uint64_t req = g / m.gpr;
+  const uint64_t *q = m.q[req];
+  uint64_t acc_a, acc_b;
+  direct_chains(a.db + (g * a.n_l * a.P + k) * mn + cs, q + cs, q + mn + cs, a.n_l, (uint64_t)a.P * mn, 2 * mn,
+                a.ms.p[c], a.ms.delta[c], acc_a, acc_b);
+  uint64_t *o = m.per_req_out ? m.out[req] + (gk - req * m.gpr * a.P) * 2 * mn + cs : a.out + gk * 2 * mn + cs;
   o[0] = acc_a;
   o[mn] = acc_b;
 }
@@ -988,4 +1027,39 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   timing_end(ctx, k0, k1, 1);
   timing_end(ctx, e0, e1);
   return rc;
+}
+
+int launch_mac_level_multi(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
+                           uint32_t nreq, uint64_t n_l, uint64_t gpr, uint32_t P, uint64_t *const *out_per_req,
+                           uint64_t *out) {
+  if (nreq == 0 || nreq > (uint32_t)kMaxReqs) return set_err(ctx, XPG_EINVAL, "1..64 requests per grouped launch");
+  MultiArgs m{};
+  m.base.ms = modset_of(prm);
+  m.base.db = db;
+  m.base.q = nullptr;
+  m.base.out = out;
+  m.base.n_l = n_l;
+  m.base.groups = gpr * nreq;
+  m.base.P = P;
+  m.base.n = prm->n;
+  m.base.m = prm->m;
+  m.base.logn = prm->logn;
+  for (uint32_t i = 0; i < nreq; i++) {
+    m.q[i] = q[i];
+    m.out[i] = out_per_req ? out_per_req[i] : nullptr;
+  }
+  m.gpr = gpr;
+  m.per_req_out = out_per_req ? 1u : 0u;
+  const uint64_t lanes = m.base.groups * P * (uint64_t)prm->m * prm->n;
+  const uint64_t blocks = (lanes + 255) / 256;
+  if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  timing_begin(ctx, &e0, &e1);
+  hipEvent_t k0 = nullptr, k1 = nullptr;
+  timing_begin(ctx, &k0, &k1);
+  hipLaunchKernelGGL(mac_direct_multi_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, m);
+  timing_end(ctx, k0, k1, 1);
+  timing_end(ctx, e0, e1);
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "grouped MAC launch");
+  return XPG_OK;
 }
