@@ -26,6 +26,9 @@ def test_import_argument_errors(gpu_ctx):
     _raises(lambda: gpu_ctx.import_db(prm, data, 8, 64, [8, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1]), -1)  # d > 10
     prm7 = xpir_amd.params(C2, 7)
     _raises(lambda: gpu_ctx.import_db(prm7, data, 8, 64, [7]), -1)               # grid too small
+    empty = np.zeros(0, dtype=np.uint8)
+    _raises(lambda: gpu_ctx.import_db(prm, empty, 0, 64, [8]), -1)                # no elements
+    _raises(lambda: gpu_ctx.import_db(prm, empty, 8, 0, [8]), -1)                 # empty elements
     # the context still works
     db = gpu_ctx.import_db(prm, data, 8, 64, [8])
     assert db.polys == 1
@@ -44,5 +47,21 @@ def test_reply_argument_errors(gpu_ctx):
         _raises(lambda: gpu_ctx.set_option("no_such_knob", 1), -1)
         rep = gpu_ctx.reply(db, np.zeros(db.query_words, dtype=np.uint64))          # still usable
         assert rep.size == db.reply_words
+    finally:
+        db.free()
+
+
+def test_empty_batch_and_bad_requests(gpu_ctx):
+    import xpir_amd
+    gpu_ctx.reply_batch_dev([])                                                     # nothing to do
+    prm = xpir_amd.params(C2, 8)
+    db = gpu_ctx.import_db(prm, np.arange(8 * 64, dtype=np.uint8), 8, 64, [8])
+    try:
+        lib = xpir_amd.capi.lib()
+        req = (xpir_amd.capi.XpgRequest * 1)()
+        req[0].db, req[0].query_dev, req[0].reply_dev = db.h, None, None
+        assert lib.xpg_reply_batch_dev(gpu_ctx.h, req, 1) == -1                    # null buffers
+        assert lib.xpg_autotune(gpu_ctx.h, db.h, 65, None, 0, None) == -1          # batch above 64
+        assert lib.xpg_autotune(gpu_ctx.h, None, 0, None, 0, None) == -1
     finally:
         db.free()

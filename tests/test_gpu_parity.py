@@ -183,3 +183,36 @@ def test_recursive_full_size_decrypts(gpu_ctx):
     for b in (skd, qd, rd):
         b.free()
     db.free()
+
+
+@pytest.mark.parametrize("desc,n_elems,elem_bytes,dims", [
+    ("LWE:80:512:60", 20, 700, [20]),          # smallest ring the reference allows (kMinPolyDegree)
+    ("LWE:80:512:60", 24, 300, [6, 4]),
+    ("LWE:128:8192:240", 6, 20000, [6]),       # largest ring and four moduli (A = 105: two sub-chunks)
+    ("LWE:128:8192:240", 8, 5000, [4, 2]),
+])
+def test_extreme_ring_sizes_match_oracle(gpu_ctx, desc, n_elems, elem_bytes, dims):
+    """n = 512 (1 modulus) and n = 8192 (4 moduli): import words, reply words and the
+    decrypted element, against the oracle."""
+    import xpir_amd
+    orc = Oracle(desc, dims[0])
+    prm = xpir_amd.params(desc, dims[0])
+    assert (prm.n, prm.m, prm.A) == (orc.n, orc.m, orc.A)
+    data = _db_bytes(orc, 7, n_elems, elem_bytes)
+    units = int(np.prod(dims))
+    words, P = _oracle_import(orc, data, n_elems, elem_bytes, 1, units)
+    db = gpu_ctx.import_db(prm, data, n_elems, elem_bytes, dims)
+    try:
+        assert db.polys == P
+        assert np.array_equal(db.export_words(), words)
+        sk = orc.keygen(8)
+        index = n_elems - 2
+        qlevels = orc.make_query(sk, dims, index, 9)
+        want, Pd = orc.generate_reply(words, P, dims, qlevels)
+        got = gpu_ctx.reply(db, np.concatenate(qlevels))
+        assert db.reply_polys == Pd
+        assert np.array_equal(got, want)
+        assert extract_reply(orc, sk, got, dims, elem_bytes) == \
+            data[index * elem_bytes:(index + 1) * elem_bytes].tobytes()
+    finally:
+        db.free()
