@@ -169,9 +169,16 @@ def main():
     t0 = time.perf_counter()
     db = ctx.import_db_dev(prm, dbb, n, L, dims)
     ctx.sync()
+    import_first_s = time.perf_counter() - t0
+    # again, with the context's NTT tables for (n, m) already built: the per-database cost
+    t0 = time.perf_counter()
+    db2 = ctx.import_db_dev(prm, dbb, n, L, dims)
+    ctx.sync()
     import_s = time.perf_counter() - t0
+    db2.free()
     dbb.free()
-    log(f"import (absorb + NTT) {import_s * 1e3:.1f} ms -> {raw / import_s / 1e9:.2f} GB/s raw")
+    log(f"import (absorb + NTT) {import_s * 1e3:.1f} ms -> {raw / import_s / 1e9:.2f} GB/s raw "
+        f"(first import, building the NTT tables: {import_first_s * 1e3:.1f} ms)")
 
     sk = xpir_amd.keygen(prm, cfg["key_seed"])
     skd = ctx.alloc(sk.nbytes).upload(sk)
@@ -273,7 +280,7 @@ def main():
                    "query_bytes": q_bytes, "ntt_db_bytes": ntt_bytes},
         "bit_exact": bool(bit_exact),
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
-        "import_gbs": raw / import_s / 1e9,
+        "import_gbs": raw / import_s / 1e9, "import_first_ms": import_first_s * 1e3,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_all_cores": cpu_all,
