@@ -129,7 +129,7 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="recursive/batched/serving: skip the MAC autotuner (fixed kernel rules)")
+                    help="skip the MAC autotuner (fixed kernel rules); it runs before any timed region")
     ap.add_argument("--tune-file", help="install this saved tuning table instead of tuning when it exists; "
                                         "else tune and save the table there")
     ap.add_argument("--host-io", action="store_true",
@@ -180,6 +180,7 @@ def main():
     log(f"import (absorb + NTT) {import_s * 1e3:.1f} ms -> {raw / import_s / 1e9:.2f} GB/s raw "
         f"(first import, building the NTT tables: {import_first_s * 1e3:.1f} ms)")
 
+    tuned = autotune_db(ctx, db, 0, args, cfg["name"])
     sk = xpir_amd.keygen(prm, cfg["key_seed"])
     skd = ctx.alloc(sk.nbytes).upload(sk)
     qwords = db.query_words
@@ -227,15 +228,16 @@ def main():
     q_bytes = qwords * 8
     reply_bytes = db.reply_words * 8
     alg_bytes = ntt_bytes + q_bytes + reply_bytes
-    # the dominant kernel: the MAC's block-summary kernel, one launch per reply, timed with
-    # HIP events on the context stream; it reads every DB and query word once, and the
-    # reply words are written by the small scan kernel that follows it
+    # the dominant kernel: the MAC's main kernel (by default the block-summary kernel), one
+    # launch per reply, timed with HIP events on the context stream; it reads every DB and
+    # query word once, and the reply words are written by the small scan kernel after it
+    kern_name = main_kernel_name(tuned, dims[0])
     kern_s = main_ms / max(1, main_launches) / 1e3
     kern_bytes = ntt_bytes + q_bytes
     achieved = kern_bytes / kern_s / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["name"]),
-                "kernel": "mac_stream_kernel", "kernel_ms": main_ms / max(1, main_launches),
+                "kernel": kern_name, "kernel_ms": main_ms / max(1, main_launches),
                 "launches_per_reply": main_launches / args.steps, "alg_bytes": kern_bytes,
                 "reply_mac_ms": mac_ms / args.steps,
                 "reply_mac_gbs": alg_bytes / (mac_ms / args.steps / 1e3) / 1e9}
@@ -281,6 +283,7 @@ def main():
         "bit_exact": bool(bit_exact),
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "import_gbs": raw / import_s / 1e9, "import_first_ms": import_first_s * 1e3,
+        "autotune": tuned,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_all_cores": cpu_all,
@@ -292,6 +295,23 @@ def main():
         b.free()
     db.free()
     ctx.close()
+
+
+def main_kernel_name(tuned, n_l):
+    """The kernel a one-group level of n_l elements runs on: the tuning table's choice when
+    the autotuner recorded one, else the fixed rule's (streaming block summaries, chunks
+    of 8, for the single and plain workloads)."""
+    for e in (tuned or {}).get("entries", []):
+        if e["kind"] == 0 and e["n_l"] == n_l and e["groups"] == 1:
+            k, a, b = e["kernel"], e["a"], e["b"]
+            if k == 1:
+                return "mac_direct_kernel"
+            if k == 2:
+                return f"mac_seg_kernel<{a},{b}>"
+            if k == 3:
+                return f"mac_stream_kernel<{a or 8},{'true' if b else 'false'}>"
+            return f"mac_batch2_kernel<{a},{b}>"
+    return "mac_stream_kernel<8,false>"
 
 
 def autotune_db(ctx, db, max_batch, args, label):
