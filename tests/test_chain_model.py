@@ -59,6 +59,38 @@ def block_summary(rs, p):
     return rho, J, Jmax, Rs
 
 
+ZBIAS = 1 << 21  # blocks hold at most 2^20 elements: |Z| < 2^81 < ZBIAS * p
+
+
+def z_to_j_r(z, p):
+    """J = ceil(z / p) and R = J * p - z in [0, p), for |z| < ZBIAS * p, as the kernel
+    computes them: N = -z + ZBIAS * p >= 0 split as H * 2^61 + L = H * p + (H * delta + L)."""
+    delta = (1 << 61) - p
+    N = -z + ZBIAS * p
+    assert 0 <= N < 1 << 96
+    H, L = N >> 61, N & ((1 << 61) - 1)
+    S = H * delta + L
+    b = 1 if S >= p else 0
+    q = H + b - ZBIAS  # floor(-z / p)
+    return -q, S - b * p
+
+
+def block_summary_z(rs, p):
+    """The streaming kernel's per-step state: Z_t = t * p - A_t (= J_t * p - R_t), one add
+    of p - r per step, and its running maximum; (J, R) pairs recovered at the block end.
+    max_t Z_t is reached exactly where J_t = Jmax with the smallest residue, so it gives
+    (Jmax, Rstar)."""
+    z, zmax = 0, None
+    for r in rs:
+        z += p - r
+        assert -(1 << 95) <= z < 1 << 95  # the kernel's 96-bit word
+        if zmax is None or z > zmax:
+            zmax = z
+    Jend, rho0 = z_to_j_r(z, p)
+    Jmax, Rs = z_to_j_r(zmax, p)
+    return rho0, Jend, Jmax, Rs
+
+
 def streaming_chain(rs, p, cuts):
     """Streaming form: block summaries computed blind, then scanned with the residues."""
     rho_in, j = 0, 0
@@ -111,6 +143,23 @@ def test_segmented_and_streaming_forms_equal_sequential_chain(seed):
         want = sequential(rs, p)
         assert maxplus_chain(rs, p, cuts) == want
         assert streaming_chain(rs, p, cuts) == want
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_running_max_form_equals_block_summary(seed):
+    for p, rs, cuts in _chains(seed, 1500):
+        assert block_summary_z(rs, p) == block_summary(rs, p)
+    rng = random.Random(seed)
+    for p in MODULI:  # long blocks at the extremes of the 96-bit range
+        for rs in ([0] * 4096, [2 * p - 1] * 4096, [p] * 3000, [rng.choice([0, 2 * p - 1]) for _ in range(5000)]):
+            assert block_summary_z(rs, p) == block_summary(rs, p)
+
+
+def test_z_to_j_r_at_the_edges():
+    for p in MODULI:
+        for z in (0, 1, -1, p - 1, p, p + 1, -p, -p + 1, -p - 1, (1 << 81) - 1, -(1 << 81), 5 * p, -7 * p + 3):
+            J, R = z_to_j_r(z, p)
+            assert 0 <= R < p and J * p - z == R
 
 
 def test_special_form_step_is_division_by_p():

@@ -559,22 +559,41 @@ struct StreamArgs {
   int2 *jj;             // (Jend, Jmax), same layout
 };
 
+// Block summary of one chain read from zero: Z_t = t*p - A_t, A_t the prefix sum of its
+// lazy products (so Z_t = J_t*p - R_t with J_t = t - floor(A_t/p), R_t = A_t mod p), as a
+// 96-bit two's-complement word (low 64 bits z, signed high word zh), and its running
+// maximum (m, mh) over t = 1..E.  max_t Z_t is reached exactly where J_t is largest and,
+// among those t, R_t smallest, so it carries (Jmax, Rstar); a step is one 96-bit add and
+// one 96-bit compare, with no reduction modulo p.
 struct ChainSum {
-  uint64_t rho, rs;
-  int32_t J, Jmax;
+  uint64_t z, m;
+  int32_t zh, mh;
 };
 
-// one element of a block summary (chain from zero), given u = rho + r < 3p:
-// u = k * p + rho' with k = h + [v >= p], u = h * 2^61 + l, v = l + h * delta < 2p; J += 1 - k
-__device__ __forceinline__ void sum_step_u(ChainSum &s, uint64_t u, uint64_t p, uint32_t delta) {
-  const uint32_t h = (uint32_t)(u >> 61);
-  const uint64_t v = (u & ((1ULL << 61) - 1)) + (uint64_t)h * delta;
-  const bool b = v >= p;
-  s.rho = b ? v - p : v;
-  s.J += 1 - (int32_t)h - (b ? 1 : 0);
-  const bool take = s.J > s.Jmax || (s.J == s.Jmax && s.rho < s.rs);
-  s.Jmax = take ? s.J : s.Jmax;
-  s.rs = take ? s.rho : s.rs;
+// one element: Z += p - r (r < 2p), then the running maximum
+__device__ __forceinline__ void zsum_step(ChainSum &s, uint64_t r, uint64_t p) {
+  uint64_t u, v;
+  const bool c = __builtin_add_overflow(s.z, p, &u);
+  const bool b = __builtin_sub_overflow(u, r, &v);
+  s.zh += (int32_t)c - (int32_t)b;
+  s.z = v;
+  const bool take = s.zh > s.mh || (s.zh == s.mh && s.z > s.m);
+  s.mh = take ? s.zh : s.mh;
+  s.m = take ? s.z : s.m;
+}
+
+// J = ceil(Z/p) and R = J*p - Z in [0, p), for |Z| < 2^81 (blocks of at most 2^20
+// elements): N = -Z + 2^21*p >= 0 is split as H*2^61 + L = H*p + (H*delta + L), and
+// H*delta + L < 2p, so floor(N/p) = H + [H*delta + L >= p].
+__device__ __forceinline__ void z_to_j_r(uint64_t z, int32_t zh, uint64_t p, uint32_t delta, int32_t &J,
+                                         uint64_t &R) {
+  const __int128 Z = (__int128)(((unsigned __int128)(__int128)zh << 64) | z);
+  const unsigned __int128 N = (unsigned __int128)(((__int128)p << 21) - Z);
+  const uint64_t H = (uint64_t)(N >> 61), L = (uint64_t)N & ((1ULL << 61) - 1);
+  const uint64_t S = H * delta + L;
+  const uint64_t b = S >= p ? 1 : 0;
+  J = (int32_t)((1 << 21) - (int64_t)(H + b));
+  R = S - b * p;
 }
 
 // Loads of one chunk of U elements (x, a, b words of this lane); the elements at or past
@@ -607,15 +626,15 @@ __device__ __forceinline__ void stream_sum(ChainSum &sa, ChainSum &sb, int valid
   if (valid >= U) {
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      sum_step_u(sa, add_shoup_lazy_qf(sa.rho, xv[j], av[j], p, delta), p, delta);
-      sum_step_u(sb, add_shoup_lazy_qf(sb.rho, xv[j], bv[j], p, delta), p, delta);
+      zsum_step(sa, shoup_lazy_qf(xv[j], av[j], p, delta), p);
+      zsum_step(sb, shoup_lazy_qf(xv[j], bv[j], p, delta), p);
     }
   } else {
 #pragma unroll
     for (int j = 0; j < U; j++) {
       if (j < valid) {
-        sum_step_u(sa, add_shoup_lazy_qf(sa.rho, xv[j], av[j], p, delta), p, delta);
-        sum_step_u(sb, add_shoup_lazy_qf(sb.rho, xv[j], bv[j], p, delta), p, delta);
+        zsum_step(sa, shoup_lazy_qf(xv[j], av[j], p, delta), p);
+        zsum_step(sb, shoup_lazy_qf(xv[j], bv[j], p, delta), p);
       }
     }
   }
@@ -641,7 +660,7 @@ __global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
   const int64_t n = (int64_t)(min<uint64_t>(a.n_l, e0 + a.E) - e0);  // elements in this block
   const uint64_t *xp = a.db + ((uint64_t)g * a.n_l + e0) * xstride + (uint64_t)cw * 64 + lane;
   const uint64_t *qp = a.q + e0 * qstride + (uint64_t)c * a.n + sb * 64 + lane;
-  ChainSum sa{0, 0, 0, kNegInf}, sbs{0, 0, 0, kNegInf};
+  ChainSum sa{0, 0, 0, INT32_MIN}, sbs{0, 0, 0, INT32_MIN};
   if (!PF) {
     // one register set: the chunk's loads, then its arithmetic (other waves fill the gap)
     for (int64_t e = 0; e < n; e += U) {
@@ -665,13 +684,20 @@ __global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
     stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e - U, U), x1, a1, b1, p, delta);
   }
   }
+  // (rho0, Jend) from Z at the block end, (Rstar, Jmax) from its maximum
+  int32_t ja, jma, jb, jmb;
+  uint64_t ra, rsa, rb, rsb;
+  z_to_j_r(sa.z, sa.zh, p, delta, ja, ra);
+  z_to_j_r(sa.m, sa.mh, p, delta, jma, rsa);
+  z_to_j_r(sbs.z, sbs.zh, p, delta, jb, rb);
+  z_to_j_r(sbs.m, sbs.mh, p, delta, jmb, rsb);
   const uint64_t i = ((uint64_t)gw * 2) * 64 + lane;  // gw = blk * columns + column
-  a.rho0[i] = sa.rho;
-  a.rho0[i + 64] = sbs.rho;
-  a.rstar[i] = sa.rs;
-  a.rstar[i + 64] = sbs.rs;
-  a.jj[i] = make_int2(sa.J, sa.Jmax);
-  a.jj[i + 64] = make_int2(sbs.J, sbs.Jmax);
+  a.rho0[i] = ra;
+  a.rho0[i + 64] = rb;
+  a.rstar[i] = rsa;
+  a.rstar[i + 64] = rsb;
+  a.jj[i] = make_int2(ja, jma);
+  a.jj[i + 64] = make_int2(jb, jmb);
 }
 
 // Scan the block summaries of every chain in element order and emit the reply words.
@@ -836,6 +862,7 @@ int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t gro
   st.columns = (uint32_t)(groups * st.cpg);
   if (U != 4 && U != 8 && U != 16) return set_err(ctx, XPG_EINVAL, "streaming MAC chunk not built");
   if (!E) E = auto_e(ctx->cu_count, a.n_l, st.columns, U);
+  E = std::min<uint64_t>(E, 1u << 20);  // the block summary's 96-bit word holds 2^20 steps
   E = (E + U - 1) / U * U;
   if (E > 0xffffffffULL || (a.n_l + E - 1) / E > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.E = (uint32_t)E;
