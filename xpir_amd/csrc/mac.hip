@@ -582,6 +582,34 @@ __device__ __forceinline__ void zsum_step(ChainSum &s, uint64_t r, uint64_t p) {
   s.m = take ? s.z : s.m;
 }
 
+// Z += d for a signed 64-bit d (96-bit add: low word with carry, high word plus the carry
+// and d's sign extension)
+__device__ __forceinline__ void add96(uint64_t z, int32_t zh, int64_t d, uint64_t &oz, int32_t &ozh) {
+  uint64_t u;
+  const bool c = __builtin_add_overflow(z, (uint64_t)d, &u);
+  ozh = zh + (int32_t)c + (d < 0 ? -1 : 0);
+  oz = u;
+}
+
+// four elements at once: with s_k = sum_{j<=k} (p - r_j) (|s_k| <= 4p < 2^63), the four
+// new Z values are Z + s_k, so their maximum is Z + max_k s_k: one 96-bit compare per
+// four steps instead of four
+__device__ __forceinline__ void zsum_group4(ChainSum &s, uint64_t r0, uint64_t r1, uint64_t r2, uint64_t r3,
+                                            uint64_t pv) {
+  const int64_t s1 = (int64_t)(pv - r0);
+  const int64_t s2 = s1 + (int64_t)(pv - r1);
+  const int64_t s3 = s2 + (int64_t)(pv - r2);
+  const int64_t s4 = s3 + (int64_t)(pv - r3);
+  const int64_t mx = max(max(s1, s2), max(s3, s4));
+  uint64_t cl;
+  int32_t ch;
+  add96(s.z, s.zh, mx, cl, ch);
+  const bool take = ch > s.mh || (ch == s.mh && cl > s.m);
+  s.mh = take ? ch : s.mh;
+  s.m = take ? cl : s.m;
+  add96(s.z, s.zh, s4, s.z, s.zh);
+}
+
 // J = ceil(Z/p) and R = J*p - Z in [0, p), for |Z| < 2^81 (blocks of at most 2^20
 // elements): N = -Z + 2^21*p >= 0 is split as H*2^61 + L = H*p + (H*delta + L), and
 // H*delta + L < 2p, so floor(N/p) = H + [H*delta + L >= p].
@@ -622,12 +650,19 @@ __device__ __forceinline__ void stream_load(const uint64_t *xp, const uint64_t *
 template <int U>
 __device__ __forceinline__ void stream_sum(ChainSum &sa, ChainSum &sb, int valid, const uint64_t (&xv)[U],
                                            const uint64_t (&av)[U], const uint64_t (&bv)[U], uint64_t p,
-                                           uint32_t delta) {
+                                           uint64_t pv, uint32_t delta) {
   if (valid >= U) {
+    static_assert(U % 4 == 0, "chunks are whole groups of four elements");
 #pragma unroll
-    for (int j = 0; j < U; j++) {
-      zsum_step(sa, shoup_lazy_qf(xv[j], av[j], p, delta), p);
-      zsum_step(sb, shoup_lazy_qf(xv[j], bv[j], p, delta), p);
+    for (int j = 0; j < U; j += 4) {
+      uint64_t ra[4], rb[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        ra[i] = shoup_lazy_qf(xv[j + i], av[j + i], p, delta);
+        rb[i] = shoup_lazy_qf(xv[j + i], bv[j + i], p, delta);
+      }
+      zsum_group4(sa, ra[0], ra[1], ra[2], ra[3], pv);
+      zsum_group4(sb, rb[0], rb[1], rb[2], rb[3], pv);
     }
   } else {
 #pragma unroll
@@ -653,6 +688,8 @@ __global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
   const uint32_t g = column / a.cpg, cw = column - g * a.cpg;
   const uint32_t c = (cw >> a.log_sb) % a.m, sb = cw & (a.sblocks - 1);
   const uint64_t p = a.ms.p[c];
+  uint64_t pv = p;
+  asm volatile("" : "+v"(pv));  // the modulus in VGPRs too, for the p - r of every step
   const uint32_t delta = a.ms.delta[c];
   const uint64_t mn = (uint64_t)a.m * a.n;
   const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
@@ -667,7 +704,7 @@ __global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
       uint64_t xv[U], av[U], bv[U];
       const int valid = (int)min<int64_t>(n - e, U);
       stream_load<U>(xp + e * xstride, qp + e * qstride, xstride, qstride, mn, valid, xv, av, bv);
-      stream_sum<U>(sa, sbs, valid, xv, av, bv, p, delta);
+      stream_sum<U>(sa, sbs, valid, xv, av, bv, p, pv, delta);
     }
   } else {
   uint64_t x0[U], a0[U], b0[U], x1[U], a1[U], b1[U];
@@ -676,12 +713,12 @@ __global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
     if (e + U < n)
       stream_load<U>(xp + (e + U) * xstride, qp + (e + U) * qstride, xstride, qstride, mn,
                      (int)min<int64_t>(n - e - U, U), x1, a1, b1);
-    stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e, U), x0, a0, b0, p, delta);
+    stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e, U), x0, a0, b0, p, pv, delta);
     if (e + U >= n) break;
     if (e + 2 * U < n)
       stream_load<U>(xp + (e + 2 * U) * xstride, qp + (e + 2 * U) * qstride, xstride, qstride, mn,
                      (int)min<int64_t>(n - e - 2 * U, U), x0, a0, b0);
-    stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e - U, U), x1, a1, b1, p, delta);
+    stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e - U, U), x1, a1, b1, p, pv, delta);
   }
   }
   // (rho0, Jend) from Z at the block end, (Rstar, Jmax) from its maximum
