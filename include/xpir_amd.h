@@ -92,7 +92,10 @@ int xpg_d2h(xpg_ctx *ctx, void *dst, const void *src_dev, uint64_t bytes);
  * alpha elements form one import unit (DBHandler::readAggregatedStream,
  * DBHandler.cpp:3-26).  dims[0..d-1] is the query grid; prod(dims) >= units; the grid's
  * missing units are zero polys (PIRReplyGeneratorNFL_internal.cpp:146-154).
- * The NTT-domain database stays resident in HBM until xpg_db_free. */
+ * The NTT-domain database stays resident in HBM until xpg_db_free, also past
+ * xpg_ctx_destroy of the importing context, and serves replies on any context of the
+ * same GPU (an imported_database outlives its PIRReplyGenerator in the reference's
+ * keep-database sessions, apps/server/PIRServer.cpp:73-96). */
 int xpg_db_import(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems, uint64_t n_elems,
                   uint64_t elem_bytes, uint32_t alpha, uint32_t d, const uint32_t *dims,
                   xpg_db **out);

@@ -65,3 +65,28 @@ def test_empty_batch_and_bad_requests(gpu_ctx):
         assert lib.xpg_autotune(gpu_ctx.h, None, 0, None, 0, None) == -1
     finally:
         db.free()
+
+
+def test_database_outlives_its_context():
+    """A database stays resident after the context that imported it is destroyed, serves
+    replies on another context of the same GPU, and frees cleanly afterwards (the
+    reference keeps an imported_database past its session's reply generator)."""
+    import xpir_amd
+    from oracle.oracle import Oracle
+    prm = xpir_amd.params(C2, 8)
+    orc = Oracle(C2, 8)
+    rng = np.random.default_rng(11)
+    data = rng.integers(0, 256, 8 * 3000, dtype=np.uint8)
+    q = np.concatenate([rng.integers(0, p, orc.n, dtype=np.uint64)
+                        for _ in range(8) for _ in range(2) for p in orc.p])
+    first = xpir_amd.Context(0)
+    db = first.import_db(prm, data, 8, 3000, [8])
+    want = first.reply(db, q)
+    first.close()
+    with xpir_amd.Context(0) as second:
+        got = second.reply(db, q)
+        db.free()
+        again = second.import_db(prm, data, 8, 3000, [8])
+        assert np.array_equal(second.reply(again, q), want)
+        again.free()
+    assert np.array_equal(got, want)

@@ -317,6 +317,7 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   timing_collect(ctx);
+  for (xpg_db *db : ctx->dbs) db->ctx = nullptr;  // they stay resident until xpg_db_free
   for (auto &kv : ctx->tables) {
     for (uint64_t *b : {kv.second.phi, kv.second.phiq, kv.second.w, kv.second.wq, kv.second.iw, kv.second.iwq,
                         kv.second.ipt, kv.second.iptq})
@@ -429,6 +430,7 @@ static int db_setup(xpg_ctx *ctx, const xpg_params *prm, uint64_t n_elems, uint6
   if (units < real_units) return set_err(ctx, XPG_EINVAL, "grid smaller than the database");
   xpg_db *db = new xpg_db();
   db->ctx = ctx;
+  db->device = ctx->device;
   db->serial = ++g_db_serial;
   db->prm = *prm;
   db->d = d;
@@ -450,6 +452,7 @@ static int db_setup(xpg_ctx *ctx, const xpg_params *prm, uint64_t n_elems, uint6
     delete db;
     return set_err(ctx, XPG_ENOMEM, "NTT database allocation");
   }
+  ctx->dbs.insert(db);
   *out = db;
   return XPG_OK;
 }
@@ -529,10 +532,11 @@ uint64_t xpg_db_query_words(const xpg_db *db) {
 
 void xpg_db_free(xpg_db *db) {
   if (!db) return;
+  (void)hipSetDevice(db->device);
   if (db->ctx) {
-    (void)hipSetDevice(db->ctx->device);
     (void)hipStreamSynchronize(db->ctx->stream);
     graph_cache_clear(db->ctx, db->serial);  // other contexts' entries never match again
+    db->ctx->dbs.erase(db);
   }
   if (db->words) (void)hipFree(db->words);
   delete db;
@@ -657,7 +661,7 @@ static int reply_graph(xpg_ctx *ctx, const xpg_db *db, const uint64_t *q, uint64
 
 int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev) {
   if (!ctx || !db || !query_dev || !reply_dev) return set_err(ctx, XPG_EINVAL, "null argument");
-  if (db->ctx->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
+  if (db->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
   if (ctx->graphs && !ctx->timing) return reply_graph(ctx, db, query_dev, reply_dev);
   return run_levels(ctx, db, 0, db->words, query_dev, nullptr, reply_dev);
 }
@@ -771,7 +775,7 @@ int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs) 
   std::map<const xpg_db *, std::vector<const xpg_request *>> by_db;
   for (uint32_t i = 0; i < n_reqs; i++) {
     if (!reqs[i].db || !reqs[i].query_dev || !reqs[i].reply_dev) return set_err(ctx, XPG_EINVAL, "null request field");
-    if (reqs[i].db->ctx->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
+    if (reqs[i].db->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
     if (!by_db.count(reqs[i].db)) order.push_back(reqs[i].db);
     by_db[reqs[i].db].push_back(&reqs[i]);
   }

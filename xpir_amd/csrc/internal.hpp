@@ -6,6 +6,7 @@
 
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -45,7 +46,10 @@ struct EncSrc {
 };
 
 struct xpg_db {
+  // the importing context while it lives (xpg_ctx_destroy detaches its databases: a
+  // database outlives its context and serves any context of the same GPU)
   xpg_ctx *ctx = nullptr;
+  int device = 0;
   xpg_params prm{};
   uint32_t d = 1;
   uint32_t dims[XPG_MAX_LEVELS] = {0};
@@ -75,6 +79,7 @@ struct GraphEntry {
 struct xpg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  std::set<xpg_db *> dbs;  // live databases imported here
   std::string err;
   std::map<uint64_t, DevTables> tables;  // key (n << 8) | m
   // scratch reused across calls (grown on demand): block summaries of the streaming MAC

@@ -241,7 +241,7 @@ void xpg_server_destroy(xpg_server *s) {
 
 int xpg_server_submit(xpg_server *s, const xpg_db *db, const uint64_t *query_dev, uint64_t *ticket) {
   if (!s || !db || !query_dev || !ticket) return XPG_EINVAL;
-  if (db->ctx->device != s->device) return XPG_EINVAL;
+  if (db->device != s->device) return XPG_EINVAL;
   Req r{};
   r.db = db;
   r.query = query_dev;

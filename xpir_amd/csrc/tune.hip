@@ -317,7 +317,7 @@ int xpg_tune_clear(xpg_ctx *ctx) {
 int xpg_autotune(xpg_ctx *ctx, const xpg_db *db, uint32_t max_batch, xpg_tune_entry *out, uint32_t cap,
                  uint32_t *count) {
   if (!ctx || !db || (!out && cap)) return XPG_EINVAL;
-  if (db->ctx->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
+  if (db->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
   if (max_batch > 64) return set_err(ctx, XPG_EINVAL, "batches hold at most 64 queries");
   (void)hipSetDevice(ctx->device);
   TimingOff quiet(ctx);
