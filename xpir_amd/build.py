@@ -50,18 +50,24 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
-    # the C++ facade (reference-shaped server API) and its end-to-end demo program
+    # the C++ facade (reference-shaped server API and session protocol), the server
+    # program, and the end-to-end demo program
     facade = os.path.join(HERE, "libpir_amd.so")
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", facade + ".tmp",
-           os.path.join(HERE, "cpp", "libpir_amd.cpp"), "-L", HERE, "-lxpir_amd", "-Wl,-rpath,$ORIGIN"]
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", facade + ".tmp",
+           os.path.join(HERE, "cpp", "libpir_amd.cpp"), os.path.join(HERE, "cpp", "pir_session.cpp"),
+           "-L", HERE, "-lxpir_amd", "-Wl,-rpath,$ORIGIN"]
     _run(cmd)
     os.replace(facade + ".tmp", facade)
+    server = os.path.join(OBJ, "pir_server")
+    cmd = ["g++", "-O2", "-std=c++17", "-pthread", "-o", server, os.path.join(HERE, "cpp", "pir_server.cpp"),
+           "-L", HERE, "-lpir_amd", "-lxpir_amd", "-Wl,-rpath,$ORIGIN/.."]
+    _run(cmd)
     demo = os.path.join(OBJ, "simple_pir")
     cmd = ["g++", "-O2", "-std=c++17", "-o", demo, os.path.join(ROOT, "tests", "cpp", "simple_pir.cpp"),
            "-L", HERE, "-lpir_amd", "-lxpir_amd", "-Wl,-rpath,$ORIGIN/.."]
     _run(cmd)
     if verbose:
-        print("built", LIB, facade, demo)
+        print("built", LIB, facade, server, demo)
     return LIB
 
 

@@ -57,6 +57,21 @@ uint64_t DBVectorProcessor::readStream(uint64_t streamNb, char *buf, uint64_t si
 
 void DBVectorProcessor::closeStream(uint64_t streamNb) { offsets_.erase(streamNb); }
 
+std::string DBVectorProcessor::getCatalog(bool typeOfCatalog) {
+  if (!typeOfCatalog) return "1\n" + std::to_string(getNbStream()) + "\n" + std::to_string(max_bytes_) + "\n";
+  std::string buf = "0\n" + std::to_string(getNbStream()) + "\n";
+  for (const element_t &e : elements_) buf += e.name + "\n" + std::to_string(e.data_size) + "\n";
+  return buf;
+}
+
+std::string DBHandler::getCatalog(bool typeOfCatalog) {
+  const uint64_t nb = getNbStream(), size = getmaxFileBytesize();
+  if (!typeOfCatalog) return "1\n" + std::to_string(nb) + "\n" + std::to_string(size) + "\n";
+  std::string buf = "0\n" + std::to_string(nb) + "\n";
+  for (uint64_t i = 0; i < nb; i++) buf += std::to_string(i) + "\n" + std::to_string(size) + "\n";
+  return buf;
+}
+
 void DBHandler::readAggregatedStream(uint64_t streamNb, uint64_t alpha, uint64_t offset, uint64_t bytes_per_file,
                                      char *rawBits) {
   const uint64_t maxb = getmaxFileBytesize();

@@ -83,6 +83,11 @@ class DBHandler {
   virtual uint64_t readStream(uint64_t streamNb, char *buf, uint64_t size) = 0;
   virtual void closeStream(uint64_t streamNb) = 0;
   virtual ~DBHandler() {}
+  // The catalog a session sends its client (PIRSession.cpp:123-152): typeOfCatalog true
+  // lists the elements, "0\n<count>\n" then "<name>\n<size>\n" each; false is the compact
+  // "1\n<count>\n<maxFileBytesize>\n".  This default names the elements "0", "1", ...
+  // with size maxFileBytesize, as DBGenerator does (DBGenerator.cpp:29-34,46-63).
+  virtual std::string getCatalog(bool typeOfCatalog);
   // The alpha streams of aggregate `streamNb`, each read from `offset` and zero-padded to
   // min(bytes_per_file, maxFileBytesize - offset) bytes, back to back in rawBits; slots
   // past the last stream are zeroed (DBHandler.cpp:3-26, whose bzero starts at the last
@@ -100,6 +105,9 @@ class DBVectorProcessor : public DBHandler {
   bool openStream(uint64_t streamNb, uint64_t requested_offset) override;
   uint64_t readStream(uint64_t streamNb, char *buf, uint64_t size) override;
   void closeStream(uint64_t streamNb) override;
+  // element names and sizes (DBVectorProcessor.cpp:50-66; the compact form there lacks the
+  // newline after the count, which the client's parser needs: written with it here)
+  std::string getCatalog(bool typeOfCatalog) override;
 
  private:
   std::vector<element_t> &elements_;
@@ -136,7 +144,7 @@ class DBDirectoryProcessor : public DBHandler {
   ~DBDirectoryProcessor() override;
   // typeOfCatalog true: "0\n<count>\n" then "<name>\n<size>\n" per element; false:
   // "1\n<count>\n<maxFileBytesize>\n" (DBDirectoryProcessor.cpp:127-151)
-  std::string getCatalog(bool typeOfCatalog);
+  std::string getCatalog(bool typeOfCatalog) override;
   uint64_t getDBSizeBits() { return max_bytes_ * file_list_.size() * 8; }
   uint64_t getNbStream() override { return file_list_.size(); }
   uint64_t getmaxFileBytesize() override { return max_bytes_; }

@@ -1,0 +1,226 @@
+// pir_server.cpp -- the reference's PIR server program (apps/server/main.cpp,
+// apps/server/PIRServer.cpp) with the reply on the GPU.
+//
+// Same options and session sequence as the reference: -z/--driven [file] configures the
+// parameters with a first client-driven session, keeps that session's imported database
+// resident in HBM and saves the parameters, then serves every later client server-driven
+// from that database; -L/--load_file file serves server-driven with saved parameters; with
+// neither, every session is client-driven.  The database is the files of a directory
+// (DBDirectoryProcessor, -s N to split the first file), or --db-generator.
+//
+// The sockets come from outside: --listen-fd N accepts clients on an inherited listening
+// socket (a supervisor's, e.g. systemd socket activation), --fd N serves one client on
+// an inherited connected socket (inetd-style).  Sessions are served one after another;
+// the reference runs them on concurrent threads, which a single GPU would serialise.
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/socket.h>
+#include <unistd.h>
+
+#include <string>
+
+#include "libpir_amd.hpp"
+#include "pir_session.hpp"
+
+using namespace xpir_amd;
+
+namespace {
+
+const char *kDefaultParamsFile = "exp/PIRParams.cfg";  // apps/server/main.cpp:19
+
+void usage(const char *prog) {
+  fprintf(stderr,
+          "usage: %s (--listen-fd N | --fd N) [options]\n"
+          "  -z, --driven [file]            configure with the first client, save to file (default %s)\n"
+          "  -L, --load_file file           serve with the PIR parameters saved in file\n"
+          "  -s, --split_file N             split the first file of the database directory into N elements\n"
+          "  -n, --db-generator-files N     elements of the generated database (default 10)\n"
+          "  -l, --db-generator-filesize B  bytes per generated element (default 12800000)\n"
+          "      --db-generator             generate the database instead of reading the directory\n"
+          "      --db-dir DIR               database directory (default db/)\n"
+          "      --optim-dir DIR            optimizer cache directory (default exp/)\n"
+          "      --no-pipeline              import the database after the query upload\n"
+          "      --sessions K               exit after K client sessions (default: serve until killed)\n"
+          "      --device N                 GPU (default 0)\n",
+          prog, kDefaultParamsFile);
+}
+
+bool number(const char *s, uint64_t *out) {
+  if (!s || !*s) return false;
+  char *end = nullptr;
+  *out = strtoull(s, &end, 10);
+  return *end == '\0';
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  int driven = -1;
+  std::string params_file = kDefaultParamsFile, db_dir = "db/", optim_dir = "exp/";
+  uint64_t split = 1, gen_n = 10, gen_l = 12800000, sessions = 0, device = 0;
+  bool use_generator = false, no_pipeline = false;
+  long listen_fd = -1, conn_fd = -1;
+  for (int i = 1; i < argc; i++) {
+    const std::string a = argv[i];
+    auto value = [&](uint64_t *out) {
+      if (i + 1 >= argc || !number(argv[i + 1], out)) {
+        fprintf(stderr, "%s needs a number\n", a.c_str());
+        exit(2);
+      }
+      i++;
+    };
+    auto text = [&](std::string *out) {
+      if (i + 1 >= argc) {
+        fprintf(stderr, "%s needs a value\n", a.c_str());
+        exit(2);
+      }
+      *out = argv[++i];
+    };
+    uint64_t v = 0;
+    if (a == "-h" || a == "--help") {
+      usage(argv[0]);
+      return 0;
+    } else if (a == "-z" || a == "--driven") {
+      driven = 1;
+      if (i + 1 < argc && argv[i + 1][0] != '-') params_file = argv[++i];
+    } else if (a == "-L" || a == "--load_file") {
+      driven = 0;
+      text(&params_file);
+    } else if (a == "-s" || a == "--split_file") {
+      value(&split);
+    } else if (a == "-n" || a == "--db-generator-files") {
+      value(&gen_n);
+    } else if (a == "-l" || a == "--db-generator-filesize") {
+      value(&gen_l);
+    } else if (a == "--db-generator") {
+      use_generator = true;
+    } else if (a == "--db-dir") {
+      text(&db_dir);
+    } else if (a == "--optim-dir") {
+      text(&optim_dir);
+    } else if (a == "--no-pipeline") {
+      no_pipeline = true;
+    } else if (a == "--sessions") {
+      value(&sessions);
+    } else if (a == "--device") {
+      value(&device);
+    } else if (a == "--listen-fd") {
+      value(&v);
+      listen_fd = (long)v;
+    } else if (a == "--fd") {
+      value(&v);
+      conn_fd = (long)v;
+    } else if (a == "-p" || a == "--port") {
+      fprintf(stderr, "this server takes its socket from its supervisor: use --listen-fd or --fd\n");
+      return 2;
+    } else {
+      fprintf(stderr, "unknown option %s\n", a.c_str());
+      usage(argv[0]);
+      return 2;
+    }
+  }
+  if ((listen_fd < 0) == (conn_fd < 0)) {
+    usage(argv[0]);
+    return 2;
+  }
+  if (!db_dir.empty() && db_dir.back() != '/') db_dir += '/';
+  if (!optim_dir.empty() && optim_dir.back() != '/') optim_dir += '/';
+
+  // PIRServer::PIRServer (PIRServer.cpp:26-56)
+  DBHandler *db = nullptr;
+  if (use_generator) {
+    db = new DBGenerator(gen_n, gen_l);
+  } else {
+    DBDirectoryProcessor *d = split == 1 ? new DBDirectoryProcessor(db_dir) : new DBDirectoryProcessor(split, db_dir);
+    if (d->getErrorStatus()) {
+      fprintf(stderr, "pir_server: no database in %s\n", db_dir.c_str());
+      return 1;
+    }
+    db = d;
+  }
+  fprintf(stderr, "pir_server: database of %llu elements of %llu bytes\n", (unsigned long long)db->getNbStream(),
+          (unsigned long long)db->getmaxFileBytesize());
+
+  session_option_t opt;
+  opt.driven_mode = true;
+  PIRParameters saved{};
+  bool server_driven = false;
+  if (driven == 0) {  // PIRServer::readPIRParamsFromFile (:116-125)
+    if (readPIRParameters(saved, params_file)) {
+      fprintf(stderr, "PIR parameters configuration file not found, try to launch the server in driven mode.\n");
+      return 1;
+    }
+    server_driven = true;
+    opt.driven_mode = false;
+  }
+
+  // one connected client per iteration: accepted, or the inherited one once
+  bool inherited_used = false;
+  auto next_client = [&]() -> int {
+    if (conn_fd >= 0) {
+      if (inherited_used) return -1;
+      inherited_used = true;
+      return (int)conn_fd;
+    }
+    for (;;) {
+      const int c = accept((int)listen_fd, nullptr, nullptr);
+      if (c >= 0 || errno != EINTR) return c;
+    }
+  };
+  auto serve_one = [&](int fd, PIRSession **keep) -> bool {
+    FdStream stream(fd, /*owned=*/true);
+    PIRSession *s = new PIRSession(stream, db, (int)device);
+    s->setOptimDirectory(optim_dir);
+    s->no_pipeline(no_pipeline);
+    if (server_driven) s->setPIRParams(saved);
+    const bool dry_run = s->start(opt);
+    if (s->failed()) fprintf(stderr, "pir_server: session failed: %s\n", s->lastError().c_str());
+    else if (!dry_run) fprintf(stderr, "pir_server: session done, %llu reply ciphertexts\n",
+                               (unsigned long long)s->repliesSent());
+    if (keep)
+      *keep = s;
+    else
+      delete s;
+    return dry_run;
+  };
+
+  uint64_t served = 0;
+  if (driven == 1) {  // PIRServer::processDrivenSession (:70-104)
+    opt.driven_mode = true;
+    opt.keep_database = true;
+    PIRSession *s = nullptr;
+    for (;;) {
+      const int fd = next_client();
+      if (fd < 0) return 1;
+      delete s;
+      s = nullptr;
+      if (!serve_one(fd, &s)) break;  // dry runs loop
+    }
+    served++;
+    if (s->failed()) {
+      fprintf(stderr, "pir_server: the configuration session failed\n");
+      delete s;
+      return 1;
+    }
+    saved = s->getPIRParams();
+    opt.data = s->getSavedDatabase();
+    delete s;
+    opt.driven_mode = false;
+    opt.keep_database = false;
+    opt.got_preimported_database = opt.data != nullptr;
+    if (writePIRParameters(saved, params_file))
+      fprintf(stderr, "Unable to write PIR parameters configuration data, maybe target repertory doesn't exist ?\n");
+    server_driven = true;
+  }
+  // PIRServer::serve / handleAccept (:58-112)
+  while (!sessions || served < sessions) {
+    const int fd = next_client();
+    if (fd < 0) break;
+    if (!serve_one(fd, nullptr)) served++;
+  }
+  delete opt.data;
+  delete db;
+  return 0;
+}
