@@ -123,7 +123,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="single",
-                    choices=["single", "plain", "recursive", "batched", "serving", "copy"])
+                    choices=["single", "plain", "recursive", "batched", "serving", "session", "copy"])
     ap.add_argument("--scale", type=float, default=1.0, help="element-count scale of the batched/serving databases")
     ap.add_argument("--queries", type=int, default=2000, help="serving: queries in the Poisson stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
@@ -151,6 +151,8 @@ def main():
         return bench_copy(args)
     if args.workload == "recursive":
         return bench_recursive(args)
+    if args.workload == "session":
+        return bench_session(args)
 
     cfg = C2_SINGLE if args.workload == "single" else C1_PLAIN
     gold_path = os.path.join(ROOT, "tests", "golden", "golden.json")
@@ -434,6 +436,127 @@ def bench_recursive(args):
         b.free()
     db.free()
     ctx.close()
+
+
+def bench_session(args):
+    """Whole client sessions with the server program (xpir_amd/_build/pir_server) over a
+    UNIX socket, driven by the reference client's messages (xpir_amd/pir_client.py): the
+    single-query database (LWE:80:2048:120, 65536 x 4 KiB, one file that the server
+    splits into elements, its -s option) queried at d = 1 (4 GiB query) and at d = 2 over
+    256 x 256 (32 MiB query).  Per session: the client's upload time, the wait from its
+    last query byte to the first reply byte, the download, and the server's own phase
+    timings (upload, import overlapped with it, reply, send).  value = raw DB bytes /
+    session time (first query byte sent .. last reply byte received) of the d = 1
+    session.  Each reply is decrypted on the GPU back to the chosen element."""
+    import re
+    import shutil
+    import socket
+    import struct
+    import subprocess
+    import tempfile
+
+    import xpir_amd
+    from xpir_amd.pir_client import CLIENT_DRIVEN, PIRClient
+    from xpir_amd.workloads import C2_SINGLE as cfg
+    server = os.path.join(ROOT, "xpir_amd", "_build", "pir_server")
+    n, L = cfg["n_elems"], cfg["elem_bytes"]
+    raw = n * L
+    ctx = xpir_amd.Context(0)
+    dbb = ctx.alloc(raw + 16)
+    ctx.random_bytes_dev(cfg["db_seed"], 0, dbb, raw)
+    data = dbb.download(np.uint8, raw)
+    dbb.free()
+    tmp = tempfile.mkdtemp(prefix="xpir_session_")
+    sessions = []
+    try:
+        os.makedirs(os.path.join(tmp, "db"))
+        data.tofile(os.path.join(tmp, "db", "database.bin"))
+        for dims in ([n], [256, 256]):
+            prm = xpir_amd.params(cfg["crypto"], dims[0])
+            index = cfg["index"] % n
+            sk = xpir_amd.keygen(prm, cfg["key_seed"])
+            skd = ctx.alloc(sk.nbytes).upload(sk)
+            ct = 16 * prm.m * prm.n
+            abs_b = prm.A * prm.n // 8
+            qwords = sum(dims) * ct // 8
+            qd = ctx.alloc(qwords * 8)
+            ctx.query_gen_dev(prm, skd, dims, index, cfg["noise_seed"], qd)
+            q = qd.download(np.uint64, qwords)
+            qd.free()
+            count = -(-L // abs_b)  # reply ciphertexts (computeReplySizeInChunks)
+            for _ in range(1, len(dims)):
+                count = -(-count * ct // abs_b)
+            path = os.path.join(tmp, "pir.sock")
+            lst = socket.socket(socket.AF_UNIX, socket.SOCK_STREAM)
+            lst.bind(path)
+            lst.listen(1)
+            logp = os.path.join(tmp, "server.log")
+            with open(logp, "w") as logf:
+                proc = subprocess.Popen([server, "--listen-fd", str(lst.fileno()), "--db-dir", os.path.join(tmp, "db"),
+                                         "-s", str(n), "--sessions", "1"], pass_fds=[lst.fileno()], stdout=logf,
+                                        stderr=subprocess.STDOUT)
+            try:
+                log(f"session dims {dims}: handshake")
+                c = PIRClient(path, timeout=300)
+                c.hello()
+                kind, _, cnt, maxb = c.catalog()
+                assert (cnt, maxb) == (n, L), (cnt, maxb)
+                assert c.exchange_method() == CLIENT_DRIVEN
+                c.optim_exit()
+                c.send_crypto_params(cfg["crypto"] + ":?")
+                c.send_key(struct.pack(f"<{prm.m}Q", *[prm.moduli[i] for i in range(prm.m)]))
+                c.send_pir_params(len(dims), 1, dims)
+                qb = memoryview(q.view(np.uint8))
+                log(f"session dims {dims}: uploading {len(qb) / 2**20:.0f} MiB of query")
+                t0 = time.perf_counter()
+                for i in range(0, len(qb), ct):
+                    c.send(qb[i:i + ct])
+                t1 = time.perf_counter()
+                first = c.recv(ct)
+                t_first = time.perf_counter()
+                rest = c.recv((count - 1) * ct)
+                t2 = time.perf_counter()
+                assert c.at_eof()
+                c.close()
+                rc = proc.wait(timeout=300)
+            finally:
+                if proc.poll() is None:
+                    proc.kill()
+                    proc.wait()
+                lst.close()
+                os.unlink(path)
+            text = open(logp).read()
+            if rc != 0 or "session done" not in text:
+                raise RuntimeError(f"server failed (rc {rc}): {text}")
+            m = re.search(r"query upload ([0-9.]+) ms, import ([0-9.]+) ms.*?, reply ([0-9.]+) ms, send ([0-9.]+) ms",
+                          text)
+            srv = {k: float(v) for k, v in zip(("upload_ms", "import_ms", "reply_ms", "send_ms"), m.groups())}
+            reply = np.frombuffer(first + rest, dtype=np.uint64)
+            rd = ctx.alloc(reply.nbytes).upload(reply)
+            element = ctx.extract_reply_dev(prm, skd, rd.ptr, dims, L)
+            rd.free()
+            skd.free()
+            ok = element == data[index * L:(index + 1) * L].tobytes()
+            session_s = t2 - t0
+            log(f"session dims {dims}: {session_s:.3f} s, server {srv}, decrypts to element: {ok}")
+            sessions.append({"dims": dims, "query_bytes": len(qb), "reply_bytes": count * ct,
+                             "client_upload_s": t1 - t0, "wait_first_reply_s": t_first - t1,
+                             "download_s": t2 - t_first, "session_s": session_s,
+                             "db_gbs": raw / session_s / 1e9, "server": srv, "decrypts_to_element": bool(ok)})
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+        ctx.close()
+    out = {
+        "metric": "session DB GB/s (first query byte sent .. last reply byte received), 1 MI355X, UNIX socket",
+        "workload": "session", "value": sessions[0]["db_gbs"], "unit": "GB/s",
+        "ms_per_step": sessions[0]["session_s"] * 1e3, "n_gpus": 1, "dtype": "u64",
+        "data": "synthetic (seeded SplitMix64 bytes)",
+        "config": {"workload": cfg["name"] + " via pir_server", "crypto": cfg["crypto"], "n_elems": n,
+                   "elem_bytes": L},
+        "sessions": sessions,
+        "all_decrypt_to_element": all(s["decrypts_to_element"] for s in sessions),
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_batched(args):

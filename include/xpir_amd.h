@@ -84,6 +84,13 @@ int xpg_dev_alloc(xpg_ctx *ctx, uint64_t bytes, void **ptr_dev);
 int xpg_dev_free(xpg_ctx *ctx, void *ptr_dev);
 int xpg_h2d(xpg_ctx *ctx, void *dst_dev, const void *src, uint64_t bytes);
 int xpg_d2h(xpg_ctx *ctx, void *dst, const void *src_dev, uint64_t bytes);
+/* page-locked host memory (hipHostMalloc) and a copy from it that returns at once,
+ * ordered on the context stream (xpg_sync waits for it): staging for callers that
+ * upload a query while it arrives.  No reference counterpart (pushQuery keeps the
+ * elements in host memory, PIRReplyGeneratorNFL_internal.cpp:738-781). */
+int xpg_host_alloc(xpg_ctx *ctx, uint64_t bytes, void **ptr);
+int xpg_host_free(xpg_ctx *ctx, void *ptr);
+int xpg_h2d_async(xpg_ctx *ctx, void *dst_dev, const void *src_pinned, uint64_t bytes);
 
 /* ---- database import (PIRReplyGenerator::importData, pir/libpir.cpp:90-103 ->
  *      PIRReplyGeneratorNFL_internal::importDataNFL, :79-158) ----

@@ -95,6 +95,28 @@ int main(int argc, char **argv) {
   sp.readStream(3, piece.data(), 140);
   sp.closeStream(3);
   for (size_t k = 0; k < 140; k++) CHECK(piece[k] == big[3 * 142 + 2 + k]);
+  // a stream read twice in a row continues where it stopped
+  CHECK(sp.openStream(5, 0));
+  sp.readStream(5, piece.data(), 100);
+  sp.readStream(5, piece.data() + 100, 42);
+  sp.closeStream(5);
+  for (size_t k = 0; k < 142; k++) CHECK(piece[k] == big[5 * 142 + k]);
+  // the block read of whole elements equals the element-by-element reads; other shapes decline
+  std::vector<char> block(5 * 142, 'z'), one(142);
+  CHECK(sp.readStreams(2, 5, 0, 142, block.data()));
+  for (uint64_t i = 0; i < 5; i++) {
+    CHECK(sp.openStream(2 + i, 0));
+    sp.readStream(2 + i, one.data(), 142);
+    sp.closeStream(2 + i);
+    CHECK(memcmp(one.data(), block.data() + i * 142, 142) == 0);
+  }
+  CHECK(!sp.readStreams(0, 7, 1, 141, block.data()));  // from an offset: stream by stream
+  CHECK(!sp.readStreams(3, 5, 0, 142, block.data()));  // past the last element
+  CHECK(!db.readStreams(0, 3, 0, 300, agg.data()));    // one file per element
+  DBGenerator gen(4, 16);
+  std::vector<char> g(64, 0);
+  CHECK(gen.readStreams(0, 4, 0, 16, g.data()));
+  for (char ch : g) CHECK((unsigned char)ch == 0xaa);
   DBDirectoryProcessor too_many(2000, sdir);  // under one byte per element
   CHECK(too_many.getErrorStatus());
   DBDirectoryProcessor missing(root + "no_such_dir/");

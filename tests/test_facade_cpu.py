@@ -1,6 +1,7 @@
 """Host-only parts of the C++ facade (xpir_amd/cpp/libpir_amd.hpp), built and run without
 a GPU: DBDirectoryProcessor (one file per element and file splitting), its catalog
-strings and zero padding, and DBHandler::readAggregatedStream (tests/cpp/db_directory.cpp)."""
+strings and zero padding, DBHandler::readAggregatedStream, and the block reads the
+import uses (readStreams) against element-by-element reads (tests/cpp/db_directory.cpp)."""
 import os
 import subprocess
 

@@ -1,6 +1,6 @@
 """Whole sessions with the server program (xpir_amd/_build/pir_server, the reference's
 PIRSession and PIRServer) on the GPU, driven by the reference client's side of the
-protocol (session_proto.py): the query goes up element by element, the reply comes
+protocol (xpir_amd/pir_client.py): the query goes up element by element, the reply comes
 back ciphertext by ciphertext, equals the oracle's reply bytes and decrypts to the
 chosen element.  Covers the three server modes (client-driven, -z with its saved
 parameters and kept database, -L) and the import overlapped with the upload or not."""
@@ -13,9 +13,10 @@ import pytest
 
 from oracle.oracle import Oracle, extract_reply
 
+from xpir_amd.pir_client import CLIENT_DRIVEN, SERVER_DRIVEN
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-from session_proto import CLIENT_DRIVEN, SERVER_DRIVEN  # noqa: E402
 from test_session import Server  # noqa: E402
 
 pytestmark = pytest.mark.gpu

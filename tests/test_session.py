@@ -1,19 +1,17 @@
 """The session protocol of the server program (xpir_amd/_build/pir_server: the
 reference's PIRSession and PIRServer, apps/server/) up to the query upload, driven by the
-reference client's side of it (session_proto.py): optimizer dry runs, catalogs, the
+reference client's side of it (xpir_amd/pir_client.py): optimizer dry runs, catalogs, the
 exchange of crypto and PIR parameters in both modes, the saved-parameters file, and
 sessions that end early.  The query upload and the reply are in test_gpu_session.py."""
 import os
 import socket
 import subprocess
-import sys
 
 import pytest
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, HERE)
-from session_proto import CLIENT_DRIVEN, MEGA_BYTE, SERVER_DRIVEN, PIRClient  # noqa: E402
+from xpir_amd.pir_client import CLIENT_DRIVEN, MEGA_BYTE, SERVER_DRIVEN, PIRClient
 
+HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SERVER = os.path.join(ROOT, "xpir_amd", "_build", "pir_server")
 

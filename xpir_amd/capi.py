@@ -19,7 +19,8 @@ XPG_OK, XPG_EINVAL, XPG_ENOMEM, XPG_EDEVICE, XPG_ESIZE, XPG_EBUSY = 0, -1, -2, -
 EXPORTS = [
     "xpg_params_from_string", "xpg_polys_for_bytes", "xpg_reply_polys", "xpg_ctx_create",
     "xpg_ctx_destroy", "xpg_last_error", "xpg_ctx_stream", "xpg_ctx_set_option", "xpg_sync", "xpg_dev_alloc",
-    "xpg_dev_free", "xpg_h2d", "xpg_d2h", "xpg_db_import", "xpg_db_import_dev",
+    "xpg_dev_free", "xpg_h2d", "xpg_d2h", "xpg_host_alloc", "xpg_host_free", "xpg_h2d_async",
+    "xpg_db_import", "xpg_db_import_dev",
     "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
     "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev",
     "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_timing_read_main", "xpg_graph_stats", "xpg_autotune", "xpg_tune_set",
@@ -80,6 +81,9 @@ def lib() -> C.CDLL:
         "xpg_dev_free": (C.c_int, [vp, vp]),
         "xpg_h2d": (C.c_int, [vp, vp, vp, u64]),
         "xpg_d2h": (C.c_int, [vp, vp, vp, u64]),
+        "xpg_host_alloc": (C.c_int, [vp, u64, C.POINTER(vp)]),
+        "xpg_host_free": (C.c_int, [vp, vp]),
+        "xpg_h2d_async": (C.c_int, [vp, vp, vp, u64]),
         "xpg_db_import": (C.c_int, [vp, P, vp, u64, u64, u32, u32, C.POINTER(u32), C.POINTER(vp)]),
         "xpg_db_import_dev": (C.c_int, [vp, P, vp, u64, u64, u32, u32, C.POINTER(u32), C.POINTER(vp)]),
         "xpg_db_import_ntt_words": (C.c_int, [vp, P, vp, u64, u32, u64, u32, C.POINTER(u32), C.POINTER(vp)]),

@@ -2,11 +2,16 @@
 #include "libpir_amd.hpp"
 
 #include <dirent.h>
+#include <errno.h>
+#include <fcntl.h>
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <memory>
+#include <new>
 
 namespace xpir_amd {
 
@@ -145,6 +150,7 @@ DBDirectoryProcessor::DBDirectoryProcessor(uint64_t nbStreams, const std::string
 
 DBDirectoryProcessor::~DBDirectoryProcessor() {
   for (auto &kv : fd_pool_) fclose(kv.second);
+  if (split_fd_ >= 0) close(split_fd_);
 }
 
 std::string DBDirectoryProcessor::getCatalog(bool typeOfCatalog) {
@@ -159,7 +165,12 @@ std::string DBDirectoryProcessor::getCatalog(bool typeOfCatalog) {
 }
 
 bool DBDirectoryProcessor::openStream(uint64_t streamNb, uint64_t requested_offset) {
-  if (fd_pool_.count(streamNb) || streamNb >= file_list_.size()) return false;
+  if (fd_pool_.count(streamNb) || split_pos_.count(streamNb) || streamNb >= file_list_.size()) return false;
+  if (splitting_) {  // every element is a slice of one file: one descriptor, no reopening
+    if (split_fd_ < 0 && (split_fd_ = open(real_file_.c_str(), O_RDONLY)) < 0) return false;
+    split_pos_[streamNb] = streamNb * max_bytes_ + requested_offset;
+    return true;
+  }
   FILE *f = fopen(splitting_ ? real_file_.c_str() : (directory_ + file_list_[streamNb]).c_str(), "rb");
   if (!f) return false;
   const uint64_t at = (splitting_ ? streamNb * max_bytes_ : 0) + requested_offset;
@@ -172,13 +183,41 @@ bool DBDirectoryProcessor::openStream(uint64_t streamNb, uint64_t requested_offs
 }
 
 uint64_t DBDirectoryProcessor::readStream(uint64_t streamNb, char *buf, uint64_t size) {
-  auto it = fd_pool_.find(streamNb);
-  const uint64_t got = it == fd_pool_.end() ? 0 : fread(buf, 1, size, it->second);
+  uint64_t got = 0;
+  auto sp = split_pos_.find(streamNb);
+  if (sp != split_pos_.end()) {
+    while (got < size) {
+      const ssize_t r = pread(split_fd_, buf + got, size - got, (off_t)(sp->second + got));
+      if (r < 0 && errno == EINTR) continue;
+      if (r <= 0) break;
+      got += (uint64_t)r;
+    }
+    sp->second += got;
+  } else {
+    auto it = fd_pool_.find(streamNb);
+    got = it == fd_pool_.end() ? 0 : fread(buf, 1, size, it->second);
+  }
   memset(buf + got, 0, size - got);  // short file: zero padding
   return size;
 }
 
+bool DBDirectoryProcessor::readStreams(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) {
+  if (!splitting_ || offset != 0 || bytes != max_bytes_ || first + count > file_list_.size()) return false;
+  if (split_fd_ < 0 && (split_fd_ = open(real_file_.c_str(), O_RDONLY)) < 0) return false;
+  const uint64_t total = count * bytes, at = first * max_bytes_;
+  uint64_t got = 0;
+  while (got < total) {
+    const ssize_t r = pread(split_fd_, out + got, total - got, (off_t)(at + got));
+    if (r < 0 && errno == EINTR) continue;
+    if (r <= 0) break;
+    got += (uint64_t)r;
+  }
+  memset(out + got, 0, total - got);
+  return true;
+}
+
 void DBDirectoryProcessor::closeStream(uint64_t streamNb) {
+  split_pos_.erase(streamNb);
   auto it = fd_pool_.find(streamNb);
   if (it == fd_pool_.end()) return;
   fclose(it->second);
@@ -192,11 +231,16 @@ uint64_t DBGenerator::readStream(uint64_t, char *buf, uint64_t size) {
   return size;
 }
 
+bool DBGenerator::readStreams(uint64_t, uint64_t count, uint64_t, uint64_t bytes, char *out) {
+  memset(out, 0xaa, count * bytes);
+  return true;
+}
+
 imported_database::~imported_database() { xpg_db_free(imported_database_ptr); }
 
 PIRReplyGenerator::PIRReplyGenerator(PIRParameters &param, HomomorphicCrypto &cryptoMethod, DBHandler *db,
                                      int device)
-    : param_(param), crypto_(cryptoMethod), db_(db) {
+    : param_(param), crypto_(cryptoMethod), db_(db), device_(device) {
   // setPirParams: A depends on n[0] (PIRReplyGeneratorNFL_internal.cpp:809-816)
   crypto_.setandgetAbsBitPerCiphertext(param_.n[0]);
   prm_ = crypto_.params();
@@ -207,6 +251,12 @@ PIRReplyGenerator::PIRReplyGenerator(PIRParameters &param, HomomorphicCrypto &cr
 PIRReplyGenerator::~PIRReplyGenerator() {
   freeQueries();
   for (uint64_t i = next_reply_; i < replies_.size(); i++) free(replies_[i]);
+  if (qctx_) {
+    for (char *s : stage_)
+      if (s) xpg_host_free(qctx_, s);
+    if (qdev_) xpg_dev_free(qctx_, qdev_);
+    xpg_ctx_destroy(qctx_);
+  }
   xpg_ctx_destroy(ctx_);
 }
 
@@ -220,11 +270,61 @@ int PIRReplyGenerator::fail(int code, const std::string &msg) {
   return code;
 }
 
-void PIRReplyGenerator::pushQuery(char *rawQuery) { queries_.push_back(rawQuery); }
+// Copies the filled chunk to its place in HBM without waiting and switches to the other
+// chunk, whose own copy (the only one in flight, issued a chunk ago) is waited for first.
+int PIRReplyGenerator::stageFlush() {
+  if (stage_fill_ == 0) return XPG_OK;
+  int rc;
+  if (stage_pending_ && (rc = xpg_sync(qctx_)) != XPG_OK) return rc;
+  const uint64_t at = qbytes_ - stage_fill_;
+  rc = xpg_h2d_async(qctx_, reinterpret_cast<char *>(qdev_) + at, stage_[stage_cur_], stage_fill_);
+  if (rc != XPG_OK) return rc;
+  stage_pending_ = true;
+  stage_cur_ ^= 1;
+  stage_fill_ = 0;
+  return XPG_OK;
+}
+
+void PIRReplyGenerator::pushQuery(char *rawQuery) {
+  const uint64_t ct = prm_.ct_bytes;
+  if (stage_status_ == XPG_OK && !qctx_) {
+    uint64_t elems = 0;
+    for (unsigned l = 0; l < param_.d && l < MAX_REC_LVL; l++) elems += param_.n[l];
+    qcap_ = elems * ct;
+    void *p = nullptr;
+    if ((stage_status_ = xpg_ctx_create(device_, &qctx_)) != XPG_OK) {
+      stage_error_ = "no GPU context for the query";
+      qctx_ = nullptr;
+    } else if ((stage_status_ = xpg_dev_alloc(qctx_, qcap_, &p)) != XPG_OK) {
+      stage_error_ = "query allocation in HBM";
+    } else {
+      qdev_ = static_cast<uint64_t *>(p);
+      for (char *&s : stage_) {
+        void *h = nullptr;
+        if ((stage_status_ = xpg_host_alloc(qctx_, kStageBytes, &h)) != XPG_OK) {
+          stage_error_ = "pinned staging allocation";
+          break;
+        }
+        s = static_cast<char *>(h);
+      }
+    }
+  }
+  if (stage_status_ == XPG_OK && qbytes_ + ct > qcap_) stage_overflow_ = true;  // more elements than the grid
+  if (stage_status_ == XPG_OK && !stage_overflow_) {
+    if (stage_fill_ + ct > kStageBytes && (stage_status_ = stageFlush()) != XPG_OK) stage_error_ = "query upload";
+    if (stage_status_ == XPG_OK) {
+      memcpy(stage_[stage_cur_] + stage_fill_, rawQuery, ct);
+      stage_fill_ += ct;
+      qbytes_ += ct;
+    }
+  }
+  free(rawQuery);
+}
 
 void PIRReplyGenerator::freeQueries() {
-  for (char *q : queries_) free(q);
-  queries_.clear();
+  if (qctx_ && stage_pending_) xpg_sync(qctx_);
+  qbytes_ = stage_fill_ = 0;
+  stage_pending_ = stage_overflow_ = false;
 }
 
 imported_database *PIRReplyGenerator::importData(uint64_t offset, uint64_t bytes_per_db_element) {
@@ -240,14 +340,21 @@ imported_database *PIRReplyGenerator::importData(uint64_t offset, uint64_t bytes
     fail(XPG_EINVAL, "empty database");
     return nullptr;
   }
-  std::vector<uint8_t> staging(nb * file_bytes);
-  for (uint64_t i = 0; i < nb; i++) {
-    db_->openStream(i, offset);
-    db_->readStream(i, reinterpret_cast<char *>(staging.data() + i * file_bytes), file_bytes);
-    db_->closeStream(i);
+  std::unique_ptr<char[]> staging(new (std::nothrow) char[nb * file_bytes]);  // every byte is read below
+  if (!staging) {
+    fail(XPG_ENOMEM, "host staging of the database");
+    return nullptr;
+  }
+  if (!db_->readStreams(0, nb, offset, file_bytes, staging.get())) {
+    for (uint64_t i = 0; i < nb; i++) {
+      db_->openStream(i, offset);
+      db_->readStream(i, staging.get() + i * file_bytes, file_bytes);
+      db_->closeStream(i);
+    }
   }
   xpg_db *h = nullptr;
-  int rc = xpg_db_import(ctx_, &prm_, staging.data(), nb, file_bytes, param_.alpha, param_.d, param_.n, &h);
+  int rc = xpg_db_import(ctx_, &prm_, reinterpret_cast<const uint8_t *>(staging.get()), nb, file_bytes,
+                         param_.alpha, param_.d, param_.n, &h);
   if (rc != XPG_OK) {
     fail(rc, "importData");
     return nullptr;
@@ -272,15 +379,28 @@ void PIRReplyGenerator::generateReply(const imported_database *database) {
   const xpg_db *h = database->imported_database_ptr;
   const uint64_t qwords = xpg_db_query_words(h);
   const uint64_t ct_words = 2ULL * prm_.m * prm_.n;
-  if (queries_.size() * ct_words != qwords) {
+  if (stage_status_ != XPG_OK) {
+    fail(stage_status_, stage_error_);
+    return;
+  }
+  if (stage_overflow_ || qbytes_ != qwords * 8) {
     fail(XPG_ESIZE, "pushed query elements do not match the database grid");
     return;
   }
-  std::vector<uint64_t> q(qwords);
-  for (size_t i = 0; i < queries_.size(); i++) memcpy(q.data() + i * ct_words, queries_[i], ct_words * 8);
+  int rc = stageFlush();
+  if (rc == XPG_OK) rc = xpg_sync(qctx_);  // the whole query is in HBM
+  if (rc != XPG_OK) {
+    fail(rc, "query upload");
+    return;
+  }
+  stage_pending_ = false;
   const uint64_t nrep = xpg_db_reply_polys(h);
   std::vector<uint64_t> r(nrep * ct_words);
-  int rc = xpg_reply(ctx_, h, q.data(), qwords, r.data(), r.size());
+  void *rdev = nullptr;
+  rc = xpg_dev_alloc(ctx_, r.size() * 8, &rdev);
+  if (rc == XPG_OK) rc = xpg_reply_dev(ctx_, h, qdev_, static_cast<uint64_t *>(rdev));
+  if (rc == XPG_OK) rc = xpg_d2h(ctx_, r.data(), rdev, r.size() * 8);
+  if (rdev) xpg_dev_free(ctx_, rdev);
   if (rc != XPG_OK) {
     fail(rc, "generateReply");
     return;

@@ -88,6 +88,13 @@ class DBHandler {
   // "1\n<count>\n<maxFileBytesize>\n".  This default names the elements "0", "1", ...
   // with size maxFileBytesize, as DBGenerator does (DBGenerator.cpp:29-34,46-63).
   virtual std::string getCatalog(bool typeOfCatalog);
+  // Elements first .. first+count-1, `bytes` bytes each from `offset`, back to back into
+  // out, when the handler reads them as one block; false: read them stream by stream.
+  // (No reference counterpart: importDataNFL reads element by element, :102-135.)
+  virtual bool readStreams(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) {
+    (void)first, (void)count, (void)offset, (void)bytes, (void)out;
+    return false;
+  }
   // The alpha streams of aggregate `streamNb`, each read from `offset` and zero-padded to
   // min(bytes_per_file, maxFileBytesize - offset) bytes, back to back in rawBits; slots
   // past the last stream are zeroed (DBHandler.cpp:3-26, whose bzero starts at the last
@@ -124,6 +131,7 @@ class DBGenerator : public DBHandler {
   bool openStream(uint64_t, uint64_t) override { return true; }
   uint64_t readStream(uint64_t, char *buf, uint64_t size) override;
   void closeStream(uint64_t) override {}
+  bool readStreams(uint64_t, uint64_t count, uint64_t, uint64_t bytes, char *out) override;
 
  private:
   uint64_t nb_, bytes_;
@@ -152,6 +160,8 @@ class DBDirectoryProcessor : public DBHandler {
   bool openStream(uint64_t streamNb, uint64_t requested_offset) override;
   uint64_t readStream(uint64_t streamNb, char *buf, uint64_t size) override;
   void closeStream(uint64_t streamNb) override;
+  // split mode, whole elements: one positional read of the file's slice
+  bool readStreams(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) override;
   static uint64_t getFileSize(const std::string &filePath);
   const std::vector<std::string> &fileList() const { return file_list_; }
 
@@ -161,6 +171,9 @@ class DBDirectoryProcessor : public DBHandler {
   std::map<uint64_t, FILE *> fd_pool_;
   bool splitting_ = false, error_ = false;
   std::string real_file_;  // the split file
+  // split mode: one descriptor of the split file, read positionally per stream
+  int split_fd_ = -1;
+  std::map<uint64_t, uint64_t> split_pos_;
   uint64_t max_bytes_ = 0;
 };
 
@@ -178,7 +191,9 @@ class PIRReplyGenerator {
  public:
   PIRReplyGenerator(PIRParameters &param, HomomorphicCrypto &cryptoMethod, DBHandler *db, int device = 0);
   ~PIRReplyGenerator();
-  // one query element (16*n*m bytes, a||b), level after level; ownership passes here
+  // one query element (16*n*m bytes, a||b), level after level; ownership passes here.
+  // The element goes to HBM at once (through pinned staging on a stream of its own, so
+  // pushing may overlap importData on another thread) and its buffer is freed.
   void pushQuery(char *rawQuery);
   imported_database *importData(uint64_t offset, uint64_t bytes_per_db_element);
   void generateReply(const imported_database *database);
@@ -192,13 +207,25 @@ class PIRReplyGenerator {
 
  private:
   int fail(int code, const std::string &msg);
+  int stageFlush();
   PIRParameters param_;
   HomomorphicCrypto &crypto_;
   xpg_params prm_;
   DBHandler *db_;
+  int device_;
   xpg_ctx *ctx_ = nullptr;
-  std::vector<char *> queries_;
+  // query staging: its own context (stream), the query in HBM (every level back to back)
+  // and two pinned chunks filled in turn while the other one's copy runs
+  xpg_ctx *qctx_ = nullptr;
+  uint64_t *qdev_ = nullptr;
+  uint64_t qcap_ = 0, qbytes_ = 0, stage_fill_ = 0;
+  char *stage_[2] = {nullptr, nullptr};
+  int stage_cur_ = 0;
+  bool stage_pending_ = false, stage_overflow_ = false;
+  int stage_status_ = XPG_OK;
+  std::string stage_error_;
   std::vector<char *> replies_;
+  static const uint64_t kStageBytes = 32ull << 20;
   uint64_t next_reply_ = 0, generated_ = 0;
   int status_ = XPG_OK;
   std::string error_;

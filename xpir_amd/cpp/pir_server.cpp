@@ -177,8 +177,13 @@ int main(int argc, char **argv) {
     if (server_driven) s->setPIRParams(saved);
     const bool dry_run = s->start(opt);
     if (s->failed()) fprintf(stderr, "pir_server: session failed: %s\n", s->lastError().c_str());
-    else if (!dry_run) fprintf(stderr, "pir_server: session done, %llu reply ciphertexts\n",
-                               (unsigned long long)s->repliesSent());
+    else if (!dry_run)
+      fprintf(stderr,
+              "pir_server: session done, %llu reply ciphertexts; query upload %.3f ms, import %.3f ms%s, "
+              "reply %.3f ms, send %.3f ms\n",
+              (unsigned long long)s->repliesSent(), s->times().upload_ms, s->times().import_ms,
+              opt.got_preimported_database ? " (kept database)" : (no_pipeline ? "" : " (during the upload)"),
+              s->times().reply_ms, s->times().send_ms);
     if (keep)
       *keep = s;
     else

@@ -16,6 +16,9 @@
 namespace xpir_amd {
 
 namespace {
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
 const size_t kMegaByte = 1000000;                  // pir/replyGenerator/PIROptimizer.hpp:33
 const uint32_t kMaxParamString = 1 << 16;          // bounds on what a peer may announce
 const uint32_t kMaxKeyBytes = 1 << 24;
@@ -237,13 +240,16 @@ bool PIRSession::queryAndReply(const session_option_t &opt) {
   imported_database *db = nullptr;
   std::string import_error;
   auto import = [&]() {
+    const auto t0 = std::chrono::steady_clock::now();
     db = gen.importData(0, db_->getmaxFileBytesize());
     if (!db) import_error = gen.lastError();
+    times_.import_ms = ms_since(t0);
   };
   std::thread importer;
   if (!preimported && !no_pipeline_) importer = std::thread(import);
 
   bool ok = true;
+  std::chrono::steady_clock::time_point t_up;
   for (unsigned j = 0; ok && j < pir_.d; j++)
     for (unsigned i = 0; ok && i < pir_.n[j]; i++) {
       char *buf = static_cast<char *>(malloc(qbytes));
@@ -252,11 +258,13 @@ bool PIRSession::queryAndReply(const session_option_t &opt) {
         break;
       }
       ok = rd(buf, qbytes, "a query element");
+      if (j == 0 && i == 0) t_up = std::chrono::steady_clock::now();  // the upload has begun
       if (ok)
         gen.pushQuery(buf);  // the generator owns it now
       else
         free(buf);
     }
+  times_.upload_ms = ms_since(t_up);
   if (importer.joinable()) importer.join();
   if (!preimported && no_pipeline_ && ok) import();
   if (!ok) {
@@ -265,7 +273,9 @@ bool PIRSession::queryAndReply(const session_option_t &opt) {
   }
   if (!preimported && !db) return fail("database import: " + import_error);
 
+  const auto t_reply = std::chrono::steady_clock::now();
   gen.generateReply(preimported ? opt.data : db);
+  times_.reply_ms = ms_since(t_reply);
   if (gen.lastStatus() != XPG_OK) {
     delete db;
     return fail("reply generation: " + gen.lastError());
@@ -277,6 +287,7 @@ bool PIRSession::queryAndReply(const session_option_t &opt) {
     delete db;
   }
   const uint64_t bytes = gen.getReplyElementBytesize();
+  const auto t_send = std::chrono::steady_clock::now();
   char *r = nullptr;
   while (gen.popReply(&r)) {
     const bool sent = wr(r, bytes, "a reply ciphertext");
@@ -284,6 +295,7 @@ bool PIRSession::queryAndReply(const session_option_t &opt) {
     if (!sent) return false;
     sent_++;
   }
+  times_.send_ms = ms_since(t_send);
   gen.freeQueries();
   return true;
 }
@@ -292,6 +304,7 @@ bool PIRSession::start(const session_option_t &opt) {
   failed_ = false;
   error_.clear();
   sent_ = 0;
+  times_ = SessionTimes();
   int32_t is_client = 0;
   if (!rd(&is_client, 4, "the client or optimizer choice")) {
     s_.close();

@@ -59,6 +59,15 @@ class FdStream : public ByteStream {
   bool owned_, closed_ = false;
 };
 
+// Wall-clock milliseconds of a session's phases (no reference counterpart; the reference
+// prints its PERF_TIMERS figures instead, PIRSession.cpp:384-451).
+struct SessionTimes {
+  double upload_ms = 0;  // first query element read .. last one read
+  double import_ms = 0;  // database import (on its own thread, during the upload)
+  double reply_ms = 0;   // generateReply: query to HBM, every level, reply back
+  double send_ms = 0;    // reply ciphertexts written
+};
+
 // apps/server/PIRSession.hpp session_option_t
 struct session_option_t {
   bool driven_mode = true;                // client-driven: the client picks the parameters
@@ -92,6 +101,7 @@ class PIRSession {
   bool failed() const { return failed_; }
   const std::string &lastError() const { return error_; }
   uint64_t repliesSent() const { return sent_; }
+  const SessionTimes &times() const { return times_; }
 
  private:
   bool fail(const std::string &msg);
@@ -116,6 +126,7 @@ class PIRSession {
   bool send_catalog_ = true, no_pipeline_ = false, failed_ = false;
   std::string error_;
   uint64_t sent_ = 0;
+  SessionTimes times_;
 };
 
 // ServerService.cpp:24-66: "d <d>\ndim <n0> <n1> ... \n<crypto_params>\n"; both return

@@ -410,6 +410,29 @@ int xpg_h2d(xpg_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
   return XPG_OK;
 }
 
+int xpg_host_alloc(xpg_ctx *ctx, uint64_t bytes, void **ptr) {
+  if (!ctx || !ptr) return XPG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  if (hipHostMalloc(ptr, bytes ? bytes : 8, hipHostMallocDefault) != hipSuccess) {
+    *ptr = nullptr;
+    return set_err(ctx, XPG_ENOMEM, "pinned host allocation");
+  }
+  return XPG_OK;
+}
+
+int xpg_host_free(xpg_ctx *ctx, void *ptr) {
+  if (!ctx) return XPG_EINVAL;
+  HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");  // no copy from it in flight
+  HIP_OK(hipHostFree(ptr), "pinned host free");
+  return XPG_OK;
+}
+
+int xpg_h2d_async(xpg_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx || !dst || !src) return XPG_EINVAL;
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D copy");
+  return XPG_OK;
+}
+
 int xpg_d2h(xpg_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
   if (!ctx) return XPG_EINVAL;
   HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "D2H copy");

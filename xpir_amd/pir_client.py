@@ -1,4 +1,5 @@
-"""The client side of the reference's session protocol, restated for the tests from
+"""The client side of the reference's session protocol (for tests, load generation and
+Python callers of the server program), restated from
 apps/client/PIRClient.cpp (hello :59-60, catalog :73-100, crypto parameters :215-266,
 PIR parameters :137-163, exchange method :177, query upload :327-345, reply download
 :387-420), apps/client/DESC.cpp:38-100 (catalog parsing) and the optimizer's client
