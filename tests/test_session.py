@@ -147,6 +147,24 @@ def test_unsupported_crypto_parameters_end_the_session(tmp_path):
         assert 'unsupported crypto parameters "LWE:80:1000:120"' in srv.log()
 
 
+def test_grid_smaller_than_the_database_ends_the_session(tmp_path):
+    """12 elements on a 3 x 3 grid: refused right after the PIR parameters, before any
+    query byte (and before the GPU is touched)."""
+    with Server(tmp_path, "--db-generator", "-n", 12, "-l", 100, "--sessions", 1) as srv:
+        c = srv.client()
+        c.hello()
+        c.catalog()
+        c.exchange_method()
+        c.optim_exit()
+        c.send_crypto_params("LWE:80:2048:120:?")
+        c.send_key(bytes(16))
+        c.send_pir_params(2, 1, [3, 3])
+        assert c.at_eof()
+        c.close()
+        assert srv.wait() == 0
+        assert "the PIR grid holds 9 elements, the database 12" in srv.log()
+
+
 @pytest.mark.parametrize("content", [None, "d 3\ndim 4 3 \nLWE:80:2048:120\n", "d 1\ndim 4 \n"])
 def test_bad_params_file(tmp_path, content):
     cfg = tmp_path / "PIRParams.cfg"

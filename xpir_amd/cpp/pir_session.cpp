@@ -7,6 +7,7 @@
 #include <sys/socket.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <fstream>
 #include <sstream>
@@ -232,6 +233,13 @@ bool PIRSession::sendPirParams() {
 // (from the database this session imports, or the one it was handed), then its
 // ciphertexts in order and the end of the stream.
 bool PIRSession::queryAndReply(const session_option_t &opt) {
+  // the grid must hold every aggregate of alpha elements (the reference finds out in the
+  // import; here the client learns it before uploading its query)
+  const uint64_t units = (db_->getNbStream() + pir_.alpha - 1) / pir_.alpha;
+  uint64_t grid = 1;
+  for (unsigned l = 0; l < pir_.d; l++) grid = std::min<uint64_t>(grid * pir_.n[l], 1ull << 62);
+  if (grid < units)
+    return fail("the PIR grid holds " + std::to_string(grid) + " elements, the database " + std::to_string(units));
   PIRReplyGenerator gen(pir_, *crypto_, db_, device_);
   if (gen.lastStatus() != XPG_OK) return fail("reply generator: " + gen.lastError());
   const uint64_t qbytes = crypto_->params().ct_bytes;  // after setandgetAbsBitPerCiphertext(n[0])
