@@ -74,3 +74,17 @@ def test_simple_pir_from_a_directory(tmp_path):
     got = np.fromfile(out, dtype=np.uint64)
     assert np.array_equal(got, want)
     assert extract_reply(orc, sk, got, dims, L) == elems[index].tobytes()
+
+
+def test_pushed_query_staging(tmp_path):
+    """PIRReplyGenerator::pushQuery stages elements through pinned chunks into HBM: a query
+    over three chunks, a second reply, a new query after freeQueries, two levels, and
+    element counts off by one (tests/cpp/facade_staging.cpp), against the C ABI's reply."""
+    exe = tmp_path / "facade_staging"
+    pkg = os.path.join(ROOT, "xpir_amd")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "cpp", "facade_staging.cpp"),
+                    "-L", pkg, "-lpir_amd", "-lxpir_amd", f"-Wl,-rpath,{pkg}"], check=True)
+    print("run: facade_staging", flush=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr + r.stdout
