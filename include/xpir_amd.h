@@ -179,7 +179,10 @@ int xpg_server_release(xpg_server *srv, uint64_t ticket);
 double xpg_server_now_us(xpg_server *srv);
 int xpg_server_stats(xpg_server *srv, uint64_t *batches, uint64_t *batched_requests, uint64_t *budget_used);
 
-/* ---- kernel timing: HIP events around every MAC launch on the context stream ---- */
+/* ---- kernel timing: HIP events around every MAC launch on the context stream ----
+ * on = 1: around each MAC level and each level's main kernel; on = 2: around the main
+ * kernels alone (fewer event records between launches: no gaps around a level's scan,
+ * for timing the main kernel inside a throughput loop); 0: off */
 int xpg_timing_enable(xpg_ctx *ctx, int on);
 /* total milliseconds and launch count of the MAC kernels since the last reset */
 int xpg_timing_read(xpg_ctx *ctx, double *mac_ms, uint64_t *mac_launches, int reset);

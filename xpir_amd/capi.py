@@ -302,8 +302,10 @@ class Context:
             arr[i].db, arr[i].query_dev, arr[i].reply_dev = db.h, q.ptr, r.ptr
         self._check(lib().xpg_reply_batch_dev(self.h, arr, len(reqs)))
 
-    def timing(self, on: bool = True):
-        self._check(lib().xpg_timing_enable(self.h, 1 if on else 0))
+    def timing(self, on=True):
+        """True/1: events around every MAC level and main kernel; 2: the main kernels
+        alone; False/0: off"""
+        self._check(lib().xpg_timing_enable(self.h, 2 if on == 2 else 1 if on else 0))
 
     def timing_read(self, reset: bool = True):
         ms, n = C.c_double(), C.c_uint64()

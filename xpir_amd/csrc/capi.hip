@@ -214,14 +214,18 @@ void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial) {
   }
 }
 
-void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b) {
-  if (!ctx->timing) return;
+void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b, int tag) {
+  // mode 2 times the main kernels alone: no events around whole levels, whose records
+  // would put gaps between a level's launches
+  if (!ctx->timing || (ctx->timing == 2 && tag != 1)) return;
   for (hipEvent_t *e : {a, b}) {
     if (!ctx->ev_pool.empty()) {
       *e = ctx->ev_pool.back();
       ctx->ev_pool.pop_back();
     } else {
-      (void)hipEventCreate(e);
+      // timing only: no system-scope fence at the record (no cache writeback between the
+      // launches it separates)
+      (void)hipEventCreateWithFlags(e, hipEventDisableSystemFence);
     }
   }
   (void)hipEventRecord(*a, ctx->stream);
@@ -833,7 +837,7 @@ int xpg_reply(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query, uint64_t qu
 // ---- timing ----
 int xpg_timing_enable(xpg_ctx *ctx, int on) {
   if (!ctx) return XPG_EINVAL;
-  ctx->timing = on != 0;
+  ctx->timing = on == 2 ? 2 : on != 0 ? 1 : 0;
   return XPG_OK;
 }
 

@@ -107,7 +107,7 @@ struct xpg_ctx {
   int cu_count = 256;              // compute units of the device
   uint64_t group_budget = 0;       // bytes for a batch's grouped levels (0 = default)
   // MAC timing
-  bool timing = false;
+  int timing = 0;  // 0 off, 1 every MAC level and main kernel, 2 main kernels alone
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::tuple<hipEvent_t, hipEvent_t, int>> pending;  // (start, end, 0 = whole MAC | 1 = main kernel)
   double mac_ms = 0.0;
@@ -133,7 +133,7 @@ int ensure_io_buf(xpg_ctx *ctx, size_t bytes);
 int ensure_dec_buf(xpg_ctx *ctx, size_t bytes);
 void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial);  // 0 = every entry
 ModSet modset_of(const xpg_params *prm);
-void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b);
+void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b, int tag = 0);
 void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b, int tag = 0);
 
 // ---- tuning (tune.hip) ----

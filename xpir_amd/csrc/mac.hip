@@ -752,20 +752,44 @@ __global__ void __launch_bounds__(128 * FIX_PARTS) mac_stream_fixup_kernel(Strea
   const uint32_t per = (a.nblk + FIX_PARTS - 1) / FIX_PARTS;
   const uint32_t t0 = min(a.nblk, pi * per), t1 = min(a.nblk, t0 + per);
   auto idx = [&](uint32_t t) { return ((uint64_t)t * a.columns + column) * 128 + cl; };
+  // a run of up to KB blocks (the usual case: about 64 blocks per column) is loaded into
+  // registers at once, so its loads are in flight together instead of one per iteration
+  constexpr uint32_t KB = 8;
+  const bool cached = t1 - t0 <= KB;
+  uint64_t r0v[KB], rsv[KB];
+  int2 jv[KB];
   uint64_t s = 0;
-  for (uint32_t t = t0; t < t1; t++) s = addmod_c(s, a.rho0[idx(t)], p);
+  if (cached) {
+#pragma unroll
+    for (uint32_t i = 0; i < KB; i++)
+      if (t0 + i < t1) {
+        r0v[i] = a.rho0[idx(t0 + i)];
+        rsv[i] = a.rstar[idx(t0 + i)];
+        jv[i] = a.jj[idx(t0 + i)];
+      }
+#pragma unroll
+    for (uint32_t i = 0; i < KB; i++)
+      if (t0 + i < t1) s = addmod_c(s, r0v[i], p);
+  } else {
+    for (uint32_t t = t0; t < t1; t++) s = addmod_c(s, a.rho0[idx(t)], p);
+  }
   part_rho[pi][cl] = s;
   __syncthreads();
   uint64_t rho_in = 0;
   for (uint32_t i = 0; i < pi; i++) rho_in = addmod_c(rho_in, part_rho[i][cl], p);
   int2 f = make_int2(0, kNegInf);
-  for (uint32_t t = t0; t < t1; t++) {
-    const uint64_t r0 = a.rho0[idx(t)], rs = a.rstar[idx(t)];
-    const int2 j = a.jj[idx(t)];
+  auto step = [&](uint64_t r0, uint64_t rs, int2 j) {
     const int32_t D = (rho_in + r0 >= p ? 1 : 0) - j.x;
     const int32_t M = D + j.y - (rho_in + rs >= p ? 1 : 0);
     f = make_int2(f.x + D, max(f.y + D, M));
     rho_in = addmod_c(rho_in, r0, p);
+  };
+  if (cached) {
+#pragma unroll
+    for (uint32_t i = 0; i < KB; i++)
+      if (t0 + i < t1) step(r0v[i], rsv[i], jv[i]);
+  } else {
+    for (uint32_t t = t0; t < t1; t++) step(a.rho0[idx(t)], a.rstar[idx(t)], a.jj[idx(t)]);
   }
   part_map[pi][cl] = f;
   __syncthreads();
@@ -815,7 +839,7 @@ int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
   int rc = seg_prepare(ctx, sa, (uint64_t)S * W);
   if (rc) return rc;
   hipEvent_t k0 = nullptr, k1 = nullptr;
-  timing_begin(ctx, &k0, &k1);
+  timing_begin(ctx, &k0, &k1, 1);
   hipLaunchKernelGGL((mac_seg_kernel<S, W>), dim3((uint32_t)(sa.tiles * sa.columns)), dim3(64 * W), 0, ctx->stream, sa);
   timing_end(ctx, k0, k1, 1);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
@@ -857,7 +881,7 @@ int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
   st.rstar = st.rho0 + recs;
   st.jj = (int2 *)(st.rstar + recs);
   hipEvent_t k0 = nullptr, k1 = nullptr;
-  timing_begin(ctx, &k0, &k1);
+  timing_begin(ctx, &k0, &k1, 1);
   hipLaunchKernelGGL((mac_stream_kernel<U, PF>), dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, ctx->stream, st);
   timing_end(ctx, k0, k1, 1);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
@@ -999,7 +1023,7 @@ int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
     const uint64_t blocks = (lanes + 255) / 256;
     if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
     hipEvent_t k0 = nullptr, k1 = nullptr;
-    timing_begin(ctx, &k0, &k1);
+    timing_begin(ctx, &k0, &k1, 1);
     hipLaunchKernelGGL(mac_direct_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
     timing_end(ctx, k0, k1, 1);
     if (hipGetLastError() != hipSuccess) rc = set_err(ctx, XPG_EDEVICE, "MAC launch");
@@ -1076,7 +1100,7 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
   hipEvent_t k0 = nullptr, k1 = nullptr;
-  timing_begin(ctx, &k0, &k1);
+  timing_begin(ctx, &k0, &k1, 1);
   switch (key) {
     case 82: hipLaunchKernelGGL((mac_batch2_kernel<8, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
     case 81: hipLaunchKernelGGL((mac_batch2_kernel<8, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
@@ -1120,7 +1144,7 @@ int launch_mac_level_multi(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
   hipEvent_t k0 = nullptr, k1 = nullptr;
-  timing_begin(ctx, &k0, &k1);
+  timing_begin(ctx, &k0, &k1, 1);
   hipLaunchKernelGGL(mac_direct_multi_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, m);
   timing_end(ctx, k0, k1, 1);
   timing_end(ctx, e0, e1);
