@@ -340,21 +340,51 @@ imported_database *PIRReplyGenerator::importData(uint64_t offset, uint64_t bytes
     fail(XPG_EINVAL, "empty database");
     return nullptr;
   }
-  std::unique_ptr<char[]> staging(new (std::nothrow) char[nb * file_bytes]);  // every byte is read below
-  if (!staging) {
-    fail(XPG_ENOMEM, "host staging of the database");
-    return nullptr;
-  }
-  if (!db_->readStreams(0, nb, offset, file_bytes, staging.get())) {
-    for (uint64_t i = 0; i < nb; i++) {
+  // elements [first, first + count) into out: one block read when the handler has one
+  auto read = [&](uint64_t first, uint64_t count, char *out) {
+    if (db_->readStreams(first, count, offset, file_bytes, out)) return;
+    for (uint64_t i = first; i < first + count; i++) {
       db_->openStream(i, offset);
-      db_->readStream(i, staging.get() + i * file_bytes, file_bytes);
+      db_->readStream(i, out + (i - first) * file_bytes, file_bytes);
       db_->closeStream(i);
     }
-  }
+  };
   xpg_db *h = nullptr;
-  int rc = xpg_db_import(ctx_, &prm_, reinterpret_cast<const uint8_t *>(staging.get()), nb, file_bytes,
-                         param_.alpha, param_.d, param_.n, &h);
+  int rc = XPG_OK;
+  // the elements go to HBM through two pinned chunks, each read while the other's copy
+  // runs, then the import reads them there; a plain host buffer if pinning fails
+  const uint64_t per = std::max<uint64_t>(1, kStageBytes / file_bytes), chunk = per * file_bytes;
+  void *pin[2] = {nullptr, nullptr}, *dev = nullptr;
+  bool staged = xpg_host_alloc(ctx_, chunk, &pin[0]) == XPG_OK && xpg_host_alloc(ctx_, chunk, &pin[1]) == XPG_OK &&
+                xpg_dev_alloc(ctx_, nb * file_bytes + 16, &dev) == XPG_OK;
+  if (staged) {
+    bool pending = false;
+    int k = 0;
+    for (uint64_t first = 0; first < nb && rc == XPG_OK; first += per, k ^= 1) {
+      const uint64_t count = std::min(per, nb - first);
+      read(first, count, static_cast<char *>(pin[k]));
+      if (pending) rc = xpg_sync(ctx_);  // the other chunk's copy, issued a chunk ago
+      if (rc == XPG_OK)
+        rc = xpg_h2d_async(ctx_, static_cast<char *>(dev) + first * file_bytes, pin[k], count * file_bytes);
+      pending = true;
+    }
+    if (rc == XPG_OK)
+      rc = xpg_db_import_dev(ctx_, &prm_, static_cast<const uint8_t *>(dev), nb, file_bytes, param_.alpha, param_.d,
+                             param_.n, &h);
+  }
+  for (void *p : pin)
+    if (p) xpg_host_free(ctx_, p);
+  if (dev) xpg_dev_free(ctx_, dev);
+  if (!staged) {
+    std::unique_ptr<char[]> host(new (std::nothrow) char[nb * file_bytes]);  // every byte is read below
+    if (!host) {
+      fail(XPG_ENOMEM, "host staging of the database");
+      return nullptr;
+    }
+    read(0, nb, host.get());
+    rc = xpg_db_import(ctx_, &prm_, reinterpret_cast<const uint8_t *>(host.get()), nb, file_bytes, param_.alpha,
+                       param_.d, param_.n, &h);
+  }
   if (rc != XPG_OK) {
     fail(rc, "importData");
     return nullptr;
