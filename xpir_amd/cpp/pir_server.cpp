@@ -169,49 +169,54 @@ int main(int argc, char **argv) {
       if (c >= 0 || errno != EINTR) return c;
     }
   };
-  auto serve_one = [&](int fd, PIRSession **keep) -> bool {
+  // one session on a connected descriptor; what a configuration session leaves behind
+  // (its parameters and kept database) is taken before the stream closes
+  struct Outcome {
+    bool dry_run = false, failed = false;
+    PIRParameters params{};
+    imported_database *kept = nullptr;
+  };
+  auto serve_one = [&](int fd) -> Outcome {
     FdStream stream(fd, /*owned=*/true);
-    PIRSession *s = new PIRSession(stream, db, (int)device);
-    s->setOptimDirectory(optim_dir);
-    s->no_pipeline(no_pipeline);
-    if (server_driven) s->setPIRParams(saved);
-    const bool dry_run = s->start(opt);
-    if (s->failed()) fprintf(stderr, "pir_server: session failed: %s\n", s->lastError().c_str());
-    else if (!dry_run)
+    PIRSession s(stream, db, (int)device);
+    s.setOptimDirectory(optim_dir);
+    s.no_pipeline(no_pipeline);
+    if (server_driven) s.setPIRParams(saved);
+    Outcome o;
+    o.dry_run = s.start(opt);
+    o.failed = s.failed();
+    if (o.failed) fprintf(stderr, "pir_server: session failed: %s\n", s.lastError().c_str());
+    else if (!o.dry_run)
       fprintf(stderr,
               "pir_server: session done, %llu reply ciphertexts; query upload %.3f ms, import %.3f ms%s, "
               "reply %.3f ms, send %.3f ms\n",
-              (unsigned long long)s->repliesSent(), s->times().upload_ms, s->times().import_ms,
+              (unsigned long long)s.repliesSent(), s.times().upload_ms, s.times().import_ms,
               opt.got_preimported_database ? " (kept database)" : (no_pipeline ? "" : " (during the upload)"),
-              s->times().reply_ms, s->times().send_ms);
-    if (keep)
-      *keep = s;
-    else
-      delete s;
-    return dry_run;
+              s.times().reply_ms, s.times().send_ms);
+    o.params = s.getPIRParams();
+    o.kept = s.getSavedDatabase();
+    return o;
   };
 
   uint64_t served = 0;
   if (driven == 1) {  // PIRServer::processDrivenSession (:70-104)
     opt.driven_mode = true;
     opt.keep_database = true;
-    PIRSession *s = nullptr;
-    for (;;) {
+    Outcome o;
+    do {  // dry runs loop
       const int fd = next_client();
       if (fd < 0) return 1;
-      delete s;
-      s = nullptr;
-      if (!serve_one(fd, &s)) break;  // dry runs loop
-    }
+      delete o.kept;
+      o = serve_one(fd);
+    } while (o.dry_run);
     served++;
-    if (s->failed()) {
+    if (o.failed) {
       fprintf(stderr, "pir_server: the configuration session failed\n");
-      delete s;
+      delete o.kept;
       return 1;
     }
-    saved = s->getPIRParams();
-    opt.data = s->getSavedDatabase();
-    delete s;
+    saved = o.params;
+    opt.data = o.kept;
     opt.driven_mode = false;
     opt.keep_database = false;
     opt.got_preimported_database = opt.data != nullptr;
@@ -223,7 +228,9 @@ int main(int argc, char **argv) {
   while (!sessions || served < sessions) {
     const int fd = next_client();
     if (fd < 0) break;
-    if (!serve_one(fd, nullptr)) served++;
+    const Outcome o = serve_one(fd);
+    delete o.kept;  // only a keep_database session keeps one
+    if (!o.dry_run) served++;
   }
   delete opt.data;
   delete db;
