@@ -553,6 +553,7 @@ def bench_session(args):
                              "client_upload_s": t1 - t0, "wait_first_reply_s": t_first - t1,
                              "download_s": t2 - t_first, "session_s": session_s,
                              "db_gbs": raw / session_s / 1e9, "server": srv, "decrypts_to_element": bool(ok)})
+        concurrent = session_clients(ctx, server, tmp, cfg, data, n, L, clients=4)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
         ctx.close()
@@ -564,9 +565,100 @@ def bench_session(args):
         "config": {"workload": cfg["name"] + " via pir_server", "crypto": cfg["crypto"], "n_elems": n,
                    "elem_bytes": L},
         "sessions": sessions,
-        "all_decrypt_to_element": all(s["decrypts_to_element"] for s in sessions),
+        "concurrent": concurrent,
+        "all_decrypt_to_element": all(s["decrypts_to_element"] for s in sessions) and concurrent["all_decrypt"],
     }
     print(json.dumps(out), flush=True)
+
+
+def session_clients(ctx, server, tmp, cfg, data, n, L, clients):
+    """`clients` d = 1 sessions at once against pir_server --concurrent (each client on a
+    thread of its own, each with its own query index): wall time from the first query
+    byte sent to the last reply byte received, and the aggregate raw-DB rate."""
+    import socket
+    import struct
+    import subprocess
+    import threading
+
+    import xpir_amd
+    from xpir_amd.pir_client import PIRClient
+    dims = [n]
+    prm = xpir_amd.params(cfg["crypto"], n)
+    sk = xpir_amd.keygen(prm, cfg["key_seed"])
+    skd = ctx.alloc(sk.nbytes).upload(sk)
+    ct = 16 * prm.m * prm.n
+    qwords = n * ct // 8
+    indices = [(cfg["index"] + 9973 * k) % n for k in range(clients)]
+    queries = []
+    for k, index in enumerate(indices):
+        qd = ctx.alloc(qwords * 8)
+        ctx.query_gen_dev(prm, skd, dims, index, cfg["noise_seed"] + k, qd)
+        queries.append(qd.download(np.uint64, qwords))
+        qd.free()
+    path = os.path.join(tmp, "pir_c.sock")
+    lst = socket.socket(socket.AF_UNIX, socket.SOCK_STREAM)
+    lst.bind(path)
+    lst.listen(clients)
+    logp = os.path.join(tmp, "server_c.log")
+    with open(logp, "w") as logf:
+        proc = subprocess.Popen([server, "--listen-fd", str(lst.fileno()), "--db-dir", os.path.join(tmp, "db"),
+                                 "-s", str(n), "--sessions", str(clients), "--concurrent", str(clients)],
+                                pass_fds=[lst.fileno()], stdout=logf, stderr=subprocess.STDOUT)
+    replies, errors = [None] * clients, []
+    try:
+        conns = []
+        for _ in range(clients):
+            c = PIRClient(path, timeout=600)
+            c.hello()
+            c.catalog()
+            c.exchange_method()
+            c.optim_exit()
+            c.send_crypto_params(cfg["crypto"] + ":?")
+            c.send_key(struct.pack(f"<{prm.m}Q", *[prm.moduli[i] for i in range(prm.m)]))
+            c.send_pir_params(1, 1, dims)
+            conns.append(c)
+        count = -(-L // (prm.A * prm.n // 8))
+
+        def run(k):
+            try:
+                qb = memoryview(queries[k].view(np.uint8))
+                for i in range(0, len(qb), ct):
+                    conns[k].send(qb[i:i + ct])
+                replies[k] = conns[k].recv(count * ct)
+            except Exception as e:  # reported in the result
+                errors.append(repr(e))
+
+        log(f"{clients} concurrent d = 1 sessions: uploading {clients} x {qwords * 8 / 2**30:.0f} GiB")
+        threads = [threading.Thread(target=run, args=(k,)) for k in range(clients)]
+        t0 = time.perf_counter()
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        wall = time.perf_counter() - t0
+        for c in conns:
+            c.close()
+        rc = proc.wait(timeout=300)
+    finally:
+        if proc.poll() is None:
+            proc.kill()
+            proc.wait()
+        lst.close()
+    ok = []
+    for k, index in enumerate(indices):
+        if replies[k] is None:
+            ok.append(False)
+            continue
+        rep = np.frombuffer(replies[k], dtype=np.uint64)
+        rd = ctx.alloc(rep.nbytes).upload(rep)
+        ok.append(ctx.extract_reply_dev(prm, skd, rd.ptr, dims, L) == data[index * L:(index + 1) * L].tobytes())
+        rd.free()
+    skd.free()
+    log(f"{clients} concurrent sessions: {wall:.3f} s wall, {clients * n * L / wall / 1e9:.2f} GB/s raw DB "
+        f"aggregate, server rc {rc}, decrypt {ok}, errors {errors}")
+    return {"clients": clients, "dims": dims, "wall_s": wall, "aggregate_db_gbs": clients * n * L / wall / 1e9,
+            "query_gbs_aggregate": clients * qwords * 8 / wall / 1e9, "server_rc": rc, "errors": errors,
+            "all_decrypt": bool(all(ok)) and not errors}
 
 
 def bench_batched(args):

@@ -159,3 +159,46 @@ def test_server_driven_from_params_file(tmp_path):
         assert srv.wait(120) == 0, srv.log()
     assert np.array_equal(got, want)
     assert extract_reply(orc, sk, got, dims, unit) == elem
+
+
+def test_concurrent_client_sessions(tmp_path):
+    """three clients at once (--concurrent 3): uploads overlap, every session imports the
+    database through the shared, locked handler, and each reply decrypts to its element"""
+    import threading
+    crypto, n, L, dims = "LWE:80:2048:120", 30, 4096, [6, 5]
+    orc = Oracle(crypto, dims[0])
+    db, files = _make_db(tmp_path, orc, n, L)
+    sk = orc.keygen(4)
+    indices = [3, 17, 29]
+    results, errors = {}, []
+    with Server(tmp_path, "--db-dir", db, "--sessions", 3, "--concurrent", 3) as srv:
+        # the catalog order, from a first handshake, fixes what each index means
+        clients = [srv.client(timeout=180) for _ in indices]
+        names = None
+        for c in clients:
+            c.hello()
+            names = _client_driven(c, orc, crypto, 2, 1, dims, files, L)
+        expected = {i: _expected(orc, files, names, L, 1, dims, sk, i, 3 + i) for i in indices}
+
+        def run(c, i):
+            try:
+                q, want, elem, unit = expected[i]
+                results[i] = _exchange_query(c, orc, q, want)
+            except Exception as e:  # reported below
+                errors.append((i, repr(e)))
+
+        print("sessions:", indices, flush=True)
+        threads = [threading.Thread(target=run, args=(c, i)) for c, i in zip(clients, indices)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=300)
+        for c in clients:
+            c.close()
+        assert srv.wait(120) == 0, srv.log()
+        assert srv.log().count("session done") == 3, srv.log()
+    assert not errors, errors
+    for i in indices:
+        q, want, elem, unit = expected[i]
+        assert np.array_equal(results[i], want), i
+        assert extract_reply(orc, sk, results[i], dims, unit) == elem, i

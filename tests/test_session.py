@@ -180,3 +180,39 @@ def test_usage_errors():
     for args in ([], ["--listen-fd", "3", "--fd", "0"], ["--fd", "0", "-p", "1234"], ["--fd", "x"]):
         r = subprocess.run([SERVER, *args], capture_output=True, text=True, timeout=30, stdin=subprocess.DEVNULL)
         assert r.returncode == 2, args
+
+
+def test_concurrent_sessions(tmp_path):
+    """--concurrent 2 serves a second client while the first one's session is still open;
+    the default (one at a time) leaves the second waiting until the first ends."""
+    import socket as _socket
+    with Server(tmp_path, "--db-generator", "-n", 5, "-l", 64, "--concurrent", 2) as srv:
+        a = srv.client()
+        a.hello(is_client=False)
+        assert a.optim_data() == (5, 64)  # a's session is open
+        b = srv.client(timeout=10)
+        b.hello(is_client=False)
+        assert b.optim_data() == (5, 64)  # served beside it
+        b.optim_exit()
+        assert b.at_eof()
+        a.optim_exit()
+        assert a.at_eof()
+        a.close()
+        b.close()
+    (tmp_path / "pir.sock").unlink()
+    with Server(tmp_path, "--db-generator", "-n", 5, "-l", 64) as srv:
+        a = srv.client()
+        a.hello(is_client=False)
+        assert a.optim_data() == (5, 64)
+        b = srv.client(timeout=1.0)
+        b.hello(is_client=False)
+        b._pack("Q", 1)  # DATA: not answered while a's session holds the server
+        with pytest.raises(_socket.timeout):
+            b.recv(8)
+        a.optim_exit()
+        assert a.at_eof()
+        b.s.settimeout(10)
+        assert b.recv(16)  # now b's session runs
+        b.optim_exit()
+        a.close()
+        b.close()

@@ -10,16 +10,24 @@
 //
 // The sockets come from outside: --listen-fd N accepts clients on an inherited listening
 // socket (a supervisor's, e.g. systemd socket activation), --fd N serves one client on
-// an inherited connected socket (inetd-style).  Sessions are served one after another;
-// the reference runs them on concurrent threads, which a single GPU would serialise.
+// an inherited connected socket (inetd-style).  Sessions are served one after another,
+// or up to N at once with --concurrent N (the reference runs every session on a thread of
+// its own): their uploads and downloads overlap, their GPU work shares the one GPU, and
+// their database imports take turns on the database handler.
 #include <errno.h>
+#include <poll.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <sys/socket.h>
 #include <unistd.h>
 
+#include <atomic>
+#include <condition_variable>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "libpir_amd.hpp"
 #include "pir_session.hpp"
@@ -43,9 +51,44 @@ void usage(const char *prog) {
           "      --optim-dir DIR            optimizer cache directory (default exp/)\n"
           "      --no-pipeline              import the database after the query upload\n"
           "      --sessions K               exit after K client sessions (default: serve until killed)\n"
+          "      --concurrent N             serve up to N sessions at once (default 1)\n"
           "      --device N                 GPU (default 0)\n",
           prog, kDefaultParamsFile);
 }
+
+// Every call of a shared database handler under one lock: concurrent sessions import
+// through it, and the handlers keep per-stream state (DBHandler.hpp makes no promise of
+// thread safety; importDataNFL is documented as not thread-safe).
+class LockedDB : public DBHandler {
+ public:
+  explicit LockedDB(DBHandler *inner) : in_(inner) {}
+  uint64_t getNbStream() override { return in_->getNbStream(); }
+  uint64_t getmaxFileBytesize() override { return in_->getmaxFileBytesize(); }
+  bool openStream(uint64_t s, uint64_t off) override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->openStream(s, off);
+  }
+  uint64_t readStream(uint64_t s, char *buf, uint64_t size) override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->readStream(s, buf, size);
+  }
+  void closeStream(uint64_t s) override {
+    std::lock_guard<std::mutex> l(mu_);
+    in_->closeStream(s);
+  }
+  std::string getCatalog(bool full) override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->getCatalog(full);
+  }
+  bool readStreams(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->readStreams(first, count, offset, bytes, out);
+  }
+
+ private:
+  DBHandler *in_;
+  std::mutex mu_;
+};
 
 bool number(const char *s, uint64_t *out) {
   if (!s || !*s) return false;
@@ -59,7 +102,7 @@ bool number(const char *s, uint64_t *out) {
 int main(int argc, char **argv) {
   int driven = -1;
   std::string params_file = kDefaultParamsFile, db_dir = "db/", optim_dir = "exp/";
-  uint64_t split = 1, gen_n = 10, gen_l = 12800000, sessions = 0, device = 0;
+  uint64_t split = 1, gen_n = 10, gen_l = 12800000, sessions = 0, device = 0, concurrent = 1;
   bool use_generator = false, no_pipeline = false;
   long listen_fd = -1, conn_fd = -1;
   for (int i = 1; i < argc; i++) {
@@ -104,6 +147,12 @@ int main(int argc, char **argv) {
       no_pipeline = true;
     } else if (a == "--sessions") {
       value(&sessions);
+    } else if (a == "--concurrent") {
+      value(&concurrent);
+      if (concurrent < 1 || concurrent > 256) {
+        fprintf(stderr, "--concurrent takes 1 .. 256\n");
+        return 2;
+      }
     } else if (a == "--device") {
       value(&device);
     } else if (a == "--listen-fd") {
@@ -142,6 +191,7 @@ int main(int argc, char **argv) {
   }
   fprintf(stderr, "pir_server: database of %llu elements of %llu bytes\n", (unsigned long long)db->getNbStream(),
           (unsigned long long)db->getmaxFileBytesize());
+  LockedDB shared_db(db);
 
   session_option_t opt;
   opt.driven_mode = true;
@@ -156,7 +206,11 @@ int main(int argc, char **argv) {
     opt.driven_mode = false;
   }
 
-  // one connected client per iteration: accepted, or the inherited one once
+  // client sessions finished (dry runs excluded); the accept loop stops at `sessions`
+  std::atomic<uint64_t> served{0};
+  auto done = [&]() { return sessions && served.load() >= sessions; };
+  // one connected client per call: accepted, or the inherited one once; -1 when there is
+  // none or the session count is reached
   bool inherited_used = false;
   auto next_client = [&]() -> int {
     if (conn_fd >= 0) {
@@ -165,8 +219,14 @@ int main(int argc, char **argv) {
       return (int)conn_fd;
     }
     for (;;) {
+      if (done()) return -1;
+      pollfd pf{(int)listen_fd, POLLIN, 0};
+      const int ready = poll(&pf, 1, 100);  // wake now and then to see the count
+      if (ready < 0 && errno != EINTR) return -1;
+      if (ready <= 0) continue;
       const int c = accept((int)listen_fd, nullptr, nullptr);
-      if (c >= 0 || errno != EINTR) return c;
+      if (c >= 0) return c;
+      if (errno != EINTR && errno != EAGAIN && errno != ECONNABORTED) return -1;
     }
   };
   // one session on a connected descriptor; what a configuration session leaves behind
@@ -178,7 +238,7 @@ int main(int argc, char **argv) {
   };
   auto serve_one = [&](int fd) -> Outcome {
     FdStream stream(fd, /*owned=*/true);
-    PIRSession s(stream, db, (int)device);
+    PIRSession s(stream, &shared_db, (int)device);
     s.setOptimDirectory(optim_dir);
     s.no_pipeline(no_pipeline);
     if (server_driven) s.setPIRParams(saved);
@@ -198,7 +258,6 @@ int main(int argc, char **argv) {
     return o;
   };
 
-  uint64_t served = 0;
   if (driven == 1) {  // PIRServer::processDrivenSession (:70-104)
     opt.driven_mode = true;
     opt.keep_database = true;
@@ -224,13 +283,35 @@ int main(int argc, char **argv) {
       fprintf(stderr, "Unable to write PIR parameters configuration data, maybe target repertory doesn't exist ?\n");
     server_driven = true;
   }
-  // PIRServer::serve / handleAccept (:58-112)
-  while (!sessions || served < sessions) {
+  // PIRServer::serve / handleAccept (:58-112): a session per client, up to `concurrent`
+  // at once
+  std::mutex mu;
+  std::condition_variable cv;
+  uint64_t active = 0;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> l(mu);
+      cv.wait(l, [&] { return active < concurrent; });
+    }
     const int fd = next_client();
     if (fd < 0) break;
-    const Outcome o = serve_one(fd);
-    delete o.kept;  // only a keep_database session keeps one
-    if (!o.dry_run) served++;
+    {
+      std::lock_guard<std::mutex> l(mu);
+      active++;
+    }
+    // detached: a long-running server must not keep a finished thread per session
+    std::thread([&, fd] {
+      const Outcome o = serve_one(fd);
+      delete o.kept;  // only a keep_database session keeps one
+      if (!o.dry_run) served++;
+      std::lock_guard<std::mutex> l(mu);
+      active--;
+      cv.notify_all();
+    }).detach();
+  }
+  {
+    std::unique_lock<std::mutex> l(mu);
+    cv.wait(l, [&] { return active == 0; });  // every session has ended
   }
   delete opt.data;
   delete db;
