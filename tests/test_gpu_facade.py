@@ -88,3 +88,18 @@ def test_pushed_query_staging(tmp_path):
     print("run: facade_staging", flush=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr + r.stdout
+
+
+def test_client_classes_round_trip(tmp_path):
+    """PIRQueryGenerator and PIRReplyExtraction of the facade (on the GPU) around
+    PIRReplyGenerator, as simplePIR uses them: d = 1 and 2, alpha = 3 with a partial last
+    aggregate, uneven elements; reproducible seeds; refused inputs
+    (tests/cpp/client_classes.cpp)."""
+    exe = tmp_path / "client_classes"
+    pkg = os.path.join(ROOT, "xpir_amd")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "cpp", "client_classes.cpp"),
+                    "-L", pkg, "-lpir_amd", "-lxpir_amd", f"-Wl,-rpath,{pkg}"], check=True)
+    print("run: client_classes", flush=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr + r.stdout

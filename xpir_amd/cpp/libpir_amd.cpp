@@ -10,8 +10,10 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <cmath>
 #include <memory>
 #include <new>
+#include <random>
 
 namespace xpir_amd {
 
@@ -29,6 +31,206 @@ long HomomorphicCrypto::setandgetAbsBitPerCiphertext(unsigned int elt_nbr) {
 
 unsigned int HomomorphicCrypto::getAbsorptionBitsize() const { return (unsigned)(prm_.A * prm_.n); }
 unsigned int HomomorphicCrypto::getCiphertextBitsize() const { return (unsigned)(prm_.ct_bytes * 8); }
+
+void HomomorphicCrypto::setKeySeed(uint64_t seed) {
+  key_seed_ = seed;
+  key_seeded_ = true;
+  sk_.clear();
+}
+
+const std::vector<uint64_t> &HomomorphicCrypto::secretKey() {
+  if (sk_.empty() && ok_) {
+    if (!key_seeded_) {
+      std::random_device rd;
+      key_seed_ = ((uint64_t)rd() << 32) ^ rd();
+      key_seeded_ = true;
+    }
+    sk_.resize((size_t)prm_.m * prm_.n);
+    if (xpg_keygen(&prm_, key_seed_, sk_.data()) != XPG_OK) sk_.clear();
+  }
+  return sk_;
+}
+
+// Device copy of the client's secret key on ctx (freed by the caller with xpg_dev_free).
+static int upload_key(xpg_ctx *ctx, HomomorphicCrypto &crypto, uint64_t **sk_dev) {
+  const std::vector<uint64_t> &sk = crypto.secretKey();
+  if (sk.empty()) return XPG_EINVAL;
+  void *p = nullptr;
+  int rc = xpg_dev_alloc(ctx, sk.size() * 8, &p);
+  if (rc == XPG_OK) rc = xpg_h2d(ctx, p, sk.data(), sk.size() * 8);
+  *sk_dev = static_cast<uint64_t *>(p);
+  return rc;
+}
+
+PIRQueryGenerator::PIRQueryGenerator(PIRParameters &pirParameters, HomomorphicCrypto &cryptoMethod, int device)
+    : param_(pirParameters), crypto_(cryptoMethod) {
+  crypto_.setandgetAbsBitPerCiphertext(param_.n[0]);
+  const int rc = xpg_ctx_create(device, &ctx_);
+  if (rc != XPG_OK) fail(rc, "no GPU context");
+}
+
+PIRQueryGenerator::~PIRQueryGenerator() {
+  for (size_t i = next_; i < elems_.size(); i++) free(elems_[i]);
+  xpg_ctx_destroy(ctx_);
+}
+
+int PIRQueryGenerator::fail(int code, const std::string &msg) {
+  status_ = code;
+  error_ = msg;
+  if (ctx_ && code != XPG_OK) {
+    const char *e = xpg_last_error(ctx_);
+    if (e && *e) error_ += std::string(": ") + e;
+  }
+  return code;
+}
+
+void PIRQueryGenerator::setNoiseSeed(uint64_t seed) {
+  noise_seed_ = seed;
+  noise_seeded_ = true;
+}
+
+uint64_t PIRQueryGenerator::getQueryElementBytesize() { return crypto_.params().ct_bytes; }
+
+void PIRQueryGenerator::generateQuery(uint64_t chosenElement) {
+  status_ = XPG_OK;
+  for (size_t i = next_; i < elems_.size(); i++) free(elems_[i]);
+  elems_.clear();
+  next_ = 0;
+  if (!ctx_) {
+    fail(XPG_EDEVICE, "no GPU context");
+    return;
+  }
+  const xpg_params &prm = crypto_.params();
+  uint64_t units = 1, nelems = 0;
+  for (unsigned l = 0; l < param_.d; l++) {
+    units *= param_.n[l];
+    nelems += param_.n[l];
+  }
+  if (param_.d == 0 || param_.d > MAX_REC_LVL || chosenElement >= units) {
+    fail(XPG_EINVAL, "chosen element outside the PIR grid");
+    return;
+  }
+  if (!noise_seeded_) {
+    std::random_device rd;
+    noise_seed_ = ((uint64_t)rd() << 32) ^ rd();
+  }
+  noise_seeded_ = false;  // a seed set by the caller serves one query
+  uint64_t *sk = nullptr;
+  void *q = nullptr;
+  const uint64_t bytes = nelems * prm.ct_bytes;
+  std::vector<char> host(bytes);
+  int rc = upload_key(ctx_, crypto_, &sk);
+  if (rc == XPG_OK) rc = xpg_dev_alloc(ctx_, bytes, &q);
+  if (rc == XPG_OK)
+    rc = xpg_query_gen_dev(ctx_, &prm, sk, param_.d, param_.n, chosenElement, noise_seed_, static_cast<uint64_t *>(q));
+  if (rc == XPG_OK) rc = xpg_d2h(ctx_, host.data(), q, bytes);
+  if (q) xpg_dev_free(ctx_, q);
+  if (sk) xpg_dev_free(ctx_, sk);
+  if (rc != XPG_OK) {
+    fail(rc, "generateQuery");
+    return;
+  }
+  for (uint64_t e = 0; e < nelems; e++) {
+    char *b = static_cast<char *>(malloc(prm.ct_bytes));
+    memcpy(b, host.data() + e * prm.ct_bytes, prm.ct_bytes);
+    elems_.push_back(b);
+  }
+}
+
+bool PIRQueryGenerator::popQuery(char **query) {
+  if (next_ >= elems_.size()) return false;
+  *query = elems_[next_];
+  elems_[next_++] = nullptr;
+  return true;
+}
+
+PIRReplyExtraction::PIRReplyExtraction(PIRParameters &pirParameters, HomomorphicCrypto &cryptoMethod, int device)
+    : param_(pirParameters), crypto_(cryptoMethod) {
+  crypto_.setandgetAbsBitPerCiphertext(param_.n[0]);
+  const int rc = xpg_ctx_create(device, &ctx_);
+  if (rc != XPG_OK) fail(rc, "no GPU context");
+}
+
+PIRReplyExtraction::~PIRReplyExtraction() {
+  for (char *r : replies_) free(r);
+  for (size_t i = next_; i < chunks_.size(); i++) free(chunks_[i]);
+  xpg_ctx_destroy(ctx_);
+}
+
+int PIRReplyExtraction::fail(int code, const std::string &msg) {
+  status_ = code;
+  error_ = msg;
+  if (ctx_ && code != XPG_OK) {
+    const char *e = xpg_last_error(ctx_);
+    if (e && *e) error_ += std::string(": ") + e;
+  }
+  return code;
+}
+
+void PIRReplyExtraction::pushEncryptedReply(char *rawBytes) { replies_.push_back(rawBytes); }
+
+uint64_t PIRReplyExtraction::getPlaintextReplyBytesize() { return crypto_.params().abs_bytes; }
+
+uint64_t PIRReplyExtraction::getnbPlaintextReplies(uint64_t maxFileBytesize) {
+  return (uint64_t)ceil((float)maxFileBytesize * param_.alpha / getPlaintextReplyBytesize());
+}
+
+void PIRReplyExtraction::extractReply(uint64_t maxFileBytesize) {
+  status_ = XPG_OK;
+  for (size_t i = next_; i < chunks_.size(); i++) free(chunks_[i]);
+  chunks_.clear();
+  next_ = 0;
+  if (!ctx_) {
+    fail(XPG_EDEVICE, "no GPU context");
+    return;
+  }
+  const xpg_params &prm = crypto_.params();
+  const uint64_t ct = prm.ct_bytes, ab = prm.abs_bytes;
+  // ciphertexts per level, bottom up: level-l plaintexts are the level-(l-1) ciphertexts
+  std::vector<uint64_t> counts{(maxFileBytesize * param_.alpha + ab - 1) / ab};
+  for (unsigned l = 1; l < param_.d; l++) counts.push_back((counts.back() * ct + ab - 1) / ab);
+  if (replies_.size() != counts.back()) {
+    fail(XPG_ESIZE, "pushed reply ciphertexts do not match the element size");
+    return;
+  }
+  uint64_t *sk = nullptr;
+  void *cur = nullptr, *out = nullptr;
+  int rc = upload_key(ctx_, crypto_, &sk);
+  if (rc == XPG_OK) rc = xpg_dev_alloc(ctx_, replies_.size() * ct, &cur);
+  for (size_t i = 0; rc == XPG_OK && i < replies_.size(); i++)
+    rc = xpg_h2d(ctx_, static_cast<char *>(cur) + i * ct, replies_[i], ct);
+  for (int l = (int)param_.d - 1; l >= 0 && rc == XPG_OK; l--) {
+    rc = xpg_dev_alloc(ctx_, counts[l] * ab + 8, &out);
+    if (rc == XPG_OK)
+      rc = xpg_decrypt_dev(ctx_, &prm, sk, static_cast<const uint64_t *>(cur), counts[l], static_cast<uint8_t *>(out));
+    xpg_dev_free(ctx_, cur);
+    cur = out;
+    out = nullptr;
+  }
+  std::vector<char> plain(counts[0] * ab);
+  if (rc == XPG_OK) rc = xpg_d2h(ctx_, plain.data(), cur, plain.size());
+  if (cur) xpg_dev_free(ctx_, cur);
+  if (sk) xpg_dev_free(ctx_, sk);
+  for (char *r : replies_) free(r);
+  replies_.clear();
+  if (rc != XPG_OK) {
+    fail(rc, "extractReply");
+    return;
+  }
+  const uint64_t nb = std::min<uint64_t>(getnbPlaintextReplies(maxFileBytesize), counts[0]);
+  for (uint64_t i = 0; i < nb; i++) {
+    char *b = static_cast<char *>(malloc(ab));
+    memcpy(b, plain.data() + i * ab, ab);
+    chunks_.push_back(b);
+  }
+}
+
+bool PIRReplyExtraction::popPlaintextResult(char **result) {
+  if (next_ >= chunks_.size()) return false;
+  *result = chunks_[next_];
+  chunks_[next_++] = nullptr;
+  return true;
+}
 
 HomomorphicCrypto *HomomorphicCryptoFactory::getCryptoMethod(std::string cryptoParams) {
   HomomorphicCrypto *c = new HomomorphicCrypto(cryptoParams);

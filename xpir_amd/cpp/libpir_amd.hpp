@@ -53,11 +53,19 @@ class HomomorphicCrypto {
   const std::string &descriptor() const { return desc_; }
   const xpg_params &params() const { return prm_; }
   bool valid() const { return ok_; }
+  // The client's secret key (m*n NTT-domain words), drawn on first use from a seed the
+  // system's random source gives (the reference keys its generator from /dev/urandom,
+  // crypto/prng/fastrandombytes.cpp:21-24), or from setKeySeed, for reproducible keys.
+  void setKeySeed(uint64_t seed);
+  const std::vector<uint64_t> &secretKey();
 
  private:
   std::string desc_;
   xpg_params prm_;
   bool ok_;
+  std::vector<uint64_t> sk_;
+  uint64_t key_seed_ = 0;
+  bool key_seeded_ = false;
 };
 
 class HomomorphicCryptoFactory {
@@ -185,6 +193,67 @@ class imported_database {
   uint64_t nbElements = 0;
   uint64_t polysPerElement = 0;
   uint64_t beforeImportElementBytesize = 0;
+};
+
+// Client side, on the GPU (libpir.hpp:41-91, pir/libpir.cpp:42-69): the query for one
+// aggregate of the PIR grid, as dims[l] encryptions of 0 or 1 per level
+// (PIRQueryGenerator_internal.cpp:37-70), handed out one malloc'd element at a time.
+class PIRQueryGenerator {
+ public:
+  PIRQueryGenerator(PIRParameters &pirParameters, HomomorphicCrypto &cryptoMethod, int device = 0);
+  ~PIRQueryGenerator();
+  // chosenElement: the aggregate's index on the grid (element index / alpha)
+  void generateQuery(uint64_t chosenElement);
+  // one query element (16*n*m bytes, a||b), level after level; ownership passes to the
+  // caller; false when none is left
+  bool popQuery(char **query);
+  uint64_t getQueryElementBytesize();
+  // the encryption noise stream of the next query (default: drawn from the system)
+  void setNoiseSeed(uint64_t seed);
+  int lastStatus() const { return status_; }
+  const std::string &lastError() const { return error_; }
+
+ private:
+  int fail(int code, const std::string &msg);
+  PIRParameters param_;
+  HomomorphicCrypto &crypto_;
+  xpg_ctx *ctx_ = nullptr;
+  std::vector<char *> elems_;
+  size_t next_ = 0;
+  uint64_t noise_seed_ = 0;
+  bool noise_seeded_ = false;
+  int status_ = XPG_OK;
+  std::string error_;
+};
+
+// Client side, on the GPU (libpir.hpp:171-193, pir/libpir.cpp:171-203): the reply
+// ciphertexts decrypted level by level (PIRReplyExtraction_internal.cpp:41-106) into
+// plaintext chunks of A*n/8 bytes, handed out one malloc'd chunk at a time.
+class PIRReplyExtraction {
+ public:
+  PIRReplyExtraction(PIRParameters &pirParameters, HomomorphicCrypto &cryptoMethod, int device = 0);
+  ~PIRReplyExtraction();
+  // one reply ciphertext (16*n*m bytes) as popReply gave it; ownership passes here
+  void pushEncryptedReply(char *rawBytes);
+  // decrypts the pushed reply of an aggregate of alpha elements of maxFileBytesize bytes
+  void extractReply(uint64_t maxFileBytesize);
+  bool popPlaintextResult(char **result);
+  uint64_t getPlaintextReplyBytesize();
+  // ceil(maxFileBytesize * alpha / getPlaintextReplyBytesize()), in float as the reference
+  uint64_t getnbPlaintextReplies(uint64_t maxFileBytesize);
+  int lastStatus() const { return status_; }
+  const std::string &lastError() const { return error_; }
+
+ private:
+  int fail(int code, const std::string &msg);
+  PIRParameters param_;
+  HomomorphicCrypto &crypto_;
+  xpg_ctx *ctx_ = nullptr;
+  std::vector<char *> replies_;
+  std::vector<char *> chunks_;
+  size_t next_ = 0;
+  int status_ = XPG_OK;
+  std::string error_;
 };
 
 class PIRReplyGenerator {
