@@ -554,6 +554,7 @@ def bench_session(args):
                              "download_s": t2 - t_first, "session_s": session_s,
                              "db_gbs": raw / session_s / 1e9, "server": srv, "decrypts_to_element": bool(ok)})
         concurrent = session_clients(ctx, server, tmp, cfg, data, n, L, clients=4)
+        kept = session_kept_db(ctx, server, tmp, cfg, data, n, L)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
         ctx.close()
@@ -566,9 +567,96 @@ def bench_session(args):
                    "elem_bytes": L},
         "sessions": sessions,
         "concurrent": concurrent,
-        "all_decrypt_to_element": all(s["decrypts_to_element"] for s in sessions) and concurrent["all_decrypt"],
+        "kept_database": kept,
+        "all_decrypt_to_element": (all(s["decrypts_to_element"] for s in sessions) and concurrent["all_decrypt"]
+                                   and kept["decrypts_to_element"]),
     }
     print(json.dumps(out), flush=True)
+
+
+def session_kept_db(ctx, server, tmp, cfg, data, n, L):
+    """The reference's -z mode: a first client configures d = 2 over 256 x 256 and its
+    imported database stays resident; a second client is then told the parameters and
+    answered from it (no import in its session).  Times the second session."""
+    import socket
+    import struct
+    import subprocess
+
+    import xpir_amd
+    from xpir_amd.pir_client import PIRClient
+    dims = [256, 256]
+    prm = xpir_amd.params(cfg["crypto"], dims[0])
+    sk = xpir_amd.keygen(prm, cfg["key_seed"])
+    skd = ctx.alloc(sk.nbytes).upload(sk)
+    ct = 16 * prm.m * prm.n
+    abs_b = prm.A * prm.n // 8
+    count = -(-(-(-L // abs_b)) * ct // abs_b)  # two levels of reply ciphertexts
+    key = struct.pack(f"<{prm.m}Q", *[prm.moduli[i] for i in range(prm.m)])
+    qwords = sum(dims) * ct // 8
+
+    def query(index, seed):
+        qd = ctx.alloc(qwords * 8)
+        ctx.query_gen_dev(prm, skd, dims, index, seed, qd)
+        q = qd.download(np.uint64, qwords)
+        qd.free()
+        return memoryview(q.view(np.uint8))
+
+    path = os.path.join(tmp, "pir_k.sock")
+    lst = socket.socket(socket.AF_UNIX, socket.SOCK_STREAM)
+    lst.bind(path)
+    lst.listen(2)
+    logp = os.path.join(tmp, "server_k.log")
+    with open(logp, "w") as logf:
+        proc = subprocess.Popen([server, "--listen-fd", str(lst.fileno()), "--db-dir", os.path.join(tmp, "db"),
+                                 "-s", str(n), "--sessions", "2", "-z", os.path.join(tmp, "PIRParams.cfg")],
+                                pass_fds=[lst.fileno()], stdout=logf, stderr=subprocess.STDOUT)
+    try:
+        c = PIRClient(path, timeout=300)  # the configuring session
+        c.hello()
+        c.catalog()
+        c.exchange_method()
+        c.optim_exit()
+        c.send_crypto_params(cfg["crypto"] + ":?")
+        c.send_key(key)
+        c.send_pir_params(2, 1, dims)
+        qb = query(cfg["index"] % n, cfg["noise_seed"])
+        for i in range(0, len(qb), ct):
+            c.send(qb[i:i + ct])
+        c.recv(count * ct)
+        c.close()
+        index = (cfg["index"] + 4321) % n
+        qb = query(index, cfg["noise_seed"] + 1)
+        log("kept database: second session")
+        t0 = time.perf_counter()
+        c = PIRClient(path, timeout=300)
+        c.hello()
+        c.catalog()
+        c.exchange_method()
+        c.recv_crypto_params()
+        c.send_key(key)
+        c.recv_pir_params()
+        t1 = time.perf_counter()
+        for i in range(0, len(qb), ct):
+            c.send(qb[i:i + ct])
+        reply = c.recv(count * ct)
+        t2 = time.perf_counter()
+        c.close()
+        rc = proc.wait(timeout=120)
+    finally:
+        if proc.poll() is None:
+            proc.kill()
+            proc.wait()
+        lst.close()
+    rd = ctx.alloc(len(reply)).upload(np.frombuffer(reply, dtype=np.uint64))
+    ok = ctx.extract_reply_dev(prm, skd, rd.ptr, dims, L) == data[index * L:(index + 1) * L].tobytes()
+    rd.free()
+    skd.free()
+    text = open(logp).read().strip().splitlines()
+    log(f"kept database: second session {t2 - t0:.3f} s (handshake {t1 - t0:.4f} s), server rc {rc}, "
+        f"decrypts to element: {ok}; {text[-1] if text else ''}")
+    return {"dims": dims, "session_s": t2 - t0, "handshake_s": t1 - t0, "query_and_reply_s": t2 - t1,
+            "db_gbs": n * L / (t2 - t0) / 1e9, "server_rc": rc, "server_line": text[-1] if text else "",
+            "decrypts_to_element": bool(ok)}
 
 
 def session_clients(ctx, server, tmp, cfg, data, n, L, clients):

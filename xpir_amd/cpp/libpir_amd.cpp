@@ -11,11 +11,47 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <random>
 
 namespace xpir_amd {
+
+namespace {
+// Idle GPU contexts, kept for the next generator on the same device: a context keeps its
+// NTT tables and grown buffers, so a session after the first starts warm.  Never freed
+// (the process's end releases them); at most kPoolMax are kept.
+std::mutex g_pool_mu;
+std::multimap<int, xpg_ctx *> *g_pool = new std::multimap<int, xpg_ctx *>();
+const size_t kPoolMax = 8;
+
+int acquire_ctx(int device, xpg_ctx **out) {
+  {
+    std::lock_guard<std::mutex> l(g_pool_mu);
+    auto it = g_pool->find(device);
+    if (it != g_pool->end()) {
+      *out = it->second;
+      g_pool->erase(it);
+      return XPG_OK;
+    }
+  }
+  return xpg_ctx_create(device, out);
+}
+
+void release_ctx(int device, xpg_ctx *ctx) {
+  if (!ctx) return;
+  if (xpg_sync(ctx) == XPG_OK) {  // nothing of the last owner still runs on it
+    std::lock_guard<std::mutex> l(g_pool_mu);
+    if (g_pool->size() < kPoolMax) {
+      g_pool->emplace(device, ctx);
+      return;
+    }
+  }
+  xpg_ctx_destroy(ctx);
+}
+}  // namespace
 
 HomomorphicCrypto::HomomorphicCrypto(const std::string &desc) : desc_(desc) {
   memset(&prm_, 0, sizeof(prm_));
@@ -446,7 +482,7 @@ PIRReplyGenerator::PIRReplyGenerator(PIRParameters &param, HomomorphicCrypto &cr
   // setPirParams: A depends on n[0] (PIRReplyGeneratorNFL_internal.cpp:809-816)
   crypto_.setandgetAbsBitPerCiphertext(param_.n[0]);
   prm_ = crypto_.params();
-  int rc = xpg_ctx_create(device, &ctx_);
+  int rc = acquire_ctx(device, &ctx_);
   if (rc != XPG_OK) fail(rc, "no GPU context");
 }
 
@@ -457,9 +493,14 @@ PIRReplyGenerator::~PIRReplyGenerator() {
     for (char *s : stage_)
       if (s) xpg_host_free(qctx_, s);
     if (qdev_) xpg_dev_free(qctx_, qdev_);
-    xpg_ctx_destroy(qctx_);
+    release_ctx(device_, qctx_);
   }
-  xpg_ctx_destroy(ctx_);
+  // a context that imported a database the caller may still hold is destroyed (which
+  // detaches the database); one that only answered goes back to the pool
+  if (imported_any_)
+    xpg_ctx_destroy(ctx_);
+  else
+    release_ctx(device_, ctx_);
 }
 
 int PIRReplyGenerator::fail(int code, const std::string &msg) {
@@ -494,7 +535,7 @@ void PIRReplyGenerator::pushQuery(char *rawQuery) {
     for (unsigned l = 0; l < param_.d && l < MAX_REC_LVL; l++) elems += param_.n[l];
     qcap_ = elems * ct;
     void *p = nullptr;
-    if ((stage_status_ = xpg_ctx_create(device_, &qctx_)) != XPG_OK) {
+    if ((stage_status_ = acquire_ctx(device_, &qctx_)) != XPG_OK) {
       stage_error_ = "no GPU context for the query";
       qctx_ = nullptr;
     } else if ((stage_status_ = xpg_dev_alloc(qctx_, qcap_, &p)) != XPG_OK) {
@@ -591,6 +632,7 @@ imported_database *PIRReplyGenerator::importData(uint64_t offset, uint64_t bytes
     fail(rc, "importData");
     return nullptr;
   }
+  imported_any_ = true;
   imported_database *out = new imported_database();
   out->imported_database_ptr = h;
   out->nbElements = (nb + param_.alpha - 1) / param_.alpha;

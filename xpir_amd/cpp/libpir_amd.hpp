@@ -283,6 +283,7 @@ class PIRReplyGenerator {
   DBHandler *db_;
   int device_;
   xpg_ctx *ctx_ = nullptr;
+  bool imported_any_ = false;
   // query staging: its own context (stream), the query in HBM (every level back to back)
   // and two pinned chunks filled in turn while the other one's copy runs
   xpg_ctx *qctx_ = nullptr;
