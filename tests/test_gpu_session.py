@@ -202,3 +202,56 @@ def test_concurrent_client_sessions(tmp_path):
         q, want, elem, unit = expected[i]
         assert np.array_equal(results[i], want), i
         assert extract_reply(orc, sk, results[i], dims, unit) == elem, i
+
+
+def test_concurrent_sessions_on_the_kept_database(tmp_path):
+    """-z with --concurrent 2: after the configuring client, two clients at once are
+    answered from the one kept database (each session on its own pooled GPU context)"""
+    import threading
+    crypto, n, L, dims = "LWE:80:2048:120", 30, 4096, [6, 5]
+    orc = Oracle(crypto, dims[0])
+    db, files = _make_db(tmp_path, orc, n, L)
+    cfg = tmp_path / "PIRParams.cfg"
+    sk = orc.keygen(4)
+    with Server(tmp_path, "-z", cfg, "--db-dir", db, "--sessions", 3, "--concurrent", 2) as srv:
+        c = srv.client(timeout=180)
+        c.hello()
+        names = _client_driven(c, orc, crypto, 2, 1, dims, files, L)
+        q, want, elem, unit = _expected(orc, files, names, L, 1, dims, sk, 5, 3)
+        assert np.array_equal(_exchange_query(c, orc, q, want), want)
+        c.close()
+        indices = [11, 26]
+        expected = {i: _expected(orc, files, names, L, 1, dims, sk, i, 7 + i) for i in indices}
+        clients = []
+        for _ in indices:
+            c = srv.client(timeout=180)
+            c.hello()
+            c.catalog()
+            assert c.exchange_method() == SERVER_DRIVEN
+            c.recv_crypto_params()
+            c.send_key(struct.pack(f"<{orc.m}Q", *orc.p))
+            assert c.recv_pir_params() == (2, 1, dims)
+            clients.append(c)
+        results, errors = {}, []
+
+        def run(c, i):
+            try:
+                q, want, _, _ = expected[i]
+                results[i] = _exchange_query(c, orc, q, want)
+            except Exception as e:  # reported below
+                errors.append((i, repr(e)))
+
+        print("sessions on the kept database:", indices, flush=True)
+        threads = [threading.Thread(target=run, args=(c, i)) for c, i in zip(clients, indices)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=300)
+        for c in clients:
+            c.close()
+        assert srv.wait(120) == 0, srv.log()
+    assert not errors, errors
+    for i in indices:
+        q, want, elem, unit = expected[i]
+        assert np.array_equal(results[i], want), i
+        assert extract_reply(orc, sk, results[i], dims, unit) == elem, i
