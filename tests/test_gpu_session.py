@@ -77,6 +77,8 @@ def _client_driven(c, orc, crypto, d, alpha, dims, files, maxb):
 @pytest.mark.parametrize("crypto,n,L,alpha,dims,no_pipeline", [
     ("LWE:80:2048:120", 30, 4096, 1, [6, 5], False),
     ("LWE:80:1024:60", 12, 3000, 3, [4], True),
+    ("LWE:100:4096:180", 20, 2000, 1, [5, 4], False),  # three moduli, A > 56 absorb
+    ("LWE:80:2048:120", 24, 1000, 1, [3, 4, 2], False),  # three levels
 ])
 def test_client_driven_session(tmp_path, crypto, n, L, alpha, dims, no_pipeline):
     orc = Oracle(crypto, dims[0])
