@@ -137,6 +137,7 @@ int main() {
   run_case("LWE:80:2048:120", {20}, 1, 20, 3000, 13);
   run_case("LWE:80:2048:120", {6, 5}, 1, 30, 4096, 22);
   run_case("LWE:80:1024:60", {4}, 3, 11, 2500, 3);  // the last aggregate is partial
+  run_case("LWE:100:4096:180", {5, 4}, 1, 20, 2000, 17);  // three moduli, A = 78 > 64
   if (failures) return 1;
   printf("ok\n");
   return 0;
