@@ -318,8 +318,6 @@ def main_kernel_name(tuned, n_l):
             k, a, b = e["kernel"], e["a"], e["b"]
             if k == 1:
                 return "mac_direct_kernel"
-            if k == 2:
-                return f"mac_seg_kernel<{a},{b}>"
             if k == 3:
                 return f"mac_stream_kernel<{a or 8},{'true' if b else 'false'}>"
             return f"mac_batch2_kernel<{a},{b}>"
@@ -343,7 +341,7 @@ def autotune_db(ctx, db, max_batch, args, label):
     dt = time.perf_counter() - t0
     if args.tune_file:
         ctx.tune_save(args.tune_file)
-    names = {1: "one-chain", 2: "segmented", 3: "streaming", 4: "blocked"}
+    names = {1: "one-chain", 3: "streaming", 4: "blocked"}
     for e in entries:
         what = f"level n_l={e['n_l']} groups={e['groups']} P={e['P']}" if e["kind"] == 0 else \
             f"batch of {e['qn']} on n_l={e['n_l']} groups={e['groups']} P={e['P']}"

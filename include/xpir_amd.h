@@ -65,10 +65,8 @@ const char *xpg_last_error(const xpg_ctx *ctx);
 void *xpg_ctx_stream(xpg_ctx *ctx);
 int xpg_sync(xpg_ctx *ctx);
 
-/* tuning knobs: "mac_kernel" = 0 (by shape), 1 (one chain per lane), 2 (segmented chains
- * with look-back), 3 (streaming block summaries; what 0 picks for few, long chains);
- * "seg_s" / "seg_w" = segmented tile shape (elements per lane 4|8|12|16|24|32,
- * wavefronts per workgroup 2|4|8|16; 0 = by shape); "stream_e" = elements per streaming
+/* tuning knobs: "mac_kernel" = 0 (by shape), 1 (one chain per lane), 3 (streaming block
+ * summaries; what 0 picks for few, long chains); "stream_e" = elements per streaming
  * block (0 = sized for one pass), "stream_u" = elements per load chunk 4|8|16 (0 = 8),
  * "stream_pf" = 1 to double-buffer the chunks; "graphs" = 1 to replay each reply's
  * launches (every level, the recursion between them) from a HIP graph captured on the
@@ -187,7 +185,7 @@ int xpg_timing_enable(xpg_ctx *ctx, int on);
 /* total milliseconds and launch count of the MAC kernels since the last reset */
 int xpg_timing_read(xpg_ctx *ctx, double *mac_ms, uint64_t *mac_launches, int reset);
 /* the same for the dominant kernel of each MAC alone (the streaming block-summary,
- * segmented, one-chain-per-lane or batched kernel, without its scan or fixup) */
+ * one-chain-per-lane or batched kernel, without its scan) */
 int xpg_timing_read_main(xpg_ctx *ctx, double *ms, uint64_t *launches, int reset);
 
 /* ---- MAC autotuning (no reference counterpart: the reference runs one CPU loop nest
@@ -199,8 +197,7 @@ int xpg_timing_read_main(xpg_ctx *ctx, double *ms, uint64_t *launches, int reset
  * differs), and records the fastest per shape in a table shared by every context on the
  * device; later launches of a recorded shape use it unless an option pins a kernel.
  * kind: 0 = one query's level, 1 = a batch (qn = batch size rounded up to a power of
- * two).  kernel: 1 = one chain per lane, 2 = segmented (a x b = elements per lane x
- * wavefronts), 3 = streaming (a = chunk, b = double-buffered, c = elements per block,
+ * two).  kernel: 1 = one chain per lane, 3 = streaming (a = chunk, b = double-buffered, c = elements per block,
  * 0 = one pass), 4 = register blocks (a groups x b queries per lane); a kind-1 entry
  * with kernel 0 records that the batch runs faster as single-query launches (the batch
  * call then takes that path for batches of this size on this database shape).  A

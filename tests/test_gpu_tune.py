@@ -96,9 +96,9 @@ def test_tune_set_refuses_unbuilt_shapes():
     ctx = xpir_amd.Context(0)
     try:
         base = {"kind": 0, "n": 2048, "m": 2, "P": 1, "n_l": 100, "groups": 4, "qn": 0,
-                "kernel": 2, "a": 8, "b": 4, "c": 0, "ms": 1.0, "default_ms": 1.0}
+                "kernel": 3, "a": 8, "b": 0, "c": 0, "ms": 1.0, "default_ms": 1.0}
         ctx.tune_set([base])
-        for bad in ({"kernel": 2, "a": 5}, {"kernel": 3, "a": 32}, {"kernel": 7},
+        for bad in ({"kernel": 2, "a": 8, "b": 4}, {"kernel": 3, "a": 32}, {"kernel": 7},
                     {"kind": 1, "qn": 4, "kernel": 4, "a": 8, "b": 4}, {"kind": 0, "qn": 3}):
             with pytest.raises(xpir_amd.capi.XpgError):
                 ctx.tune_set([dict(base, **bad)])

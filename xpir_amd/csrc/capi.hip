@@ -172,21 +172,6 @@ int ensure_batch_buf(xpg_ctx *ctx, size_t bytes) {
   return XPG_OK;
 }
 
-int ensure_seg_scratch(xpg_ctx *ctx, size_t bytes) {
-  if (ctx->seg_scratch_bytes >= bytes) return XPG_OK;
-  if (ctx->seg_scratch) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(ctx->seg_scratch);
-  }
-  ctx->seg_scratch = nullptr;
-  ctx->seg_scratch_bytes = 0;
-  ctx->seg_capacity = 0;
-  ctx->epoch++;
-  HIP_OK(hipMalloc(&ctx->seg_scratch, bytes), "segmented MAC status region");
-  ctx->seg_scratch_bytes = bytes;
-  return XPG_OK;
-}
-
 int ensure_scratch(xpg_ctx *ctx, size_t bytes) {
   if (ctx->scratch_bytes >= bytes) return XPG_OK;
   if (ctx->scratch) {
@@ -330,7 +315,6 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
   for (int i = 0; i < 2; i++)
     if (ctx->level_buf[i]) (void)hipFree(ctx->level_buf[i]);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
-  if (ctx->seg_scratch) (void)hipFree(ctx->seg_scratch);
   if (ctx->batch_buf) (void)hipFree(ctx->batch_buf);
   if (ctx->io_buf) (void)hipFree(ctx->io_buf);
   if (ctx->dec_buf) (void)hipFree(ctx->dec_buf);
@@ -353,7 +337,7 @@ static int set_option(xpg_ctx *ctx, const char *key, int64_t value) {
     if (!ctx->graphs) graph_cache_clear(ctx, 0);
     return XPG_OK;
   }
-  if (strcmp(key, "mac_kernel") == 0 && value >= 0 && value <= 3) {
+  if (strcmp(key, "mac_kernel") == 0 && (value == 0 || value == 1 || value == 3)) {
     ctx->mac_mode = (int)value;
     return XPG_OK;
   }
@@ -367,14 +351,6 @@ static int set_option(xpg_ctx *ctx, const char *key, int64_t value) {
   }
   if (strcmp(key, "stream_e") == 0 && value >= 0 && value <= (1 << 20)) {
     ctx->stream_e = (int)value;
-    return XPG_OK;
-  }
-  if (strcmp(key, "seg_s") == 0 && (value == 0 || value == 4 || value == 8 || value == 12 || value == 16 || value == 24 || value == 32)) {
-    ctx->seg_s = (int)value;
-    return XPG_OK;
-  }
-  if (strcmp(key, "seg_w") == 0 && (value == 0 || value == 2 || value == 4 || value == 8 || value == 16)) {
-    ctx->seg_w = (int)value;
     return XPG_OK;
   }
   return set_err(ctx, XPG_EINVAL, std::string("unknown option or value: ") + key);

@@ -85,11 +85,6 @@ struct xpg_ctx {
   // scratch reused across calls (grown on demand): block summaries of the streaming MAC
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
-  // the segmented MAC's ticket and status words (zero between launches) and its maps;
-  // a region of its own, so no other kernel disturbs the zeros
-  void *seg_scratch = nullptr;
-  size_t seg_scratch_bytes = 0;
-  uint64_t seg_capacity = 0;
   uint64_t *batch_buf = nullptr;  // level-0 replies of a batch (d > 1)
   size_t batch_bytes = 0;
   uint64_t *level_buf[2] = {nullptr, nullptr};
@@ -98,10 +93,8 @@ struct xpg_ctx {
   size_t io_bytes = 0;
   uint64_t *dec_buf = nullptr;    // residues and lifted coefficients of xpg_decrypt_dev
   size_t dec_bytes = 0;
-  // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 2 = segmented chains with
-  // look-back, 3 = streaming block summaries
+  // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 3 = streaming block summaries
   int mac_mode = 0;
-  int seg_s = 0, seg_w = 0;  // segmented tile shape (0 = by shape)
   int stream_e = 0, stream_u = 0;  // streaming MAC: elements per block, per load chunk (0 = default)
   int stream_pf = 0;               // streaming MAC: double-buffered chunk loads
   int cu_count = 256;              // compute units of the device
@@ -126,7 +119,6 @@ struct xpg_ctx {
 int set_err(xpg_ctx *ctx, int code, const std::string &msg);
 const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m);
 int ensure_scratch(xpg_ctx *ctx, size_t bytes);
-int ensure_seg_scratch(xpg_ctx *ctx, size_t bytes);
 int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
 int ensure_io_buf(xpg_ctx *ctx, size_t bytes);
@@ -137,9 +129,8 @@ void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b, int tag = 0);
 void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b, int tag = 0);
 
 // ---- tuning (tune.hip) ----
-// Kernel configuration of one MAC launch.  kernel: 1 = one chain per lane, 2 = segmented
-// chains with look-back (a = elements per lane, b = wavefronts per workgroup), 3 =
-// streaming block summaries (a = elements per load chunk, b = double-buffered, c =
+// Kernel configuration of one MAC launch.  kernel: 1 = one chain per lane, 3 = streaming
+// block summaries (a = elements per load chunk, b = double-buffered, c =
 // elements per block, 0 = sized for one pass), 4 = register-blocked (a = groups per lane,
 // b = queries per lane).
 struct LevelCfg {
@@ -194,7 +185,6 @@ int launch_mac_level_multi(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *
                            uint64_t *out);
 // what launch_mac_level runs for this shape with the context's options and no tuning
 LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P);
-bool seg_shape_built(int S, int W);     // segmented tile shapes compiled in
 bool batch_shape_built(int GR, int QR);  // register-block shapes compiled in
 // explicit options (a kernel or a tile shape asked for) win over the tuning table
 bool options_pinned(const xpg_ctx *ctx);

@@ -244,291 +244,22 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
 }
 
 // ------------------------------------------------------------------------------------
-// Segmented exact chains.  With few chains (a single query: 8192 chains of 65536 steps)
-// the element axis has to be split.  Write acc = rho + j*p with rho = acc mod p.  A step
-// with lazy product r = rc + b*p (rc < p, b in {0,1}) gives
+// Exact chains split along the element axis.  With few chains (a single query: 8192
+// chains of 65536 steps) the element axis has to be split.  Write acc = rho + j*p with
+// rho = acc mod p.  A step with lazy product r = rc + b*p (rc < p, b in {0,1}) gives
 //     rho' = (rho + rc) mod p,   carry c = [rho + rc >= p],
 //     j'   = max(j + b + c - 1, 0)
 // (acc + r >= p  <=>  j + b + c >= 1).  The rho chain is a plain modular sum; the j chain
 // is a max-plus map j -> max(j + D, M) whose composition is associative:
-//     (D1,M1) then (D2,M2) = (D1 + D2, max(M1 + D2, M2)).
-// Its per-step coefficients need the incoming rho, which a decoupled look-back over the
-// tiles' residue sums provides while the tile's products wait in registers, so every DB
-// and query word is read from HBM exactly once.  A fixup kernel folds the tiles' (D,M)
-// maps in element order: final word = rho_final + j_final * p, identical to the
-// sequential chain for every input.
-//
-// Tile = W wavefronts x S elements of one column (64 consecutive slots of one
-// (group, poly, modulus)); lane = slot; each lane carries the a and the b chain.
-// Status words (one per tile, lane, chain): state in bits 62-63 (1 = tile aggregate,
-// 2 = inclusive prefix), residue in the low 61 bits; zero = not yet published.  The
-// fixup kernel returns every status word and the tile ticket to zero.
+//     (D1,M1) then (D2,M2) = (D1 + D2, max(M1 + D2, M2)),
+// so the final word rho_final + j_final * p is identical to the sequential chain's.
 
 constexpr int32_t kNegInf = -(1 << 30);
-
-struct SegArgs {
-  ModSet ms;
-  const uint64_t *db;
-  const uint64_t *q;
-  uint64_t *out;
-  uint64_t n_l;
-  uint32_t columns;   // groups * P * m * (n / 64)
-  uint32_t cpg;       // columns per group: P * m * (n / 64)
-  uint64_t tiles;     // per column
-  uint32_t P, n, m, sblocks, log_sb;  // sblocks = n / 64, a power of two
-  uint64_t *status;   // tiles * columns * 128
-  int2 *dm;           // tiles * columns * 128
-  uint32_t *ticket;
-};
+constexpr int FIX_PARTS = 8;  // threads per chain in the scan kernel
 
 __device__ __forceinline__ uint64_t addmod_c(uint64_t a, uint64_t b, uint64_t p) {
   uint64_t s = a + b;
   return s >= p ? s - p : s;
-}
-
-// Residue sum of lazy products (each below 2^62) without a reduction per term: a 64-bit
-// sum and a count of its wraps past 2^64.
-__device__ __forceinline__ void lazy_add(uint64_t &s, uint32_t &wraps, uint64_t r) {
-  s += r;
-  wraps += s < r ? 1u : 0u;
-}
-
-// (s + wraps * 2^64) mod p, from 2^61 = delta and 2^64 = 8 * delta (mod p); wraps < 2^20
-// keeps the folded value below 2p.
-__device__ __forceinline__ uint64_t lazy_sum_mod(uint64_t s, uint32_t wraps, uint64_t p, uint32_t delta) {
-  const uint64_t v = (s & ((1ULL << 61) - 1)) + (uint64_t)((uint32_t)(s >> 61) + 8 * wraps) * delta;
-  return v >= p ? v - p : v;
-}
-
-// One element of a chain in (rho, D, M) form: rho in [0, p) is the running residue and r
-// in [0, 2p) the lazy product.  rho + r = h * 2^61 + l = h * p + v with v = l + h * delta
-// below 2p, so rho + r = k * p + rho' with k = h + [v >= p]; the element changes the
-// chain's count of subtracted-late p's by d = k - 1 (max-plus map (d, 0)).
-__device__ __forceinline__ void chain_step(uint64_t &rho, int32_t &D, int32_t &M, uint64_t r, uint64_t p,
-                                           uint32_t delta) {
-  const uint64_t u = rho + r;
-  const uint32_t h = (uint32_t)(u >> 61);
-  const uint64_t v = (u & ((1ULL << 61) - 1)) + (uint64_t)h * delta;
-  const bool b = v >= p;
-  rho = b ? v - p : v;
-  const int32_t d = (int32_t)h - (b ? 0 : 1);
-  D += d;
-  M = max(M + d, 0);
-}
-
-__device__ __forceinline__ uint64_t ld_status(const uint64_t *ptr) {
-  return __hip_atomic_load(ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_status(uint64_t *ptr, uint64_t v) {
-  __hip_atomic_store(ptr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <int S, int W>
-__global__ void __launch_bounds__(64 * W) mac_seg_kernel(SegArgs a) {
-  __shared__ uint32_t tile_sh;
-  __shared__ uint64_t rsum[W][2][64];
-  __shared__ int2 dmsh[W][2][64];
-  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (threadIdx.x == 0) tile_sh = atomicAdd(a.ticket, 1u);
-  __syncthreads();
-  // the ticket is workgroup-uniform: its decomposition runs once, in 32-bit scalar math.
-  // ticket = tile * columns + column;  column = g * cpg + cw;  cw = (k * m + c) * sblocks + sb
-  const uint32_t ticket = __builtin_amdgcn_readfirstlane(tile_sh);
-  const uint32_t tile = ticket / a.columns, column = ticket - tile * a.columns;
-  const uint32_t g = column / a.cpg, cw = column - g * a.cpg;
-  const uint32_t c = (cw >> a.log_sb) % a.m, sb = cw & (a.sblocks - 1);
-  const uint64_t mn = (uint64_t)a.m * a.n;
-  const uint64_t p = a.ms.p[c];
-  const uint32_t delta = a.ms.delta[c];
-
-  // phase A: lazy products of this lane's S elements, kept in registers
-  const uint64_t e0 = ((uint64_t)tile * W + w) * S;
-  const int valid = e0 >= a.n_l ? 0 : (int)min<uint64_t>(S, a.n_l - e0);
-  const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
-  // (k * m + c) * n + sb * 64 = cw * 64: the column's slots inside its element row
-  const uint64_t *xp = a.db + ((uint64_t)g * a.n_l + e0) * xstride + (uint64_t)cw * 64 + lane;
-  const uint64_t *qp = a.q + e0 * qstride + (uint64_t)c * a.n + sb * 64 + lane;
-  uint64_t ra[S], rb[S];
-  uint64_t sa = 0, sbm = 0;
-  uint32_t wa = 0, wb = 0;
-  if (valid == S) {
-    // all 3*S loads in flight before any product: the tile's HBM requests overlap
-    uint64_t xv[S], av[S], bv[S];
-#pragma unroll
-    for (int j = 0; j < S; j++) {
-      xv[j] = __builtin_nontemporal_load(xp + j * xstride);
-      av[j] = __builtin_nontemporal_load(qp + j * qstride);
-      bv[j] = __builtin_nontemporal_load(qp + j * qstride + mn);
-    }
-#pragma unroll
-    for (int j = 0; j < S; j++) {
-      ra[j] = shoup_lazy_qf(xv[j], av[j], p, delta);
-      rb[j] = shoup_lazy_qf(xv[j], bv[j], p, delta);
-      lazy_add(sa, wa, ra[j]);
-      lazy_add(sbm, wb, rb[j]);
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < S; j++) {
-      ra[j] = rb[j] = 0;
-      if (j < valid) {
-        const uint64_t xv = xp[j * xstride], av = qp[j * qstride], bv = qp[j * qstride + mn];
-        ra[j] = shoup_lazy_qf(xv, av, p, delta);
-        rb[j] = shoup_lazy_qf(xv, bv, p, delta);
-        lazy_add(sa, wa, ra[j]);
-        lazy_add(sbm, wb, rb[j]);
-      }
-    }
-  }
-  rsum[w][0][lane] = lazy_sum_mod(sa, wa, p, delta);
-  rsum[w][1][lane] = lazy_sum_mod(sbm, wb, p, delta);
-  __syncthreads();
-
-  // phase B: tile aggregate, decoupled look-back over earlier tiles of this column
-  if (w == 0) {
-    uint64_t ta = 0, tb = 0;
-#pragma unroll
-    for (int i = 0; i < W; i++) {
-      ta = addmod_c(ta, rsum[i][0][lane], p);
-      tb = addmod_c(tb, rsum[i][1][lane], p);
-    }
-    uint64_t *my = a.status + (uint64_t)ticket * 128;
-    uint64_t exa = 0, exb = 0;
-    if (tile > 0) {
-      st_status(my + lane, (1ULL << 62) | ta);
-      st_status(my + 64 + lane, (1ULL << 62) | tb);
-      // look back over up to LB predecessor tiles per round trip: the loads are issued
-      // together, then folded from the nearest tile until an inclusive prefix is found
-      constexpr int LB = 4;
-      bool done_a = false, done_b = false;
-      int64_t pa = (int64_t)tile - 1, pb = (int64_t)tile - 1;
-      while (true) {
-        uint64_t va[LB], vb[LB];
-#pragma unroll
-        for (int i = 0; i < LB; i++) {
-          va[i] = (!done_a && pa - i >= 0) ? ld_status(a.status + ((uint64_t)(pa - i) * a.columns + column) * 128 + lane) : 0;
-          vb[i] = (!done_b && pb - i >= 0) ? ld_status(a.status + ((uint64_t)(pb - i) * a.columns + column) * 128 + 64 + lane) : 0;
-        }
-        // fold the contiguous published run from the nearest tile
-        if (!done_a) {
-#pragma unroll
-          for (int i = 0; i < LB; i++) {
-            const uint64_t st = va[i] >> 62;
-            if (st == 0 || done_a || pa < 0) break;
-            exa = addmod_c(exa, va[i] & ((1ULL << 61) - 1), p);
-            if (st == 2) done_a = true; else pa--;
-          }
-        }
-        if (!done_b) {
-#pragma unroll
-          for (int i = 0; i < LB; i++) {
-            const uint64_t st = vb[i] >> 62;
-            if (st == 0 || done_b || pb < 0) break;
-            exb = addmod_c(exb, vb[i] & ((1ULL << 61) - 1), p);
-            if (st == 2) done_b = true; else pb--;
-          }
-        }
-        if (__all(done_a && done_b)) break;
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    st_status(my + lane, (2ULL << 62) | addmod_c(exa, ta, p));
-    st_status(my + 64 + lane, (2ULL << 62) | addmod_c(exb, tb, p));
-    // exclusive residue at each wave's first element
-#pragma unroll
-    for (int i = 0; i < W; i++) {
-      const uint64_t va = rsum[i][0][lane], vb = rsum[i][1][lane];
-      rsum[i][0][lane] = exa;
-      rsum[i][1][lane] = exb;
-      exa = addmod_c(exa, va, p);
-      exb = addmod_c(exb, vb, p);
-    }
-  }
-  __syncthreads();
-
-  // phase C: carries from the now-known residues -> this lane's max-plus maps
-  {
-    uint64_t rho_a = rsum[w][0][lane], rho_b = rsum[w][1][lane];
-    int32_t Da = 0, Ma = kNegInf, Db = 0, Mb = kNegInf;
-    if (valid == S) {
-#pragma unroll
-      for (int j = 0; j < S; j++) {
-        chain_step(rho_a, Da, Ma, ra[j], p, delta);
-        chain_step(rho_b, Db, Mb, rb[j], p, delta);
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < S; j++) {
-        if (j < valid) {
-          chain_step(rho_a, Da, Ma, ra[j], p, delta);
-          chain_step(rho_b, Db, Mb, rb[j], p, delta);
-        }
-      }
-    }
-    dmsh[w][0][lane] = make_int2(Da, Ma);
-    dmsh[w][1][lane] = make_int2(Db, Mb);
-  }
-  __syncthreads();
-
-  // phase D: fold the W wave maps in element order; one map per tile, lane, chain
-  if (w == 0) {
-    int2 fa = make_int2(0, kNegInf), fb = make_int2(0, kNegInf);
-#pragma unroll
-    for (int i = 0; i < W; i++) {
-      const int2 x = dmsh[i][0][lane], y = dmsh[i][1][lane];
-      fa = make_int2(fa.x + x.x, max(fa.y + x.x, x.y));
-      fb = make_int2(fb.x + y.x, max(fb.y + y.x, y.y));
-    }
-    int2 *o = a.dm + (uint64_t)ticket * 128;
-    o[lane] = fa;
-    o[64 + lane] = fb;
-  }
-}
-
-// Fold the tile maps of every chain in element order, emit the reply words, and leave
-// the status words and the ticket at zero for the next launch.  Block = one column:
-// 128 chains x FIX_PARTS threads; each thread folds a contiguous run of tiles (the map
-// loads are independent of the fold, so they stream), then the runs are folded in order.
-constexpr int FIX_PARTS = 8;
-
-__global__ void __launch_bounds__(128 * FIX_PARTS) mac_fixup_kernel(SegArgs a) {
-  __shared__ int2 part[FIX_PARTS][128];
-  const uint64_t column = blockIdx.x;
-  const uint32_t cl = threadIdx.x & 127, pi = threadIdx.x >> 7;
-  if (column == 0 && threadIdx.x == 0) *a.ticket = 0;
-  const uint64_t per = (a.tiles + FIX_PARTS - 1) / FIX_PARTS;
-  const uint64_t t0 = pi * per, t1 = min<uint64_t>(a.tiles, t0 + per);
-  int2 f = make_int2(0, kNegInf);
-  uint64_t t = t0;
-  for (; t + 8 <= t1; t += 8) {
-    int2 v[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = a.dm[((t + i) * a.columns + column) * 128 + cl];
-#pragma unroll
-    for (int i = 0; i < 8; i++) f = make_int2(f.x + v[i].x, max(f.y + v[i].x, v[i].y));
-  }
-  for (; t < t1; t++) {
-    const int2 v = a.dm[(t * a.columns + column) * 128 + cl];
-    f = make_int2(f.x + v.x, max(f.y + v.x, v.y));
-  }
-  part[pi][cl] = f;
-  // the last tile's inclusive residue is the chain's final residue; read it before the
-  // status words are cleared
-  uint64_t rho = 0;
-  if (pi == FIX_PARTS - 1) rho = a.status[((a.tiles - 1) * a.columns + column) * 128 + cl] & ((1ULL << 61) - 1);
-  __syncthreads();
-  for (uint64_t tt = t0; tt < t1; tt++) a.status[(tt * a.columns + column) * 128 + cl] = 0;
-  if (pi != FIX_PARTS - 1) return;
-  int32_t j = 0;
-#pragma unroll
-  for (int i = 0; i < FIX_PARTS; i++) j = max(j + part[i][cl].x, part[i][cl].y);
-  const uint32_t chain = cl >> 6, lane = cl & 63;
-  const uint32_t sb = (uint32_t)(column % a.sblocks);
-  uint64_t r = column / a.sblocks;
-  const uint32_t c = (uint32_t)(r % a.m);
-  const uint64_t gk = r / a.m;
-  const uint64_t mn = (uint64_t)a.m * a.n;
-  a.out[gk * 2 * mn + chain * mn + (uint64_t)c * a.n + sb * 64 + lane] = rho + (uint64_t)j * a.ms.p[c];
 }
 
 // ---- streaming MAC: per-block chain summaries, no inter-workgroup communication ----
@@ -807,68 +538,6 @@ __global__ void __launch_bounds__(128 * FIX_PARTS) mac_stream_fixup_kernel(Strea
 }  // namespace
 
 namespace {
-// Tiles per column and the scratch layout for a segmented launch of tile_elems elements per tile.
-int seg_prepare(xpg_ctx *ctx, SegArgs &sa, uint64_t tile_elems) {
-  sa.tiles = (sa.n_l + tile_elems - 1) / tile_elems;
-  const uint64_t words = sa.tiles * sa.columns * 128;
-  if (sa.tiles * sa.columns > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
-  // region layout: [ticket, 256 B][status words, capacity][dm maps, capacity].  The
-  // ticket and the whole status region are zero between launches: zeroed when the
-  // region is (re)allocated, and every launch's fixup zeroes exactly the words it used.
-  if (ctx->seg_capacity < words) {
-    int rc = ensure_seg_scratch(ctx, 256 + 2 * words * 8);
-    if (rc) return rc;
-    rc = launch_fill_zero(ctx, (uint64_t *)ctx->seg_scratch, (256 + words * 8) / 8);
-    if (rc) return rc;
-    ctx->seg_capacity = words;
-  }
-  sa.ticket = (uint32_t *)ctx->seg_scratch;
-  sa.status = (uint64_t *)((char *)ctx->seg_scratch + 256);
-  sa.dm = (int2 *)(sa.status + ctx->seg_capacity);
-  return XPG_OK;
-}
-
-int launch_fixup(xpg_ctx *ctx, SegArgs &sa) {
-  hipLaunchKernelGGL(mac_fixup_kernel, dim3((uint32_t)sa.columns), dim3(128 * FIX_PARTS), 0, ctx->stream, sa);
-  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC fixup launch");
-  return XPG_OK;
-}
-
-template <int S, int W>
-int launch_seg(xpg_ctx *ctx, SegArgs &sa) {
-  int rc = seg_prepare(ctx, sa, (uint64_t)S * W);
-  if (rc) return rc;
-  hipEvent_t k0 = nullptr, k1 = nullptr;
-  timing_begin(ctx, &k0, &k1, 1);
-  hipLaunchKernelGGL((mac_seg_kernel<S, W>), dim3((uint32_t)(sa.tiles * sa.columns)), dim3(64 * W), 0, ctx->stream, sa);
-  timing_end(ctx, k0, k1, 1);
-  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "MAC launch");
-  return launch_fixup(ctx, sa);
-}
-
-// Segmented launch of tile shape S x W (the instantiated shapes only).
-int launch_seg_shape(xpg_ctx *ctx, SegArgs &sa, int S, int W) {
-  if (!seg_shape_built(S, W)) return set_err(ctx, XPG_EINVAL, "segmented MAC tile shape not built");
-  switch (S * 100 + W) {
-    case 408: return launch_seg<4, 8>(ctx, sa);
-    case 416: return launch_seg<4, 16>(ctx, sa);
-    case 804: return launch_seg<8, 4>(ctx, sa);
-    case 808: return launch_seg<8, 8>(ctx, sa);
-    case 816: return launch_seg<8, 16>(ctx, sa);
-    case 1202: return launch_seg<12, 2>(ctx, sa);
-    case 1204: return launch_seg<12, 4>(ctx, sa);
-    case 1208: return launch_seg<12, 8>(ctx, sa);
-    case 1602: return launch_seg<16, 2>(ctx, sa);
-    case 1604: return launch_seg<16, 4>(ctx, sa);
-    case 1608: return launch_seg<16, 8>(ctx, sa);
-    case 2402: return launch_seg<24, 2>(ctx, sa);
-    case 2404: return launch_seg<24, 4>(ctx, sa);
-    case 3202: return launch_seg<32, 2>(ctx, sa);
-    case 3204: return launch_seg<32, 4>(ctx, sa);
-    default: return set_err(ctx, XPG_EINVAL, "segmented MAC tile shape not built");
-  }
-}
-
 // Streaming launch: block summaries into scratch, then the per-column scan.
 template <int U, bool PF>
 int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
@@ -937,16 +606,6 @@ int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t gro
 
 }  // namespace
 
-bool seg_shape_built(int S, int W) {
-  switch (S * 100 + W) {
-    case 408: case 416: case 804: case 808: case 816: case 1202: case 1204: case 1208:
-    case 1602: case 1604: case 1608: case 2402: case 2404: case 3202: case 3204:
-      return true;
-    default:
-      return false;
-  }
-}
-
 void batch_default_shape(uint32_t qn, uint64_t groups, int *gr, int *qr) {
   if (!*qr) *qr = qn >= 4 ? 4 : (qn >= 2 ? 2 : 1);
   if (!*gr) *gr = groups >= 8 && *qr <= 2 ? 8 : (groups >= 4 ? 4 : 1);
@@ -966,8 +625,8 @@ uint64_t stream_auto_e(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, 
 
 LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P) {
   const uint64_t lanes = groups * P * (uint64_t)prm->m * prm->n;
-  // the segmented and streaming kernels' columns are 64 slots wide: degrees below 64
-  // always take one chain per lane
+  // the streaming kernel's columns are 64 slots wide: degrees below 64 always take one
+  // chain per lane
   const bool one_chain_per_lane =
       prm->n < 64 || ctx->mac_mode == 1 || (ctx->mac_mode == 0 && (lanes >= kDirectLanes || n_l <= 64));
   LevelCfg c;
@@ -975,21 +634,13 @@ LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n
     // many groups share each query word: register-blocked over groups
     c.kernel = 4;
     c.b = 1;
-  } else if (!one_chain_per_lane && (ctx->mac_mode == 3 || ctx->mac_mode == 0)) {
+  } else if (one_chain_per_lane) {
+    c.kernel = 1;
+  } else {
     c.kernel = 3;
     c.a = ctx->stream_u ? ctx->stream_u : 8;
     c.b = ctx->stream_pf;
     c.c = ctx->stream_e;
-  } else if (one_chain_per_lane) {
-    c.kernel = 1;
-  } else {
-    // tile shape: S elements per lane, W wavefronts per workgroup.  Measured on the
-    // 65536-element single query: 32 x 4 streams at 5.8 TB/s (more loads in flight per
-    // lane, fewer tiles to look back over) against 5.6 for 24 x 4 and 5.3 for 16 x 4.
-    c.kernel = 2;
-    c.a = ctx->seg_s ? ctx->seg_s : (n_l >= 4096 ? 32 : (n_l >= 512 ? 16 : 8));
-    c.b = ctx->seg_w ? ctx->seg_w : 4;
-    if (!seg_shape_built(c.a, c.b)) c.a = c.b = 8;
   }
   return c;
 }
@@ -1000,8 +651,8 @@ int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
     uint64_t *outp = out;
     return launch_mac_batch_cfg(ctx, prm, db, &q, &outp, 1, n_l, groups, P, cfg.a, 1);
   }
-  if (cfg.kernel < 1 || cfg.kernel > 3) return set_err(ctx, XPG_EINVAL, "unknown MAC kernel");
-  if (cfg.kernel != 1 && prm->n < 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n >= 64");
+  if (cfg.kernel != 1 && cfg.kernel != 3) return set_err(ctx, XPG_EINVAL, "unknown MAC kernel");
+  if (cfg.kernel == 3 && prm->n < 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n >= 64");
   MacArgs a;
   a.ms = modset_of(prm);
   a.db = db;
@@ -1027,22 +678,6 @@ int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
     hipLaunchKernelGGL(mac_direct_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
     timing_end(ctx, k0, k1, 1);
     if (hipGetLastError() != hipSuccess) rc = set_err(ctx, XPG_EDEVICE, "MAC launch");
-  } else {
-    SegArgs sa{};
-    sa.ms = a.ms;
-    sa.db = db;
-    sa.q = q;
-    sa.out = out;
-    sa.n_l = n_l;
-    sa.P = P;
-    sa.n = prm->n;
-    sa.m = prm->m;
-    sa.sblocks = prm->n / 64;
-    sa.log_sb = (uint32_t)__builtin_ctz(sa.sblocks);
-    sa.cpg = P * prm->m * sa.sblocks;
-    if ((uint64_t)groups * sa.cpg > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
-    sa.columns = (uint32_t)(groups * sa.cpg);
-    rc = launch_seg_shape(ctx, sa, cfg.a, cfg.b);
   }
   timing_end(ctx, e0, e1);
   return rc;
@@ -1050,7 +685,7 @@ int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
 
 // Explicit options (a kernel or a tile shape asked for) win over the tuning table.
 bool options_pinned(const xpg_ctx *ctx) {
-  return ctx->mac_mode || ctx->seg_s || ctx->seg_w || ctx->stream_e || ctx->stream_u || ctx->stream_pf;
+  return ctx->mac_mode || ctx->stream_e || ctx->stream_u || ctx->stream_pf;
 }
 
 int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,

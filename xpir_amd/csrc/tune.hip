@@ -2,7 +2,7 @@
 //
 // Every MAC launch shape (one query's reply level, or a batch of queries on level 0) has
 // several kernels that all give the same words: one chain per lane, register blocks over
-// groups and queries, segmented chains with look-back, streaming block summaries, each
+// groups and queries, streaming block summaries, each
 // with tile parameters.  Which is fastest depends on the shape (chain length, number of
 // groups sharing a query word, polys, batch size).  xpg_autotune times every candidate on
 // the database's real level shapes, checks that each candidate's words equal the untuned
@@ -134,14 +134,10 @@ bool same_cfg(const LevelCfg &x, const LevelCfg &y) {
   return x.kernel == y.kernel && x.a == y.a && x.b == y.b && x.c == y.c;
 }
 
-// Scratch bytes a candidate needs (segmented status + maps, streaming summaries).
+// Scratch bytes a candidate needs (the streaming kernel's block summaries).
 uint64_t cfg_scratch_bytes(const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, const LevelCfg &c,
                            uint64_t auto_e8) {
   const uint64_t columns = groups * P * (uint64_t)prm->m * (prm->n / 64);
-  if (c.kernel == 2) {
-    const uint64_t tiles = (n_l + (uint64_t)c.a * c.b - 1) / ((uint64_t)c.a * c.b);
-    return 256 + 2 * tiles * columns * 128 * 8;
-  }
   if (c.kernel == 3) {
     const uint64_t E = c.c ? (uint64_t)c.c : auto_e8;
     return ((n_l + E - 1) / E) * columns * 128 * 24;
@@ -174,8 +170,6 @@ std::vector<LevelCfg> level_candidates(const xpg_ctx *ctx, const xpg_params *prm
         for (uint64_t e : {e0, 2 * e0, std::max<uint64_t>(u, e0 / 2)})
           if (e <= (1u << 20)) add(LevelCfg{3, u, pf, (int)e});
       }
-    const int shapes[][2] = {{8, 4}, {16, 4}, {32, 4}, {16, 8}, {8, 8}, {32, 2}};
-    for (const auto &s : shapes) add(LevelCfg{2, s[0], s[1], 0});
   }
   return v;
 }
@@ -205,7 +199,6 @@ bool entry_valid(const xpg_tune_entry &e) {
   if (e.kind != 0 || e.qn != 0) return false;
   switch (e.kernel) {
     case 1: return true;
-    case 2: return seg_shape_built(e.a, e.b);
     case 3: return (e.a == 4 || e.a == 8 || e.a == 16) && (e.b == 0 || e.b == 1) && e.c >= 0 && e.c <= (1 << 20);
     case 4: return (e.a == 0 || e.a == 1 || e.a == 4 || e.a == 8) && e.b == 1;
     default: return false;
