@@ -135,6 +135,21 @@ int xpg_reply(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query, uint64_t qu
 /* device-resident query and reply, asynchronous on the context stream */
 int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev);
 
+/* xpg_reply_dev without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No reference counterpart.  The same reply words, enqueued on the context stream; no
+ * hipMalloc, hipFree, event or graph is created.  The intermediate-level buffers and the
+ * streaming MAC's block summaries come from one static device array of
+ * XPG_NOALLOC_SCRATCH_BYTES per GPU (held from library load until the process ends);
+ * a reply that needs more returns XPG_ESIZE before anything is enqueued.  For d > 1 the
+ * context must already hold the NTT tables of the database's parameters (any import or
+ * ordinary reply on it builds them), else XPG_EINVAL.  Every call on one GPU shares
+ * that GPU's array: calls on different contexts (streams) of the same GPU that overlap
+ * in time race on it; calls on one context are ordered by its stream.  Nothing in the
+ * array needs to start at zero (every word is written before it is read). */
+#define XPG_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int xpg_reply_dev_noalloc(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev);
+
 /* ---- batched replies: many queries over resident databases, the DB tiles of each
  *      database streamed from HBM once per query group.  No reference counterpart
  *      (the reference answers one query per generator); each reply is identical to

@@ -22,7 +22,7 @@ EXPORTS = [
     "xpg_dev_free", "xpg_h2d", "xpg_d2h", "xpg_host_alloc", "xpg_host_free", "xpg_h2d_async",
     "xpg_db_import", "xpg_db_import_dev",
     "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
-    "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev",
+    "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev", "xpg_reply_dev_noalloc",
     "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_timing_read_main", "xpg_graph_stats", "xpg_autotune", "xpg_tune_set",
     "xpg_tune_get", "xpg_tune_clear", "xpg_keygen",
     "xpg_query_gen_dev", "xpg_random_bytes_dev", "xpg_decrypt_dev", "xpg_server_create", "xpg_server_destroy",
@@ -95,6 +95,7 @@ def lib() -> C.CDLL:
         "xpg_db_free": (None, [vp]),
         "xpg_reply": (C.c_int, [vp, vp, vp, u64, vp, u64]),
         "xpg_reply_dev": (C.c_int, [vp, vp, vp, vp]),
+        "xpg_reply_dev_noalloc": (C.c_int, [vp, vp, vp, vp]),
         "xpg_reply_batch_dev": (C.c_int, [vp, C.POINTER(XpgRequest), u32]),
         "xpg_timing_enable": (C.c_int, [vp, C.c_int]),
         "xpg_timing_read": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
@@ -295,6 +296,11 @@ class Context:
 
     def reply_dev(self, db: Database, query_dev: DeviceBuffer, reply_dev: DeviceBuffer):
         self._check(lib().xpg_reply_dev(self.h, db.h, query_dev.ptr, reply_dev.ptr))
+
+    def reply_dev_noalloc(self, db: Database, query_dev: DeviceBuffer, reply_dev: DeviceBuffer):
+        """xpg_reply_dev_noalloc: the same reply with nothing allocated (scratch from the
+        library's static device array); raises XpgError with XPG_ESIZE when it needs more."""
+        self._check(lib().xpg_reply_dev_noalloc(self.h, db.h, query_dev.ptr, reply_dev.ptr))
 
     def reply_batch_dev(self, reqs):
         arr = (XpgRequest * len(reqs))()

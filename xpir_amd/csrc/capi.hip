@@ -6,10 +6,52 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 
 #include "internal.hpp"
 
 static std::atomic<uint64_t> g_db_serial{0};
+
+// ---- scratch of the no-allocation reply (xpg_reply_dev_noalloc) ----
+// That entry point takes its level buffers and the streaming MAC's block summaries from
+// this array instead of hipMalloc.  Every GPU the library runs on holds its own copy
+// until the process ends.  In host code the name `scratch` is only a stand-in: a
+// device's copy is reached through the address hipGetSymbolAddress returns with that
+// device current, looked up once per device when the first context on it is created.
+static __device__ __align__(256) uint8_t scratch[XPG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// With `device` current: its copy's address, once.  A failed lookup is kept and reported
+// by xpg_reply_dev_noalloc; nothing else depends on the array.
+void map_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess && p) {
+    g_scratch_ptr[device] = static_cast<uint8_t *>(p);
+    g_scratch_err[device] = hipSuccess;
+  } else {
+    g_scratch_err[device] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+  }
+}
+
+uint8_t *scratch_of(int device, hipError_t *err) {
+  if (device < 0 || device >= kMaxDevices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  *err = g_scratch_err[device];
+  return g_scratch_ptr[device];
+}
+}  // namespace
 
 namespace {
 
@@ -59,6 +101,10 @@ const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m) {
   const uint64_t key = ((uint64_t)n << 8) | m;
   auto it = ctx->tables.find(key);
   if (it != ctx->tables.end()) return &it->second;
+  if (ctx->ws) {
+    set_err(ctx, XPG_EINVAL, "no NTT tables for these parameters on this context (the no-allocation reply builds none)");
+    return nullptr;
+  }
   uint32_t logn = 0;
   while ((1u << logn) < n) logn++;
   std::vector<uint64_t> phi(n * m), phiq(n * m), w(n * m), wq(n * m), iw(n * m), iwq(n * m), ipt(n * m),
@@ -186,6 +232,19 @@ int ensure_scratch(xpg_ctx *ctx, size_t bytes) {
   return XPG_OK;
 }
 
+int get_scratch(xpg_ctx *ctx, size_t bytes, void **p) {
+  if (ctx->ws) {
+    if (bytes > ctx->ws_cap - ctx->ws_used)
+      return set_err(ctx, XPG_ESIZE, "the no-allocation reply's scratch array is too small for this launch");
+    *p = ctx->ws + ctx->ws_used;  // per launch: later launches on the stream reuse it
+    return XPG_OK;
+  }
+  const int rc = ensure_scratch(ctx, bytes);
+  if (rc) return rc;
+  *p = ctx->scratch;
+  return XPG_OK;
+}
+
 void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial) {
   auto &c = ctx->graph_cache;
   for (size_t i = 0; i < c.size();) {
@@ -201,8 +260,8 @@ void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial) {
 
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b, int tag) {
   // mode 2 times the main kernels alone: no events around whole levels, whose records
-  // would put gaps between a level's launches
-  if (!ctx->timing || (ctx->timing == 2 && tag != 1)) return;
+  // would put gaps between a level's launches; the no-allocation reply creates no events
+  if (!ctx->timing || ctx->ws || (ctx->timing == 2 && tag != 1)) return;
   for (hipEvent_t *e : {a, b}) {
     if (!ctx->ev_pool.empty()) {
       *e = ctx->ev_pool.back();
@@ -297,6 +356,7 @@ int xpg_ctx_create(int device, xpg_ctx **out) {
     delete ctx;
     return XPG_EDEVICE;
   }
+  map_scratch(device);
   *out = ctx;
   return XPG_OK;
 }
@@ -546,6 +606,18 @@ void xpg_db_free(xpg_db *db) {
 }
 
 // ---- replies (PIRReplyGeneratorNFL_internal.cpp:340-445) ----
+// Bytes of the two intermediate-level buffers of levels [from, d): the largest level
+// reply (groups x P x 2 x m x n) and the largest next-level NTT database.
+static void level_buf_sizes(const xpg_db *db, uint32_t from, size_t *need_reply, size_t *need_db) {
+  const uint64_t mn = (uint64_t)db->prm.m * db->prm.n;
+  *need_reply = *need_db = 0;
+  for (uint32_t l = from; l + 1 < db->d; l++) {
+    const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
+    *need_reply = std::max<size_t>(*need_reply, groups * db->Pl[l] * 2 * mn * 8);
+    *need_db = std::max<size_t>(*need_db, groups * db->Pl[l + 1] * mn * 8);
+  }
+}
+
 // Levels [first, d) of one query.  `data` holds level `first`'s NTT database (P polys per
 // unit); `q` points at level `first`'s query part.  When first > 0 the caller passes the
 // previous level's reply in `prev_reply` and this routine starts with the recursion.
@@ -553,18 +625,25 @@ static int run_levels(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const uint
                       const uint64_t *prev_reply, uint64_t *reply_dev) {
   const xpg_params *prm = &db->prm;
   const uint64_t mn = (uint64_t)prm->m * prm->n;
-  // intermediate levels: level_buf[0] holds a level's reply, level_buf[1] the next
-  // level's NTT database (sized before any launch reads them)
+  // intermediate levels: lvl_reply holds a level's reply, lvl_db the next level's NTT
+  // database (sized before any launch reads them): the context's level buffers, or
+  // inside xpg_reply_dev_noalloc the front of its scratch array
   size_t need_reply = 0, need_db = 0;
-  for (uint32_t l = (first ? first - 1 : 0); l + 1 < db->d; l++) {
-    const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
-    need_reply = std::max<size_t>(need_reply, groups * db->Pl[l] * 2 * mn * 8);
-    need_db = std::max<size_t>(need_db, groups * db->Pl[l + 1] * mn * 8);
-  }
-  if (need_db) {
+  level_buf_sizes(db, first ? first - 1 : 0, &need_reply, &need_db);
+  uint64_t *lvl_reply = nullptr, *lvl_db = nullptr;
+  if (need_db && ctx->ws) {
+    const size_t r_al = (need_reply + 255) & ~(size_t)255, d_al = (need_db + 255) & ~(size_t)255;
+    if (r_al + d_al > ctx->ws_cap - ctx->ws_used)
+      return set_err(ctx, XPG_ESIZE, "the no-allocation reply's scratch array is too small for the level buffers");
+    lvl_reply = reinterpret_cast<uint64_t *>(ctx->ws + ctx->ws_used);
+    lvl_db = reinterpret_cast<uint64_t *>(ctx->ws + ctx->ws_used + r_al);
+    ctx->ws_used += r_al + d_al;
+  } else if (need_db) {
     int rc = ensure_level_buf(ctx, 0, need_reply);
     if (!rc) rc = ensure_level_buf(ctx, 1, need_db);
     if (rc) return rc;
+    lvl_reply = ctx->level_buf[0];
+    lvl_db = ctx->level_buf[1];
   }
   if (first > 0) {
     // fromResulttoInData on the previous level's reply
@@ -572,25 +651,25 @@ static int run_levels(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const uint
     const uint64_t groups = prod_dims(db->d, db->dims, first);
     const uint64_t unit = (uint64_t)db->Pl[l] * 2 * mn * 8;
     ByteSrc src{(const uint8_t *)prev_reply, unit, unit, groups * unit};
-    int rc = launch_import_bytes(ctx, prm, src, groups, groups, db->Pl[first], ctx->level_buf[1]);
+    int rc = launch_import_bytes(ctx, prm, src, groups, groups, db->Pl[first], lvl_db);
     if (rc) return rc;
-    data = ctx->level_buf[1];
+    data = lvl_db;
   }
   for (uint32_t l = first; l < db->d; l++) {
     const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
     const uint32_t P = db->Pl[l];
     const bool last = (l + 1 == db->d);
-    uint64_t *out = last ? reply_dev : ctx->level_buf[0];
+    uint64_t *out = last ? reply_dev : lvl_reply;
     int rc = launch_mac_level(ctx, prm, data, q, db->dims[l], groups, P, out);
     if (rc) return rc;
     q += (uint64_t)db->dims[l] * 2 * mn;
     if (!last) {
       // fromResulttoInData: each group's P ciphertexts, as bytes, become one unit
       const uint64_t unit = (uint64_t)P * 2 * mn * 8;
-      ByteSrc src{(const uint8_t *)ctx->level_buf[0], unit, unit, groups * unit};
-      rc = launch_import_bytes(ctx, prm, src, groups, groups, db->Pl[l + 1], ctx->level_buf[1]);
+      ByteSrc src{(const uint8_t *)lvl_reply, unit, unit, groups * unit};
+      rc = launch_import_bytes(ctx, prm, src, groups, groups, db->Pl[l + 1], lvl_db);
       if (rc) return rc;
-      data = ctx->level_buf[1];
+      data = lvl_db;
     }
   }
   return XPG_OK;
@@ -667,6 +746,46 @@ int xpg_reply_dev(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uin
   if (db->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
   if (ctx->graphs && !ctx->timing) return reply_graph(ctx, db, query_dev, reply_dev);
   return run_levels(ctx, db, 0, db->words, query_dev, nullptr, reply_dev);
+}
+
+// The reply of xpg_reply_dev without any allocation: the levels run eagerly on the
+// context stream, their intermediate buffers and block summaries carved from the
+// device's static scratch array (level buffers first, each launch's summaries after
+// them).  Every need is checked before the first launch, so a reply that does not fit
+// fails with nothing enqueued.
+int xpg_reply_dev_noalloc(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_dev, uint64_t *reply_dev) {
+  if (!ctx || !db || !query_dev || !reply_dev) return set_err(ctx, XPG_EINVAL, "null argument");
+  if (db->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
+  hipError_t herr = hipSuccess;
+  uint8_t *base = scratch_of(ctx->device, &herr);
+  if (!base)
+    return set_err(ctx, XPG_EDEVICE, std::string("no scratch array address on this device: ") + hipGetErrorString(herr));
+  const xpg_params *prm = &db->prm;
+  size_t need_reply = 0, need_db = 0;
+  level_buf_sizes(db, 0, &need_reply, &need_db);
+  const size_t lvl = need_db ? ((need_reply + 255) & ~(size_t)255) + ((need_db + 255) & ~(size_t)255) : 0;
+  size_t scr = 0;
+  for (uint32_t l = 0; l < db->d; l++) {
+    const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
+    const LevelCfg cfg = resolve_level_cfg(ctx, prm, db->dims[l], groups, db->Pl[l]);
+    scr = std::max<size_t>(scr, level_scratch_bytes(ctx, prm, db->dims[l], groups, db->Pl[l], cfg));
+  }
+  if (lvl > XPG_NOALLOC_SCRATCH_BYTES || scr > XPG_NOALLOC_SCRATCH_BYTES - lvl)
+    return set_err(ctx, XPG_ESIZE,
+                   "reply needs " + std::to_string(lvl + scr) + " bytes of scratch; the no-allocation entry point has " +
+                       std::to_string((uint64_t)XPG_NOALLOC_SCRATCH_BYTES));
+  if (db->d > 1 && !ctx->tables.count(((uint64_t)prm->n << 8) | prm->m))
+    return set_err(ctx, XPG_EINVAL,
+                   "no NTT tables for these parameters on this context: the recursion needs them (an import or "
+                   "an ordinary reply on this context builds them)");
+  (void)hipSetDevice(ctx->device);
+  ctx->ws = base;
+  ctx->ws_cap = XPG_NOALLOC_SCRATCH_BYTES;
+  ctx->ws_used = 0;
+  const int rc = run_levels(ctx, db, 0, db->words, query_dev, nullptr, reply_dev);
+  ctx->ws = nullptr;
+  ctx->ws_cap = ctx->ws_used = 0;
+  return rc;
 }
 
 // Level 0 of every request on one database in one batched launch, then each request's

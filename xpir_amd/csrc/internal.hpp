@@ -89,6 +89,10 @@ struct xpg_ctx {
   size_t batch_bytes = 0;
   uint64_t *level_buf[2] = {nullptr, nullptr};
   size_t level_bytes[2] = {0, 0};
+  // set only inside xpg_reply_dev_noalloc: the device's static scratch array, from which
+  // that call takes its level buffers and block summaries (nothing is allocated then)
+  uint8_t *ws = nullptr;
+  size_t ws_cap = 0, ws_used = 0;
   uint64_t *io_buf = nullptr;     // device copies of a host query and its reply (xpg_reply)
   size_t io_bytes = 0;
   uint64_t *dec_buf = nullptr;    // residues and lifted coefficients of xpg_decrypt_dev
@@ -119,6 +123,9 @@ struct xpg_ctx {
 int set_err(xpg_ctx *ctx, int code, const std::string &msg);
 const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m);
 int ensure_scratch(xpg_ctx *ctx, size_t bytes);
+// scratch for one launch: the context's grown buffer, or during xpg_reply_dev_noalloc the
+// free part of the static scratch array (nothing allocated; XPG_ESIZE if it is too small)
+int get_scratch(xpg_ctx *ctx, size_t bytes, void **p);
 int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
 int ensure_io_buf(xpg_ctx *ctx, size_t bytes);
@@ -183,6 +190,12 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
 int launch_mac_level_multi(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                            uint32_t nreq, uint64_t n_l, uint64_t gpr, uint32_t P, uint64_t *const *out_per_req,
                            uint64_t *out);
+// what launch_mac_level runs for this shape: the tuning table's choice unless an option
+// pins a kernel, else default_level_cfg
+LevelCfg resolve_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P);
+// device scratch a level launch with cfg takes (the streaming kernel's block summaries)
+uint64_t level_scratch_bytes(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P,
+                             const LevelCfg &cfg);
 // what launch_mac_level runs for this shape with the context's options and no tuning
 LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P);
 bool batch_shape_built(int GR, int QR);  // register-block shapes compiled in

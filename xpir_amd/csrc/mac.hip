@@ -544,9 +544,10 @@ int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
   const uint64_t recs = (uint64_t)st.nblk * st.columns * 128;
   const uint64_t waves = (uint64_t)st.nblk * st.columns;
   if (waves > 0xffffffffULL - 3) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
-  int rc = ensure_scratch(ctx, recs * 24);
+  void *scr = nullptr;
+  int rc = get_scratch(ctx, recs * 24, &scr);
   if (rc) return rc;
-  st.rho0 = (uint64_t *)ctx->scratch;
+  st.rho0 = (uint64_t *)scr;
   st.rstar = st.rho0 + recs;
   st.jj = (int2 *)(st.rstar + recs);
   hipEvent_t k0 = nullptr, k1 = nullptr;
@@ -573,6 +574,14 @@ uint64_t auto_e(int cu_count, uint64_t n_l, uint64_t columns, int U) {
   return (E + U - 1) / U * U;
 }
 
+// Elements per streaming block: E as asked (0 = auto_e), at most 2^20 (the block
+// summary's 96-bit word holds 2^20 steps), a whole number of chunks.
+uint64_t stream_block_elems(int cu_count, uint64_t n_l, uint64_t columns, int U, uint64_t E) {
+  if (!E) E = auto_e(cu_count, n_l, columns, U);
+  E = std::min<uint64_t>(E, 1u << 20);
+  return (E + U - 1) / U * U;
+}
+
 // The streaming MAC for one level: U elements per load chunk, PF double-buffered chunks,
 // E elements per block (0 = auto_e).
 int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t groups, int U, bool pf, uint64_t E) {
@@ -591,9 +600,7 @@ int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t gro
   if (groups * st.cpg > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.columns = (uint32_t)(groups * st.cpg);
   if (U != 4 && U != 8 && U != 16) return set_err(ctx, XPG_EINVAL, "streaming MAC chunk not built");
-  if (!E) E = auto_e(ctx->cu_count, a.n_l, st.columns, U);
-  E = std::min<uint64_t>(E, 1u << 20);  // the block summary's 96-bit word holds 2^20 steps
-  E = (E + U - 1) / U * U;
+  E = stream_block_elems(ctx->cu_count, a.n_l, st.columns, U, E);
   if (E > 0xffffffffULL || (a.n_l + E - 1) / E > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.E = (uint32_t)E;
   st.nblk = (uint32_t)((a.n_l + E - 1) / E);
@@ -621,6 +628,22 @@ static const uint64_t kDirectLanes = 256ull * 32 * 64;
 
 uint64_t stream_auto_e(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, int u) {
   return auto_e(ctx->cu_count, n_l, columns_of(prm, groups, P), u ? u : 8);
+}
+
+uint64_t level_scratch_bytes(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P,
+                             const LevelCfg &cfg) {
+  if (cfg.kernel != 3) return 0;
+  const uint64_t columns = columns_of(prm, groups, P);
+  const int U = cfg.a ? cfg.a : 8;
+  const uint64_t E = stream_block_elems(ctx->cu_count, n_l, columns, U, (uint64_t)cfg.c);
+  return ((n_l + E - 1) / E) * columns * 128 * 24;
+}
+
+LevelCfg resolve_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P) {
+  LevelCfg cfg;
+  const TuneKey key{0, prm->n, prm->m, P, n_l, groups, 0};
+  if (options_pinned(ctx) || !tune_lookup(ctx->device, key, &cfg)) cfg = default_level_cfg(ctx, prm, n_l, groups, P);
+  return cfg;
 }
 
 LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P) {
@@ -690,10 +713,7 @@ bool options_pinned(const xpg_ctx *ctx) {
 
 int launch_mac_level(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *q,
                      uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out) {
-  LevelCfg cfg;
-  const TuneKey key{0, prm->n, prm->m, P, n_l, groups, 0};
-  if (options_pinned(ctx) || !tune_lookup(ctx->device, key, &cfg)) cfg = default_level_cfg(ctx, prm, n_l, groups, P);
-  return launch_mac_level_cfg(ctx, prm, db, q, n_l, groups, P, out, cfg);
+  return launch_mac_level_cfg(ctx, prm, db, q, n_l, groups, P, out, resolve_level_cfg(ctx, prm, n_l, groups, P));
 }
 
 int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,

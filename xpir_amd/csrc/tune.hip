@@ -134,17 +134,6 @@ bool same_cfg(const LevelCfg &x, const LevelCfg &y) {
   return x.kernel == y.kernel && x.a == y.a && x.b == y.b && x.c == y.c;
 }
 
-// Scratch bytes a candidate needs (the streaming kernel's block summaries).
-uint64_t cfg_scratch_bytes(const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, const LevelCfg &c,
-                           uint64_t auto_e8) {
-  const uint64_t columns = groups * P * (uint64_t)prm->m * (prm->n / 64);
-  if (c.kernel == 3) {
-    const uint64_t E = c.c ? (uint64_t)c.c : auto_e8;
-    return ((n_l + E - 1) / E) * columns * 128 * 24;
-  }
-  return 0;
-}
-
 const uint64_t kScratchCap = 2ull << 30;  // candidates needing more scratch are skipped
 const float kKeep = 0.97f;                // a candidate must beat the fixed rule by 3 %
 
@@ -154,8 +143,7 @@ std::vector<LevelCfg> level_candidates(const xpg_ctx *ctx, const xpg_params *prm
   auto add = [&](LevelCfg c) {
     for (const LevelCfg &x : v)
       if (same_cfg(x, c)) return;
-    const uint64_t e8 = stream_auto_e(ctx, prm, n_l, groups, P, 8);
-    if (cfg_scratch_bytes(prm, n_l, groups, P, c, e8) > kScratchCap) return;
+    if (level_scratch_bytes(ctx, prm, n_l, groups, P, c) > kScratchCap) return;
     v.push_back(c);
   };
   add(LevelCfg{1, 0, 0, 0});
