@@ -74,7 +74,10 @@ int xpg_sync(xpg_ctx *ctx);
  * eagerly and sizes the context buffers; any option change or buffer growth re-captures;
  * with timing on, replies run eagerly).  No reference counterpart; every setting yields
  * the same reply bytes.  "group_budget" = device bytes a batch's levels 1..d-1 may take
- * at once (requests are grouped under it; 0 = 24 GiB). */
+ * at once (requests are grouped under it; 0 = 24 GiB).  "alpha_zero_pad" = 1: a partial
+ * last alpha aggregate holds its real elements and zeros (the default 0 gives it the
+ * reference's bytes, see xpg_db_import); this one changes reply bytes when n_elems is
+ * not a multiple of alpha. */
 int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value);
 
 /* device memory plumbing for callers without a GPU runtime of their own */
@@ -95,7 +98,12 @@ int xpg_h2d_async(xpg_ctx *ctx, void *dst_dev, const void *src_pinned, uint64_t 
  * elems: n_elems elements of elem_bytes, back to back (shorter elements zero-padded by
  * the caller, as DBVectorProcessor::readStream does, DBVectorProcessor.cpp:88-106).
  * alpha elements form one import unit (DBHandler::readAggregatedStream,
- * DBHandler.cpp:3-26).  dims[0..d-1] is the query grid; prod(dims) >= units; the grid's
+ * DBHandler.cpp:3-26).  When n_elems mod alpha = r > 0 the last unit gets the bytes the
+ * reference gives it (its buffer is shared by every unit, and the bzero at
+ * DBHandler.cpp:21-24 starts at the last real element's slot): elements 0..r-2 of the
+ * unit, zeros in slots r-1..alpha-2, and in slot alpha-1 the previous unit's last element
+ * (zeros if there is none); option "alpha_zero_pad" keeps the r elements and zeros the
+ * rest instead.  dims[0..d-1] is the query grid; prod(dims) >= units; the grid's
  * missing units are zero polys (PIRReplyGeneratorNFL_internal.cpp:146-154).
  * The NTT-domain database stays resident in HBM until xpg_db_free, also past
  * xpg_ctx_destroy of the importing context, and serves replies on any context of the

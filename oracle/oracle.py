@@ -125,6 +125,30 @@ class Oracle:
     def next_polys(self, P: int) -> int:
         return int(self.lib().xo_next_polys(C.byref(self.prm), P))
 
+    # --- database staging ---
+    @staticmethod
+    def aggregate_units(data: np.ndarray, n_elems: int, elem_bytes: int, alpha: int,
+                        zero_pad: bool = False) -> np.ndarray:
+        """Import units of alpha elements each (rows of alpha * elem_bytes bytes), as
+        importDataNFL (PIRReplyGeneratorNFL_internal.cpp:86-105) fills them with
+        DBHandler::readAggregatedStream (DBHandler.cpp:3-26): one buffer reused for every
+        unit; a partial last unit (r = n_elems mod alpha elements) reads its r elements
+        into slots 0..r-1, then zeroes alpha - r slots starting at slot r-1 (the last real
+        element's), so slot alpha-1 keeps the previous unit's last element (zero when
+        there is no previous unit).  zero_pad: the last unit's missing slots are zero
+        and its elements kept instead."""
+        data = np.ascontiguousarray(data, dtype=np.uint8)[: n_elems * elem_bytes]
+        units = -(-n_elems // alpha)
+        out = np.zeros((units, alpha * elem_bytes), dtype=np.uint8)
+        out.reshape(-1)[: data.size] = data
+        r = n_elems % alpha
+        if r and not zero_pad:
+            last = out[units - 1]
+            last[(r - 1) * elem_bytes:] = 0
+            if units > 1:
+                last[(alpha - 1) * elem_bytes:] = out[units - 2][(alpha - 1) * elem_bytes:]
+        return out
+
     # --- kernels ---
     def absorb(self, data: np.ndarray) -> np.ndarray:
         data = np.ascontiguousarray(data, dtype=np.uint8)
@@ -289,6 +313,12 @@ class RefNFL:
                                          u64p, C.c_int]
             L.xr_reply_level_timed.argtypes = [C.c_void_p, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint,
                                                u64p, C.c_int, C.POINTER(C.c_double)]
+            L.xr_aggregate.restype = C.c_int64
+            L.xr_aggregate.argtypes = [u8p, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, u8p,
+                                       C.c_uint64]
+            L.xr_level0_stream.restype = C.c_int
+            L.xr_level0_stream.argtypes = [C.c_void_p, C.c_uint, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                           C.c_uint64, C.c_uint64, C.c_uint, u64p, u64p, C.c_int]
             cls._lib = L
         return cls._lib
 
@@ -317,6 +347,36 @@ class RefNFL:
         x = np.array(poly, dtype=np.uint64, copy=True)
         self.lib().xr_ntt_powphi(self.h, x)
         return x
+
+    @classmethod
+    def aggregate(cls, elements, alpha: int, offset: int = 0, bytes_per_file=None) -> np.ndarray:
+        """The reference's alpha aggregation (DBHandler::readAggregatedStream over a
+        DBVectorProcessor, in importDataNFL's loop with one reused buffer): one row of
+        alpha * fileByteSize bytes per aggregate."""
+        sizes = np.array([len(e) for e in elements], dtype=np.uint64)
+        offs = np.zeros(len(elements), dtype=np.uint64)
+        offs[1:] = np.cumsum(sizes)[:-1]
+        data = np.frombuffer(b"".join(bytes(e) for e in elements) + b"\0", dtype=np.uint8).copy()
+        maxb = int(sizes.max())
+        fb = min(maxb if bytes_per_file is None else bytes_per_file, maxb - offset)
+        runs = -(-len(elements) // alpha)
+        out = np.zeros(runs * alpha * fb + 1, dtype=np.uint8)
+        got = cls.lib().xr_aggregate(data, offs, sizes, len(elements), alpha, offset,
+                                     maxb if bytes_per_file is None else bytes_per_file, out, out.size)
+        assert got == runs, (got, runs)
+        return out[:runs * alpha * fb].reshape(runs, alpha * fb)
+
+    def level0_stream(self, A: int, db_seed: int, n_elems: int, elem_bytes: int, n0: int, g0: int, g1: int,
+                      P0: int, q0: np.ndarray, threads: int = 8) -> np.ndarray:
+        """Level-0 replies of groups [g0, g1) of a d >= 2 grid, the groups' elements made
+        from the seeded byte stream and imported one group at a time (the reference's
+        deserializeDataNFL + mulandaddShoup), so the whole NTT database is never held."""
+        q0 = np.ascontiguousarray(q0, dtype=np.uint64)
+        assert q0.size == n0 * 2 * self.mn
+        out = np.zeros((g1 - g0) * P0 * 2 * self.mn, dtype=np.uint64)
+        rc = self.lib().xr_level0_stream(self.h, A, db_seed, n_elems, elem_bytes, n0, g0, g1, P0, q0, out, threads)
+        assert rc == 0, "an element imported to another poly count"
+        return out
 
     def invntt(self, poly):
         x = np.array(poly, dtype=np.uint64, copy=True)

@@ -1,10 +1,11 @@
-// ref_driver.cpp -- thin C entry points over the REFERENCE's own NFLlib, compiled from
-// the unmodified sources under the reference tree by oracle/Makefile into
-// oracle/_ref/libxpir_ref.so.  TEST INFRASTRUCTURE ONLY (the checker).
+// ref_driver.cpp -- thin C entry points over the REFERENCE's own NFLlib and database
+// handlers, compiled from the unmodified sources under the reference tree by
+// oracle/Makefile into oracle/_ref/libxpir_ref.so.  TEST INFRASTRUCTURE ONLY (the checker).
 //
 // The arithmetic all comes from the reference (crypto/NFLlib.hpp / NFLlib.cpp):
 //   NFLlib::setNewParameters, deserializeDataNFL (bitsplitter + nttAndPowPhi),
-//   nttAndPowPhi, invnttAndPowInvPhi, mulandaddShoup.
+//   nttAndPowPhi, invnttAndPowInvPhi, mulandaddShoup; the alpha aggregation from
+//   DBHandler::readAggregatedStream over DBVectorProcessor (pir/dbhandlers/).
 // The reference's NFLLWE.cpp and reply generator need Boost, which the image lacks,
 // so the two loops they wrap are driven here in the reference's own order:
 //   NFLLWE::mulandadd (crypto/NFLLWE.cpp:176-205): per modulus, the a chain over all
@@ -12,8 +13,11 @@
 //   PIRReplyGeneratorNFL_internal::generateReply (PIRReplyGeneratorNFL_internal.cpp:
 //   531-569): per poly (OpenMP), elements ascending.
 #include "NFLlib.hpp"
+#include "DBVectorProcessor.hpp"
 #include <cstdint>
+#include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include <omp.h>
@@ -118,6 +122,109 @@ void xr_reply_level_timed(void *hp, const uint64_t *db, const uint64_t *q, uint6
     }
   }
   if (mac_seconds) *mac_seconds = omp_get_wtime() - t0;
+}
+
+// The alpha aggregation of importDataNFL (PIRReplyGeneratorNFL_internal.cpp:79-105): one
+// rawBits buffer, calloc'd once and reused by every readAggregatedStream call in turn,
+// over a DBVectorProcessor holding the elements (element i = `sizes[i]` bytes at
+// data + offs[i]).  Each aggregate's rawBits content after its call is copied to
+// out (nbruns x alpha * fileByteSize bytes, fileByteSize = min(bytes_per_file,
+// maxFileBytesize - offset)).  Returns nbruns, or -1 if out_cap is too small.
+int64_t xr_aggregate(const uint8_t *data, const uint64_t *offs, const uint64_t *sizes, uint64_t n_elems,
+                     uint64_t alpha, uint64_t offset, uint64_t bytes_per_file, uint8_t *out, uint64_t out_cap) {
+  std::vector<element_t> elems;
+  elems.reserve(n_elems);
+  for (uint64_t i = 0; i < n_elems; i++)
+    elems.emplace_back(std::to_string(i), sizes[i], (char *)(data + offs[i]));
+  DBVectorProcessor dbh(elems);
+  const uint64_t file_bytes = std::min(bytes_per_file, dbh.getmaxFileBytesize() - offset);
+  const uint64_t nbruns = (uint64_t)ceil((double)dbh.getNbStream() / alpha);
+  if (nbruns * alpha * file_bytes > out_cap) return -1;
+  char *raw = (char *)calloc(file_bytes * alpha, sizeof(char));
+  for (uint64_t i = 0; i < nbruns; i++) {
+    dbh.readAggregatedStream(i, alpha, offset, bytes_per_file, raw);
+    memcpy(out + i * alpha * file_bytes, raw, alpha * file_bytes);
+  }
+  free(raw);
+  for (auto &e : elems) free(e.data);
+  return (int64_t)nbruns;
+}
+
+// Row-streamed level 0 of a d >= 2 reply over a database too large to hold: for groups
+// [g0, g1), the group's n0 elements (element g*n0 + e, e ascending; elements at or past
+// n_elems are the zero padding polys of importDataNFL, :146-154) are produced from the
+// seeded byte stream, imported one by one with deserializeDataNFL (A bits per
+// coefficient), and accumulated against the level-0 query q0 (n0 x {a,b} x m x n,
+// canonical) with mulandaddShoup, each chain in element order as NFLLWE::mulandadd
+// (crypto/NFLLWE.cpp:176-205) and generateReply (:531-569) run it.  out: (g1 - g0) x P0
+// x {a,b} x m x n.  Groups are spread over `threads` OpenMP threads (each holds one
+// group's NTT words).  Returns 0, or -1 when an element imports to other than P0 polys.
+static inline uint64_t splitmix_word(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+int xr_level0_stream(void *hp, unsigned A, uint64_t db_seed, uint64_t n_elems, uint64_t elem_bytes, uint64_t n0,
+                     uint64_t g0, uint64_t g1, unsigned P0, const uint64_t *q0, uint64_t *out, int threads) {
+  RefHandle *h = static_cast<RefHandle *>(hp);
+  const uint64_t n = h->n, m = h->m, mn = n * m;
+  const uint64_t *moduli = h->nfl.getmoduli();
+  std::vector<uint64_t> qq(n0 * 2 * mn);
+  for (uint64_t i = 0; i < n0 * 2 * mn; i++) qq[i] = ((uint128_t)q0[i] << 64) / moduli[(i % mn) / n];
+  int bad = 0;
+  if (threads <= 0) threads = 1;
+#pragma omp parallel num_threads(threads)
+  {
+    std::vector<uint64_t> words(n0 * P0 * mn);
+    unsigned char *buf = static_cast<unsigned char *>(calloc(elem_bytes + 64, 1));
+#pragma omp for schedule(dynamic, 1)
+    for (uint64_t g = g0; g < g1; g++) {
+      for (uint64_t e = 0; e < n0; e++) {
+        uint64_t *dst = words.data() + e * P0 * mn;
+        const uint64_t idx = g * n0 + e;
+        if (idx >= n_elems) {
+          memset(dst, 0, P0 * mn * sizeof(uint64_t));
+          continue;
+        }
+        const uint64_t b0 = idx * elem_bytes;
+        for (uint64_t j = 0; j < elem_bytes; j++) {
+          const uint64_t b = b0 + j;
+          buf[j] = (unsigned char)(splitmix_word(db_seed, b >> 3) >> (8 * (b & 7)));
+        }
+        unsigned char *bufs[1] = {buf};
+        uint64_t polys = 0;
+        poly64 *res = h->nfl.deserializeDataNFL(bufs, 1, elem_bytes * 8, A, polys);
+        if (polys != P0) {
+#pragma omp atomic write
+          bad = 1;
+        } else {
+          memcpy(dst, res[0], P0 * mn * sizeof(uint64_t));
+        }
+        free(res[0]);
+        free(res);
+      }
+      uint64_t *og = out + (g - g0) * P0 * 2 * mn;
+      memset(og, 0, P0 * 2 * mn * sizeof(uint64_t));
+      for (unsigned k = 0; k < P0; k++) {
+        uint64_t *ra = og + k * 2 * mn, *rb = ra + mn;
+        for (uint64_t e = 0; e < n0; e++) {
+          const uint64_t *x = words.data() + (e * P0 + k) * mn;
+          const uint64_t *ya = q0 + e * 2 * mn, *yb = ya + mn;
+          const uint64_t *yaq = qq.data() + e * 2 * mn, *ybq = yaq + mn;
+          for (uint64_t c = 0; c < m; c++) {
+            for (uint64_t s = 0; s < n; s++)
+              NFLlib::mulandaddShoup(ra[c * n + s], x[c * n + s], ya[c * n + s], yaq[c * n + s], moduli[c]);
+            for (uint64_t s = 0; s < n; s++)
+              NFLlib::mulandaddShoup(rb[c * n + s], x[c * n + s], yb[c * n + s], ybq[c * n + s], moduli[c]);
+          }
+        }
+      }
+    }
+    free(buf);
+  }
+  return bad ? -1 : 0;
 }
 
 void xr_reply_level(void *hp, const uint64_t *db, const uint64_t *q, uint64_t n_l,

@@ -107,12 +107,16 @@ static void run_case(const char *crypto, std::vector<unsigned> dims, unsigned al
   }
   CHECK(chunks == ext.getnbPlaintextReplies(maxb));
   CHECK(plain.size() >= maxb * alpha);
-  // the aggregate: elements chosen*alpha .. +alpha-1, each zero-padded to maxb (missing
-  // elements past the database are zero)
+  // the aggregate as the reference's importDataNFL builds it: readAggregatedStream for
+  // aggregates 0..chosen in turn over one buffer (a partial last aggregate loses its last
+  // real element and keeps the previous aggregate's last slot, DBHandler.cpp:3-26)
   std::vector<char> want(maxb * alpha, 0);
-  for (unsigned a = 0; a < alpha; a++) {
-    const uint64_t e = chosen * alpha + a;
-    if (e < n_elems) memcpy(want.data() + a * maxb, data[e].data(), data[e].size());
+  for (uint64_t u = 0; u <= chosen; u++) db.readAggregatedStream(u, alpha, 0, maxb, want.data());
+  if (alpha == 3 && n_elems == 11 && chosen == 3) {  // elements 9, 10: [9, zeros, 8]
+    std::vector<char> ref(maxb * alpha, 0);
+    memcpy(ref.data(), data[9].data(), data[9].size());
+    memcpy(ref.data() + 2 * maxb, data[8].data(), data[8].size());
+    CHECK(want == ref);
   }
   CHECK(plain.size() >= want.size() && memcmp(plain.data(), want.data(), want.size()) == 0);
 

@@ -1,6 +1,7 @@
 // db_directory.cpp -- host-only checks of the facade's database handlers (no GPU):
 // DBDirectoryProcessor in one-file-per-element and file-splitting modes, its catalog
-// strings, zero padding of short files, and DBHandler::readAggregatedStream.
+// strings, zero padding of short files, DBHandler::readAggregatedStream, and two threads
+// reading one database through LockedDBHandler.
 // usage: db_directory SCRATCH_DIR   (prints "ok" and exits 0 when every check holds)
 #include <stdio.h>
 #include <string.h>
@@ -8,6 +9,7 @@
 
 #include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../xpir_amd/cpp/libpir_amd.hpp"
@@ -71,14 +73,27 @@ int main(int argc, char **argv) {
     for (size_t k = 0; k < 296; k++) CHECK(buf[k] == (k + 4 < want.size() ? want[k + 4] : 0));
     CHECK(buf[296] == 'x');
   }
-  // aggregation: streams 0..2 in aggregates of 2, the missing slot zeroed
+  // aggregation as DBHandler.cpp:3-26 does it: streams 0..2 in aggregates of 2.  The
+  // full aggregate 0 holds streams 0 and 1; the partial aggregate 1 reads stream 2 into
+  // slot 0, then zeroes one slot from slot 0 (the last real stream's), so stream 2 is
+  // erased and slot 1 keeps what the buffer held
+  auto elem = [&](uint64_t i) {
+    const std::string &f = db.fileList()[i];
+    std::vector<char> w = pattern(f == "alpha" ? 10 : f == "beta" ? 300 : 0, f == "alpha" ? 1 : f == "beta" ? 2 : 3);
+    w.resize(300, 0);
+    return w;
+  };
   std::vector<char> agg(2 * 300, 'y');
+  db.readAggregatedStream(0, 2, 0, 300, agg.data());
+  CHECK(memcmp(agg.data(), elem(0).data(), 300) == 0 && memcmp(agg.data() + 300, elem(1).data(), 300) == 0);
   db.readAggregatedStream(1, 2, 0, 300, agg.data());
-  const std::string &last = db.fileList()[2];
-  const std::vector<char> lw = pattern(last == "alpha" ? 10 : last == "beta" ? 300 : 0,
-                                       last == "alpha" ? 1 : last == "beta" ? 2 : 3);
-  for (size_t k = 0; k < 300; k++) CHECK(agg[k] == (k < lw.size() ? lw[k] : 0));
-  for (size_t k = 300; k < 600; k++) CHECK(agg[k] == 0);
+  for (size_t k = 0; k < 300; k++) CHECK(agg[k] == 0);
+  CHECK(memcmp(agg.data() + 300, elem(1).data(), 300) == 0);
+  std::vector<char> agg3(3 * 300, 'y');  // one aggregate of 3 over 3 streams: nothing zeroed
+  db.readAggregatedStream(0, 3, 0, 300, agg3.data());
+  for (uint64_t i = 0; i < 3; i++) CHECK(memcmp(agg3.data() + i * 300, elem(i).data(), 300) == 0);
+  db.readAggregatedStream(1, 3, 0, 300, agg3.data());  // past the last stream: untouched
+  CHECK(memcmp(agg3.data(), elem(0).data(), 300) == 0);
 
   // splitting: the first regular file into 7 elements of floor(1000 / 7) = 142 bytes
   const std::string sdir = root + "split/";
@@ -117,6 +132,43 @@ int main(int argc, char **argv) {
   std::vector<char> g(64, 0);
   CHECK(gen.readStreams(0, 4, 0, 16, g.data()));
   for (char ch : g) CHECK((unsigned char)ch == 0xaa);
+  // two threads importing the same many-file database through one locked handler, as
+  // concurrent server sessions do: every element read equals its file, every time
+  const std::string mdir = root + "many/";
+  mkdir(mdir.c_str(), 0755);
+  const int kFiles = 240, kBytes = 4096;
+  for (int i = 0; i < kFiles; i++) {
+    char name[32];
+    snprintf(name, sizeof(name), "f%04d", i);
+    write_file(mdir + name, pattern(kBytes, 100 + i));
+  }
+  DBDirectoryProcessor many(mdir);
+  CHECK(!many.getErrorStatus() && many.getNbStream() == (uint64_t)kFiles);
+  LockedDBHandler shared(&many);
+  std::vector<std::vector<char>> want(kFiles);
+  for (int i = 0; i < kFiles; i++) {
+    const std::string &f = many.fileList()[i];
+    want[i] = pattern(kBytes, 100 + atoi(f.c_str() + 1));
+  }
+  int bad[2] = {0, 0};
+  auto importer = [&](int t) {
+    std::vector<char> all((size_t)kFiles * kBytes);
+    for (int rep = 0; rep < 20; rep++) {
+      std::fill(all.begin(), all.end(), 'q');
+      for (int first = 0; first < kFiles; first += 16) {
+        const int cnt = std::min(16, kFiles - first);
+        if (!shared.readElements(first, cnt, 0, kBytes, all.data() + (size_t)first * kBytes)) bad[t]++;
+      }
+      for (int i = 0; i < kFiles; i++)
+        if (memcmp(all.data() + (size_t)i * kBytes, want[i].data(), kBytes) != 0) bad[t]++;
+    }
+  };
+  std::thread t0(importer, 0), t1(importer, 1);
+  t0.join();
+  t1.join();
+  CHECK(bad[0] == 0 && bad[1] == 0);
+  CHECK(!many.readElements(kFiles, 1, 0, kBytes, agg.data()));  // no such stream: refused
+
   DBDirectoryProcessor too_many(2000, sdir);  // under one byte per element
   CHECK(too_many.getErrorStatus());
   DBDirectoryProcessor missing(root + "no_such_dir/");

@@ -11,7 +11,7 @@ PKG = os.path.join(ROOT, "xpir_amd")
 
 def test_db_directory_processor(tmp_path):
     exe = tmp_path / "db_directory"
-    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-o", str(exe),
+    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-pthread", "-o", str(exe),
                     os.path.join(ROOT, "tests", "cpp", "db_directory.cpp"),
                     os.path.join(PKG, "cpp", "libpir_amd.cpp"),
                     "-L", PKG, "-lxpir_amd", f"-Wl,-rpath,{PKG}"], check=True)

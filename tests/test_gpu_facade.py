@@ -18,10 +18,11 @@ DEMO = os.path.join(ROOT, "xpir_amd", "_build", "simple_pir")
     ("LWE:80:1024:60", 50, 3000, 1, [50]),
     ("LWE:80:2048:120", 30, 4096, 1, [6, 5]),
     ("LWE:80:2048:120", 12, 1000, 3, [4]),
+    ("LWE:80:2048:120", 13, 1000, 3, [5]),   # partial last aggregate: the reference's bytes
 ])
 def test_simple_pir_through_the_facade(tmp_path, crypto, n, L, alpha, dims):
     out = tmp_path / "reply.bin"
-    index = n // alpha - 1
+    index = -(-n // alpha) - 1
     args = [DEMO, crypto, str(n), str(L), str(alpha), str(len(dims)), *map(str, dims), "1", str(index), "3", "4",
             str(out)]
     print("run:", " ".join(args[1:]), flush=True)
@@ -32,8 +33,7 @@ def test_simple_pir_through_the_facade(tmp_path, crypto, n, L, alpha, dims):
     data = orc.rand_bytes(1, 0, n * L)
     unit = L * alpha
     units = -(-n // alpha)
-    padded = np.zeros(units * unit, dtype=np.uint8)
-    padded[:data.size] = data
+    padded = Oracle.aggregate_units(data, n, L, alpha).reshape(-1)
     P = orc.polys(unit)
     words = np.zeros(int(np.prod(dims)) * P * orc.mn, dtype=np.uint64)
     words[:units * P * orc.mn] = np.concatenate([orc.import_element(padded[u * unit:(u + 1) * unit])

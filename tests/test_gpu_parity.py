@@ -19,39 +19,78 @@ def _db_bytes(orc, seed, n_elems, elem_bytes):
     return orc.rand_bytes(seed, 0, n_elems * elem_bytes)
 
 
-def _oracle_import(orc, data, n_elems, elem_bytes, alpha, units_total):
+def _oracle_import(orc, data, n_elems, elem_bytes, alpha, units_total, zero_pad=False):
     unit = elem_bytes * alpha
     P = orc.polys(unit)
-    units = -(-n_elems // alpha)
-    padded = np.zeros(units * unit, dtype=np.uint8)
-    padded[: data.size] = data
-    words = [orc.import_element(padded[u * unit:(u + 1) * unit]) for u in range(units)]
+    rows = Oracle.aggregate_units(data, n_elems, elem_bytes, alpha, zero_pad)
+    units = rows.shape[0]
+    words = [orc.import_element(r) for r in rows]
     out = np.zeros(units_total * P * orc.mn, dtype=np.uint64)
     if words:
         out[: units * P * orc.mn] = np.concatenate(words)
     return out, P
 
 
-@pytest.mark.parametrize("desc,n_elems,elem_bytes,alpha,dims", [
-    (C1, 20, 10240, 1, [20]),
-    (C1, 13, 1000, 1, [16]),          # ragged element size, padding units
-    (C2, 9, 4096, 2, [5]),            # alpha aggregation with a short last unit
-    (C2, 7, 4101, 1, [7]),
-    (C3, 5, 1024, 1, [6]),            # A = 74: two sub-chunks per coefficient
-    (C3, 3, 40000, 1, [3]),
+@pytest.mark.parametrize("desc,n_elems,elem_bytes,alpha,dims,zero_pad", [
+    (C1, 20, 10240, 1, [20], 0),
+    (C1, 13, 1000, 1, [16], 0),       # ragged element size, padding units
+    (C2, 9, 4096, 2, [5], 0),         # alpha aggregation with a short last unit: the reference's bytes
+    (C2, 9, 4096, 2, [5], 1),         # ... and with option alpha_zero_pad
+    (C2, 2, 4096, 3, [1], 0),         # one partial unit, no unit before it
+    (C1, 11, 3000, 4, [3, 2], 0),     # partial last unit of 3, padding units after it
+    (C2, 7, 4101, 1, [7], 0),
+    (C3, 5, 1024, 1, [6], 0),         # A = 78: two sub-chunks per coefficient
+    (C3, 5, 1024, 1, [1024], 0),      # n0 = 1024: A = 74, the recursive configuration's
+    (C3, 3, 40000, 1, [3], 0),
 ])
-def test_import_matches_oracle(gpu_ctx, desc, n_elems, elem_bytes, alpha, dims):
+def test_import_matches_oracle(gpu_ctx, desc, n_elems, elem_bytes, alpha, dims, zero_pad):
     import xpir_amd
     orc = Oracle(desc, dims[0])
     prm = xpir_amd.params(desc, dims[0])
     assert prm.A == orc.A
+    if dims == [1024]:
+        assert prm.A == 74
     data = _db_bytes(orc, 11, n_elems, elem_bytes)
-    db = gpu_ctx.import_db(prm, data, n_elems, elem_bytes, dims, alpha=alpha)
-    want, P = _oracle_import(orc, data, n_elems, elem_bytes, alpha, int(np.prod(dims)))
+    gpu_ctx.set_option("alpha_zero_pad", zero_pad)
+    try:
+        db = gpu_ctx.import_db(prm, data, n_elems, elem_bytes, dims, alpha=alpha)
+    finally:
+        gpu_ctx.set_option("alpha_zero_pad", 0)
+    want, P = _oracle_import(orc, data, n_elems, elem_bytes, alpha, int(np.prod(dims)), bool(zero_pad))
     assert db.polys == P
     got = db.export_words()
     assert np.array_equal(got, want)
     db.free()
+
+
+def test_alpha_aggregation_matches_reference_golden(gpu_ctx):
+    """n_elems not a multiple of alpha: the GPU import's words equal the reference's
+    (readAggregatedStream in importDataNFL's loop, then deserializeDataNFL; golden
+    hashes), and a reply over them equals the oracle's."""
+    import hashlib
+    import json
+    import os
+    import xpir_amd
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))["alpha"]
+    for g in gold:
+        orc = Oracle(g["crypto"], g["n0"])
+        prm = xpir_amd.params(g["crypto"], g["n0"])
+        n, L, alpha, units = g["n_elems"], g["elem_bytes"], g["alpha"], g["units"]
+        dims = [g["n0"]] if g["n0"] >= units else [g["n0"], -(-units // g["n0"])]
+        data = orc.rand_bytes(g["seed"], 0, n * L)
+        db = gpu_ctx.import_db(prm, data, n, L, dims, alpha=alpha)
+        try:
+            words = db.export_words()
+            per = db.polys * orc.mn
+            assert db.polys == g["polys"]
+            assert hashlib.sha256(words[: units * per].tobytes()).hexdigest() == g["ntt_sha256"], g
+            assert not words[units * per:].any()
+            sk = orc.keygen(5)
+            q = orc.make_query(sk, dims, units - 1, 6)
+            want, _ = orc.generate_reply(words, db.polys, dims, q)
+            assert np.array_equal(gpu_ctx.reply(db, np.concatenate(q)), want)
+        finally:
+            db.free()
 
 
 @pytest.mark.parametrize("desc,dims", [(C1, [7]), (C2, [3, 4]), (C3, [2, 3])])
@@ -145,43 +184,6 @@ def test_recursive_scaled_matches_golden(gpu_ctx):
     assert hashlib.sha256(rep.tobytes()).hexdigest() == gold["reply_sha256"]
     data = orc.rand_bytes(cfg["db_seed"], cfg["index"] * L, L)
     assert extract_reply(orc, sk, rep, cfg["dims"], L) == data.tobytes()
-    db.free()
-
-
-@pytest.mark.skipif(not __import__("os").environ.get("XPIR_AMD_FULL"),
-                    reason="full-size recursive configuration (96 GiB NTT database): set XPIR_AMD_FULL=1")
-def test_recursive_full_size_decrypts(gpu_ctx):
-    """LWE:100:4096:180, d=2 over 1024x1024, 2^20 x 1 KiB: the reply (6 ciphertexts)
-    decrypts to the chosen element.  The oracle cannot replay this size on the CPU; the
-    scaled grid above pins the bytes."""
-    import time
-    import xpir_amd
-    from xpir_amd.workloads import C3_RECURSIVE as cfg
-    orc = Oracle(cfg["crypto"], cfg["dims"][0])
-    prm = xpir_amd.params(cfg["crypto"], cfg["dims"][0])
-    n, L = cfg["n_elems"], cfg["elem_bytes"]
-    raw = gpu_ctx.alloc(n * L + 16)
-    gpu_ctx.random_bytes_dev(cfg["db_seed"], 0, raw, n * L)
-    t0 = time.perf_counter()
-    db = gpu_ctx.import_db_dev(prm, raw, n, L, cfg["dims"])
-    gpu_ctx.sync()
-    print(f"import of 2^20 x 1 KiB: {time.perf_counter() - t0:.2f} s", flush=True)
-    raw.free()
-    sk = orc.keygen(cfg["key_seed"])
-    skd = gpu_ctx.alloc(sk.nbytes).upload(sk)
-    qd = gpu_ctx.alloc(db.query_words * 8)
-    gpu_ctx.query_gen_dev(prm, skd, cfg["dims"], cfg["index"], cfg["noise_seed"], qd)
-    rd = gpu_ctx.alloc(db.reply_words * 8)
-    t0 = time.perf_counter()
-    gpu_ctx.reply_dev(db, qd, rd)
-    gpu_ctx.sync()
-    print(f"reply: {time.perf_counter() - t0:.2f} s", flush=True)
-    rep = rd.download()
-    assert db.reply_polys == 6
-    data = orc.rand_bytes(cfg["db_seed"], cfg["index"] * L, L)
-    assert extract_reply(orc, sk, rep, cfg["dims"], L) == data.tobytes()
-    for b in (skd, qd, rd):
-        b.free()
     db.free()
 
 

@@ -1,8 +1,8 @@
 """Serving loop (xpg_server_*): a seeded Poisson stream over several resident databases
 with Zipf database choice, d in {1, 2}, admission by reply memory (a budget small enough
 to refuse some submissions), continuous batching on two workers.  Every reply must be
-the reply of the same query computed alone (plain path), bit for bit, and decrypt to the
-chosen element."""
+the reply of the same query computed alone (plain path), bit for bit; every reply
+computed alone equals the oracle's and decrypts to the chosen element."""
 import time
 
 import numpy as np
@@ -25,14 +25,19 @@ def test_serving_mix_equals_alone(gpu_ctx):
         prm = xpir_amd.params(C2, dims[0])
         data = orc.rand_bytes(500 + si, 0, n * L)
         db = gpu_ctx.import_db(prm, data, n, L, dims)
+        P = orc.polys(L)
+        words = np.zeros(int(np.prod(dims)) * P * orc.mn, dtype=np.uint64)
+        words[: n * P * orc.mn] = np.concatenate([orc.import_element(data[e * L:(e + 1) * L]) for e in range(n)])
         sk = orc.keygen(600 + si)
         # a pool of distinct queries per database, each with its reply computed alone
         pool = []
         for qi in range(6):
             index = int(rng.integers(0, n))
-            q = np.concatenate(orc.make_query(sk, dims, index, 7000 + 10 * si + qi))
+            qlevels = orc.make_query(sk, dims, index, 7000 + 10 * si + qi)
+            q = np.concatenate(qlevels)
             qd = gpu_ctx.alloc(q.nbytes).upload(q)
             alone = gpu_ctx.reply(db, q)
+            assert np.array_equal(alone, orc.generate_reply(words, P, dims, qlevels)[0])
             assert extract_reply(orc, sk, alone, dims, L) == data[index * L:(index + 1) * L].tobytes()
             pool.append((qd, alone))
         dbs.append((db, pool))

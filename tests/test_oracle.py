@@ -297,3 +297,71 @@ def test_planted_mistakes_are_caught():
     out = o.absorb(data)
     assert int(out[1]) == (big >> o.A) & ((1 << o.A) - 1)
     assert int(out[1]) != (big >> (o.A + 1)) & ((1 << o.A) - 1) or int(out[2]) != (big >> (2 * o.A + 1)) & ((1 << o.A) - 1)
+
+
+@pytest.mark.parametrize("vec", GOLD["alpha"], ids=lambda v: f"{v['crypto']}-{v['n_elems']}by{v['alpha']}")
+def test_golden_alpha_aggregation(vec):
+    """The oracle's alpha aggregation (and import) against the reference's
+    readAggregatedStream + deserializeDataNFL bytes; zero-padding the last unit instead
+    (the plausible 'fixed' reading) must not match."""
+    o = Oracle(vec["crypto"], vec["n0"])
+    n, L, a = vec["n_elems"], vec["elem_bytes"], vec["alpha"]
+    data = o.rand_bytes(vec["seed"], 0, n * L)
+    units = Oracle.aggregate_units(data, n, L, a)
+    assert units.shape[0] == vec["units"]
+    assert hashlib.sha256(units.tobytes()).hexdigest() == vec["units_sha256"]
+    words = np.concatenate([o.import_element(u) for u in units])
+    assert hashlib.sha256(words.tobytes()).hexdigest() == vec["ntt_sha256"]
+    zp = Oracle.aggregate_units(data, n, L, a, zero_pad=True)
+    assert hashlib.sha256(zp.tobytes()).hexdigest() != vec["units_sha256"]
+
+
+@pytest.mark.skipif(not RefNFL.available(), reason="reference build (oracle/_ref) absent")
+def test_alpha_aggregation_against_reference_build():
+    """Every (n_elems, alpha, element size) up to 13 x 5: the oracle's units equal the
+    reference's DBHandler/DBVectorProcessor output, ragged element sizes included."""
+    rng = np.random.default_rng(12)
+    for n in range(1, 14):
+        for a in range(1, 6):
+            sizes = rng.integers(1, 9, n)
+            els = [rng.integers(0, 256, int(s), dtype=np.uint8).tobytes() for s in sizes]
+            L = int(sizes.max())
+            padded = np.concatenate([np.frombuffer(e + bytes(L - len(e)), dtype=np.uint8) for e in els])
+            assert np.array_equal(Oracle.aggregate_units(padded, n, L, a), RefNFL.aggregate(els, a)), (n, a)
+
+
+@pytest.mark.skipif(not RefNFL.available(), reason="reference build (oracle/_ref) absent")
+@pytest.mark.parametrize("desc,dims,n,L", [(C3, [16, 12], 180, 1024), (C2, [20, 9], 171, 16384)])
+def test_row_streamed_level0_matches_whole_database(desc, dims, n, L):
+    """The row-streamed level 0 behind the full-size goldens (one group's elements made,
+    imported and accumulated at a time) equals the reference's arithmetic over the whole
+    imported database, padding units and partial group ranges included."""
+    o = Oracle(desc, dims[0])
+    r = RefNFL(o.n, o.m * 60)
+    P0 = o.polys(L)
+    data = o.rand_bytes(5, 0, n * L)
+    words = np.zeros(dims[0] * dims[1] * P0 * o.mn, dtype=np.uint64)
+    for e in range(n):
+        words[e * P0 * o.mn:(e + 1) * P0 * o.mn] = r.import_buffers(data[e * L:(e + 1) * L], 1, o.A)
+    q = o.make_query(o.keygen(3), dims, 7, 9)
+    want = r.reply_level(words, q[0], dims[0], dims[1], P0, 4)
+    assert np.array_equal(r.level0_stream(o.A, 5, n, L, dims[0], 0, dims[1], P0, q[0], 4), want)
+    parts = [r.level0_stream(o.A, 5, n, L, dims[0], g, min(g + 5, dims[1]), P0, q[0], 2)
+             for g in range(0, dims[1], 5)]
+    assert np.array_equal(np.concatenate(parts), want)
+    assert np.array_equal(o.reply_level(words, q[0], dims[0], dims[1], P0), want)
+
+
+def test_full_size_golden_entries():
+    """The full-size fixtures describe the configurations they claim (A, polys, grids)."""
+    from xpir_amd.workloads import C3_RECURSIVE, batch_databases, serving_d1_views
+    c3 = GOLD["c3_recursive"]
+    assert (c3["A"], c3["P0"], c3["Pd"], c3["dims"], c3["n_elems"]) == (74, 1, 6, [1024, 1024], 1 << 20)
+    assert c3["index"] == C3_RECURSIVE["index"]
+    for g, spec in zip(GOLD["c4_batched"], batch_databases()):
+        o = Oracle(spec["crypto"], spec["dims"][0])
+        assert (g["dims"], g["n_elems"], g["elem_bytes"], g["A"]) == (spec["dims"], spec["n_elems"],
+                                                                      spec["elem_bytes"], o.A)
+        assert g["P0"] == o.polys(spec["elem_bytes"]) and g["Pd"] == o.next_polys(g["P0"])
+    for g, v in zip(GOLD["c5_d1_views"], serving_d1_views()):
+        assert (g["dims"], g["index"], g["noise_seed"]) == (v["dims"], v["index"], v["noise_seed"])

@@ -317,18 +317,32 @@ std::string DBHandler::getCatalog(bool typeOfCatalog) {
 
 void DBHandler::readAggregatedStream(uint64_t streamNb, uint64_t alpha, uint64_t offset, uint64_t bytes_per_file,
                                      char *rawBits) {
-  const uint64_t maxb = getmaxFileBytesize();
+  const uint64_t maxb = getmaxFileBytesize(), nb = getNbStream();
   const uint64_t file_bytes = offset >= maxb ? 0 : std::min(bytes_per_file, maxb - offset);
-  const uint64_t first = streamNb * alpha, nb = getNbStream();
-  for (uint64_t s = 0; s < alpha; s++) {
-    char *dst = rawBits + s * file_bytes;
-    if (first + s < nb && openStream(first + s, offset)) {
-      readStream(first + s, dst, file_bytes);
-      closeStream(first + s);
+  if (alpha == 0 || streamNb * alpha >= nb) return;
+  const uint64_t first = streamNb * alpha, end = first + alpha;  // streams [first, end)
+  const uint64_t last = std::min(end, nb) - 1;
+  for (uint64_t i = first; i <= last; i++) {
+    char *dst = rawBits + (i % alpha) * file_bytes;
+    if (openStream(i, offset)) {
+      readStream(i, dst, file_bytes);
+      closeStream(i);
     } else {
       memset(dst, 0, file_bytes);
     }
   }
+  const uint64_t missing = end > nb ? end - nb : 0;
+  if (missing) memset(rawBits + (last % alpha) * file_bytes, 0, missing * file_bytes);
+}
+
+bool DBHandler::readElements(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) {
+  if (readStreams(first, count, offset, bytes, out)) return true;
+  for (uint64_t i = first; i < first + count; i++) {
+    if (!openStream(i, offset)) return false;
+    readStream(i, out + (i - first) * bytes, bytes);
+    closeStream(i);
+  }
+  return true;
 }
 
 static std::string with_slash(const std::string &d) {
@@ -583,14 +597,11 @@ imported_database *PIRReplyGenerator::importData(uint64_t offset, uint64_t bytes
     fail(XPG_EINVAL, "empty database");
     return nullptr;
   }
-  // elements [first, first + count) into out: one block read when the handler has one
+  // elements [first, first + count) into out: one block read when the handler has one;
+  // a stream that does not open fails the import (never a silently zeroed element)
+  bool read_ok = true;
   auto read = [&](uint64_t first, uint64_t count, char *out) {
-    if (db_->readStreams(first, count, offset, file_bytes, out)) return;
-    for (uint64_t i = first; i < first + count; i++) {
-      db_->openStream(i, offset);
-      db_->readStream(i, out + (i - first) * file_bytes, file_bytes);
-      db_->closeStream(i);
-    }
+    if (read_ok && !db_->readElements(first, count, offset, file_bytes, out)) read_ok = false;
   };
   xpg_db *h = nullptr;
   int rc = XPG_OK;
@@ -606,7 +617,11 @@ imported_database *PIRReplyGenerator::importData(uint64_t offset, uint64_t bytes
     for (uint64_t first = 0; first < nb && rc == XPG_OK; first += per, k ^= 1) {
       const uint64_t count = std::min(per, nb - first);
       read(first, count, static_cast<char *>(pin[k]));
-      if (pending) rc = xpg_sync(ctx_);  // the other chunk's copy, issued a chunk ago
+      if (!read_ok) rc = XPG_EINVAL;
+      if (pending) {
+        const int rs = xpg_sync(ctx_);  // the other chunk's copy, issued a chunk ago
+        if (rc == XPG_OK) rc = rs;
+      }
       if (rc == XPG_OK)
         rc = xpg_h2d_async(ctx_, static_cast<char *>(dev) + first * file_bytes, pin[k], count * file_bytes);
       pending = true;
@@ -625,11 +640,13 @@ imported_database *PIRReplyGenerator::importData(uint64_t offset, uint64_t bytes
       return nullptr;
     }
     read(0, nb, host.get());
-    rc = xpg_db_import(ctx_, &prm_, reinterpret_cast<const uint8_t *>(host.get()), nb, file_bytes, param_.alpha,
+    rc = read_ok ? XPG_OK : XPG_EINVAL;
+    if (rc == XPG_OK)
+      rc = xpg_db_import(ctx_, &prm_, reinterpret_cast<const uint8_t *>(host.get()), nb, file_bytes, param_.alpha,
                        param_.d, param_.n, &h);
   }
   if (rc != XPG_OK) {
-    fail(rc, "importData");
+    fail(rc, read_ok ? "importData" : "importData: a database stream did not open");
     return nullptr;
   }
   imported_any_ = true;

@@ -21,6 +21,7 @@
 #include <stdio.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -103,10 +104,17 @@ class DBHandler {
     (void)first, (void)count, (void)offset, (void)bytes, (void)out;
     return false;
   }
+  // Elements first .. first+count-1 into out as importData needs them: readStreams when
+  // the handler reads them as one block, else stream by stream (open, read, close each).
+  // False when a stream does not open.  A handler shared between threads overrides this
+  // to hold its lock across the whole sequence.  (No reference counterpart.)
+  virtual bool readElements(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out);
   // The alpha streams of aggregate `streamNb`, each read from `offset` and zero-padded to
-  // min(bytes_per_file, maxFileBytesize - offset) bytes, back to back in rawBits; slots
-  // past the last stream are zeroed (DBHandler.cpp:3-26, whose bzero starts at the last
-  // real stream's slot instead, erasing it when the last aggregate is partial).
+  // fileByteSize = min(bytes_per_file, maxFileBytesize - offset) bytes, into slot
+  // (stream mod alpha) of rawBits; when the aggregate runs past the last stream, as many
+  // slots as are missing are zeroed starting at the last real stream's slot, which
+  // erases that stream and leaves the slot after the zeroed ones as rawBits held it
+  // (DBHandler.cpp:3-26: importDataNFL passes the same buffer for every aggregate).
   void readAggregatedStream(uint64_t streamNb, uint64_t alpha, uint64_t offset, uint64_t bytes_per_file,
                             char *rawBits);
 };
@@ -183,6 +191,46 @@ class DBDirectoryProcessor : public DBHandler {
   int split_fd_ = -1;
   std::map<uint64_t, uint64_t> split_pos_;
   uint64_t max_bytes_ = 0;
+};
+
+// A handler shared by threads (concurrent server sessions importing one database):
+// every call of the inner handler under one lock, and readElements's whole
+// open/read/close sequence under it at once, so that two imports never interleave on the
+// per-stream state the handlers keep (DBHandler.hpp promises no thread safety;
+// importDataNFL is documented as not thread-safe, PIRReplyGeneratorNFL_internal.cpp:101).
+class LockedDBHandler : public DBHandler {
+ public:
+  explicit LockedDBHandler(DBHandler *inner) : in_(inner) {}
+  uint64_t getNbStream() override { return in_->getNbStream(); }
+  uint64_t getmaxFileBytesize() override { return in_->getmaxFileBytesize(); }
+  bool openStream(uint64_t s, uint64_t off) override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->openStream(s, off);
+  }
+  uint64_t readStream(uint64_t s, char *buf, uint64_t size) override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->readStream(s, buf, size);
+  }
+  void closeStream(uint64_t s) override {
+    std::lock_guard<std::mutex> l(mu_);
+    in_->closeStream(s);
+  }
+  std::string getCatalog(bool full) override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->getCatalog(full);
+  }
+  bool readStreams(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->readStreams(first, count, offset, bytes, out);
+  }
+  bool readElements(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->readElements(first, count, offset, bytes, out);
+  }
+
+ private:
+  DBHandler *in_;
+  std::mutex mu_;
 };
 
 // The pre-processed (NTT-domain) database, resident in HBM until destroyed.

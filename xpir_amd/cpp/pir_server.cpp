@@ -56,40 +56,6 @@ void usage(const char *prog) {
           prog, kDefaultParamsFile);
 }
 
-// Every call of a shared database handler under one lock: concurrent sessions import
-// through it, and the handlers keep per-stream state (DBHandler.hpp makes no promise of
-// thread safety; importDataNFL is documented as not thread-safe).
-class LockedDB : public DBHandler {
- public:
-  explicit LockedDB(DBHandler *inner) : in_(inner) {}
-  uint64_t getNbStream() override { return in_->getNbStream(); }
-  uint64_t getmaxFileBytesize() override { return in_->getmaxFileBytesize(); }
-  bool openStream(uint64_t s, uint64_t off) override {
-    std::lock_guard<std::mutex> l(mu_);
-    return in_->openStream(s, off);
-  }
-  uint64_t readStream(uint64_t s, char *buf, uint64_t size) override {
-    std::lock_guard<std::mutex> l(mu_);
-    return in_->readStream(s, buf, size);
-  }
-  void closeStream(uint64_t s) override {
-    std::lock_guard<std::mutex> l(mu_);
-    in_->closeStream(s);
-  }
-  std::string getCatalog(bool full) override {
-    std::lock_guard<std::mutex> l(mu_);
-    return in_->getCatalog(full);
-  }
-  bool readStreams(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) override {
-    std::lock_guard<std::mutex> l(mu_);
-    return in_->readStreams(first, count, offset, bytes, out);
-  }
-
- private:
-  DBHandler *in_;
-  std::mutex mu_;
-};
-
 bool number(const char *s, uint64_t *out) {
   if (!s || !*s) return false;
   char *end = nullptr;
@@ -191,7 +157,7 @@ int main(int argc, char **argv) {
   }
   fprintf(stderr, "pir_server: database of %llu elements of %llu bytes\n", (unsigned long long)db->getNbStream(),
           (unsigned long long)db->getmaxFileBytesize());
-  LockedDB shared_db(db);
+  LockedDBHandler shared_db(db);
 
   session_option_t opt;
   opt.driven_mode = true;

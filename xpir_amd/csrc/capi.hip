@@ -392,6 +392,10 @@ static int set_option(xpg_ctx *ctx, const char *key, int64_t value) {
     ctx->group_budget = (uint64_t)value;
     return XPG_OK;
   }
+  if (strcmp(key, "alpha_zero_pad") == 0 && (value == 0 || value == 1)) {
+    ctx->alpha_zero_pad = (int)value;
+    return XPG_OK;
+  }
   if (strcmp(key, "graphs") == 0 && (value == 0 || value == 1)) {
     ctx->graphs = value != 0;
     if (!ctx->graphs) graph_cache_clear(ctx, 0);
@@ -520,6 +524,42 @@ static int db_setup(xpg_ctx *ctx, const xpg_params *prm, uint64_t n_elems, uint6
   return XPG_OK;
 }
 
+// The last import unit when n_elems is not a multiple of alpha, with the bytes the
+// reference's importDataNFL gives it (PIRReplyGeneratorNFL_internal.cpp:86-105 over
+// DBHandler::readAggregatedStream, DBHandler.cpp:3-26): the buffer is shared by all
+// units; the last unit's r real elements go to slots 0..r-1, then alpha - r slots from
+// slot r-1 are zeroed, so the last real element is erased and slot alpha-1 keeps the
+// previous unit's last element (zeros when there is no previous unit).  The unit is
+// staged in a small device buffer and imported with the grid's padding units after it.
+static int import_partial_aggregate(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems_dev, uint64_t n_elems,
+                                    uint64_t elem_bytes, uint32_t alpha, uint32_t P0, xpg_db *db) {
+  const uint64_t r = n_elems % alpha, k = n_elems / alpha;  // k full units before the last
+  const uint64_t unit_bytes = elem_bytes * alpha;
+  uint8_t *tmp = nullptr;
+  HIP_OK(hipMalloc((void **)&tmp, unit_bytes + 16), "aggregate staging");
+  int rc = XPG_OK;
+  const uint8_t *first = elems_dev + k * unit_bytes;
+  if (r > 1 && hipMemcpyAsync(tmp, first, (r - 1) * elem_bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+    rc = set_err(ctx, XPG_EDEVICE, "aggregate staging copy");
+  if (!rc && hipMemsetAsync(tmp + (r - 1) * elem_bytes, 0, (alpha - r) * elem_bytes, ctx->stream) != hipSuccess)
+    rc = set_err(ctx, XPG_EDEVICE, "aggregate staging clear");
+  uint8_t *last_slot = tmp + (alpha - 1) * elem_bytes;
+  if (!rc && k > 0 &&
+      hipMemcpyAsync(last_slot, elems_dev + (k * alpha - 1) * elem_bytes, elem_bytes, hipMemcpyDeviceToDevice,
+                     ctx->stream) != hipSuccess)
+    rc = set_err(ctx, XPG_EDEVICE, "aggregate staging copy");
+  if (!rc && k == 0 && hipMemsetAsync(last_slot, 0, elem_bytes, ctx->stream) != hipSuccess)
+    rc = set_err(ctx, XPG_EDEVICE, "aggregate staging clear");
+  if (!rc) {
+    ByteSrc src{tmp, unit_bytes, unit_bytes, unit_bytes};
+    const uint64_t per_unit = (uint64_t)P0 * prm->m * prm->n;
+    rc = launch_import_bytes(ctx, prm, src, 1, db->units - k, P0, db->words + k * per_unit);
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(tmp);
+  return rc;
+}
+
 int xpg_db_import_dev(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems_dev, uint64_t n_elems,
                       uint64_t elem_bytes, uint32_t alpha, uint32_t d, const uint32_t *dims, xpg_db **out) {
   if (!ctx || !prm || !out) return XPG_EINVAL;
@@ -532,7 +572,14 @@ int xpg_db_import_dev(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems_
   int rc = db_setup(ctx, prm, n_elems, elem_bytes, alpha, d, dims, P0, real_units, out);
   if (rc) return rc;
   ByteSrc src{elems_dev, unit_bytes, unit_bytes, n_elems * elem_bytes};
-  rc = launch_import_bytes(ctx, prm, src, real_units, (*out)->units, P0, (*out)->words);
+  const uint64_t r = n_elems % alpha;
+  if (r == 0 || ctx->alpha_zero_pad) {
+    rc = launch_import_bytes(ctx, prm, src, real_units, (*out)->units, P0, (*out)->words);
+  } else {
+    // units before the last as they lie; the last one staged as the reference fills it
+    rc = launch_import_bytes(ctx, prm, src, real_units - 1, real_units - 1, P0, (*out)->words);
+    if (!rc) rc = import_partial_aggregate(ctx, prm, elems_dev, n_elems, elem_bytes, alpha, P0, *out);
+  }
   if (rc) {
     xpg_db_free(*out);
     *out = nullptr;

@@ -103,6 +103,9 @@ struct xpg_ctx {
   int stream_pf = 0;               // streaming MAC: double-buffered chunk loads
   int cu_count = 256;              // compute units of the device
   uint64_t group_budget = 0;       // bytes for a batch's grouped levels (0 = default)
+  // a partial last alpha aggregate: 0 = the reference's bytes (readAggregatedStream's
+  // bzero from the last real element's slot), 1 = the missing elements read as zeros
+  int alpha_zero_pad = 0;
   // MAC timing
   int timing = 0;  // 0 off, 1 every MAC level and main kernel, 2 main kernels alone
   std::vector<hipEvent_t> ev_pool;

@@ -69,6 +69,23 @@ def batch_databases(scale: float = 1.0):
     return out
 
 
+def batch_golden_query(i: int, spec: dict) -> dict:
+    """The seeded query whose reply the golden fixtures hold for batched database i."""
+    return dict(index=_index(4300 + i, spec["n_elems"]), noise_seed=4400 + i, key_seed=spec["key_seed"])
+
+
+def serving_d1_views():
+    """d = 1 views of the two smallest batched databases (their d = 1 queries, 625 MiB and
+    1.5 GiB, fit a serving budget; A depends on the first dimension, so each view is a
+    transform of its own), with the seeded query the golden fixtures answer."""
+    out = []
+    for i, spec in enumerate(batch_databases()[:2]):
+        v = dict(spec, dims=[spec["n_elems"]], name=f"serving-d1-db{i}-{spec['n_elems']}x{spec['elem_bytes']}")
+        v.update(index=_index(4500 + i, spec["n_elems"]), noise_seed=4600 + i)
+        out.append(v)
+    return out
+
+
 def batch_query_counts(seed: int = 41, n_db: int = 8, lo: int = 1, hi: int = 64):
     import numpy as np
     return [int(x) for x in np.random.default_rng(seed).integers(lo, hi + 1, size=n_db)]
