@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
 """bench.py -- NFLLWE PIR reply throughput on one MI355X.
 
-Workload (default): single query, LWE:80:2048:120 (n=2048, two 60-bit moduli), d=1,
-alpha=1, 65536 elements x 4 KiB of seeded random bytes (256 MiB), one seeded query.
-The database is imported (absorb + NTT) once and stays resident in HBM; the query
-(65536 ciphertexts, 4 GiB) is generated on the GPU and resident before timing starts.
+Default workload (the metric, "DB GB/s per reply, batched vs plain; p50/p99 reply ms"):
+the batched uneven configuration -- eight resident LWE:80:2048:120 databases of
+10 000 .. 1 000 000 elements of 256 B .. 16 KiB (seeded bytes), d = 2 grids, and the
+seeded number of queries for each (334 in all), generated on the GPU and resident in HBM
+before timing starts.
 
-A step is one whole reply: xpg_reply_dev (query in HBM -> reply in HBM).
-value = raw DB bytes / reply time (decimal GB/s), the reference's "DB processing
-throughput" (PIRReplyGeneratorNFL_internal.cpp:286-288) in bytes instead of bits.
-
-Also reported: p50/p99 reply latency, import GB/s, bit-exactness against the golden
-reply bytes (tests/golden/golden.json, written from the reference's own arithmetic),
-the MAC kernel's roofline (HIP events around every MAC launch on the context stream)
-and the reference's CPU reply on this host (oracle/_ref, same inputs, same run).
+A step is one batched pass: every query answered by one xpg_reply_batch_dev call (query
+in HBM -> reply in HBM).  value = raw DB bytes of all replies / pass time (decimal GB/s),
+the reference's "DB processing throughput" (PIRReplyGeneratorNFL_internal.cpp:286-288)
+in bytes.  Beside it: the plain path (each query alone through xpg_reply_dev, one after
+another) on the same queries, and p50/p99 reply latency from a seeded Poisson stream
+through the continuous-batching server over the same databases (plus d = 1 views of the
+two smallest).  Bit-exactness: each database's golden query against the reference's
+reply bytes (tests/golden/golden.json), every batched reply against the plain reply.
+roofline: the batched level-0 kernel (mac_batch2_kernel), integer-issue bound: exact
+chain steps/s (HIP events around its launches in the timed passes) x VALU lane
+instructions per step (rocprofv3 SQ_INSTS_VALU, profiles/) against 39.3 T lane ops/s.
+cpu_baseline: the reference's own arithmetic (oracle/_ref) on this host on a bounded
+sample of the same replies.  The single-query configuration (65536 x 4 KiB, HBM-bound)
+is reported under "single_query".
 """
 from __future__ import annotations
 
@@ -121,12 +128,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="single",
-                    choices=["single", "plain", "recursive", "batched", "serving", "session", "copy"])
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 3 for metric, else 20)")
+    ap.add_argument("--workload", default="metric",
+                    choices=["metric", "batched-pass", "single", "plain", "recursive", "batched", "serving",
+                             "session", "copy"])
     ap.add_argument("--scale", type=float, default=1.0, help="element-count scale of the batched/serving databases")
-    ap.add_argument("--queries", type=int, default=2000, help="serving: queries in the Poisson stream")
-    ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
+    ap.add_argument("--queries", type=int, default=None,
+                    help="serving: queries in the Poisson stream (default 2000; metric: 3000)")
+    ap.add_argument("--rate", type=float, default=None, help="serving: arrivals per second (default 300)")
+    ap.add_argument("--no-single", action="store_true", help="metric: skip the single-query sub-report")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the MAC autotuner (fixed kernel rules); it runs before any timed region")
@@ -143,6 +153,16 @@ def main():
     import xpir_amd
     from xpir_amd.workloads import C1_PLAIN, C2_SINGLE
 
+    if args.warmup is None:
+        args.warmup = 3 if args.workload == "metric" else 20
+    if args.queries is None:
+        args.queries = 3000 if args.workload == "metric" else 2000
+    if args.rate is None:
+        args.rate = 300.0
+    if args.workload == "metric":
+        return bench_metric(args)
+    if args.workload == "batched-pass":
+        return bench_batched_pass(args)
     if args.workload == "batched":
         return bench_batched(args)
     if args.workload == "serving":
@@ -155,11 +175,22 @@ def main():
         return bench_session(args)
 
     cfg = C2_SINGLE if args.workload == "single" else C1_PLAIN
-    gold_path = os.path.join(ROOT, "tests", "golden", "golden.json")
-    gold = json.load(open(gold_path)) if os.path.exists(gold_path) else {}
-    gold = gold.get("c2_single" if args.workload == "single" else "c1_plain")
-
+    gold = load_gold().get("c2_single" if args.workload == "single" else "c1_plain")
     ctx = xpir_amd.Context(0)
+    out = run_single(ctx, args, cfg, gold)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def load_gold():
+    path = os.path.join(ROOT, "tests", "golden", "golden.json")
+    return json.load(open(path)) if os.path.exists(path) else {}
+
+
+def run_single(ctx, args, cfg, gold):
+    """One database, one query: import, autotune, K replies back to back (value), the MAC
+    kernel's HBM roofline, latency, the CPU baselines.  Returns the result dict."""
+    import xpir_amd
     prm = xpir_amd.params(cfg["crypto"], cfg["dims"][0])
     n, L, dims = cfg["n_elems"], cfg["elem_bytes"], cfg["dims"]
     raw = n * L
@@ -302,11 +333,10 @@ def main():
     }
     if host_io is not None:
         out["host_io"] = host_io
-    print(json.dumps(out), flush=True)
     for b in (skd, qd, rd):
         b.free()
     db.free()
-    ctx.close()
+    return out
 
 
 def main_kernel_name(tuned, n_l):
@@ -925,6 +955,350 @@ def bench_serving(args):
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # lane ops/s: CUs x SIMDs x lanes x clock = 39.3 T
+
+
+def batch_kernel_profile():
+    """VALU lane instructions per exact chain step of the batched level-0 kernel, and its
+    HBM bytes per pass, from the rocprofv3 counter runs committed under profiles/ (the
+    command that made them is recorded there); None when absent."""
+    path = os.path.join(ROOT, "profiles", "mac_batch2_kernel_batched_uneven_8dbs.json")
+    try:
+        return json.load(open(path))
+    except (OSError, ValueError):
+        return None
+
+
+def make_batch_queries(ctx, dbs, counts, rng):
+    """Per database: its golden query first, then count-1 seeded queries (indices from
+    rng, noise seeds 9000 + 100 i + j), all generated on the GPU."""
+    from xpir_amd.workloads import batch_golden_query
+    reqs = []
+    for i, (d, nq) in enumerate(zip(dbs, counts)):
+        d["reqs"] = []
+        gq = batch_golden_query(i, d["spec"])
+        for j in range(nq):
+            idx = gq["index"] if j == 0 else int(rng.integers(0, d["spec"]["n_elems"]))
+            seed = gq["noise_seed"] if j == 0 else 9000 + 100 * i + j
+            qd = ctx.alloc(d["db"].query_words * 8)
+            ctx.query_gen_dev(d["prm"], d["skd"], d["spec"]["dims"], idx, seed, qd)
+            rd = ctx.alloc(d["db"].reply_words * 8)
+            d["reqs"].append((d["db"], qd, rd))
+        reqs += d["reqs"]
+    ctx.sync()
+    return reqs
+
+
+def tune_batch_dbs(ctx, dbs, counts, args):
+    tuned, secs = [], 0.0
+    for i, (d, nq) in enumerate(zip(dbs, counts)):
+        bucket = 1
+        while bucket < min(nq, 64):
+            bucket *= 2
+        r = autotune_db(ctx, d["db"], bucket if bucket >= 2 else 0, args, f"db{i}")
+        tuned.append(r)
+        secs += r["seconds"] if r else 0.0
+    return tuned, secs
+
+
+def bench_batched_pass(args):
+    """One batched pass over the eight databases and nothing else on the GPU's MAC path
+    after setup (the tuning table loaded from --tune-file, no tuning launches): the run
+    that rocprofv3 profiles for the batched kernel's counters and its kernel trace."""
+    import xpir_amd
+    from xpir_amd.workloads import batch_query_counts
+    ctx = xpir_amd.Context(0)
+    dbs = _load_batch_dbs(ctx, args.scale)
+    counts = batch_query_counts()
+    reqs = make_batch_queries(ctx, dbs, counts, np.random.default_rng(42))
+    if args.tune_file and os.path.exists(args.tune_file):
+        ctx.tune_load(args.tune_file)
+    order = np.random.default_rng(43).permutation(len(reqs))
+    batch = [reqs[k] for k in order]
+    ctx.timing(2)
+    ctx.timing_read_batched(reset=True)
+    passes = max(1, args.steps)
+    for _ in range(passes):
+        ctx.reply_batch_dev(batch)
+    ctx.sync()
+    ms, launches, steps = ctx.timing_read_batched(reset=True)
+    print(json.dumps({"workload": "batched-pass", "passes": passes, "batch_kernel_ms_per_pass": ms / passes,
+                      "batch_kernel_launches_per_pass": launches / passes,
+                      "exact_steps_per_pass": steps / passes, "queries": len(reqs)}), flush=True)
+    ctx.close()
+
+
+def serving_latency(ctx, dbs, args):
+    """p50/p99 reply latency (submit -> reply in HBM) of a seeded Poisson stream over the
+    resident databases through the continuous-batching server: Zipf(1.1) database choice,
+    d in {1, 2} (d = 1 through views of the two smallest databases), every reply compared
+    with its query answered alone."""
+    import xpir_amd
+    from xpir_amd.workloads import serving_d1_views
+    views = list(dbs)
+    d1 = []
+    for spec in serving_d1_views():
+        prm = xpir_amd.params(spec["crypto"], spec["dims"][0])
+        raw = spec["n_elems"] * spec["elem_bytes"]
+        buf = ctx.alloc(raw + 16)
+        ctx.random_bytes_dev(spec["db_seed"], 0, buf, raw)
+        db = ctx.import_db_dev(prm, buf, spec["n_elems"], spec["elem_bytes"], spec["dims"])
+        buf.free()
+        sk = xpir_amd.keygen(prm, spec["key_seed"])
+        d1.append(dict(spec=spec, prm=prm, db=db, raw=raw, skd=ctx.alloc(sk.nbytes).upload(sk)))
+    views += d1
+    rng = np.random.default_rng(51)
+    bufs = []
+    for vi, v in enumerate(views):
+        v["pool"] = []
+        for j in range(4):
+            qd = ctx.alloc(v["db"].query_words * 8)
+            ctx.query_gen_dev(v["prm"], v["skd"], v["spec"]["dims"], int(rng.integers(0, v["spec"]["n_elems"])),
+                              5000 + 100 * vi + j, qd)
+            rd = ctx.alloc(v["db"].reply_words * 8)
+            ctx.reply_dev(v["db"], qd, rd)
+            v["pool"].append((qd, rd.download()))
+            bufs += [qd, rd]
+    w = np.array([1.0 / (k + 1) ** 1.1 for k in range(8)])
+    w /= w.sum()
+    n = args.queries
+    choice = rng.choice(8, size=n, p=w)
+    use_d1 = rng.integers(0, 2, size=n)
+    pick = rng.integers(0, 4, size=n)
+    gaps = rng.exponential(1.0 / args.rate, size=n)
+    srv = xpir_amd.Server(0, reply_pool_bytes=8 << 30, workers=2, max_batch=64)
+    expected, lat, refused, served_raw, mismatches, i = {}, [], 0, 0, 0, 0
+    t0 = t_next = last = time.perf_counter()
+    while i < n or expected:
+        now = time.perf_counter()
+        while i < n and now >= t_next:
+            v = views[8 + choice[i]] if choice[i] < 2 and use_d1[i] else views[choice[i]]
+            qd, alone = v["pool"][pick[i]]
+            t = srv.submit(v["db"], qd)
+            if t is None:
+                refused += 1
+                break
+            expected[t] = (alone, v["raw"])
+            t_next += gaps[i]
+            i += 1
+        for c in srv.poll(timeout_us=200):
+            alone, raw = expected.pop(c.ticket)
+            mismatches += not np.array_equal(ctx.download_ptr(c.reply_dev, c.reply_words), alone)
+            lat.append((c.done_us - c.submit_us) / 1e3)
+            served_raw += raw
+            srv.release(c.ticket)
+        if now - last > 10:
+            log(f"serving: {len(lat)}/{n} served, {len(expected)} in flight, {refused} refused")
+            last = now
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    for b in bufs:
+        b.free()
+    for v in d1:
+        v["skd"].free()
+        v["db"].free()
+    lat = np.array(lat)
+    return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+            "queries": n, "rate_per_s": args.rate, "served_gbs": served_raw / makespan / 1e9,
+            "makespan_s": makespan, "batches": st["batches"], "refused_submits": refused,
+            "bit_exact_vs_alone": mismatches == 0,
+            "stream": "seeded Poisson arrivals, Zipf(1.1) over the 8 databases, d in {1,2} (d=1 views of the "
+                      "two smallest), continuous-batching server: 2 workers, batches of <= 64, 8 GiB reply budget"}
+
+
+def cpu_baseline_batched(dbs, counts, golden_replies, budget_steps=4.0e8):
+    """The reference's reply arithmetic (oracle/_ref: its NFLlib deserializeDataNFL and
+    mulandaddShoup, threads as its MULTI_THREAD build uses them: OpenMP over the polys of
+    an element) on this host, for one query per database, on a bounded sample: level 0 on
+    the first k groups of each grid (k groups cover ~budget_steps chain steps; the whole
+    grid for the smallest database), scaled to all groups; the recursion import and the
+    last level whole, as generateReply times them (PIRReplyGeneratorNFL_internal.cpp:359,444).
+    value = raw bytes of the batch's replies / the summed CPU reply times."""
+    from oracle.oracle import Oracle, RefNFL
+    if not RefNFL.available():
+        return {"error": "reference build (oracle/_ref) absent"}
+    total_t, total_raw, cores, parts = 0.0, 0, 1, []
+    for i, (d, nq) in enumerate(zip(dbs, counts)):
+        spec, prm = d["spec"], d["prm"]
+        d0, d1 = spec["dims"]
+        orc = Oracle(spec["crypto"], d0)
+        ref = RefNFL(prm.n, prm.m * 60)
+        P0, mn = d["db"].polys, prm.m * prm.n
+        q = d["golden_query"]
+        q0, q1 = q[: d0 * 2 * mn], q[d0 * 2 * mn:]
+        per_group = P0 * mn * 2 * d0
+        k = d1 if i == 0 else max(1, min(d1, int(budget_steps // per_group)))
+        words = ref.import_range(orc.A, spec["db_seed"], spec["n_elems"], spec["elem_bytes"], 0, k * d0, P0)
+        rep0, t_l0 = ref.reply_level_timed(words, q0, d0, k, P0, P0)
+        del words
+        t_l0 *= d1 / k
+        # the recursion import over all groups' level-0 ciphertexts (k sampled groups' real
+        # words, the rest a stand-in of the same size: the import's cost is data-blind)
+        ct = P0 * 2 * mn
+        lvl0 = np.resize(rep0, d1 * ct)
+        P1 = orc.next_polys(P0)
+        nxt = np.zeros(d1 * P1 * mn, dtype=np.uint64)
+        rb = lvl0.view(np.uint8)
+        t1 = time.perf_counter()
+        for g in range(d1):
+            nxt[g * P1 * mn:(g + 1) * P1 * mn] = ref.import_buffers(rb[g * ct * 8:(g + 1) * ct * 8], P0, orc.A)
+        t_rec = time.perf_counter() - t1
+        final, t_l1 = ref.reply_level_timed(nxt, q1, d1, 1, P1, P1)
+        t = t_l0 + t_rec + t_l1
+        match = bool(np.array_equal(final, golden_replies[i])) if k == d1 else None
+        parts.append({"db": i, "groups_sampled": k, "of_groups": d1, "reply_s": t, "level0_s": t_l0,
+                      "recursion_s": t_rec, "level1_s": t_l1, "matches_gpu_reply": match,
+                      "reply_sha256": hashlib.sha256(final.tobytes()).hexdigest() if k == d1 else None})
+        total_t += t * nq
+        total_raw += d["raw"] * nq
+        cores = max(cores, P1)
+        log(f"cpu baseline db{i}: {t:.3f} s per reply (level 0 from {k}/{d1} groups)")
+    return {"value": total_raw / total_t / 1e9, "unit": "GB/s", "cores": cores, "kind": "reference",
+            "seconds_per_batch": total_t, "per_db": parts,
+            "sample": "one query per database: level 0 over the first k groups of its grid (all 100 for the "
+                      "10000 x 256 B database, whose whole reply is compared with the GPU's), scaled to every "
+                      "group; the recursion import and the last level whole; weighted by the batch's query "
+                      "counts; threads = polys per element (the reference's OpenMP split)"}
+
+
+def bench_metric(args):
+    """The metric: batched vs plain over the eight uneven databases, p50/p99 latency of a
+    serving stream over them, bit-exactness against the reference's replies."""
+    import xpir_amd
+    from xpir_amd.workloads import C2_SINGLE, batch_query_counts
+    gold = load_gold()
+    ctx = xpir_amd.Context(0)
+    t_setup = time.perf_counter()
+    dbs = _load_batch_dbs(ctx, args.scale)
+    counts = batch_query_counts()
+    reqs = make_batch_queries(ctx, dbs, counts, np.random.default_rng(42))
+    log(f"{len(reqs)} queries over {len(dbs)} databases, {sum(r[1].nbytes for r in reqs) / 2**30:.1f} GiB of queries "
+        f"resident; setup {time.perf_counter() - t_setup:.1f} s")
+    tuned, tune_s = tune_batch_dbs(ctx, dbs, counts, args)
+    order = np.random.default_rng(43).permutation(len(reqs))
+    batch = [reqs[k] for k in order]
+    total_raw = sum(d["raw"] * nq for d, nq in zip(dbs, counts))
+
+    # plain path: every query alone through xpg_reply_dev, one after another; per-query
+    # latency (each synchronised) from a second pass
+    plain = {}
+    ctx.sync()
+    t0 = time.perf_counter()
+    for db, qd, rd in batch:
+        ctx.reply_dev(db, qd, rd)
+    ctx.sync()
+    plain_s = time.perf_counter() - t0
+    for db, qd, rd in reqs:
+        plain[rd.ptr] = rd.download()
+    plain_lat = []
+    for db, qd, rd in batch:
+        t0 = time.perf_counter()
+        ctx.reply_dev(db, qd, rd)
+        ctx.sync()
+        plain_lat.append((time.perf_counter() - t0) * 1e3)
+    log(f"plain path: {plain_s:.3f} s for {len(reqs)} queries -> {total_raw / plain_s / 1e9:.1f} GB/s")
+
+    for _ in range(args.warmup):
+        ctx.reply_batch_dev(batch)
+    ctx.sync()
+    ctx.timing(2)
+    ctx.timing_read_batched(reset=True)
+    ctx.timing_read_main(reset=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.reply_batch_dev(batch)
+    ctx.sync()
+    total = time.perf_counter() - t0
+    k_ms, k_launches, k_steps = ctx.timing_read_batched(reset=True)
+    ctx.timing_read_main(reset=True)
+    ctx.timing(False)
+    pass_s = total / args.steps
+    value = total_raw / pass_s / 1e9
+    log(f"{args.steps} batched passes: {pass_s * 1e3:.1f} ms per pass -> {value:.1f} GB/s "
+        f"({plain_s / pass_s:.2f}x the plain path)")
+
+    # bit-exactness: every batched reply equals the plain reply; each database's golden
+    # query equals the reference's reply bytes
+    exact_vs_plain = all(np.array_equal(rd.download(), plain[rd.ptr]) for _, _, rd in reqs)
+    golden_ok, golden_replies = [], []
+    for i, d in enumerate(dbs):
+        rep = d["reqs"][0][2].download()
+        golden_replies.append(rep)
+        g = gold.get("c4_batched", [None] * 8)[i]
+        golden_ok.append(g is not None and hashlib.sha256(rep.tobytes()).hexdigest() == g["reply_sha256"])
+        d["golden_query"] = d["reqs"][0][1].download()
+    log(f"bit-exact: batched == plain {exact_vs_plain}, golden queries {sum(golden_ok)}/{len(golden_ok)}")
+
+    prof = batch_kernel_profile()
+    steps_s = k_steps / (k_ms / 1e3)
+    roofline = {"bound": "valu", "unit": "TOP/s", "peak": VALU_PEAK_TOPS, "kernel": "mac_batch2_kernel",
+                "kernel_ms_per_pass": k_ms / args.steps, "launches_per_pass": k_launches / args.steps,
+                "exact_steps_per_pass": k_steps / args.steps, "exact_steps_per_s": steps_s,
+                "peak_derivation": "256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 32-bit lane ops/s "
+                                   "(= 157.3 TFLOP/s fp32 / 2 for FMA / 2 for dual issue)"}
+    if prof:
+        vps = prof["valu_lane_ops_per_step"]
+        roofline.update(achieved=steps_s * vps / 1e12, frac=steps_s * vps / 1e12 / VALU_PEAK_TOPS,
+                        valu_lane_ops_per_step=vps, traffic=prof.get("hbm_bytes_per_pass"),
+                        hbm_gbs=(prof["hbm_bytes_per_pass"] / (k_ms / args.steps / 1e3) / 1e9
+                                 if prof.get("hbm_bytes_per_pass") else None),
+                        counters_from="profiles/mac_batch2_kernel_batched_uneven_8dbs.json (rocprofv3 "
+                                      "SQ_INSTS_VALU, FETCH_SIZE, WRITE_SIZE; command recorded there)")
+    else:
+        roofline.update(achieved=None, frac=None, traffic=None)
+
+    latency = serving_latency(ctx, dbs, args)
+    log(f"serving stream: p50 {latency['p50']:.2f} ms p99 {latency['p99']:.2f} ms, "
+        f"{latency['served_gbs']:.1f} GB/s, exact {latency['bit_exact_vs_alone']}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: the reference's reply arithmetic on this host, bounded sample")
+        cpu = cpu_baseline_batched(dbs, counts, golden_replies)
+    for _, qd, rd in reqs:
+        qd.free()
+        rd.free()
+    for d in dbs:
+        d["skd"].free()
+        d["db"].free()
+
+    single = None
+    if not args.no_single:
+        log("single-query configuration (65536 x 4 KiB) for the HBM-bound kernel")
+        sargs = argparse.Namespace(**vars(args))
+        sargs.steps, sargs.warmup = max(5, min(args.steps, 20)), 5
+        single = run_single(ctx, sargs, C2_SINGLE, gold.get("c2_single"))
+        single = {k: single[k] for k in ("value", "ms_per_step", "bit_exact", "latency_ms", "import_gbs",
+                                         "roofline", "cpu_baseline", "cpu_all_cores", "config") if k in single}
+    ctx.close()
+    plain_lat = np.array(plain_lat)
+    out = {
+        "metric": "DB GB/s per reply, batched vs plain, 1 MI355X, bit-exact; p50/p99 reply ms",
+        "value": value, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": pass_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic (seeded SplitMix64 bytes)",
+        "config": {"workload": "batched uneven: 8 resident LWE:80:2048:120 databases, 10k..1M elements x "
+                               "256 B..16 KiB, d=2 grids, 1-64 seeded queries each, one batched pass",
+                   "crypto": "LWE:80:2048:120", "queries": len(reqs), "counts": counts,
+                   "dbs": [[d["spec"]["n_elems"], d["spec"]["elem_bytes"], d["spec"]["dims"]] for d in dbs],
+                   "raw_db_bytes_per_pass": total_raw},
+        "plain_value": total_raw / plain_s / 1e9, "plain_ms_per_pass": plain_s * 1e3,
+        "batched_over_plain": plain_s / pass_s,
+        "bit_exact": bool(exact_vs_plain and all(golden_ok)),
+        "bit_exact_detail": {"batched_vs_plain": bool(exact_vs_plain), "golden_queries": golden_ok},
+        "latency_ms": {"p50": latency["p50"], "p99": latency["p99"], "source": "serving stream"},
+        "serving": latency,
+        "plain_latency_ms": {"p50": float(np.percentile(plain_lat, 50)), "p99": float(np.percentile(plain_lat, 99))},
+        "autotune_s": tune_s,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "single_query": single,
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -210,6 +210,10 @@ int xpg_timing_read(xpg_ctx *ctx, double *mac_ms, uint64_t *mac_launches, int re
 /* the same for the dominant kernel of each MAC alone (the streaming block-summary,
  * one-chain-per-lane or batched kernel, without its scan) */
 int xpg_timing_read_main(xpg_ctx *ctx, double *ms, uint64_t *launches, int reset);
+/* the same for the register-blocked kernel's launches alone (level 0 of d >= 2 replies
+ * and of batches), with the exact chain steps they ran (queries x groups x P x m x n x 2
+ * chains x n_l elements), for its integer-issue roofline */
+int xpg_timing_read_batched(xpg_ctx *ctx, double *ms, uint64_t *launches, uint64_t *steps, int reset);
 
 /* ---- MAC autotuning (no reference counterpart: the reference runs one CPU loop nest
  *      for every shape, pir/replyGenerator/PIRReplyGeneratorNFL_internal.cpp:511-595) ----

@@ -316,6 +316,9 @@ class RefNFL:
             L.xr_aggregate.restype = C.c_int64
             L.xr_aggregate.argtypes = [u8p, u64p, u64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, u8p,
                                        C.c_uint64]
+            L.xr_import_range.restype = C.c_int
+            L.xr_import_range.argtypes = [C.c_void_p, C.c_uint, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                          C.c_uint64, C.c_uint, u64p, C.c_int]
             L.xr_level0_stream.restype = C.c_int
             L.xr_level0_stream.argtypes = [C.c_void_p, C.c_uint, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                            C.c_uint64, C.c_uint64, C.c_uint, u64p, u64p, C.c_int]
@@ -365,6 +368,15 @@ class RefNFL:
                                      maxb if bytes_per_file is None else bytes_per_file, out, out.size)
         assert got == runs, (got, runs)
         return out[:runs * alpha * fb].reshape(runs, alpha * fb)
+
+    def import_range(self, A: int, db_seed: int, n_elems: int, elem_bytes: int, first: int, count: int,
+                     P0: int, threads: int = 8) -> np.ndarray:
+        """NTT words of elements [first, first + count) of the seeded database (zero polys
+        past n_elems), the reference's deserializeDataNFL per element."""
+        out = np.zeros(count * P0 * self.mn, dtype=np.uint64)
+        rc = self.lib().xr_import_range(self.h, A, db_seed, n_elems, elem_bytes, first, count, P0, out, threads)
+        assert rc == 0, "an element imported to another poly count"
+        return out
 
     def level0_stream(self, A: int, db_seed: int, n_elems: int, elem_bytes: int, n0: int, g0: int, g1: int,
                       P0: int, q0: np.ndarray, threads: int = 8) -> np.ndarray:

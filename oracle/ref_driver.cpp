@@ -166,6 +166,63 @@ static inline uint64_t splitmix_word(uint64_t seed, uint64_t k) {
   return z ^ (z >> 31);
 }
 
+// Elements [first, first + count) of the seeded database (zero polys at or past
+// n_elems), each imported with deserializeDataNFL, into dst (count x P0 x m x n).
+// `buf` holds elem_bytes + 64 bytes.  Returns false when an element imports to other
+// than P0 polys.
+static bool import_range(RefHandle *h, unsigned A, uint64_t db_seed, uint64_t n_elems, uint64_t elem_bytes,
+                         uint64_t first, uint64_t count, unsigned P0, unsigned char *buf, uint64_t *dst) {
+  const uint64_t mn = (uint64_t)h->n * h->m;
+  bool ok = true;
+  for (uint64_t e = 0; e < count; e++, dst += P0 * mn) {
+    const uint64_t idx = first + e;
+    if (idx >= n_elems) {
+      memset(dst, 0, P0 * mn * sizeof(uint64_t));
+      continue;
+    }
+    const uint64_t b0 = idx * elem_bytes;
+    for (uint64_t j = 0; j < elem_bytes; j++) {
+      const uint64_t b = b0 + j;
+      buf[j] = (unsigned char)(splitmix_word(db_seed, b >> 3) >> (8 * (b & 7)));
+    }
+    unsigned char *bufs[1] = {buf};
+    uint64_t polys = 0;
+    poly64 *res = h->nfl.deserializeDataNFL(bufs, 1, elem_bytes * 8, A, polys);
+    if (polys != P0)
+      ok = false;
+    else
+      memcpy(dst, res[0], P0 * mn * sizeof(uint64_t));
+    free(res[0]);
+    free(res);
+  }
+  return ok;
+}
+
+// The NTT words of elements [first, first + count) of the seeded database, `threads`
+// OpenMP threads over the elements (untimed preparation of a CPU sample).
+int xr_import_range(void *hp, unsigned A, uint64_t db_seed, uint64_t n_elems, uint64_t elem_bytes, uint64_t first,
+                    uint64_t count, unsigned P0, uint64_t *out, int threads) {
+  RefHandle *h = static_cast<RefHandle *>(hp);
+  const uint64_t mn = (uint64_t)h->n * h->m;
+  const int saved = omp_get_max_active_levels();
+  omp_set_max_active_levels(1);  // deserializeDataNFL's own poly loop stays serial here
+  int bad = 0;
+  if (threads <= 0) threads = 1;
+#pragma omp parallel num_threads(threads)
+  {
+    unsigned char *buf = static_cast<unsigned char *>(calloc(elem_bytes + 64, 1));
+#pragma omp for schedule(dynamic, 16)
+    for (uint64_t e = 0; e < count; e++)
+      if (!import_range(h, A, db_seed, n_elems, elem_bytes, first + e, 1, P0, buf, out + e * P0 * mn)) {
+#pragma omp atomic write
+        bad = 1;
+      }
+    free(buf);
+  }
+  omp_set_max_active_levels(saved);
+  return bad ? -1 : 0;
+}
+
 int xr_level0_stream(void *hp, unsigned A, uint64_t db_seed, uint64_t n_elems, uint64_t elem_bytes, uint64_t n0,
                      uint64_t g0, uint64_t g1, unsigned P0, const uint64_t *q0, uint64_t *out, int threads) {
   RefHandle *h = static_cast<RefHandle *>(hp);
@@ -173,6 +230,8 @@ int xr_level0_stream(void *hp, unsigned A, uint64_t db_seed, uint64_t n_elems, u
   const uint64_t *moduli = h->nfl.getmoduli();
   std::vector<uint64_t> qq(n0 * 2 * mn);
   for (uint64_t i = 0; i < n0 * 2 * mn; i++) qq[i] = ((uint128_t)q0[i] << 64) / moduli[(i % mn) / n];
+  const int saved = omp_get_max_active_levels();
+  omp_set_max_active_levels(1);
   int bad = 0;
   if (threads <= 0) threads = 1;
 #pragma omp parallel num_threads(threads)
@@ -181,29 +240,9 @@ int xr_level0_stream(void *hp, unsigned A, uint64_t db_seed, uint64_t n_elems, u
     unsigned char *buf = static_cast<unsigned char *>(calloc(elem_bytes + 64, 1));
 #pragma omp for schedule(dynamic, 1)
     for (uint64_t g = g0; g < g1; g++) {
-      for (uint64_t e = 0; e < n0; e++) {
-        uint64_t *dst = words.data() + e * P0 * mn;
-        const uint64_t idx = g * n0 + e;
-        if (idx >= n_elems) {
-          memset(dst, 0, P0 * mn * sizeof(uint64_t));
-          continue;
-        }
-        const uint64_t b0 = idx * elem_bytes;
-        for (uint64_t j = 0; j < elem_bytes; j++) {
-          const uint64_t b = b0 + j;
-          buf[j] = (unsigned char)(splitmix_word(db_seed, b >> 3) >> (8 * (b & 7)));
-        }
-        unsigned char *bufs[1] = {buf};
-        uint64_t polys = 0;
-        poly64 *res = h->nfl.deserializeDataNFL(bufs, 1, elem_bytes * 8, A, polys);
-        if (polys != P0) {
+      if (!import_range(h, A, db_seed, n_elems, elem_bytes, g * n0, n0, P0, buf, words.data())) {
 #pragma omp atomic write
-          bad = 1;
-        } else {
-          memcpy(dst, res[0], P0 * mn * sizeof(uint64_t));
-        }
-        free(res[0]);
-        free(res);
+        bad = 1;
       }
       uint64_t *og = out + (g - g0) * P0 * 2 * mn;
       memset(og, 0, P0 * 2 * mn * sizeof(uint64_t));
@@ -224,6 +263,7 @@ int xr_level0_stream(void *hp, unsigned A, uint64_t db_seed, uint64_t n_elems, u
     }
     free(buf);
   }
+  omp_set_max_active_levels(saved);
   return bad ? -1 : 0;
 }
 

@@ -23,7 +23,7 @@ EXPORTS = [
     "xpg_db_import", "xpg_db_import_dev",
     "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
     "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev", "xpg_reply_dev_noalloc",
-    "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_timing_read_main", "xpg_graph_stats", "xpg_autotune", "xpg_tune_set",
+    "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_timing_read_main", "xpg_timing_read_batched", "xpg_graph_stats", "xpg_autotune", "xpg_tune_set",
     "xpg_tune_get", "xpg_tune_clear", "xpg_keygen",
     "xpg_query_gen_dev", "xpg_random_bytes_dev", "xpg_decrypt_dev", "xpg_server_create", "xpg_server_destroy",
     "xpg_server_submit", "xpg_server_poll", "xpg_server_release", "xpg_server_now_us", "xpg_server_stats",
@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
         "xpg_timing_enable": (C.c_int, [vp, C.c_int]),
         "xpg_timing_read": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
         "xpg_timing_read_main": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.c_int]),
+        "xpg_timing_read_batched": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(u64), C.POINTER(u64), C.c_int]),
         "xpg_graph_stats": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
         "xpg_autotune": (C.c_int, [vp, vp, u32, vp, u32, C.POINTER(u32)]),
         "xpg_tune_set": (C.c_int, [vp, vp, u32]),
@@ -319,10 +320,17 @@ class Context:
         return ms.value, n.value
 
     def timing_read_main(self, reset: bool = True):
-        """(ms, launches) of the dominant MAC kernel alone, without its scan or fixup."""
+        """(ms, launches) of the dominant MAC kernel alone, without its scan."""
         ms, n = C.c_double(), C.c_uint64()
         self._check(lib().xpg_timing_read_main(self.h, C.byref(ms), C.byref(n), 1 if reset else 0))
         return ms.value, n.value
+
+    def timing_read_batched(self, reset: bool = True):
+        """(ms, launches, exact chain steps) of the register-blocked MAC kernel's launches."""
+        ms, n, st = C.c_double(), C.c_uint64(), C.c_uint64()
+        self._check(lib().xpg_timing_read_batched(self.h, C.byref(ms), C.byref(n), C.byref(st),
+                                                  1 if reset else 0))
+        return ms.value, n.value, st.value
 
     def autotune(self, db: "Database", max_batch: int = 0):
         """Time every MAC kernel candidate on db's level shapes (and batches of 2..max_batch

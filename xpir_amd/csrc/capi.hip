@@ -261,7 +261,7 @@ void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial) {
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b, int tag) {
   // mode 2 times the main kernels alone: no events around whole levels, whose records
   // would put gaps between a level's launches; the no-allocation reply creates no events
-  if (!ctx->timing || ctx->ws || (ctx->timing == 2 && tag != 1)) return;
+  if (!ctx->timing || ctx->ws || (ctx->timing == 2 && tag == 0)) return;
   for (hipEvent_t *e : {a, b}) {
     if (!ctx->ev_pool.empty()) {
       *e = ctx->ev_pool.back();
@@ -286,9 +286,13 @@ static void timing_collect(xpg_ctx *ctx) {
     float ms = 0.f;
     hipEvent_t a = std::get<0>(pr), b = std::get<1>(pr);
     if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess) {
-      if (std::get<2>(pr) == 1) {
+      if (std::get<2>(pr) != 0) {
         ctx->main_ms += ms;
         ctx->main_launches++;
+        if (std::get<2>(pr) == 2) {
+          ctx->batch_ms += ms;
+          ctx->batch_launches++;
+        }
       } else {
         ctx->mac_ms += ms;
         ctx->mac_launches++;
@@ -999,6 +1003,20 @@ int xpg_graph_stats(xpg_ctx *ctx, uint64_t *captures, uint64_t *replays) {
   if (!ctx) return XPG_EINVAL;
   if (captures) *captures = ctx->graph_captures;
   if (replays) *replays = ctx->graph_launches;
+  return XPG_OK;
+}
+
+int xpg_timing_read_batched(xpg_ctx *ctx, double *ms, uint64_t *launches, uint64_t *steps, int reset) {
+  if (!ctx) return XPG_EINVAL;
+  timing_collect(ctx);
+  if (ms) *ms = ctx->batch_ms;
+  if (launches) *launches = ctx->batch_launches;
+  if (steps) *steps = ctx->batch_steps;
+  if (reset) {
+    ctx->batch_ms = 0;
+    ctx->batch_launches = 0;
+    ctx->batch_steps = 0;
+  }
   return XPG_OK;
 }
 

@@ -109,11 +109,14 @@ struct xpg_ctx {
   // MAC timing
   int timing = 0;  // 0 off, 1 every MAC level and main kernel, 2 main kernels alone
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::tuple<hipEvent_t, hipEvent_t, int>> pending;  // (start, end, 0 = whole MAC | 1 = main kernel)
+  // (start, end, 0 = whole MAC | 1 = main kernel | 2 = main kernel, register-blocked)
+  std::vector<std::tuple<hipEvent_t, hipEvent_t, int>> pending;
   double mac_ms = 0.0;
   uint64_t mac_launches = 0;
-  double main_ms = 0.0;        // the dominant kernel of each MAC (without its scan or fixup)
+  double main_ms = 0.0;        // the dominant kernel of each MAC (without its scan)
   uint64_t main_launches = 0;
+  double batch_ms = 0.0;       // the register-blocked kernel's launches alone (tag 2)
+  uint64_t batch_launches = 0, batch_steps = 0;  // ... and the exact chain steps they ran
   // captured replies (option "graphs")
   bool graphs = false;
   uint64_t epoch = 1;          // bumped whenever a context buffer moves or an option changes

@@ -755,7 +755,8 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
   hipEvent_t k0 = nullptr, k1 = nullptr;
-  timing_begin(ctx, &k0, &k1, 1);
+  timing_begin(ctx, &k0, &k1, 2);
+  if (ctx->timing && !ctx->ws) ctx->batch_steps += (uint64_t)qn * groups * P * prm->m * prm->n * 2 * n_l;
   switch (key) {
     case 82: hipLaunchKernelGGL((mac_batch2_kernel<8, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
     case 81: hipLaunchKernelGGL((mac_batch2_kernel<8, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
@@ -767,7 +768,7 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
     default: hipLaunchKernelGGL((mac_batch2_kernel<1, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
   }
   const int rc = hipGetLastError() != hipSuccess ? set_err(ctx, XPG_EDEVICE, "batched MAC launch") : XPG_OK;
-  timing_end(ctx, k0, k1, 1);
+  timing_end(ctx, k0, k1, 2);
   timing_end(ctx, e0, e1);
   return rc;
 }
