@@ -15,7 +15,7 @@ another) on the same queries, and p50/p99 reply latency from a seeded Poisson st
 through the continuous-batching server over the same databases (plus d = 1 views of the
 two smallest).  Bit-exactness: each database's golden query against the reference's
 reply bytes (tests/golden/golden.json), every batched reply against the plain reply.
-roofline: the batched level-0 kernel (mac_batch2_kernel), integer-issue bound: exact
+roofline: the batched level-0 kernel (mac_batch3_kernel), integer-issue bound: exact
 chain steps/s (HIP events around its launches in the timed passes) x VALU lane
 instructions per step (rocprofv3 SQ_INSTS_VALU, profiles/) against 39.3 T lane ops/s.
 cpu_baseline: the reference's own arithmetic (oracle/_ref) on this host on a bounded
@@ -964,7 +964,7 @@ def batch_kernel_profile():
     """VALU lane instructions per exact chain step of the batched level-0 kernel, and its
     HBM bytes per pass, from the rocprofv3 counter runs committed under profiles/ (the
     command that made them is recorded there); None when absent."""
-    path = os.path.join(ROOT, "profiles", "mac_batch2_kernel_batched_uneven_8dbs.json")
+    path = os.path.join(ROOT, "profiles", "mac_batch3_kernel_batched_uneven_8dbs.json")
     try:
         return json.load(open(path))
     except (OSError, ValueError):
@@ -1235,7 +1235,8 @@ def bench_metric(args):
 
     prof = batch_kernel_profile()
     steps_s = k_steps / (k_ms / 1e3)
-    roofline = {"bound": "valu", "unit": "TOP/s", "peak": VALU_PEAK_TOPS, "kernel": "mac_batch2_kernel",
+    roofline = {"bound": "valu", "unit": "TOP/s", "peak": VALU_PEAK_TOPS,
+                "kernel": "mac_batch3_kernel (register-blocked level-0 MAC, every launch of the pass)",
                 "kernel_ms_per_pass": k_ms / args.steps, "launches_per_pass": k_launches / args.steps,
                 "exact_steps_per_pass": k_steps / args.steps, "exact_steps_per_s": steps_s,
                 "peak_derivation": "256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 32-bit lane ops/s "
@@ -1246,7 +1247,7 @@ def bench_metric(args):
                         valu_lane_ops_per_step=vps, traffic=prof.get("hbm_bytes_per_pass"),
                         hbm_gbs=(prof["hbm_bytes_per_pass"] / (k_ms / args.steps / 1e3) / 1e9
                                  if prof.get("hbm_bytes_per_pass") else None),
-                        counters_from="profiles/mac_batch2_kernel_batched_uneven_8dbs.json (rocprofv3 "
+                        counters_from="profiles/mac_batch3_kernel_batched_uneven_8dbs.json (rocprofv3 "
                                       "SQ_INSTS_VALU, FETCH_SIZE, WRITE_SIZE; command recorded there)")
     else:
         roofline.update(achieved=None, frac=None, traffic=None)

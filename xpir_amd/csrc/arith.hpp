@@ -84,6 +84,68 @@ __device__ __forceinline__ uint64_t mac_step(uint64_t acc, uint64_t x, uint64_t 
   return acc >= p ? acc - p : acc;
 }
 
+// Per-modulus constants of the asm chain step below (wave-uniform except plo/phi, which
+// the select needs in VGPRs: a VOP3 instruction reads one SGPR at most on this GPU).
+struct StepConsts {
+  uint64_t negp;   // 2^64 - p
+  uint32_t delta;  // 2^61 - p
+  uint32_t k29;    // 0xE0000000 = high word of 2^64 - p, so lo(q)*k29 = -(lo(q) << 29) mod 2^32
+  uint32_t plo, phi;
+};
+
+__device__ __forceinline__ StepConsts step_consts(uint64_t p, uint32_t delta) {
+  StepConsts k;
+  k.negp = 0 - p;
+  k.delta = delta;
+  k.k29 = 0xE0000000u;
+  k.plo = (uint32_t)p;
+  k.phi = (uint32_t)(p >> 32);
+  asm volatile("" : "+v"(k.plo), "+v"(k.phi));  // the select's operands live in VGPRs
+  return k;
+}
+
+// One reply-chain step (crypto/NFLlib.hpp:303-321) as straight-line VALU code:
+//   acc = acc + (x*y - q*p),  q = hi64(x*yq);   if (acc >= p) acc -= p   (all mod 2^64)
+// for every x, y, yq, acc < 2^64 and p = 2^61 - delta.  19 VALU instructions:
+//   q:   t = hi32(xl*ql) (from C, as {t, 0}), A = xh*ql + t (below 2^64), B = xl*qh + A with
+//        its carry c, q = xh*qh + {hi32(B), c} -- the middle product's carry comes from
+//        v_mad_u64_u32's carry-out instead of being recombined from 32-bit halves;
+//   r-p: x*y + q*(2^64 - p) - p (mod 2^64), with 2^64 - p = 0xE0000000 * 2^32 + delta:
+//        lo = xl*yl - p + lo(q)*delta, hi += xl*yh + xh*yl + hi(q)*delta + lo(q)*0xE0000000;
+//   chain: u = acc + r - p; the reference's acc + r >= p exactly when u < 2^64 - p, then
+//        u is the new acc, else u + p.
+// Scratch v[2:9] and vcc; an s_nop separates a VALU write of vcc from its read as a lane
+// mask (a hazard on this GPU the compiler cannot see inside inline asm).
+__device__ __forceinline__ void mac_step_asm(uint64_t &acc, uint64_t x, uint64_t y, uint64_t yq,
+                                             const StepConsts &k) {
+  const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
+  const uint32_t ql = (uint32_t)yq, qh = (uint32_t)(yq >> 32);
+  const uint64_t T = __umulhi(xl, ql);
+  asm("v_mad_u64_u32 v[4:5], vcc, %[xh], %[ql], %[T]\n\t"
+      "v_mad_u64_u32 v[6:7], vcc, %[xl], %[qh], v[4:5]\n\t"
+      "v_mul_lo_u32 v2, %[xl], %[yh]\n\t"
+      "v_mul_lo_u32 v3, %[xh], %[yl]\n\t"
+      "v_mov_b32 v8, v7\n\t"
+      "v_cndmask_b32_e64 v9, 0, 1, vcc\n\t"
+      "v_mad_u64_u32 v[4:5], vcc, %[xh], %[qh], v[8:9]\n\t"
+      "v_mad_u64_u32 v[6:7], vcc, %[xl], %[yl], %[negp]\n\t"
+      "v_mad_u64_u32 v[6:7], vcc, v4, %[d], v[6:7]\n\t"
+      "v_mul_lo_u32 v8, v5, %[d]\n\t"
+      "v_mul_lo_u32 v9, v4, %[k29]\n\t"
+      "v_add3_u32 v7, v7, v2, v3\n\t"
+      "v_add3_u32 v7, v7, v8, v9\n\t"
+      "v_lshl_add_u64 %[acc], %[acc], 0, v[6:7]\n\t"
+      "v_cmp_gt_u64_e64 vcc, %[negp], %[acc]\n\t"
+      "s_nop 0\n\t"
+      "v_cndmask_b32_e64 v2, %[plo], 0, vcc\n\t"
+      "v_cndmask_b32_e64 v3, %[phi], 0, vcc\n\t"
+      "v_lshl_add_u64 %[acc], v[2:3], 0, %[acc]"
+      : [acc] "+v"(acc)
+      : [xl] "v"(xl), [xh] "v"(xh), [yl] "v"(yl), [yh] "v"(yh), [ql] "v"(ql), [qh] "v"(qh), [T] "v"(T),
+        [d] "s"(k.delta), [k29] "s"(k.k29), [negp] "s"(k.negp), [plo] "v"(k.plo), [phi] "v"(k.phi)
+      : "v2", "v3", "v4", "v5", "v6", "v7", "v8", "v9", "vcc");
+}
+
 // counter-based SplitMix64: word k of stream `seed` (the seeded generator of this
 // project; stands in for the reference's /dev/urandom-keyed Salsa20 stream).
 __host__ __device__ __forceinline__ uint64_t rand_word(uint64_t seed, uint64_t k) {

@@ -145,7 +145,8 @@ void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b, int tag = 0);
 // Kernel configuration of one MAC launch.  kernel: 1 = one chain per lane, 3 = streaming
 // block summaries (a = elements per load chunk, b = double-buffered, c =
 // elements per block, 0 = sized for one pass), 4 = register-blocked (a = groups per lane,
-// b = queries per lane).
+// b = queries per lane, c = 2: query words loaded per wave, 3: shared through LDS by the
+// waves of a workgroup, 0: by shape).
 struct LevelCfg {
   int kernel = 0;
   int a = 0, b = 0, c = 0;
@@ -189,7 +190,7 @@ int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
                          uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out, const LevelCfg &cfg);
 int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                          uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P, int gr,
-                         int qr);
+                         int qr, int family);
 // One level of up to 64 requests at once: their level databases side by side (request i's
 // groups are groups [i*gpr, (i+1)*gpr) of db), request i reading query q[i]; outputs to
 // out_per_req[i] (each request's own buffer) when given, else contiguous in out.
@@ -204,7 +205,8 @@ uint64_t level_scratch_bytes(const xpg_ctx *ctx, const xpg_params *prm, uint64_t
                              const LevelCfg &cfg);
 // what launch_mac_level runs for this shape with the context's options and no tuning
 LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P);
-bool batch_shape_built(int GR, int QR);  // register-block shapes compiled in
+bool batch_shape_built(int GR, int QR);   // register-block shapes compiled in (per-wave query loads)
+bool batch3_shape_built(int GR, int QR);  // ... and with the query words shared through LDS
 // explicit options (a kernel or a tile shape asked for) win over the tuning table
 bool options_pinned(const xpg_ctx *ctx);
 // register blocks the batched MAC picks by shape for the ones given as 0

@@ -155,6 +155,7 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
 #pragma unroll
   for (int i = 0; i < QR; i++) qa[i] = a.q[min(q0 + i, a.qn - 1)] + cs;
   const uint32_t delta = a.ms.delta[c];
+  const StepConsts kc = step_consts(p, delta);
   int gv = (int)min<uint64_t>(GR, a.groups - g0);  // groups of this block that exist
   uint64_t acc_a[GR][QR], acc_b[GR][QR];
 #pragma unroll
@@ -190,8 +191,8 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
           const uint64_t yaq = shoup_quot(ya[u][i], p, delta), ybq = shoup_quot(yb[u][i], p, delta);
 #pragma unroll
           for (int g = 0; g < GR; g++) {
-            acc_a[g][i] = mac_step(acc_a[g][i], xv[u][g], ya[u][i], yaq, p, delta);
-            acc_b[g][i] = mac_step(acc_b[g][i], xv[u][g], yb[u][i], ybq, p, delta);
+            mac_step_asm(acc_a[g][i], xv[u][g], ya[u][i], yaq, kc);
+            mac_step_asm(acc_b[g][i], xv[u][g], yb[u][i], ybq, kc);
           }
         }
       }
@@ -221,8 +222,8 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
           const uint64_t yaq = shoup_quot(ya[u][i], p, delta), ybq = shoup_quot(yb[u][i], p, delta);
 #pragma unroll
           for (int g = 0; g < GR; g++) {
-            acc_a[g][i] = mac_step(acc_a[g][i], xv[u][g], ya[u][i], yaq, p, delta);
-            acc_b[g][i] = mac_step(acc_b[g][i], xv[u][g], yb[u][i], ybq, p, delta);
+            mac_step_asm(acc_a[g][i], xv[u][g], ya[u][i], yaq, kc);
+            mac_step_asm(acc_b[g][i], xv[u][g], yb[u][i], ybq, kc);
           }
         }
       }
@@ -236,6 +237,160 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
     for (int i = 0; i < QR; i++) {
       if (q0 + i < a.qn) {
         uint64_t *o = a.out[q0 + i] + gk * 2 * mn + cs;
+        o[0] = acc_a[g][i];
+        o[mn] = acc_b[g][i];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Register blocks with the query words shared through LDS.  A workgroup of kB3Waves
+// waves takes one 64-slot column (modulus c, slots sb*64..+63), one block of QR queries
+// and kB3Waves*GR rows, a row being one (group g, poly k) of the level (row r = g*P + k);
+// lane = slot.  Every row of a column reads the same query words, so the workgroup loads
+// each query word once per chunk of kB3U elements, derives its Shoup quotient once, and
+// stages (y, y') in LDS for its waves; each wave then runs the a and b chains of its GR
+// rows x QR queries for the chunk, element by element, with the loads of the next chunk
+// (query and DB words) in flight.  Two LDS buffers alternate between chunks; one barrier
+// per chunk.  Same words as every other MAC kernel: each chain still takes its elements in
+// order through the same step.
+constexpr int kB3Waves = 4;  // waves per workgroup
+constexpr int kB3U = 2;      // elements per staged chunk
+
+struct QPair {
+  uint64_t y, yq;
+};
+
+template <int GR, int QR>
+__global__ void __launch_bounds__(64 * kB3Waves) mac_batch3_kernel(Batch2Args a) {
+  constexpr int NQ = kB3U * QR * 2;  // staged query rows (element, query, a|b) per chunk
+  constexpr int NS = NQ / kB3Waves;  // rows staged per wave
+  static_assert(NQ % kB3Waves == 0, "staged rows split evenly over the waves");
+  __shared__ QPair lds[2][NQ][64];
+  const uint64_t b = blockIdx.x;
+  const uint64_t xcd = b & 7, j = b >> 3;
+  const uint64_t cg = (j / a.qblocks) * 8 + xcd;
+  const uint32_t qb = (uint32_t)(j % a.qblocks);
+  if (cg >= a.colgroups) return;  // the whole workgroup
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t sb = (uint32_t)(cg % a.sblocks);
+  uint64_t rr = cg / a.sblocks;
+  const uint32_t c = (uint32_t)(rr % a.m);
+  const uint64_t rb = rr / a.m;
+  const uint64_t rows = a.groups * a.P;
+  const uint64_t mn = (uint64_t)a.m * a.n;
+  const uint32_t cs = c * a.n + sb * 64 + lane;  // this lane's word within a poly
+  const uint64_t p = a.ms.p[c];
+  const uint32_t delta = a.ms.delta[c];
+  const StepConsts kc = step_consts(p, delta);
+  const uint64_t xstride = (uint64_t)a.P * mn;  // next element of a row
+  const uint64_t qstride = 2 * mn;              // next element of a query
+  const uint64_t r0 = (rb * kB3Waves + w) * GR;
+  const int rv = (int)max<int64_t>(0, min<int64_t>(GR, (int64_t)rows - (int64_t)r0));  // this wave's rows
+  // wave-uniform row bases (rows past the level read the last row: their chains are
+  // computed and dropped, so no load needs a guard); the lane adds its 32-bit word offset
+  const uint64_t *xr[GR];
+#pragma unroll
+  for (int g = 0; g < GR; g++) {
+    const uint64_t r = min<uint64_t>(r0 + g, rows - 1);
+    xr[g] = a.db + ((r / a.P) * a.n_l * a.P + (r % a.P)) * mn;
+  }
+  // staging: this wave loads the query rows qrow = w + kB3Waves * s of every chunk
+  const uint32_t q0 = qb * QR;
+  const uint64_t *sq[NS];
+  uint32_t su[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const uint32_t qrow = w + kB3Waves * s;
+    const uint32_t u = qrow / (QR * 2), rem = qrow % (QR * 2), i = rem >> 1, ab = rem & 1;
+    su[s] = u;
+    sq[s] = a.q[min(q0 + i, a.qn - 1)] + ab * mn;
+  }
+  uint64_t acc_a[GR][QR], acc_b[GR][QR];
+#pragma unroll
+  for (int g = 0; g < GR; g++)
+#pragma unroll
+    for (int i = 0; i < QR; i++) acc_a[g][i] = acc_b[g][i] = 0;
+  const uint64_t nfull = a.n_l / kB3U, last = a.n_l - 1;
+
+  // loads of chunk ch, element indices clamped to the level (a chunk past the last whole
+  // one reads words it never uses)
+  auto load_q = [&](uint64_t ch, uint64_t (&qv)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+      qv[s] = __builtin_nontemporal_load(sq[s] + min<uint64_t>(ch * kB3U + su[s], last) * qstride + cs);
+  };
+  auto load_x = [&](uint64_t ch, uint64_t (&xv)[kB3U][GR]) {
+#pragma unroll
+    for (int u = 0; u < kB3U; u++) {
+      const uint64_t eo = min<uint64_t>(ch * kB3U + u, last) * xstride;
+#pragma unroll
+      for (int g = 0; g < GR; g++) xv[u][g] = __builtin_nontemporal_load(xr[g] + eo + cs);
+    }
+  };
+  auto stage = [&](int buf, const uint64_t (&qv)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) lds[buf][w + kB3Waves * s][lane] = QPair{qv[s], shoup_quot(qv[s], p, delta)};
+  };
+  auto compute = [&](int buf, const uint64_t (&xv)[kB3U][GR]) {
+#pragma unroll
+    for (int u = 0; u < kB3U; u++)
+#pragma unroll
+      for (int i = 0; i < QR; i++) {
+        const QPair ya = lds[buf][(u * QR + i) * 2][lane], yb = lds[buf][(u * QR + i) * 2 + 1][lane];
+#pragma unroll
+        for (int g = 0; g < GR; g++) {
+          mac_step_asm(acc_a[g][i], xv[u][g], ya.y, ya.yq, kc);
+          mac_step_asm(acc_b[g][i], xv[u][g], yb.y, yb.yq, kc);
+        }
+      }
+  };
+
+  uint64_t qv[NS], x0[kB3U][GR], x1[kB3U][GR];
+  load_q(0, qv);
+  load_x(0, x0);
+  stage(0, qv);
+  __syncthreads();
+  for (uint64_t ch = 0; ch < nfull; ch += 2) {
+    // whole chunk ch from buffer 0 and x0 while chunk ch + 1 loads, then chunk ch + 1 from
+    // buffer 1 and x1 (when it is whole) while chunk ch + 2 loads
+    load_q(ch + 1, qv);
+    load_x(ch + 1, x1);
+    compute(0, x0);
+    stage(1, qv);
+    __syncthreads();
+    load_q(ch + 2, qv);
+    load_x(ch + 2, x0);
+    if (ch + 1 < nfull) compute(1, x1);
+    stage(0, qv);
+    __syncthreads();
+  }
+  // the elements after the last whole chunk: each wave loads its own query words
+  for (uint64_t e = nfull * kB3U; e < a.n_l; e++) {
+    uint64_t xv[GR];
+#pragma unroll
+    for (int g = 0; g < GR; g++) xv[g] = __builtin_nontemporal_load(xr[g] + e * xstride + cs);
+#pragma unroll
+    for (int i = 0; i < QR; i++) {
+      const uint64_t *qi = a.q[min(q0 + i, a.qn - 1)] + e * qstride + cs;
+      const uint64_t ya = qi[0], yb = qi[mn];
+      const uint64_t yaq = shoup_quot(ya, p, delta), ybq = shoup_quot(yb, p, delta);
+#pragma unroll
+      for (int g = 0; g < GR; g++) {
+        mac_step_asm(acc_a[g][i], xv[g], ya, yaq, kc);
+        mac_step_asm(acc_b[g][i], xv[g], yb, ybq, kc);
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < GR; g++) {
+    if (g >= rv) continue;
+#pragma unroll
+    for (int i = 0; i < QR; i++) {
+      if (q0 + i < a.qn) {
+        uint64_t *o = a.out[q0 + i] + (r0 + g) * 2 * mn + cs;
         o[0] = acc_a[g][i];
         o[mn] = acc_b[g][i];
       }
@@ -618,6 +773,11 @@ void batch_default_shape(uint32_t qn, uint64_t groups, int *gr, int *qr) {
   if (!*gr) *gr = groups >= 8 && *qr <= 2 ? 8 : (groups >= 4 ? 4 : 1);
 }
 
+bool batch3_shape_built(int GR, int QR) {
+  const int key = GR * 10 + QR;
+  return key == 41 || key == 42 || key == 44 || key == 81 || key == 82;
+}
+
 bool batch_shape_built(int GR, int QR) {
   const int key = GR * 10 + QR;
   return key == 11 || key == 12 || key == 14 || key == 41 || key == 42 || key == 44 || key == 81 || key == 82;
@@ -672,7 +832,7 @@ int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
                          uint64_t n_l, uint64_t groups, uint32_t P, uint64_t *out, const LevelCfg &cfg) {
   if (cfg.kernel == 4) {
     uint64_t *outp = out;
-    return launch_mac_batch_cfg(ctx, prm, db, &q, &outp, 1, n_l, groups, P, cfg.a, 1);
+    return launch_mac_batch_cfg(ctx, prm, db, &q, &outp, 1, n_l, groups, P, cfg.a, 1, cfg.c);
   }
   if (cfg.kernel != 1 && cfg.kernel != 3) return set_err(ctx, XPG_EINVAL, "unknown MAC kernel");
   if (cfg.kernel == 3 && prm->n < 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n >= 64");
@@ -721,19 +881,25 @@ int launch_mac_batch(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, co
   LevelCfg cfg;
   const TuneKey key{1, prm->n, prm->m, P, n_l, groups, tune_qn_bucket(qn)};
   if (options_pinned(ctx) || !tune_lookup(ctx->device, key, &cfg) || cfg.kernel != 4) cfg = LevelCfg{};
-  return launch_mac_batch_cfg(ctx, prm, db, q, out, qn, n_l, groups, P, cfg.a, cfg.b);
+  return launch_mac_batch_cfg(ctx, prm, db, q, out, qn, n_l, groups, P, cfg.a, cfg.b, cfg.c);
 }
 
 int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                          uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P, int gr,
-                         int qr) {
+                         int qr, int family) {
   if (qn == 0 || qn > (uint32_t)kMaxBatch) return set_err(ctx, XPG_EINVAL, "batch of 1..64 queries per launch");
   if (prm->n < 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n >= 64");
-  // register blocks: GR groups x QR queries per lane (0 = by shape)
+  // register blocks: GR rows x QR queries per lane (0 = by shape); family 3 shares the
+  // query words of a workgroup's rows through LDS, 2 loads them per wave (0 = by shape)
   int QR = qr, GR = gr;
   batch_default_shape(qn, groups, &GR, &QR);
+  if (family == 0) family = batch3_shape_built(GR, QR) ? 3 : 2;
   const int key = GR * 10 + QR;
-  if (!batch_shape_built(GR, QR)) return set_err(ctx, XPG_EINVAL, "register block shape not built");
+  if (family == 3) {
+    if (!batch3_shape_built(GR, QR)) return set_err(ctx, XPG_EINVAL, "register block shape not built");
+  } else if (family != 2 || !batch_shape_built(GR, QR)) {
+    return set_err(ctx, XPG_EINVAL, "register block shape not built");
+  }
   Batch2Args b2{};
   b2.ms = modset_of(prm);
   b2.db = db;
@@ -749,7 +915,9 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   b2.sblocks = prm->n / 64;
   b2.qn = qn;
   b2.qblocks = (qn + QR - 1) / QR;
-  b2.colgroups = ((groups + GR - 1) / GR) * P * prm->m * b2.sblocks;
+  // family 2: a block = GR groups x P polys of one column; family 3: kB3Waves*GR rows
+  b2.colgroups = family == 3 ? ((groups * P + kB3Waves * GR - 1) / (kB3Waves * GR)) * prm->m * b2.sblocks
+                             : ((groups + GR - 1) / GR) * P * prm->m * b2.sblocks;
   const uint64_t blocks = ((b2.colgroups + 7) / 8) * 8 * b2.qblocks;
   if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "batched MAC grid too large");
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -757,7 +925,16 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   hipEvent_t k0 = nullptr, k1 = nullptr;
   timing_begin(ctx, &k0, &k1, 2);
   if (ctx->timing && !ctx->ws) ctx->batch_steps += (uint64_t)qn * groups * P * prm->m * prm->n * 2 * n_l;
-  switch (key) {
+  const dim3 wg3(64 * kB3Waves);
+  if (family == 3) {
+    switch (key) {
+      case 82: hipLaunchKernelGGL((mac_batch3_kernel<8, 2>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
+      case 81: hipLaunchKernelGGL((mac_batch3_kernel<8, 1>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
+      case 44: hipLaunchKernelGGL((mac_batch3_kernel<4, 4>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
+      case 42: hipLaunchKernelGGL((mac_batch3_kernel<4, 2>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
+      default: hipLaunchKernelGGL((mac_batch3_kernel<4, 1>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
+    }
+  } else switch (key) {
     case 82: hipLaunchKernelGGL((mac_batch2_kernel<8, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
     case 81: hipLaunchKernelGGL((mac_batch2_kernel<8, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
     case 44: hipLaunchKernelGGL((mac_batch2_kernel<4, 4>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;

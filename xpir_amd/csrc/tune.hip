@@ -148,8 +148,10 @@ std::vector<LevelCfg> level_candidates(const xpg_ctx *ctx, const xpg_params *prm
   };
   add(LevelCfg{1, 0, 0, 0});
   if (groups >= 2) {
-    add(LevelCfg{4, 4, 1, 0});
-    add(LevelCfg{4, 8, 1, 0});
+    for (int fam : {2, 3}) {
+      add(LevelCfg{4, 4, 1, fam});
+      add(LevelCfg{4, 8, 1, fam});
+    }
   }
   if (prm->n >= 64 && n_l >= 2) {
     for (int u : {4, 8, 16})
@@ -183,12 +185,15 @@ bool entry_valid(const xpg_tune_entry &e) {
   if (e.n < 64 || (e.n & (e.n - 1)) || e.m < 1 || e.m > XPG_MAX_M || e.P < 1 || e.n_l < 1 || e.groups < 1)
     return false;
   if (e.kind == 1)
-    return e.qn >= 1 && e.qn <= 64 && (e.kernel == 0 || (e.kernel == 4 && batch_shape_built(e.a, e.b)));
+    return e.qn >= 1 && e.qn <= 64 &&
+           (e.kernel == 0 || (e.kernel == 4 && (e.c == 0 || e.c == 2 || e.c == 3) &&
+                              (e.c == 3 ? batch3_shape_built(e.a, e.b) : batch_shape_built(e.a, e.b))));
   if (e.kind != 0 || e.qn != 0) return false;
   switch (e.kernel) {
     case 1: return true;
     case 3: return (e.a == 4 || e.a == 8 || e.a == 16) && (e.b == 0 || e.b == 1) && e.c >= 0 && e.c <= (1 << 20);
-    case 4: return (e.a == 0 || e.a == 1 || e.a == 4 || e.a == 8) && e.b == 1;
+    case 4: return (e.a == 0 || e.a == 1 || e.a == 4 || e.a == 8) && e.b == 1 && (e.c == 0 || e.c == 2 || e.c == 3) &&
+                   (e.c != 3 || e.a == 4 || e.a == 8);
     default: return false;
   }
 }
@@ -421,25 +426,34 @@ int xpg_autotune(xpg_ctx *ctx, const xpg_db *db, uint32_t max_batch, xpg_tune_en
       rp[i] = ref + i * o0w;
       cp[i] = cand + i * o0w;
     }
-    // the shape picked by the fixed rules first, then every other built shape that fits
+    // the shape picked by the fixed rules first (its family by shape: 0), then every other
+    // built shape that fits, in both families
     int dg = 0, dq = 0;
     batch_default_shape(qn, groups0, &dg, &dq);
-    std::vector<std::pair<int, int>> shapes{{dg, dq}};
+    struct Shape {
+      int gr, qr, fam;
+    };
+    std::vector<Shape> shapes{{dg, dq, 0}};
     const int built[][2] = {{1, 1}, {1, 2}, {1, 4}, {4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}};
+    const int def_fam = batch3_shape_built(dg, dq) ? 3 : 2;
     for (const auto &s : built)
-      if ((s[0] != dg || s[1] != dq) && s[1] <= (int)qn && (s[0] == 1 || groups0 >= (uint64_t)s[0] / 2))
-        shapes.emplace_back(s[0], s[1]);
+      for (int fam : {2, 3}) {
+        const bool ok = fam == 3 ? batch3_shape_built(s[0], s[1]) : batch_shape_built(s[0], s[1]);
+        if (ok && (s[0] != dg || s[1] != dq || fam != def_fam) && s[1] <= (int)qn &&
+            (s[0] == 1 || groups0 * P0 >= (uint64_t)s[0] / 2))
+          shapes.push_back(Shape{s[0], s[1], fam});
+      }
     rc = warm_up(ctx, tm, [&] {
-      return launch_mac_batch_cfg(ctx, prm, db->words, qp.data(), rp.data(), qn, n0, groups0, P0, dg, dq);
+      return launch_mac_batch_cfg(ctx, prm, db->words, qp.data(), rp.data(), qn, n0, groups0, P0, dg, dq, 0);
     });
     if (rc) return rc;
     float best_ms = -1.f, def_ms = -1.f;
-    int bg = 0, bq = 0;
+    int bg = 0, bq = 0, bf = 0;
     for (size_t i = 0; i < shapes.size(); i++) {
-      const int gr = shapes[i].first, qr = shapes[i].second;
+      const int gr = shapes[i].gr, qr = shapes[i].qr, fam = shapes[i].fam;
       std::vector<uint64_t *> &dst = i == 0 ? rp : cp;
       const float ms = time_launches(ctx, tm, [&] {
-        return launch_mac_batch_cfg(ctx, prm, db->words, qp.data(), dst.data(), qn, n0, groups0, P0, gr, qr);
+        return launch_mac_batch_cfg(ctx, prm, db->words, qp.data(), dst.data(), qn, n0, groups0, P0, gr, qr, fam);
       });
       if (ms < 0) {
         if (i == 0) return ctx->err.empty() ? set_err(ctx, XPG_EDEVICE, "autotune launch") : XPG_EDEVICE;
@@ -457,14 +471,15 @@ int xpg_autotune(xpg_ctx *ctx, const xpg_db *db, uint32_t max_batch, xpg_tune_en
         best_ms = ms;
         bg = gr;
         bq = qr;
+        bf = fam;
       }
     }
-    if ((bg != dg || bq != dq) && best_ms <= kKeep * def_ms) {
+    if ((bg != dg || bq != dq || bf != 0) && best_ms <= kKeep * def_ms) {
       const float d2 = time_launches(ctx, tm, [&] {
-        return launch_mac_batch_cfg(ctx, prm, db->words, qp.data(), cp.data(), qn, n0, groups0, P0, dg, dq);
+        return launch_mac_batch_cfg(ctx, prm, db->words, qp.data(), cp.data(), qn, n0, groups0, P0, dg, dq, 0);
       });
       const float b2 = time_launches(ctx, tm, [&] {
-        return launch_mac_batch_cfg(ctx, prm, db->words, qp.data(), cp.data(), qn, n0, groups0, P0, bg, bq);
+        return launch_mac_batch_cfg(ctx, prm, db->words, qp.data(), cp.data(), qn, n0, groups0, P0, bg, bq, bf);
       });
       if (d2 > 0 && b2 > 0) {
         def_ms = std::min(def_ms, d2);
@@ -474,10 +489,11 @@ int xpg_autotune(xpg_ctx *ctx, const xpg_db *db, uint32_t max_batch, xpg_tune_en
     if (best_ms > kKeep * def_ms) {
       bg = dg;
       bq = dq;
+      bf = 0;
       best_ms = def_ms;
     }
     // kernel 0: the batch's level 0 runs faster as qn single-query launches
-    const LevelCfg choice = best_ms > (float)qn * level0_ms ? LevelCfg{0, 0, 0, 0} : LevelCfg{4, bg, bq, 0};
+    const LevelCfg choice = best_ms > (float)qn * level0_ms ? LevelCfg{0, 0, 0, 0} : LevelCfg{4, bg, bq, bf};
     found.push_back(make_entry(1, prm, P0, n0, groups0, qn, choice, std::min(best_ms, (float)qn * level0_ms),
                                def_ms));
   }
