@@ -900,6 +900,14 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   } else if (family != 2 || !batch_shape_built(GR, QR)) {
     return set_err(ctx, XPG_EINVAL, "register block shape not built");
   }
+  // a last query block of fewer than QR queries would compute padding queries: those r
+  // queries take a launch of their own, with the shape the fixed rules give r queries
+  const uint32_t rest = qn % (uint32_t)QR;
+  if (qn > (uint32_t)QR && rest) {
+    const int rc = launch_mac_batch_cfg(ctx, prm, db, q, out, qn - rest, n_l, groups, P, GR, QR, family);
+    if (rc) return rc;
+    return launch_mac_batch_cfg(ctx, prm, db, q + (qn - rest), out + (qn - rest), rest, n_l, groups, P, 0, 0, 0);
+  }
   Batch2Args b2{};
   b2.ms = modset_of(prm);
   b2.db = db;
