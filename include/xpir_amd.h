@@ -173,13 +173,18 @@ int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs);
  *      counterpart (the reference serves one query per session thread,
  *      apps/server/PIRSession.cpp:371-493).  A dispatcher thread owns `workers` streams;
  *      pending requests are grouped per database and launched as one batch on the first
- *      idle worker, so arrivals during a batch overlap with it on the GPU.  Admission is
- *      by reply memory: each request needs its reply slot (carved from a pool of
- *      reply_pool_bytes) plus, for d > 1, its level-0 intermediates, all counted
- *      against the same budget; a request that does not fit now gets XPG_EBUSY. ---- */
+ *      idle worker, so arrivals during a batch overlap with it on the GPU.  The server's
+ *      device memory is one pool of reply_pool_bytes, allocated at create: half holds the
+ *      reply slots, the other half is split into one workspace per worker, from which a
+ *      batch's intermediate levels are carved (nothing else is allocated per request).
+ *      Admission: a request whose reply slot does not fit the free slots now gets
+ *      XPG_EBUSY (submit again after releasing replies); one whose intermediates alone
+ *      exceed a worker's workspace can never run and gets XPG_ESIZE.  A batch takes as
+ *      many of one database's pending requests as a workspace holds (at most
+ *      max_batch). ---- */
 typedef struct xpg_server xpg_server;
 typedef struct {
-  uint64_t reply_pool_bytes;  /* admission budget (device bytes)       */
+  uint64_t reply_pool_bytes;  /* the server's device memory: slots + workspaces */
   uint32_t workers;           /* batches in flight (one stream each)   */
   uint32_t max_batch;         /* queries per batch                     */
 } xpg_server_config;
@@ -196,8 +201,11 @@ void xpg_server_destroy(xpg_server *srv);
 int xpg_server_submit(xpg_server *srv, const xpg_db *db, const uint64_t *query_dev, uint64_t *ticket);
 /* up to max completions; waits up to timeout_us when none is ready */
 int xpg_server_poll(xpg_server *srv, xpg_completion *out, uint32_t max, uint32_t *n_out, int64_t timeout_us);
+/* hands a completed ticket's reply slot back; XPG_EBUSY while the ticket is pending or
+ * running (its slot may still be written), XPG_EINVAL for an unknown ticket */
 int xpg_server_release(xpg_server *srv, uint64_t ticket);
 double xpg_server_now_us(xpg_server *srv);
+/* batches launched, requests in them, reply-slot bytes now held */
 int xpg_server_stats(xpg_server *srv, uint64_t *batches, uint64_t *batched_requests, uint64_t *budget_used);
 
 /* ---- kernel timing: HIP events around every MAC launch on the context stream ----

@@ -51,8 +51,9 @@ def test_serving_mix_equals_alone(gpu_ctx):
     # pile up behind a running batch and must share the next one
     gaps = rng.exponential(1.0 / 4000.0, size=n_req)
     gaps[np.arange(n_req) % 25 != 24] = 0.0
-    # budget: room for a handful of replies at once, so admission refuses some
-    srv = xpir_amd.Server(0, reply_pool_bytes=6 << 20, workers=2, max_batch=16)
+    # pool: 8 MiB of reply slots (about twenty replies at once, so bursts are refused) and
+    # two 4 MiB workspaces (a d = 2 request here needs 2.6 MiB, so batches stay small)
+    srv = xpir_amd.Server(0, reply_pool_bytes=16 << 20, workers=2, max_batch=16)
     try:
         expected, done, refused = {}, {}, 0
         t0 = time.perf_counter()
@@ -89,3 +90,84 @@ def test_serving_mix_equals_alone(gpu_ctx):
             for qd, _ in pool:
                 qd.free()
             db.free()
+
+
+def _hip_free_bytes():
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_admission_bounds_device_memory(gpu_ctx):
+    """The server's device memory is its pool: a request whose intermediates exceed a
+    worker's workspace is refused (XPG_ESIZE), bursts of requests that fit are served in
+    batches sized to the workspace with replies equal to the plain path's, device memory
+    stays within the pool (plus the workers' NTT tables and streams), and a ticket still
+    in flight keeps its slot (release gives XPG_EBUSY)."""
+    import xpir_amd
+    from xpir_amd.capi import XpgError
+    C2 = "LWE:80:2048:120"
+    # level-0 intermediates per request: 2048 groups x 64 KiB = 128 MiB, above a 16 MiB workspace
+    big_prm = xpir_amd.params(C2, 8)
+    raw = gpu_ctx.alloc(8 * 2048 * 64 + 16)
+    gpu_ctx.random_bytes_dev(71, 0, raw, 8 * 2048 * 64)
+    big = gpu_ctx.import_db_dev(big_prm, raw, 8 * 2048, 64, [8, 2048])
+    raw.free()
+    # per request: 16 groups x 64 KiB of level-0 outputs, plus 3 MiB of level-1 database
+    # for the batch: up to 13 requests per 16 MiB workspace
+    orc = Oracle(C2, 64)
+    prm = xpir_amd.params(C2, 64)
+    n, L, dims = 1000, 300, [64, 16]
+    data = orc.rand_bytes(72, 0, n * L)
+    small = gpu_ctx.import_db(prm, data, n, L, dims)
+    sk = orc.keygen(73)
+    pool = []
+    for qi in range(4):
+        q = np.concatenate(orc.make_query(sk, dims, 37 * qi + 5, 740 + qi))
+        pool.append((gpu_ctx.alloc(q.nbytes).upload(q), gpu_ctx.reply(small, q)))
+    bq = gpu_ctx.alloc(big.query_words * 8)
+    gpu_ctx.sync()
+    free0 = _hip_free_bytes()
+    budget = 64 << 20  # 32 MiB of reply slots (85 replies of 384 KiB), two 16 MiB workspaces
+    srv = xpir_amd.Server(0, reply_pool_bytes=budget, workers=2, max_batch=64)
+    try:
+        with pytest.raises(XpgError, match="-4"):
+            srv.submit(big, bq)
+        lowest, busy_seen, done = free0, False, 0
+        for burst in range(3):
+            expected = {}
+            for k in range(40):  # back to back: they pile up behind the running batches
+                qd, alone = pool[k % 4]
+                t = srv.submit(small, qd)
+                assert t is not None
+                expected[t] = alone
+            if burst == 0:
+                try:
+                    srv.release(t)  # the burst's last request: still pending or running
+                except XpgError as e:
+                    busy_seen = "-5" in str(e)
+            lowest = min(lowest, _hip_free_bytes())
+            t0 = time.perf_counter()
+            while expected:
+                for c in srv.poll(timeout_us=2000):
+                    assert c.status == 0
+                    assert np.array_equal(gpu_ctx.download_ptr(c.reply_dev, c.reply_words), expected.pop(c.ticket))
+                    srv.release(c.ticket)
+                    done += 1
+                lowest = min(lowest, _hip_free_bytes())
+                assert time.perf_counter() - t0 < 60
+        st = srv.stats()
+        print(f"{st['batches']} batches for {done} requests; device memory used {(free0 - lowest) >> 20} MiB "
+              f"for a {budget >> 20} MiB pool")
+        assert busy_seen and done == 120
+        assert st["batches"] < 120
+        assert free0 - lowest <= budget + (32 << 20)  # the pool, the workers' tables and streams
+    finally:
+        srv.close()
+        for qd, _ in pool:
+            qd.free()
+        bq.free()
+        big.free()
+        small.free()

@@ -449,8 +449,9 @@ class Server:
         return [XpgCompletion.from_buffer_copy(self._buf[i]) for i in range(n.value)]
 
     def release(self, ticket: int):
-        if lib().xpg_server_release(self.h, ticket) != XPG_OK:
-            raise XpgError("release")
+        rc = lib().xpg_server_release(self.h, ticket)
+        if rc != XPG_OK:
+            raise XpgError(f"release: status {rc}")
 
     def now_us(self) -> float:
         return lib().xpg_server_now_us(self.h)

@@ -101,7 +101,7 @@ const DevTables *get_tables(xpg_ctx *ctx, uint32_t n, uint32_t m) {
   const uint64_t key = ((uint64_t)n << 8) | m;
   auto it = ctx->tables.find(key);
   if (it != ctx->tables.end()) return &it->second;
-  if (ctx->ws) {
+  if (ctx->noalloc) {
     set_err(ctx, XPG_EINVAL, "no NTT tables for these parameters on this context (the no-allocation reply builds none)");
     return nullptr;
   }
@@ -235,7 +235,7 @@ int ensure_scratch(xpg_ctx *ctx, size_t bytes) {
 int get_scratch(xpg_ctx *ctx, size_t bytes, void **p) {
   if (ctx->ws) {
     if (bytes > ctx->ws_cap - ctx->ws_used)
-      return set_err(ctx, XPG_ESIZE, "the no-allocation reply's scratch array is too small for this launch");
+      return set_err(ctx, XPG_ESIZE, "the workspace is too small for this launch's scratch");
     *p = ctx->ws + ctx->ws_used;  // per launch: later launches on the stream reuse it
     return XPG_OK;
   }
@@ -261,7 +261,7 @@ void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial) {
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b, int tag) {
   // mode 2 times the main kernels alone: no events around whole levels, whose records
   // would put gaps between a level's launches; the no-allocation reply creates no events
-  if (!ctx->timing || ctx->ws || (ctx->timing == 2 && tag == 0)) return;
+  if (!ctx->timing || ctx->noalloc || (ctx->timing == 2 && tag == 0)) return;
   for (hipEvent_t *e : {a, b}) {
     if (!ctx->ev_pool.empty()) {
       *e = ctx->ev_pool.back();
@@ -672,8 +672,21 @@ static void level_buf_sizes(const xpg_db *db, uint32_t from, size_t *need_reply,
 // Levels [first, d) of one query.  `data` holds level `first`'s NTT database (P polys per
 // unit); `q` points at level `first`'s query part.  When first > 0 the caller passes the
 // previous level's reply in `prev_reply` and this routine starts with the recursion.
+static int run_levels_in(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const uint64_t *data,
+                         const uint64_t *q, const uint64_t *prev_reply, uint64_t *reply_dev);
+
+// With a workspace set, the carve of one reply is returned at its end: the next reply's
+// launches queue behind this one's on the same stream, so they may reuse the bytes.
 static int run_levels(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const uint64_t *data, const uint64_t *q,
                       const uint64_t *prev_reply, uint64_t *reply_dev) {
+  const size_t used = ctx->ws_used;
+  const int rc = run_levels_in(ctx, db, first, data, q, prev_reply, reply_dev);
+  if (ctx->ws) ctx->ws_used = used;
+  return rc;
+}
+
+static int run_levels_in(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const uint64_t *data,
+                         const uint64_t *q, const uint64_t *prev_reply, uint64_t *reply_dev) {
   const xpg_params *prm = &db->prm;
   const uint64_t mn = (uint64_t)prm->m * prm->n;
   // intermediate levels: lvl_reply holds a level's reply, lvl_db the next level's NTT
@@ -685,7 +698,7 @@ static int run_levels(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const uint
   if (need_db && ctx->ws) {
     const size_t r_al = (need_reply + 255) & ~(size_t)255, d_al = (need_db + 255) & ~(size_t)255;
     if (r_al + d_al > ctx->ws_cap - ctx->ws_used)
-      return set_err(ctx, XPG_ESIZE, "the no-allocation reply's scratch array is too small for the level buffers");
+      return set_err(ctx, XPG_ESIZE, "the workspace is too small for the level buffers");
     lvl_reply = reinterpret_cast<uint64_t *>(ctx->ws + ctx->ws_used);
     lvl_db = reinterpret_cast<uint64_t *>(ctx->ws + ctx->ws_used + r_al);
     ctx->ws_used += r_al + d_al;
@@ -833,9 +846,11 @@ int xpg_reply_dev_noalloc(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_
   ctx->ws = base;
   ctx->ws_cap = XPG_NOALLOC_SCRATCH_BYTES;
   ctx->ws_used = 0;
+  ctx->noalloc = true;
   const int rc = run_levels(ctx, db, 0, db->words, query_dev, nullptr, reply_dev);
   ctx->ws = nullptr;
   ctx->ws_cap = ctx->ws_used = 0;
+  ctx->noalloc = false;
   return rc;
 }
 
@@ -859,6 +874,58 @@ static bool batch_pays(const xpg_ctx *ctx, const xpg_db *db, uint32_t qn) {
   if (!options_pinned(ctx) && tune_lookup(ctx->device, key, &c)) return c.kernel != 0;
   const uint64_t lanes = groups0 * db->P0 * (uint64_t)prm->m * prm->n;
   return lanes * ((qn + 7) / 8) >= (1ull << 16);
+}
+
+// Bytes of one buffer of a batched call: carved from the call's workspace when one is set
+// (kept until the call's end), else the context's grown buffer (which: 0 = level reply,
+// 1 = level database, 2 = level-0 outputs).
+static int batch_buffer(xpg_ctx *ctx, int which, size_t bytes, uint64_t **p) {
+  if (ctx->ws) {
+    const size_t al = (bytes + 255) & ~(size_t)255;
+    if (al > ctx->ws_cap - ctx->ws_used)
+      return set_err(ctx, XPG_ESIZE, "the workspace is too small for the batch's intermediate levels");
+    *p = reinterpret_cast<uint64_t *>(ctx->ws + ctx->ws_used);
+    ctx->ws_used += al;
+    return XPG_OK;
+  }
+  const int rc = which == 2 ? ensure_batch_buf(ctx, bytes) : ensure_level_buf(ctx, which, bytes);
+  if (rc) return rc;
+  *p = which == 2 ? ctx->batch_buf : ctx->level_buf[which];
+  return XPG_OK;
+}
+
+// bytes of the buffers one request adds to a batch's levels 1..d-1 (the level database
+// and, for d > 2, a level's outputs)
+static void batch_level_bytes(const xpg_db *db, size_t *per_db, size_t *per_out) {
+  const uint64_t mn = (uint64_t)db->prm.m * db->prm.n;
+  *per_db = *per_out = 0;
+  for (uint32_t l = 1; l < db->d; l++) {
+    *per_db = std::max<size_t>(*per_db, prod_dims(db->d, db->dims, l) * db->Pl[l] * mn * 8);
+    if (l + 1 < db->d)
+      *per_out = std::max<size_t>(*per_out, prod_dims(db->d, db->dims, l + 1) * db->Pl[l] * 2 * mn * 8);
+  }
+}
+
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+extern "C++" uint64_t batch_workspace_bytes(const xpg_ctx *ctx, const xpg_db *db, uint32_t k) {
+  const xpg_params *prm = &db->prm;
+  const uint64_t mn = (uint64_t)prm->m * prm->n;
+  // batched: every request's level-0 outputs, then one request's later levels at least
+  const uint64_t lvl0 = db->d > 1 ? al256(k * prod_dims(db->d, db->dims, 1) * db->P0 * 2 * mn * 8) : 0;
+  size_t per_db = 0, per_out = 0;
+  batch_level_bytes(db, &per_db, &per_out);
+  const uint64_t batched = lvl0 + (db->d > 1 ? al256(per_db) + (per_out ? al256(per_out) : 0) : 0);
+  // one reply at a time: its level buffers and the largest level's scratch
+  size_t need_reply = 0, need_db = 0;
+  level_buf_sizes(db, 0, &need_reply, &need_db);
+  uint64_t single = need_db ? al256(need_reply) + al256(need_db) : 0, scr = 0;
+  for (uint32_t l = 0; l < db->d; l++) {
+    const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
+    const LevelCfg cfg = resolve_level_cfg(ctx, prm, db->dims[l], groups, db->Pl[l]);
+    scr = std::max<uint64_t>(scr, level_scratch_bytes(ctx, prm, db->dims[l], groups, db->Pl[l], cfg));
+  }
+  return std::max<uint64_t>(batched, single + scr);
 }
 
 static int batch_one_db(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs) {
@@ -885,37 +952,53 @@ static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const x
   const uint64_t groups0 = prod_dims(db->d, db->dims, 1);
   const uint32_t qn = (uint32_t)reqs.size();
   const uint64_t lvl0_words = groups0 * db->P0 * 2 * mn;
+  // a workspace carve is returned at the call's end (later calls queue behind this one)
+  struct WsMark {
+    xpg_ctx *c;
+    size_t used;
+    ~WsMark() {
+      if (c->ws) c->ws_used = used;
+    }
+  } mark{ctx, ctx->ws_used};
   // level-0 outputs: the final replies when d == 1, else per-request intermediates
+  uint64_t *lvl0 = nullptr;
   if (db->d > 1) {
-    int rc0 = ensure_batch_buf(ctx, qn * lvl0_words * 8);
+    const int rc0 = batch_buffer(ctx, 2, qn * lvl0_words * 8, &lvl0);
     if (rc0) return rc0;
   }
   std::vector<const uint64_t *> qp(qn);
   std::vector<uint64_t *> op(qn);
   for (uint32_t i = 0; i < qn; i++) {
     qp[i] = reqs[i]->query_dev;
-    op[i] = db->d == 1 ? reqs[i]->reply_dev : ctx->batch_buf + i * lvl0_words;
+    op[i] = db->d == 1 ? reqs[i]->reply_dev : lvl0 + i * lvl0_words;
   }
   int rc = launch_mac_batch(ctx, prm, db->words, qp.data(), op.data(), qn, db->dims[0], groups0, db->P0);
   if (rc || db->d == 1) return rc;
   // Levels 1..d-1 of all the chunk's requests together: per level, one recursion import
   // over every request's previous-level reply (they lie side by side) and one MAC launch
   // over the requests' level databases side by side, each request reading its own query.
-  // Requests are taken in sub-chunks whose level buffers fit kGroupedBudget.
+  // Requests are taken in sub-chunks whose level buffers fit the budget: the rest of the
+  // workspace when one is set, else kGroupedBudget (option group_budget).
   const uint32_t d = db->d;
   size_t per_db = 0, per_out = 0;
-  for (uint32_t l = 1; l < d; l++) {
-    per_db = std::max<size_t>(per_db, prod_dims(d, db->dims, l) * db->Pl[l] * mn * 8);
-    if (l + 1 < d) per_out = std::max<size_t>(per_out, prod_dims(d, db->dims, l + 1) * db->Pl[l] * 2 * mn * 8);
+  batch_level_bytes(db, &per_db, &per_out);
+  const size_t per_req = al256(per_db) + (per_out ? al256(per_out) : 0);
+  const uint64_t budget = ctx->ws ? ctx->ws_cap - ctx->ws_used : ctx->group_budget ? ctx->group_budget : kGroupedBudget;
+  const uint32_t sub = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(qn, budget / per_req));
+  uint64_t *lvl_db = nullptr, *lvl_out = nullptr;
+  if (ctx->ws) {  // carved once, for the largest sub-chunk
+    rc = batch_buffer(ctx, 1, sub * per_db, &lvl_db);
+    if (!rc && per_out) rc = batch_buffer(ctx, 0, sub * per_out, &lvl_out);
+    if (rc) return rc;
   }
-  const uint64_t budget = ctx->group_budget ? ctx->group_budget : kGroupedBudget;
-  const uint32_t sub = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(qn, budget / (per_db + per_out)));
   for (uint32_t s0 = 0; s0 < qn; s0 += sub) {
     const uint32_t cnt = std::min<uint32_t>(sub, qn - s0);
-    rc = ensure_level_buf(ctx, 1, cnt * per_db);
-    if (!rc && per_out) rc = ensure_level_buf(ctx, 0, cnt * per_out);
-    if (rc) return rc;
-    const uint64_t *prev = ctx->batch_buf + s0 * lvl0_words;
+    if (!ctx->ws) {
+      rc = batch_buffer(ctx, 1, cnt * per_db, &lvl_db);
+      if (!rc && per_out) rc = batch_buffer(ctx, 0, cnt * per_out, &lvl_out);
+      if (rc) return rc;
+    }
+    const uint64_t *prev = lvl0 + s0 * lvl0_words;
     uint64_t qoff = (uint64_t)db->dims[0] * 2 * mn;
     std::vector<const uint64_t *> qs(cnt);
     std::vector<uint64_t *> outs(cnt);
@@ -924,21 +1007,32 @@ static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const x
       const uint64_t unit = (uint64_t)db->Pl[l - 1] * 2 * mn * 8;
       const uint64_t units = cnt * groups_prev;
       ByteSrc src{(const uint8_t *)prev, unit, unit, units * unit};
-      rc = launch_import_bytes(ctx, prm, src, units, units, db->Pl[l], ctx->level_buf[1]);
+      rc = launch_import_bytes(ctx, prm, src, units, units, db->Pl[l], lvl_db);
       if (rc) return rc;
       for (uint32_t i = 0; i < cnt; i++) {
         qs[i] = reqs[s0 + i]->query_dev + qoff;
         outs[i] = reqs[s0 + i]->reply_dev;
       }
       const bool last = l + 1 == d;
-      rc = launch_mac_level_multi(ctx, prm, ctx->level_buf[1], qs.data(), cnt, db->dims[l], groups_l, db->Pl[l],
-                                  last ? outs.data() : nullptr, last ? nullptr : ctx->level_buf[0]);
+      rc = launch_mac_level_multi(ctx, prm, lvl_db, qs.data(), cnt, db->dims[l], groups_l, db->Pl[l],
+                                  last ? outs.data() : nullptr, last ? nullptr : lvl_out);
       if (rc) return rc;
-      prev = ctx->level_buf[0];
+      prev = lvl_out;
       qoff += (uint64_t)db->dims[l] * 2 * mn;
     }
   }
   return XPG_OK;
+}
+
+extern "C++" int reply_batch_in_workspace(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs, uint8_t *base,
+                                         size_t bytes) {
+  ctx->ws = base;
+  ctx->ws_cap = bytes;
+  ctx->ws_used = 0;
+  const int rc = xpg_reply_batch_dev(ctx, reqs, n_reqs);
+  ctx->ws = nullptr;
+  ctx->ws_cap = ctx->ws_used = 0;
+  return rc;
 }
 
 int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs) {

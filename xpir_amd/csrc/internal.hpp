@@ -89,10 +89,14 @@ struct xpg_ctx {
   size_t batch_bytes = 0;
   uint64_t *level_buf[2] = {nullptr, nullptr};
   size_t level_bytes[2] = {0, 0};
-  // set only inside xpg_reply_dev_noalloc: the device's static scratch array, from which
-  // that call takes its level buffers and block summaries (nothing is allocated then)
+  // a call's workspace: while set, a reply's level buffers and block summaries and a
+  // batch's intermediate levels are carved from it instead of the context's grown
+  // buffers.  Set by xpg_reply_dev_noalloc (the device's static scratch array; then
+  // nothing at all is allocated: noalloc) and by the serving loop (a worker's share of
+  // the admitted pool).
   uint8_t *ws = nullptr;
   size_t ws_cap = 0, ws_used = 0;
+  bool noalloc = false;
   uint64_t *io_buf = nullptr;     // device copies of a host query and its reply (xpg_reply)
   size_t io_bytes = 0;
   uint64_t *dec_buf = nullptr;    // residues and lifted coefficients of xpg_decrypt_dev
@@ -140,6 +144,12 @@ void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial);  // 0 = every entry
 ModSet modset_of(const xpg_params *prm);
 void timing_begin(xpg_ctx *ctx, hipEvent_t *a, hipEvent_t *b, int tag = 0);
 void timing_end(xpg_ctx *ctx, hipEvent_t a, hipEvent_t b, int tag = 0);
+
+// Workspace bytes xpg_reply_batch_dev needs for k requests on db when given a workspace
+// (the serving loop sizes its batches by it).
+uint64_t batch_workspace_bytes(const xpg_ctx *ctx, const xpg_db *db, uint32_t k);
+// xpg_reply_batch_dev with every intermediate buffer carved from [base, base + bytes)
+int reply_batch_in_workspace(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs, uint8_t *base, size_t bytes);
 
 // ---- tuning (tune.hip) ----
 // Kernel configuration of one MAC launch.  kernel: 1 = one chain per lane, 3 = streaming

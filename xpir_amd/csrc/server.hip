@@ -2,15 +2,21 @@
 //
 // The reference serves one client per session thread and one query per generator
 // (apps/server/PIRSession.cpp:371-493).  Here one dispatcher thread owns W worker
-// contexts (one HIP stream each).  Requests are admitted against a reply-memory budget
-// (their reply slot plus the level-0 intermediates of a multi-level reply); pending
-// requests are grouped per database and launched as one batched reply on the first idle
-// worker, so arrivals during a running batch are picked up by the next free worker and
-// their kernels overlap on the GPU.  Completion is detected with HIP events; latency is
-// measured from submit to the event that ends the request's batch.
+// contexts (one HIP stream each).  The server's device memory is one pool of
+// reply_pool_bytes: half of it holds reply slots, the other half is split into one
+// workspace per worker, from which a batch's intermediate levels (level-0 outputs, level
+// databases, block summaries) are carved.  A request is admitted when its reply slot fits
+// (else XPG_EBUSY: try again after releasing completed replies); one whose intermediates
+// alone exceed a worker's workspace can never run (XPG_ESIZE).  Pending requests are
+// grouped per database, as many as the worker's workspace holds (at most max_batch),
+// and launched as one batched reply on the first idle worker, so arrivals during a
+// running batch are picked up by the next free worker and their kernels overlap on the
+// GPU.  Completion is detected with HIP events; latency is measured from submit to the
+// event that ends the request's batch.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <set>
 #include <thread>
 
 #include "internal.hpp"
@@ -31,6 +37,7 @@ struct Worker {
   hipEvent_t done = nullptr;
   bool busy = false;
   std::vector<Req> batch;
+  uint8_t *ws = nullptr;  // this worker's share of the pool
 };
 
 // first-fit allocator over one device slab (host bookkeeping only)
@@ -80,13 +87,16 @@ struct xpg_server {
   int device = 0;
   xpg_server_config cfg{};
   std::vector<Worker> workers;
-  Slab slab;
+  uint8_t *pool = nullptr;  // reply_pool_bytes: the reply slots, then the workspaces
+  uint64_t ws_bytes = 0;    // each worker's workspace
+  Slab slab;                // the reply slots
   uint64_t budget_used = 0;
   uint64_t next_ticket = 1;
   std::deque<Req> pending;
   std::vector<xpg_completion> done;
   std::map<uint64_t, std::pair<uint64_t, uint64_t>> slots;  // ticket -> (offset, bytes) of its reply
   std::map<uint64_t, uint64_t> footprint;                   // ticket -> admitted bytes
+  std::set<uint64_t> finished;                              // tickets whose completion is posted
   std::mutex mu;
   std::condition_variable cv, done_cv;
   std::thread th;
@@ -94,16 +104,24 @@ struct xpg_server {
   uint64_t batches = 0, batched_requests = 0;
 };
 
+// a request's reply slot (its intermediates come from the worker's workspace)
 static uint64_t request_footprint(const xpg_db *db, uint64_t *reply_words) {
   const uint64_t mn = (uint64_t)db->prm.m * db->prm.n;
   *reply_words = (uint64_t)db->Pl[db->d] * 2 * mn;
-  uint64_t bytes = *reply_words * 8;
-  if (db->d > 1) {
-    uint64_t groups0 = 1;
-    for (uint32_t l = 1; l < db->d; l++) groups0 *= db->dims[l];
-    bytes += groups0 * db->P0 * 2 * mn * 8;
-  }
-  return bytes;
+  return (*reply_words * 8 + 255) & ~255ULL;
+}
+
+static void post_completion(xpg_server *s, const Req &r, double t, int status) {
+  xpg_completion c{};
+  c.ticket = r.ticket;
+  c.reply_dev = r.reply;
+  c.reply_words = r.reply_words;
+  c.submit_us = r.submit_us;
+  c.start_us = r.start_us;
+  c.done_us = t;
+  c.status = status;
+  s->done.push_back(c);
+  s->finished.insert(r.ticket);
 }
 
 static void dispatcher(xpg_server *s) {
@@ -115,28 +133,21 @@ static void dispatcher(xpg_server *s) {
     for (Worker &w : s->workers) {
       if (!w.busy || hipEventQuery(w.done) != hipSuccess) continue;
       const double t = now_us();
-      for (const Req &r : w.batch) {
-        xpg_completion c{};
-        c.ticket = r.ticket;
-        c.reply_dev = r.reply;
-        c.reply_words = r.reply_words;
-        c.submit_us = r.submit_us;
-        c.start_us = r.start_us;
-        c.done_us = t;
-        c.status = XPG_OK;
-        s->done.push_back(c);
-      }
+      for (const Req &r : w.batch) post_completion(s, r, t, XPG_OK);
       w.batch.clear();
       w.busy = false;
       progressed = true;
     }
     if (progressed) s->done_cv.notify_all();
-    // dispatch: oldest pending request's database, up to max_batch of its requests
+    // dispatch: oldest pending request's database, up to max_batch of its requests and as
+    // many as the worker's workspace holds
     for (Worker &w : s->workers) {
       if (w.busy || s->pending.empty()) continue;
       const xpg_db *db = s->pending.front().db;
+      uint32_t fit = 1;
+      while (fit < s->cfg.max_batch && batch_workspace_bytes(w.ctx, db, fit + 1) <= s->ws_bytes) fit++;
       std::vector<Req> take;
-      for (auto it = s->pending.begin(); it != s->pending.end() && take.size() < s->cfg.max_batch;) {
+      for (auto it = s->pending.begin(); it != s->pending.end() && take.size() < fit;) {
         if (it->db == db) {
           take.push_back(*it);
           it = s->pending.erase(it);
@@ -151,23 +162,17 @@ static void dispatcher(xpg_server *s) {
         reqs[i] = xpg_request{take[i].db, take[i].query, take[i].reply};
       }
       lk.unlock();
-      int rc = xpg_reply_batch_dev(w.ctx, reqs.data(), (uint32_t)reqs.size());
+      int rc = reply_batch_in_workspace(w.ctx, reqs.data(), (uint32_t)reqs.size(), w.ws, s->ws_bytes);
       if (rc == XPG_OK && hipEventRecord(w.done, w.ctx->stream) != hipSuccess) rc = XPG_EDEVICE;
+      // a failed batch may have launched kernels already: they finish before its slots
+      // and workspace are handed back
+      if (rc != XPG_OK) (void)hipStreamSynchronize(w.ctx->stream);
       lk.lock();
       s->batches++;
       s->batched_requests += take.size();
       if (rc != XPG_OK) {
-        for (const Req &r : take) {
-          xpg_completion c{};
-          c.ticket = r.ticket;
-          c.reply_dev = r.reply;
-          c.reply_words = r.reply_words;
-          c.submit_us = r.submit_us;
-          c.start_us = r.start_us;
-          c.done_us = now_us();
-          c.status = rc;
-          s->done.push_back(c);
-        }
+        const double t1 = now_us();
+        for (const Req &r : take) post_completion(s, r, t1, rc);
         s->done_cv.notify_all();
       } else {
         w.batch = std::move(take);
@@ -207,16 +212,24 @@ int xpg_server_create(int device, const xpg_server_config *cfg, xpg_server **out
     s->workers.push_back(w);
   }
   (void)hipSetDevice(device);
-  if (hipMalloc((void **)&s->slab.base, cfg->reply_pool_bytes) != hipSuccess) {
+  // half the pool for reply slots, the other half one workspace per worker
+  s->ws_bytes = (cfg->reply_pool_bytes / 2 / cfg->workers) & ~255ULL;
+  const uint64_t slots = (cfg->reply_pool_bytes - s->ws_bytes * cfg->workers) & ~255ULL;
+  const int prc = s->ws_bytes == 0 || slots == 0 ? XPG_EINVAL
+                  : hipMalloc((void **)&s->pool, cfg->reply_pool_bytes) != hipSuccess ? XPG_ENOMEM
+                                                                                      : XPG_OK;
+  if (prc != XPG_OK) {
     for (Worker &x : s->workers) {
       (void)hipEventDestroy(x.done);
       xpg_ctx_destroy(x.ctx);
     }
     delete s;
-    return XPG_ENOMEM;
+    return prc;
   }
-  s->slab.size = cfg->reply_pool_bytes;
-  s->slab.free_[0] = cfg->reply_pool_bytes;
+  s->slab.base = s->pool;
+  s->slab.size = slots;
+  s->slab.free_[0] = slots;
+  for (uint32_t i = 0; i < cfg->workers; i++) s->workers[i].ws = s->pool + slots + i * s->ws_bytes;
   s->th = std::thread(dispatcher, s);
   *out = s;
   return XPG_OK;
@@ -235,7 +248,7 @@ void xpg_server_destroy(xpg_server *s) {
     (void)hipEventDestroy(w.done);
     xpg_ctx_destroy(w.ctx);
   }
-  (void)hipFree(s->slab.base);
+  (void)hipFree(s->pool);
   delete s;
 }
 
@@ -246,10 +259,12 @@ int xpg_server_submit(xpg_server *s, const xpg_db *db, const uint64_t *query_dev
   r.db = db;
   r.query = query_dev;
   r.footprint = request_footprint(db, &r.reply_words);
+  // a request whose intermediates exceed a worker's workspace can never run
+  if (batch_workspace_bytes(s->workers[0].ctx, db, 1) > s->ws_bytes || r.footprint > s->slab.size)
+    return XPG_ESIZE;
   std::lock_guard<std::mutex> lk(s->mu);
   uint64_t off = 0;
-  // admission: the reply slot must fit the pool and the request's intermediates the budget
-  if (s->budget_used + r.footprint > s->cfg.reply_pool_bytes) return XPG_EBUSY;
+  // admission: the reply slot must fit the slots now free
   if (!s->slab.alloc(r.reply_words * 8, &off)) return XPG_EBUSY;
   r.reply = (uint64_t *)(s->slab.base + off);
   r.ticket = s->next_ticket++;
@@ -280,6 +295,9 @@ int xpg_server_release(xpg_server *s, uint64_t ticket) {
   std::lock_guard<std::mutex> lk(s->mu);
   auto it = s->slots.find(ticket);
   if (it == s->slots.end()) return XPG_EINVAL;
+  // a ticket still pending or running keeps its slot: kernels may still write it
+  if (!s->finished.count(ticket)) return XPG_EBUSY;
+  s->finished.erase(ticket);
   s->slab.release(it->second.first, it->second.second);
   s->slots.erase(it);
   auto f = s->footprint.find(ticket);
