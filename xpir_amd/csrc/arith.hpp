@@ -106,12 +106,14 @@ __device__ __forceinline__ StepConsts step_consts(uint64_t p, uint32_t delta) {
 
 // One reply-chain step (crypto/NFLlib.hpp:303-321) as straight-line VALU code:
 //   acc = acc + (x*y - q*p),  q = hi64(x*yq);   if (acc >= p) acc -= p   (all mod 2^64)
-// for every x, y, yq, acc < 2^64 and p = 2^61 - delta.  19 VALU instructions:
+// for every x, y, yq, acc < 2^64 and p = 2^61 - delta.  18 VALU instructions:
 //   q:   t = hi32(xl*ql) (from C, as {t, 0}), A = xh*ql + t (below 2^64), B = xl*qh + A with
 //        its carry c, q = xh*qh + {hi32(B), c} -- the middle product's carry comes from
 //        v_mad_u64_u32's carry-out instead of being recombined from 32-bit halves;
 //   r-p: x*y + q*(2^64 - p) - p (mod 2^64), with 2^64 - p = 0xE0000000 * 2^32 + delta:
-//        lo = xl*yl - p + lo(q)*delta, hi += xl*yh + xh*yl + hi(q)*delta + lo(q)*0xE0000000;
+//        lo = xl*yl - p + lo(q)*delta (two 64-bit multiply-adds), and the four products
+//        that reach only the high word (xl*yh, xh*yl, hi(q)*delta, lo(q)*0xE0000000) are
+//        summed by a multiply and three multiply-adds whose low words alone are kept;
 //   chain: u = acc + r - p; the reference's acc + r >= p exactly when u < 2^64 - p, then
 //        u is the new acc, else u + p.
 // Scratch v[2:9] and vcc; an s_nop separates a VALU write of vcc from its read as a lane
@@ -124,16 +126,15 @@ __device__ __forceinline__ void mac_step_asm(uint64_t &acc, uint64_t x, uint64_t
   asm("v_mad_u64_u32 v[4:5], vcc, %[xh], %[ql], %[T]\n\t"
       "v_mad_u64_u32 v[6:7], vcc, %[xl], %[qh], v[4:5]\n\t"
       "v_mul_lo_u32 v2, %[xl], %[yh]\n\t"
-      "v_mul_lo_u32 v3, %[xh], %[yl]\n\t"
       "v_mov_b32 v8, v7\n\t"
       "v_cndmask_b32_e64 v9, 0, 1, vcc\n\t"
       "v_mad_u64_u32 v[4:5], vcc, %[xh], %[qh], v[8:9]\n\t"
+      "v_mad_u64_u32 v[2:3], vcc, %[xh], %[yl], v[2:3]\n\t"
       "v_mad_u64_u32 v[6:7], vcc, %[xl], %[yl], %[negp]\n\t"
       "v_mad_u64_u32 v[6:7], vcc, v4, %[d], v[6:7]\n\t"
-      "v_mul_lo_u32 v8, v5, %[d]\n\t"
-      "v_mul_lo_u32 v9, v4, %[k29]\n\t"
-      "v_add3_u32 v7, v7, v2, v3\n\t"
-      "v_add3_u32 v7, v7, v8, v9\n\t"
+      "v_mad_u64_u32 v[2:3], vcc, v5, %[d], v[2:3]\n\t"
+      "v_mad_u64_u32 v[2:3], vcc, v4, %[k29], v[2:3]\n\t"
+      "v_add_u32 v7, v7, v2\n\t"
       "v_lshl_add_u64 %[acc], %[acc], 0, v[6:7]\n\t"
       "v_cmp_gt_u64_e64 vcc, %[negp], %[acc]\n\t"
       "s_nop 0\n\t"

@@ -127,17 +127,26 @@ struct Batch2Args {
   const uint64_t *db;
   const uint64_t *q[kMaxBatch];
   uint64_t *out[kMaxBatch];
-  uint64_t n_l, groups, colgroups;  // colgroups = ceil(groups / GR) * P * m * sblocks
+  uint64_t n_l, groups, colgroups;  // colgroups = row blocks * m * sblocks
   uint32_t P, n, m, sblocks, qn, qblocks;
 };
 
+// Block b -> (colgroup cg, query block qb).  A colgroup is (row block, column), the column
+// (c, sb) fastest; the 8 XCDs take blocks round-robin (b & 7) and the query blocks of one
+// colgroup run back to back on one XCD, so its DB rows come from HBM once and from that
+// XCD's L2 for the other query blocks.
+__device__ __forceinline__ bool batch_tile(const Batch2Args &a, uint64_t b, uint64_t &cg, uint32_t &qb) {
+  const uint64_t xcd = b & 7, j = b >> 3;
+  qb = (uint32_t)(j % a.qblocks);
+  cg = (j / a.qblocks) * 8 + xcd;
+  return cg < a.colgroups;
+}
+
 template <int GR, int QR>
 __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
-  const uint64_t b = blockIdx.x;
-  const uint64_t xcd = b & 7, j = b >> 3;
-  const uint64_t cg = (j / a.qblocks) * 8 + xcd;
-  const uint32_t qb = (uint32_t)(j % a.qblocks);
-  if (cg >= a.colgroups) return;
+  uint64_t cg;
+  uint32_t qb;
+  if (!batch_tile(a, blockIdx.x, cg, qb)) return;
   const uint32_t lane = threadIdx.x;
   const uint32_t sb = (uint32_t)(cg % a.sblocks);
   uint64_t r = cg / a.sblocks;
@@ -268,11 +277,9 @@ __global__ void __launch_bounds__(64 * kB3Waves) mac_batch3_kernel(Batch2Args a)
   constexpr int NS = NQ / kB3Waves;  // rows staged per wave
   static_assert(NQ % kB3Waves == 0, "staged rows split evenly over the waves");
   __shared__ QPair lds[2][NQ][64];
-  const uint64_t b = blockIdx.x;
-  const uint64_t xcd = b & 7, j = b >> 3;
-  const uint64_t cg = (j / a.qblocks) * 8 + xcd;
-  const uint32_t qb = (uint32_t)(j % a.qblocks);
-  if (cg >= a.colgroups) return;  // the whole workgroup
+  uint64_t cg;
+  uint32_t qb;
+  if (!batch_tile(a, blockIdx.x, cg, qb)) return;  // the whole workgroup
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t sb = (uint32_t)(cg % a.sblocks);
@@ -780,7 +787,8 @@ bool batch3_shape_built(int GR, int QR) {
 
 bool batch_shape_built(int GR, int QR) {
   const int key = GR * 10 + QR;
-  return key == 11 || key == 12 || key == 14 || key == 41 || key == 42 || key == 44 || key == 81 || key == 82;
+  return key == 11 || key == 12 || key == 14 || key == 41 || key == 42 || key == 44 || key == 81 || key == 82 ||
+         key == 161;
 }
 
 // Lanes at which one chain per lane already fills the GPU (256 CUs x 32 wavefronts).
@@ -943,6 +951,7 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
       default: hipLaunchKernelGGL((mac_batch3_kernel<4, 1>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
     }
   } else switch (key) {
+    case 161: hipLaunchKernelGGL((mac_batch2_kernel<16, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
     case 82: hipLaunchKernelGGL((mac_batch2_kernel<8, 2>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
     case 81: hipLaunchKernelGGL((mac_batch2_kernel<8, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
     case 44: hipLaunchKernelGGL((mac_batch2_kernel<4, 4>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;

@@ -152,6 +152,7 @@ std::vector<LevelCfg> level_candidates(const xpg_ctx *ctx, const xpg_params *prm
       add(LevelCfg{4, 4, 1, fam});
       add(LevelCfg{4, 8, 1, fam});
     }
+    if (groups >= 16) add(LevelCfg{4, 16, 1, 2});
   }
   if (prm->n >= 64 && n_l >= 2) {
     for (int u : {4, 8, 16})
@@ -192,8 +193,8 @@ bool entry_valid(const xpg_tune_entry &e) {
   switch (e.kernel) {
     case 1: return true;
     case 3: return (e.a == 4 || e.a == 8 || e.a == 16) && (e.b == 0 || e.b == 1) && e.c >= 0 && e.c <= (1 << 20);
-    case 4: return (e.a == 0 || e.a == 1 || e.a == 4 || e.a == 8) && e.b == 1 && (e.c == 0 || e.c == 2 || e.c == 3) &&
-                   (e.c != 3 || e.a == 4 || e.a == 8);
+    case 4: return (e.a == 0 || e.a == 1 || e.a == 4 || e.a == 8 || e.a == 16) && e.b == 1 &&
+                   (e.c == 0 || e.c == 2 || e.c == 3) && (e.c != 3 || e.a == 4 || e.a == 8) && (e.a != 16 || e.c != 3);
     default: return false;
   }
 }
@@ -434,7 +435,7 @@ int xpg_autotune(xpg_ctx *ctx, const xpg_db *db, uint32_t max_batch, xpg_tune_en
       int gr, qr, fam;
     };
     std::vector<Shape> shapes{{dg, dq, 0}};
-    const int built[][2] = {{1, 1}, {1, 2}, {1, 4}, {4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}};
+    const int built[][2] = {{1, 1}, {1, 2}, {1, 4}, {4, 1}, {4, 2}, {4, 4}, {8, 1}, {8, 2}, {16, 1}};
     const int def_fam = batch3_shape_built(dg, dq) ? 3 : 2;
     for (const auto &s : built)
       for (int fam : {2, 3}) {
