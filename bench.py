@@ -1240,15 +1240,18 @@ def bench_metric(args):
                 "kernel_ms_per_pass": k_ms / args.steps, "launches_per_pass": k_launches / args.steps,
                 "exact_steps_per_pass": k_steps / args.steps, "exact_steps_per_s": steps_s,
                 "peak_derivation": "256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 32-bit lane ops/s "
-                                   "(= 157.3 TFLOP/s fp32 / 2 for FMA / 2 for dual issue)"}
+                                   "(= 157.3 TFLOP/s fp32 / 2 flops per FMA / 2 for packed fp32)"}
     if prof:
         vps = prof["valu_lane_ops_per_step"]
         roofline.update(achieved=steps_s * vps / 1e12, frac=steps_s * vps / 1e12 / VALU_PEAK_TOPS,
                         valu_lane_ops_per_step=vps, traffic=prof.get("hbm_bytes_per_pass"),
                         hbm_gbs=(prof["hbm_bytes_per_pass"] / (k_ms / args.steps / 1e3) / 1e9
                                  if prof.get("hbm_bytes_per_pass") else None),
+                        valu_busy=prof.get("valu_busy"), sustained_clock_ghz=prof.get("clock_ghz_from_grbm"),
                         counters_from="profiles/mac_batch3_kernel_batched_uneven_8dbs.json (rocprofv3 "
-                                      "SQ_INSTS_VALU, FETCH_SIZE, WRITE_SIZE; command recorded there)")
+                                      "SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU, GRBM_GUI_ACTIVE, FETCH_SIZE, WRITE_SIZE "
+                                      "of the same pass; commands recorded there; traffic is those counters', "
+                                      "not measured in this run)")
     else:
         roofline.update(achieved=None, frac=None, traffic=None)
 
