@@ -147,6 +147,72 @@ __device__ __forceinline__ void mac_step_asm(uint64_t &acc, uint64_t x, uint64_t
       : "v2", "v3", "v4", "v5", "v6", "v7", "v8", "v9", "vcc");
 }
 
+// The lazy Shoup product x*w - hi64(x*wq)*p (mod 2^64) for p = 2^61 - delta as 13 VALU
+// instructions (equal to shoup_lazy(x, w, wq, p) for every x, w, wq < 2^64): q as in
+// mac_step_asm; the four products reaching only the high word (xl*wh, xh*wl,
+// hi(q)*delta, lo(q)*0xE0000000) are summed by chained multiply-adds, whose low word a
+// 64-bit shift moves to the high half, and two multiply-adds add xl*wl and lo(q)*delta.
+// Scratch v[4:9] and vcc; two wait states separate the carry's write of vcc from its
+// read as a lane mask.
+__device__ __forceinline__ uint64_t shoup_lazy_asm(uint64_t x, uint64_t w, uint64_t wq, uint32_t delta) {
+  const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), wl = (uint32_t)w, wh = (uint32_t)(w >> 32);
+  const uint32_t ql = (uint32_t)wq, qh = (uint32_t)(wq >> 32);
+  const uint64_t T = __umulhi(xl, ql);
+  uint64_t r;
+  asm("v_mad_u64_u32 v[4:5], vcc, %[xh], %[ql], %[T]\n\t"
+      "v_mad_u64_u32 v[6:7], vcc, %[xl], %[qh], v[4:5]\n\t"
+      "v_mov_b32 v8, v7\n\t"
+      "s_nop 0\n\t"
+      "v_cndmask_b32_e64 v9, 0, 1, vcc\n\t"
+      "v_mad_u64_u32 v[4:5], vcc, %[xh], %[qh], v[8:9]\n\t"
+      "v_mad_u64_u32 v[8:9], vcc, %[xl], %[wh], 0\n\t"
+      "v_mad_u64_u32 v[8:9], vcc, %[xh], %[wl], v[8:9]\n\t"
+      "v_mad_u64_u32 v[8:9], vcc, v5, %[d], v[8:9]\n\t"
+      "v_mad_u64_u32 v[8:9], vcc, v4, %[k29], v[8:9]\n\t"
+      "v_lshlrev_b64 v[8:9], 32, v[8:9]\n\t"
+      "v_mad_u64_u32 %[r], vcc, %[xl], %[wl], v[8:9]\n\t"
+      "v_mad_u64_u32 %[r], vcc, v4, %[d], %[r]"
+      : [r] "=&v"(r)
+      : [xl] "v"(xl), [xh] "v"(xh), [wl] "v"(wl), [wh] "v"(wh), [ql] "v"(ql), [qh] "v"(qh), [T] "v"(T),
+        [d] "s"(delta), [k29] "s"(0xE0000000u)
+      : "v4", "v5", "v6", "v7", "v8", "v9", "vcc");
+  return r;
+}
+
+// (u + v) reduced once by m2 (m2 = 2p, negm2 = 2^64 - 2p): the sum, the sum minus m2, and a
+// select on sum < m2 -- five VALU instructions.  Scratch v[2:3], v[10:11] and vcc.
+__device__ __forceinline__ uint64_t add_mod2p_asm(uint64_t u, uint64_t v, uint64_t m2, uint64_t negm2) {
+  uint32_t lo, hi;
+  asm("v_lshl_add_u64 v[2:3], %[u], 0, %[v]\n\t"
+      "v_lshl_add_u64 v[10:11], v[2:3], 0, %[negm2]\n\t"
+      "v_cmp_gt_u64_e64 vcc, %[m2], v[2:3]\n\t"
+      "s_nop 1\n\t"
+      "v_cndmask_b32_e64 %[lo], v10, v2, vcc\n\t"
+      "v_cndmask_b32_e64 %[hi], v11, v3, vcc"
+      : [lo] "=v"(lo), [hi] "=v"(hi)
+      : [u] "v"(u), [v] "v"(v), [m2] "s"(m2), [negm2] "s"(negm2)
+      : "v2", "v3", "v10", "v11", "vcc");
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// u - v, plus m2 when u < v (the borrow selects) -- five VALU instructions.  Scratch
+// v[2:3], v[10:11] and vcc.
+__device__ __forceinline__ uint64_t sub_mod2p_asm(uint64_t u, uint64_t v, uint64_t m2) {
+  uint32_t lo, hi;
+  asm("v_sub_co_u32_e64 v2, vcc, %[ul], %[vl]\n\t"
+      "s_nop 1\n\t"
+      "v_subb_co_u32_e64 v3, vcc, %[uh], %[vh], vcc\n\t"
+      "v_lshl_add_u64 v[10:11], v[2:3], 0, %[m2]\n\t"
+      "s_nop 1\n\t"
+      "v_cndmask_b32_e64 %[lo], v2, v10, vcc\n\t"
+      "v_cndmask_b32_e64 %[hi], v3, v11, vcc"
+      : [lo] "=v"(lo), [hi] "=v"(hi)
+      : [ul] "v"((uint32_t)u), [uh] "v"((uint32_t)(u >> 32)), [vl] "v"((uint32_t)v), [vh] "v"((uint32_t)(v >> 32)),
+        [m2] "s"(m2)
+      : "v2", "v3", "v10", "v11", "vcc");
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // counter-based SplitMix64: word k of stream `seed` (the seeded generator of this
 // project; stands in for the reference's /dev/urandom-keyed Salsa20 stream).
 __host__ __device__ __forceinline__ uint64_t rand_word(uint64_t seed, uint64_t k) {
