@@ -33,6 +33,7 @@ __device__ __forceinline__ void direct_chains(const uint64_t *x, const uint64_t 
                                               uint64_t n_l, uint64_t xstride, uint64_t qstride, uint64_t p,
                                               uint32_t delta, uint64_t &acc_a, uint64_t &acc_b) {
   acc_a = acc_b = 0;
+  const StepConsts kc = step_consts(p, delta);
   uint64_t e = 0;
   for (; e + 4 <= n_l; e += 4) {
     uint64_t xv[4], av[4], bv[4];
@@ -45,14 +46,14 @@ __device__ __forceinline__ void direct_chains(const uint64_t *x, const uint64_t 
     __builtin_amdgcn_sched_barrier(0);  // the chunk's loads stay ahead of its arithmetic
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      acc_a = mac_step_qf(acc_a, xv[i], av[i], p, delta);
-      acc_b = mac_step_qf(acc_b, xv[i], bv[i], p, delta);
+      mac_step_asm(acc_a, xv[i], av[i], shoup_quot(av[i], p, delta), kc);
+      mac_step_asm(acc_b, xv[i], bv[i], shoup_quot(bv[i], p, delta), kc);
     }
   }
   for (; e < n_l; e++) {
     const uint64_t xv = x[e * xstride], av = ya[e * qstride], bv = yb[e * qstride];
-    acc_a = mac_step_qf(acc_a, xv, av, p, delta);
-    acc_b = mac_step_qf(acc_b, xv, bv, p, delta);
+    mac_step_asm(acc_a, xv, av, shoup_quot(av, p, delta), kc);
+    mac_step_asm(acc_b, xv, bv, shoup_quot(bv, p, delta), kc);
   }
 }
 
@@ -551,8 +552,8 @@ __device__ __forceinline__ void stream_sum(ChainSum &sa, ChainSum &sb, int valid
       uint64_t ra[4], rb[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        ra[i] = shoup_lazy_qf(xv[j + i], av[j + i], p, delta);
-        rb[i] = shoup_lazy_qf(xv[j + i], bv[j + i], p, delta);
+        ra[i] = shoup_lazy_asm(xv[j + i], av[j + i], shoup_quot(av[j + i], p, delta), delta);
+        rb[i] = shoup_lazy_asm(xv[j + i], bv[j + i], shoup_quot(bv[j + i], p, delta), delta);
       }
       zsum_group4(sa, ra[0], ra[1], ra[2], ra[3], pv);
       zsum_group4(sb, rb[0], rb[1], rb[2], rb[3], pv);
@@ -561,8 +562,8 @@ __device__ __forceinline__ void stream_sum(ChainSum &sa, ChainSum &sb, int valid
 #pragma unroll
     for (int j = 0; j < U; j++) {
       if (j < valid) {
-        zsum_step(sa, shoup_lazy_qf(xv[j], av[j], p, delta), p);
-        zsum_step(sb, shoup_lazy_qf(xv[j], bv[j], p, delta), p);
+        zsum_step(sa, shoup_lazy_asm(xv[j], av[j], shoup_quot(av[j], p, delta), delta), p);
+        zsum_step(sb, shoup_lazy_asm(xv[j], bv[j], shoup_quot(bv[j], p, delta), delta), p);
       }
     }
   }
