@@ -280,6 +280,8 @@ def run_single(ctx, args, cfg, gold):
     achieved = kern_bytes / kern_s / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["name"]),
+                "traffic_from": "profiles/mac_traffic.json (calibrated FETCH_SIZE + WRITE_SIZE of this "
+                                "kernel on this workload; not measured in this run)",
                 "kernel": kern_name, "kernel_ms": main_ms / max(1, main_launches),
                 "launches_per_reply": main_launches / args.steps, "alg_bytes": kern_bytes,
                 "reply_mac_ms": mac_ms / mac_steps,
@@ -1275,9 +1277,11 @@ def bench_metric(args):
         log("single-query configuration (65536 x 4 KiB) for the HBM-bound kernel")
         sargs = argparse.Namespace(**vars(args))
         sargs.steps, sargs.warmup = max(5, min(args.steps, 20)), 5
+        sargs.host_io = True  # the host-buffer rate (PCIe included) rides along as a key of its own
         single = run_single(ctx, sargs, C2_SINGLE, gold.get("c2_single"))
         single = {k: single[k] for k in ("value", "ms_per_step", "bit_exact", "latency_ms", "import_gbs",
-                                         "roofline", "cpu_baseline", "cpu_all_cores", "config") if k in single}
+                                         "roofline", "cpu_baseline", "cpu_all_cores", "host_io", "config")
+                  if k in single}
     ctx.close()
     plain_lat = np.array(plain_lat)
     out = {

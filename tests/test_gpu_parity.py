@@ -218,3 +218,36 @@ def test_extreme_ring_sizes_match_oracle(gpu_ctx, desc, n_elems, elem_bytes, dim
             data[index * elem_bytes:(index + 1) * elem_bytes].tobytes()
     finally:
         db.free()
+
+
+def test_host_buffers_over_64mib_stage_through_pinned_chunks(gpu_ctx):
+    """A query above the 64 MiB staging chunk (70 MiB: 1100 elements of LWE:80:2048:120)
+    reaches the device through the two pinned chunks of the host-buffer entry points: the
+    host-buffer reply equals the device-buffer reply, a staged upload reads back
+    unchanged, and the reply decrypts to the chosen element."""
+    import xpir_amd
+    n, L, index = 1100, 64, 517
+    orc = Oracle(C2, n)
+    prm = xpir_amd.params(C2, n)
+    data = orc.rand_bytes(91, 0, n * L)
+    db = gpu_ctx.import_db(prm, data, n, L, [n])
+    sk = xpir_amd.keygen(prm, 92)
+    skd = gpu_ctx.alloc(sk.nbytes).upload(sk)
+    qd = gpu_ctx.alloc(db.query_words * 8)
+    rd = gpu_ctx.alloc(db.reply_words * 8)
+    try:
+        gpu_ctx.query_gen_dev(prm, skd, [n], index, 93, qd)
+        q = qd.download()
+        assert q.nbytes > 64 << 20
+        gpu_ctx.reply_dev(db, qd, rd)
+        want = rd.download()
+        assert np.array_equal(gpu_ctx.reply(db, q), want)
+        rev = np.ascontiguousarray(q[::-1])
+        q2 = gpu_ctx.alloc(rev.nbytes).upload(rev)
+        assert np.array_equal(q2.download(), rev)
+        q2.free()
+        assert gpu_ctx.extract_reply_dev(prm, skd, rd.ptr, [n], L) == data[index * L:(index + 1) * L].tobytes()
+    finally:
+        for b in (skd, qd, rd):
+            b.free()
+        db.free()

@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <atomic>
 #include <mutex>
+#include <thread>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -173,6 +175,51 @@ int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes) {
   ctx->epoch++;
   HIP_OK(hipMalloc((void **)&ctx->level_buf[which], bytes), "level buffer");
   ctx->level_bytes[which] = bytes;
+  return XPG_OK;
+}
+
+// Host memcpy split over threads (a pageable source has to pass through the CPU; one
+// thread copies at a fraction of the host's memory bandwidth).
+static void copy_host_threads(void *dst, const void *src, size_t bytes) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t T = std::min<size_t>(hw ? std::min(hw, 8u) : 4, std::max<size_t>(1, bytes >> 22));  // >= 4 MiB each
+  if (T <= 1) {
+    memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t per = (bytes + T - 1) / T;
+  std::vector<std::thread> th;
+  for (size_t i = 0; i < T && i * per < bytes; i++) {
+    const size_t o = i * per, len = std::min(per, bytes - o);
+    th.emplace_back([=] { memcpy((char *)dst + o, (const char *)src + o, len); });
+  }
+  for (auto &t : th) t.join();
+}
+
+// Host -> device copy of a pageable buffer: up to 64 MiB as one hipMemcpyAsync, larger ones
+// in 64 MiB chunks through two pinned host chunks, host threads filling one while the DMA
+// engine drains the other.  The source may be reused on return (every byte has been read
+// into a pinned chunk or by the driver's own staging).
+static constexpr size_t kPinChunk = 64ull << 20;
+
+int copy_h2d_staged(xpg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (bytes <= kPinChunk) {
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D copy");
+    return XPG_OK;
+  }
+  for (int b = 0; b < 2; b++) {
+    if (!ctx->pin[b]) HIP_OK(hipHostMalloc(&ctx->pin[b], kPinChunk, hipHostMallocDefault), "pinned staging chunk");
+    if (!ctx->pin_ev[b]) HIP_OK(hipEventCreateWithFlags(&ctx->pin_ev[b], hipEventDisableTiming), "staging event");
+  }
+  size_t i = 0;
+  for (size_t off = 0; off < bytes; off += kPinChunk, i++) {
+    const int b = (int)(i & 1);
+    const size_t len = std::min(kPinChunk, bytes - off);
+    HIP_OK(hipEventSynchronize(ctx->pin_ev[b]), "staging wait");  // its last copy has read it
+    copy_host_threads(ctx->pin[b], (const char *)src + off, len);
+    HIP_OK(hipMemcpyAsync((char *)dst + off, ctx->pin[b], len, hipMemcpyHostToDevice, ctx->stream), "H2D copy");
+    HIP_OK(hipEventRecord(ctx->pin_ev[b], ctx->stream), "staging event");
+  }
   return XPG_OK;
 }
 
@@ -381,6 +428,10 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->batch_buf) (void)hipFree(ctx->batch_buf);
   if (ctx->io_buf) (void)hipFree(ctx->io_buf);
+  for (int b = 0; b < 2; b++) {
+    if (ctx->pin[b]) (void)hipHostFree(ctx->pin[b]);
+    if (ctx->pin_ev[b]) (void)hipEventDestroy(ctx->pin_ev[b]);
+  }
   if (ctx->dec_buf) (void)hipFree(ctx->dec_buf);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   graph_cache_clear(ctx, 0);
@@ -453,7 +504,8 @@ int xpg_dev_free(xpg_ctx *ctx, void *ptr) {
 
 int xpg_h2d(xpg_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
   if (!ctx) return XPG_EINVAL;
-  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D copy");
+  const int rc = copy_h2d_staged(ctx, dst, src, bytes);
+  if (rc) return rc;
   HIP_OK(hipStreamSynchronize(ctx->stream), "stream sync");
   return XPG_OK;
 }
@@ -598,9 +650,7 @@ int xpg_db_import(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems, uin
   uint8_t *tmp = nullptr;
   (void)hipSetDevice(ctx->device);
   HIP_OK(hipMalloc((void **)&tmp, bytes + 16), "staging allocation");
-  int rc = XPG_OK;
-  if (hipMemcpyAsync(tmp, elems, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-    rc = set_err(ctx, XPG_EDEVICE, "H2D copy");
+  int rc = copy_h2d_staged(ctx, tmp, elems, bytes);
   if (!rc) rc = xpg_db_import_dev(ctx, prm, tmp, n_elems, elem_bytes, alpha, d, dims, out);
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(tmp);
@@ -614,8 +664,7 @@ int xpg_db_import_ntt_words(xpg_ctx *ctx, const xpg_params *prm, const uint64_t 
   if (rc) return rc;
   xpg_db *db = *out;
   const uint64_t per = (uint64_t)P * prm->m * prm->n;
-  if (hipMemcpyAsync(db->words, words, units * per * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-    rc = set_err(ctx, XPG_EDEVICE, "H2D copy");
+  rc = copy_h2d_staged(ctx, db->words, words, units * per * 8);
   if (!rc) rc = launch_fill_zero(ctx, db->words + units * per, (db->units - units) * per);
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = set_err(ctx, XPG_EDEVICE, "sync");
   if (rc) {
@@ -1065,8 +1114,7 @@ int xpg_reply(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query, uint64_t qu
   int rc = ensure_io_buf(ctx, (qw + rw) * 8);
   if (rc) return rc;
   uint64_t *qd = ctx->io_buf, *rd = ctx->io_buf + qw;
-  if (hipMemcpyAsync(qd, query, qw * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-    rc = set_err(ctx, XPG_EDEVICE, "H2D copy");
+  rc = copy_h2d_staged(ctx, qd, query, qw * 8);
   if (!rc) rc = xpg_reply_dev(ctx, db, qd, rd);
   if (!rc && hipMemcpyAsync(reply, rd, rw * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
     rc = set_err(ctx, XPG_EDEVICE, "D2H copy");

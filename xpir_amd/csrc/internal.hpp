@@ -99,6 +99,10 @@ struct xpg_ctx {
   bool noalloc = false;
   uint64_t *io_buf = nullptr;     // device copies of a host query and its reply (xpg_reply)
   size_t io_bytes = 0;
+  // two pinned host chunks through which large host buffers reach the device (host threads
+  // fill one while the DMA engine drains the other), each with the event of its last copy
+  void *pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
   uint64_t *dec_buf = nullptr;    // residues and lifted coefficients of xpg_decrypt_dev
   size_t dec_bytes = 0;
   // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 3 = streaming block summaries
@@ -139,6 +143,7 @@ int get_scratch(xpg_ctx *ctx, size_t bytes, void **p);
 int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
 int ensure_io_buf(xpg_ctx *ctx, size_t bytes);
+int copy_h2d_staged(xpg_ctx *ctx, void *dst, const void *src, size_t bytes);
 int ensure_dec_buf(xpg_ctx *ctx, size_t bytes);
 void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial);  // 0 = every entry
 ModSet modset_of(const xpg_params *prm);
