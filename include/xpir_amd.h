@@ -234,7 +234,7 @@ int xpg_timing_read_batched(xpg_ctx *ctx, double *ms, uint64_t *launches, uint64
  * kind: 0 = one query's level, 1 = a batch (qn = batch size rounded up to a power of
  * two).  kernel: 1 = one chain per lane, 3 = streaming (a = chunk, b = double-buffered, c = elements per block,
  * 0 = one pass), 4 = register blocks (a rows x b queries per lane; c = 2: each wave
- * loads its query words, 3: a workgroup's waves share them through LDS, 0: by shape;
+ * loads its query words, 3: a workgroup's four waves share them through LDS, 0: by shape;
  * built: 1x1, 1x2, 1x4, 4x1, 4x2, 4x4, 8x1, 8x2, 16x1 with c = 2 and 4x1, 4x2, 4x4, 8x1,
  * 8x2 with c = 3); a kind-1 entry
  * with kernel 0 records that the batch runs faster as single-query launches (the batch

@@ -75,3 +75,42 @@ def test_batch_equals_plain_and_oracle(gpu_ctx, specs, budget):
     for db in dbs:
         db.free()
     gpu_ctx.set_option("group_budget", 0)
+
+
+SHAPES = [(2, g, q) for g, q in ((1, 1), (1, 2), (1, 4), (4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (16, 1))] + \
+         [(3, g, q) for g, q in ((4, 1), (4, 2), (4, 4), (8, 1), (8, 2))]
+
+
+@pytest.mark.parametrize("qn", [8, 7])  # 7: a ragged last query block takes a launch of its own
+def test_every_register_block_shape_equals_plain(gpu_ctx, qn):
+    """Each built (family, rows, queries) shape of the level-0 batch kernels, forced
+    through a tuning entry, gives every query's plain reply (family 3 stages the query
+    words through LDS for a workgroup's four waves)."""
+    import xpir_amd
+    n, L, dims = 600, 256, [30, 20]
+    orc = Oracle(C2, dims[0])
+    prm = xpir_amd.params(C2, dims[0])
+    db, _ = _setup_db(gpu_ctx, orc, prm, n, L, dims, 77)
+    sk = orc.keygen(78)
+    rng = np.random.default_rng(79)
+    reqs, plain = [], []
+    try:
+        for qi in range(qn):
+            q = np.concatenate(orc.make_query(sk, dims, int(rng.integers(0, n)), 800 + qi))
+            reqs.append((db, gpu_ctx.alloc(q.nbytes).upload(q), gpu_ctx.alloc(db.reply_words * 8)))
+            plain.append(gpu_ctx.reply(db, q))
+        for fam, gr, qr in SHAPES:
+            gpu_ctx.tune_clear()
+            gpu_ctx.tune_set([{"kind": 1, "n": prm.n, "m": prm.m, "P": db.polys, "n_l": dims[0], "groups": dims[1],
+                               "qn": 8, "kernel": 4, "a": gr, "b": qr, "c": fam, "ms": 1.0, "default_ms": 1.0}])
+            for _, _, rd in reqs:
+                rd.upload(np.zeros(db.reply_words, dtype=np.uint64))
+            gpu_ctx.reply_batch_dev(reqs)
+            for k, (_, _, rd) in enumerate(reqs):
+                assert np.array_equal(rd.download(), plain[k]), (fam, gr, qr, k)
+    finally:
+        gpu_ctx.tune_clear()
+        for _, qd, rd in reqs:
+            qd.free()
+            rd.free()
+        db.free()

@@ -222,6 +222,7 @@ uint64_t level_scratch_bytes(const xpg_ctx *ctx, const xpg_params *prm, uint64_t
 LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P);
 bool batch_shape_built(int GR, int QR);   // register-block shapes compiled in (per-wave query loads)
 bool batch3_shape_built(int GR, int QR);  // ... and with the query words shared through LDS
+bool batch_family_built(int family, int GR, int QR);  // family 2 or 3 (anything else: none)
 // explicit options (a kernel or a tile shape asked for) win over the tuning table
 bool options_pinned(const xpg_ctx *ctx);
 // register blocks the batched MAC picks by shape for the ones given as 0

@@ -132,6 +132,13 @@ struct Batch2Args {
   uint32_t P, n, m, sblocks, qn, qblocks;
 };
 
+// Word at byte offset `lane_off` from a wave-uniform base: written this way the load
+// takes the uniform base in SGPRs and the lane's 32-bit offset in a VGPR, so no per-load
+// VALU address arithmetic is needed.
+__device__ __forceinline__ const uint64_t *lane_word(const uint64_t *base, uint32_t lane_off) {
+  return reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(base) + lane_off);
+}
+
 // Block b -> (colgroup cg, query block qb).  A colgroup is (row block, column), the column
 // (c, sb) fastest; the 8 XCDs take blocks round-robin (b & 7) and the query blocks of one
 // colgroup run back to back on one XCD, so its DB rows come from HBM once and from that
@@ -159,11 +166,13 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
   const uint64_t cs = (uint64_t)c * a.n + sb * 64 + lane;
   const uint64_t p = a.ms.p[c];
   const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn, gstride = a.n_l * xstride;
-  const uint64_t *x = a.db + (g0 * a.n_l * a.P + k) * mn + cs;
+  // wave-uniform row and query bases; the lane adds its byte offset cs8 at each load
+  const uint64_t *x = a.db + (g0 * a.n_l * a.P + k) * mn;
+  const uint32_t cs8 = (uint32_t)cs * 8;
   const uint64_t *qa[QR];
   const uint32_t q0 = qb * QR;
 #pragma unroll
-  for (int i = 0; i < QR; i++) qa[i] = a.q[min(q0 + i, a.qn - 1)] + cs;
+  for (int i = 0; i < QR; i++) qa[i] = a.q[min(q0 + i, a.qn - 1)];
   const uint32_t delta = a.ms.delta[c];
   const StepConsts kc = step_consts(p, delta);
   int gv = (int)min<uint64_t>(GR, a.groups - g0);  // groups of this block that exist
@@ -175,20 +184,21 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
   // UE elements per iteration: every load of the chunk is issued before its arithmetic,
   // which then runs in element order (the chains' order)
   constexpr int UE = GR * QR >= 16 ? 1 : 4;
-  uint64_t e = 0;
+  const uint32_t nl = (uint32_t)a.n_l;  // element indices in 32 bits: scalar arithmetic
+  uint32_t e = 0;
   if (gv == GR) {
     // every group of the block exists: whole chunks without per-load guards
-    for (; e + UE <= a.n_l; e += UE) {
+    for (; e + UE <= nl; e += UE) {
       uint64_t xv[UE][GR], ya[UE][QR], yb[UE][QR];
 #pragma unroll
       for (int u = 0; u < UE; u++) {
-        const uint64_t eo = (e + u) * xstride, o = (e + u) * qstride;
+        const uint64_t eo = (uint64_t)(e + u) * xstride, o = (uint64_t)(e + u) * qstride;
 #pragma unroll
-        for (int g = 0; g < GR; g++) xv[u][g] = __builtin_nontemporal_load(x + g * gstride + eo);
+        for (int g = 0; g < GR; g++) xv[u][g] = __builtin_nontemporal_load(lane_word(x + g * gstride + eo, cs8));
 #pragma unroll
         for (int i = 0; i < QR; i++) {
-          ya[u][i] = qa[i][o];
-          yb[u][i] = qa[i][o + mn];
+          ya[u][i] = *lane_word(qa[i] + o, cs8);
+          yb[u][i] = *lane_word(qa[i] + o + mn, cs8);
         }
       }
       // keep the whole chunk's loads ahead of its arithmetic (the scheduler would
@@ -209,19 +219,20 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
     }
   }
   // the tail chunk, and blocks whose last groups lie past the level
-  for (; e < a.n_l; e += UE) {
-    const int ve = (int)min<uint64_t>(UE, a.n_l - e);
+  for (; e < nl; e += UE) {
+    const int ve = (int)min<uint32_t>(UE, nl - e);
     uint64_t xv[UE][GR], ya[UE][QR], yb[UE][QR];
 #pragma unroll
     for (int u = 0; u < UE; u++) {
       const bool in = u < ve;
-      const uint64_t eo = (e + u) * xstride, o = (e + u) * qstride;
+      const uint64_t eo = (uint64_t)(e + u) * xstride, o = (uint64_t)(e + u) * qstride;
 #pragma unroll
-      for (int g = 0; g < GR; g++) xv[u][g] = in && g < gv ? __builtin_nontemporal_load(x + g * gstride + eo) : 0;
+      for (int g = 0; g < GR; g++)
+        xv[u][g] = in && g < gv ? __builtin_nontemporal_load(lane_word(x + g * gstride + eo, cs8)) : 0;
 #pragma unroll
       for (int i = 0; i < QR; i++) {
-        ya[u][i] = in ? qa[i][o] : 0;
-        yb[u][i] = in ? qa[i][o + mn] : 0;
+        ya[u][i] = in ? *lane_word(qa[i] + o, cs8) : 0;
+        yb[u][i] = in ? *lane_word(qa[i] + o + mn, cs8) : 0;
       }
     }
 #pragma unroll
@@ -255,9 +266,9 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
 }
 
 // ------------------------------------------------------------------------------------
-// Register blocks with the query words shared through LDS.  A workgroup of kB3Waves
-// waves takes one 64-slot column (modulus c, slots sb*64..+63), one block of QR queries
-// and kB3Waves*GR rows, a row being one (group g, poly k) of the level (row r = g*P + k);
+// Register blocks with the query words shared through LDS.  A workgroup of W = 4 waves
+// takes one 64-slot column (modulus c, slots sb*64..+63), one block of QR queries and
+// W*GR rows, a row being one (group g, poly k) of the level (row r = g*P + k);
 // lane = slot.  Every row of a column reads the same query words, so the workgroup loads
 // each query word once per chunk of kB3U elements, derives its Shoup quotient once, and
 // stages (y, y') in LDS for its waves; each wave then runs the a and b chains of its GR
@@ -265,18 +276,17 @@ __global__ void __launch_bounds__(64) mac_batch2_kernel(Batch2Args a) {
 // (query and DB words) in flight.  Two LDS buffers alternate between chunks; one barrier
 // per chunk.  Same words as every other MAC kernel: each chain still takes its elements in
 // order through the same step.
-constexpr int kB3Waves = 4;  // waves per workgroup
-constexpr int kB3U = 2;      // elements per staged chunk
+constexpr int kB3U = 2;  // elements per staged chunk
 
 struct QPair {
   uint64_t y, yq;
 };
 
-template <int GR, int QR>
-__global__ void __launch_bounds__(64 * kB3Waves) mac_batch3_kernel(Batch2Args a) {
+template <int GR, int QR, int W>
+__global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
   constexpr int NQ = kB3U * QR * 2;  // staged query rows (element, query, a|b) per chunk
-  constexpr int NS = NQ / kB3Waves;  // rows staged per wave
-  static_assert(NQ % kB3Waves == 0, "staged rows split evenly over the waves");
+  constexpr int NS = NQ / W;         // rows staged per wave
+  static_assert(NQ % W == 0, "staged rows split evenly over the waves");
   __shared__ QPair lds[2][NQ][64];
   uint64_t cg;
   uint32_t qb;
@@ -295,7 +305,7 @@ __global__ void __launch_bounds__(64 * kB3Waves) mac_batch3_kernel(Batch2Args a)
   const StepConsts kc = step_consts(p, delta);
   const uint64_t xstride = (uint64_t)a.P * mn;  // next element of a row
   const uint64_t qstride = 2 * mn;              // next element of a query
-  const uint64_t r0 = (rb * kB3Waves + w) * GR;
+  const uint64_t r0 = (rb * W + w) * GR;
   const int rv = (int)max<int64_t>(0, min<int64_t>(GR, (int64_t)rows - (int64_t)r0));  // this wave's rows
   // wave-uniform row bases (rows past the level read the last row: their chains are
   // computed and dropped, so no load needs a guard); the lane adds its 32-bit word offset
@@ -305,13 +315,13 @@ __global__ void __launch_bounds__(64 * kB3Waves) mac_batch3_kernel(Batch2Args a)
     const uint64_t r = min<uint64_t>(r0 + g, rows - 1);
     xr[g] = a.db + ((r / a.P) * a.n_l * a.P + (r % a.P)) * mn;
   }
-  // staging: this wave loads the query rows qrow = w + kB3Waves * s of every chunk
+  // staging: this wave loads the query rows qrow = w + W * s of every chunk
   const uint32_t q0 = qb * QR;
   const uint64_t *sq[NS];
   uint32_t su[NS];
 #pragma unroll
   for (int s = 0; s < NS; s++) {
-    const uint32_t qrow = w + kB3Waves * s;
+    const uint32_t qrow = w + W * s;
     const uint32_t u = qrow / (QR * 2), rem = qrow % (QR * 2), i = rem >> 1, ab = rem & 1;
     su[s] = u;
     sq[s] = a.q[min(q0 + i, a.qn - 1)] + ab * mn;
@@ -321,26 +331,28 @@ __global__ void __launch_bounds__(64 * kB3Waves) mac_batch3_kernel(Batch2Args a)
   for (int g = 0; g < GR; g++)
 #pragma unroll
     for (int i = 0; i < QR; i++) acc_a[g][i] = acc_b[g][i] = 0;
-  const uint64_t nfull = a.n_l / kB3U, last = a.n_l - 1;
+  // element indices in 32 bits, so that the clamps and offsets below stay scalar
+  const uint32_t nl = (uint32_t)a.n_l, nfull = nl / kB3U, last = nl - 1;
+  const uint32_t cs8 = cs * 8;  // this lane's byte offset within a row
 
   // loads of chunk ch, element indices clamped to the level (a chunk past the last whole
   // one reads words it never uses)
-  auto load_q = [&](uint64_t ch, uint64_t (&qv)[NS]) {
+  auto load_q = [&](uint32_t ch, uint64_t (&qv)[NS]) {
 #pragma unroll
     for (int s = 0; s < NS; s++)
-      qv[s] = __builtin_nontemporal_load(sq[s] + min<uint64_t>(ch * kB3U + su[s], last) * qstride + cs);
+      qv[s] = __builtin_nontemporal_load(lane_word(sq[s] + (uint64_t)min(ch * kB3U + su[s], last) * qstride, cs8));
   };
-  auto load_x = [&](uint64_t ch, uint64_t (&xv)[kB3U][GR]) {
+  auto load_x = [&](uint32_t ch, uint64_t (&xv)[kB3U][GR]) {
 #pragma unroll
     for (int u = 0; u < kB3U; u++) {
-      const uint64_t eo = min<uint64_t>(ch * kB3U + u, last) * xstride;
+      const uint64_t eo = (uint64_t)min(ch * kB3U + u, last) * xstride;
 #pragma unroll
-      for (int g = 0; g < GR; g++) xv[u][g] = __builtin_nontemporal_load(xr[g] + eo + cs);
+      for (int g = 0; g < GR; g++) xv[u][g] = __builtin_nontemporal_load(lane_word(xr[g] + eo, cs8));
     }
   };
   auto stage = [&](int buf, const uint64_t (&qv)[NS]) {
 #pragma unroll
-    for (int s = 0; s < NS; s++) lds[buf][w + kB3Waves * s][lane] = QPair{qv[s], shoup_quot(qv[s], p, delta)};
+    for (int s = 0; s < NS; s++) lds[buf][w + W * s][lane] = QPair{qv[s], shoup_quot(qv[s], p, delta)};
   };
   auto compute = [&](int buf, const uint64_t (&xv)[kB3U][GR]) {
 #pragma unroll
@@ -361,7 +373,7 @@ __global__ void __launch_bounds__(64 * kB3Waves) mac_batch3_kernel(Batch2Args a)
   load_x(0, x0);
   stage(0, qv);
   __syncthreads();
-  for (uint64_t ch = 0; ch < nfull; ch += 2) {
+  for (uint32_t ch = 0; ch < nfull; ch += 2) {
     // whole chunk ch from buffer 0 and x0 while chunk ch + 1 loads, then chunk ch + 1 from
     // buffer 1 and x1 (when it is whole) while chunk ch + 2 loads
     load_q(ch + 1, qv);
@@ -376,14 +388,14 @@ __global__ void __launch_bounds__(64 * kB3Waves) mac_batch3_kernel(Batch2Args a)
     __syncthreads();
   }
   // the elements after the last whole chunk: each wave loads its own query words
-  for (uint64_t e = nfull * kB3U; e < a.n_l; e++) {
+  for (uint32_t e = nfull * kB3U; e < nl; e++) {
     uint64_t xv[GR];
 #pragma unroll
-    for (int g = 0; g < GR; g++) xv[g] = __builtin_nontemporal_load(xr[g] + e * xstride + cs);
+    for (int g = 0; g < GR; g++) xv[g] = __builtin_nontemporal_load(lane_word(xr[g] + (uint64_t)e * xstride, cs8));
 #pragma unroll
     for (int i = 0; i < QR; i++) {
-      const uint64_t *qi = a.q[min(q0 + i, a.qn - 1)] + e * qstride + cs;
-      const uint64_t ya = qi[0], yb = qi[mn];
+      const uint64_t *qi = a.q[min(q0 + i, a.qn - 1)] + (uint64_t)e * qstride;
+      const uint64_t ya = *lane_word(qi, cs8), yb = *lane_word(qi + mn, cs8);
       const uint64_t yaq = shoup_quot(ya, p, delta), ybq = shoup_quot(yb, p, delta);
 #pragma unroll
       for (int g = 0; g < GR; g++) {
@@ -781,16 +793,23 @@ void batch_default_shape(uint32_t qn, uint64_t groups, int *gr, int *qr) {
   if (!*gr) *gr = groups >= 8 && *qr <= 2 ? 8 : (groups >= 4 ? 4 : 1);
 }
 
-bool batch3_shape_built(int GR, int QR) {
-  const int key = GR * 10 + QR;
-  return key == 41 || key == 42 || key == 44 || key == 81 || key == 82;
-}
-
 bool batch_shape_built(int GR, int QR) {
   const int key = GR * 10 + QR;
   return key == 11 || key == 12 || key == 14 || key == 41 || key == 42 || key == 44 || key == 81 || key == 82 ||
          key == 161;
 }
+
+bool batch_family_built(int family, int GR, int QR) {
+  const int key = GR * 10 + QR;
+  switch (family) {
+    case 2: return batch_shape_built(GR, QR);
+    case 3: return key == 41 || key == 42 || key == 44 || key == 81 || key == 82;
+    default: return false;
+  }
+}
+
+bool batch3_shape_built(int GR, int QR) { return batch_family_built(3, GR, QR); }
+
 
 // Lanes at which one chain per lane already fills the GPU (256 CUs x 32 wavefronts).
 static const uint64_t kDirectLanes = 256ull * 32 * 64;
@@ -899,16 +918,13 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   if (qn == 0 || qn > (uint32_t)kMaxBatch) return set_err(ctx, XPG_EINVAL, "batch of 1..64 queries per launch");
   if (prm->n < 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n >= 64");
   // register blocks: GR rows x QR queries per lane (0 = by shape); family 3 shares the
-  // query words of a workgroup's rows through LDS, 2 loads them per wave (0 = by shape)
+  // query words of a workgroup's four waves through LDS, family 2 loads them per wave
+  // (0 = by shape)
   int QR = qr, GR = gr;
   batch_default_shape(qn, groups, &GR, &QR);
   if (family == 0) family = batch3_shape_built(GR, QR) ? 3 : 2;
   const int key = GR * 10 + QR;
-  if (family == 3) {
-    if (!batch3_shape_built(GR, QR)) return set_err(ctx, XPG_EINVAL, "register block shape not built");
-  } else if (family != 2 || !batch_shape_built(GR, QR)) {
-    return set_err(ctx, XPG_EINVAL, "register block shape not built");
-  }
+  if (!batch_family_built(family, GR, QR)) return set_err(ctx, XPG_EINVAL, "register block shape not built");
   // a last query block of fewer than QR queries would compute padding queries: those r
   // queries take a launch of their own, with the shape the fixed rules give r queries
   const uint32_t rest = qn % (uint32_t)QR;
@@ -932,8 +948,9 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   b2.sblocks = prm->n / 64;
   b2.qn = qn;
   b2.qblocks = (qn + QR - 1) / QR;
-  // family 2: a block = GR groups x P polys of one column; family 3: kB3Waves*GR rows
-  b2.colgroups = family == 3 ? ((groups * P + kB3Waves * GR - 1) / (kB3Waves * GR)) * prm->m * b2.sblocks
+  // family 2: a block = GR groups x P polys of one column; family 3: W*GR rows
+  constexpr int W = 4;
+  b2.colgroups = family == 3 ? ((groups * P + W * GR - 1) / (W * GR)) * prm->m * b2.sblocks
                              : ((groups + GR - 1) / GR) * P * prm->m * b2.sblocks;
   const uint64_t blocks = ((b2.colgroups + 7) / 8) * 8 * b2.qblocks;
   if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "batched MAC grid too large");
@@ -942,14 +959,14 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   hipEvent_t k0 = nullptr, k1 = nullptr;
   timing_begin(ctx, &k0, &k1, 2);
   if (ctx->timing && !ctx->ws) ctx->batch_steps += (uint64_t)qn * groups * P * prm->m * prm->n * 2 * n_l;
-  const dim3 wg3(64 * kB3Waves);
+  const dim3 wg3(64 * W), grid((uint32_t)blocks);
   if (family == 3) {
     switch (key) {
-      case 82: hipLaunchKernelGGL((mac_batch3_kernel<8, 2>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
-      case 81: hipLaunchKernelGGL((mac_batch3_kernel<8, 1>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
-      case 44: hipLaunchKernelGGL((mac_batch3_kernel<4, 4>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
-      case 42: hipLaunchKernelGGL((mac_batch3_kernel<4, 2>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
-      default: hipLaunchKernelGGL((mac_batch3_kernel<4, 1>), dim3((uint32_t)blocks), wg3, 0, ctx->stream, b2); break;
+      case 82: hipLaunchKernelGGL((mac_batch3_kernel<8, 2, 4>), grid, wg3, 0, ctx->stream, b2); break;
+      case 81: hipLaunchKernelGGL((mac_batch3_kernel<8, 1, 4>), grid, wg3, 0, ctx->stream, b2); break;
+      case 44: hipLaunchKernelGGL((mac_batch3_kernel<4, 4, 4>), grid, wg3, 0, ctx->stream, b2); break;
+      case 42: hipLaunchKernelGGL((mac_batch3_kernel<4, 2, 4>), grid, wg3, 0, ctx->stream, b2); break;
+      default: hipLaunchKernelGGL((mac_batch3_kernel<4, 1, 4>), grid, wg3, 0, ctx->stream, b2); break;
     }
   } else switch (key) {
     case 161: hipLaunchKernelGGL((mac_batch2_kernel<16, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;

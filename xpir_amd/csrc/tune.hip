@@ -187,8 +187,8 @@ bool entry_valid(const xpg_tune_entry &e) {
     return false;
   if (e.kind == 1)
     return e.qn >= 1 && e.qn <= 64 &&
-           (e.kernel == 0 || (e.kernel == 4 && (e.c == 0 || e.c == 2 || e.c == 3) &&
-                              (e.c == 3 ? batch3_shape_built(e.a, e.b) : batch_shape_built(e.a, e.b))));
+           (e.kernel == 0 || (e.kernel == 4 && (e.c == 0 ? batch_shape_built(e.a, e.b)
+                                                          : batch_family_built(e.c, e.a, e.b))));
   if (e.kind != 0 || e.qn != 0) return false;
   switch (e.kernel) {
     case 1: return true;
@@ -439,7 +439,7 @@ int xpg_autotune(xpg_ctx *ctx, const xpg_db *db, uint32_t max_batch, xpg_tune_en
     const int def_fam = batch3_shape_built(dg, dq) ? 3 : 2;
     for (const auto &s : built)
       for (int fam : {2, 3}) {
-        const bool ok = fam == 3 ? batch3_shape_built(s[0], s[1]) : batch_shape_built(s[0], s[1]);
+        const bool ok = batch_family_built(fam, s[0], s[1]);
         if (ok && (s[0] != dg || s[1] != dq || fam != def_fam) && s[1] <= (int)qn &&
             (s[0] == 1 || groups0 * P0 >= (uint64_t)s[0] / 2))
           shapes.push_back(Shape{s[0], s[1], fam});
