@@ -66,17 +66,6 @@ __host__ __device__ __forceinline__ uint64_t shoup_lazy_qf(uint64_t x, uint64_t 
   return shoup_lazy_sf(x, y, shoup_quot(y, p, delta), delta);
 }
 
-// acc + shoup_lazy_qf(x, y, p, delta) (mod 2^64), the addition written on 32-bit halves
-// so that it takes one add and one add-with-carry.
-__device__ __forceinline__ uint64_t add_shoup_lazy_qf(uint64_t acc, uint64_t x, uint64_t y, uint64_t p,
-                                                      uint32_t delta) {
-  const uint64_t r = shoup_lazy_qf(x, y, p, delta);
-  const uint32_t rl = (uint32_t)r, rh = (uint32_t)(r >> 32);
-  const uint32_t ul = (uint32_t)acc + rl;
-  const uint32_t uh = (uint32_t)(acc >> 32) + rh + (ul < rl ? 1u : 0u);
-  return ((uint64_t)uh << 32) | ul;
-}
-
 // One reply-chain step (crypto/NFLlib.hpp:303-321): acc += lazy product; -p if >= p.
 __device__ __forceinline__ uint64_t mac_step(uint64_t acc, uint64_t x, uint64_t y, uint64_t yq,
                                              uint64_t p, uint32_t delta) {
@@ -220,10 +209,4 @@ __host__ __device__ __forceinline__ uint64_t rand_word(uint64_t seed, uint64_t k
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
   return z ^ (z >> 31);
-}
-
-// reply-chain step with the query quotient derived on the fly
-__device__ __forceinline__ uint64_t mac_step_qf(uint64_t acc, uint64_t x, uint64_t y, uint64_t p, uint32_t delta) {
-  acc += shoup_lazy_qf(x, y, p, delta);
-  return acc >= p ? acc - p : acc;
 }
