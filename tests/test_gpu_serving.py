@@ -171,3 +171,77 @@ def test_admission_bounds_device_memory(gpu_ctx):
         bq.free()
         big.free()
         small.free()
+
+
+def test_concurrent_submitters(gpu_ctx):
+    """Four host threads submit at once (the serving API takes submits from any thread)
+    while the main thread polls and releases: every reply equals its query answered
+    alone, every ticket is unique, and nothing is lost."""
+    import threading
+
+    import xpir_amd
+    specs = [(300, 512, [30, 10]), (64, 2048, [64])]
+    dbs = []
+    for si, (n, L, dims) in enumerate(specs):
+        orc = Oracle(C2, dims[0])
+        prm = xpir_amd.params(C2, dims[0])
+        db = gpu_ctx.import_db(prm, orc.rand_bytes(900 + si, 0, n * L), n, L, dims)
+        sk = orc.keygen(910 + si)
+        pool = []
+        for qi in range(3):
+            q = np.concatenate(orc.make_query(sk, dims, 17 * qi + si, 920 + 10 * si + qi))
+            pool.append((gpu_ctx.alloc(q.nbytes).upload(q), gpu_ctx.reply(db, q)))
+        dbs.append((db, pool))
+    srv = xpir_amd.Server(0, reply_pool_bytes=64 << 20, workers=2, max_batch=16)
+    expected, lock, errors = {}, threading.Lock(), []
+    per_thread = 60
+
+    def submitter(tid):
+        try:
+            rng = np.random.default_rng(tid)
+            k = 0
+            while k < per_thread:
+                db, pool = dbs[int(rng.integers(0, 2))]
+                qd, alone = pool[int(rng.integers(0, 3))]
+                t = srv.submit(db, qd)
+                if t is None:  # reply slots full: the main thread releases, try again
+                    time.sleep(0.0005)
+                    continue
+                with lock:
+                    assert t not in expected
+                    expected[t] = alone
+                k += 1
+        except Exception as e:  # surfaced by the main thread
+            errors.append(e)
+
+    try:
+        threads = [threading.Thread(target=submitter, args=(i,)) for i in range(4)]
+        for th in threads:
+            th.start()
+        done, t0, early = 0, time.perf_counter(), []
+        while done < 4 * per_thread:
+            # a completion can arrive before its submitter has recorded the ticket: keep it
+            # until the ticket appears
+            for c in early + srv.poll(timeout_us=1000):
+                assert c.status == 0
+                with lock:
+                    alone = expected.pop(c.ticket, None)
+                if alone is None:
+                    early.append(c)
+                    continue
+                if c in early:
+                    early.remove(c)
+                assert np.array_equal(gpu_ctx.download_ptr(c.reply_dev, c.reply_words), alone)
+                srv.release(c.ticket)
+                done += 1
+            assert not errors, errors
+            assert time.perf_counter() - t0 < 60, f"{done} of {4 * per_thread} served"
+        for th in threads:
+            th.join()
+        assert not expected and srv.stats()["batched_requests"] == 4 * per_thread
+    finally:
+        srv.close()
+        for db, pool in dbs:
+            for qd, _ in pool:
+                qd.free()
+            db.free()
