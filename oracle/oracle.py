@@ -6,6 +6,9 @@ NFLlib compiled from its sources by oracle/Makefile (absent where the reference
 tree is not available; tests that need it skip).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this.
+
+Parity unpinned: query generation (``Oracle.make_query`` over xo_encrypt_bit), which
+is checked only by decryption round trips; see the header of xpir_oracle.c.
 """
 from __future__ import annotations
 

@@ -2,6 +2,13 @@
  * xpir_oracle.c -- CPU restatement of XPIR's NFLLWE reply path (TEST INFRASTRUCTURE).
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this file.
  * Declarations and the reference lines each function follows are in xpir_oracle.h.
+ *
+ * Parity unpinned: xo_encrypt_bit (query generation).  The reference's NFLLWE::enc needs
+ * Boost and keys its PRNG from /dev/urandom, so it cannot produce comparison bytes; the
+ * seeded query streams are this project's own inputs, checked only by decryption round
+ * trips (decrypt(encrypt(bit)) == bit, decrypt(reply) == the chosen element).  Every
+ * other function is pinned against the reference's own code (oracle/_ref), its golden
+ * vectors, or a closed form (tests/test_oracle.py).
  */
 #include "xpir_oracle.h"
 #include <math.h>
