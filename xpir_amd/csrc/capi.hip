@@ -199,7 +199,7 @@ static void copy_host_threads(void *dst, const void *src, size_t bytes) {
 // Host -> device copy of a pageable buffer: up to 64 MiB as one hipMemcpyAsync, larger ones
 // in 64 MiB chunks through two pinned host chunks, host threads filling one while the DMA
 // engine drains the other.  The source may be reused on return (every byte has been read
-// into a pinned chunk or by the driver's own staging).
+// into a pinned chunk or by the HIP runtime's own staging).
 static constexpr size_t kPinChunk = 64ull << 20;
 
 int copy_h2d_staged(xpg_ctx *ctx, void *dst, const void *src, size_t bytes) {
