@@ -81,7 +81,8 @@ SHAPES = [(2, g, q) for g, q in ((1, 1), (1, 2), (1, 4), (4, 1), (4, 2), (4, 4),
          [(3, g, q) for g, q in ((4, 1), (4, 2), (4, 4), (8, 1), (8, 2))]
 
 
-@pytest.mark.parametrize("qn", [8, 7])  # 7: a ragged last query block takes a launch of its own
+# 7, 6, 5: a ragged last query block of 3, 2, 1 queries takes a launch of its own
+@pytest.mark.parametrize("qn", [8, 7, 6, 5])
 def test_every_register_block_shape_equals_plain(gpu_ctx, qn):
     """Each built (family, rows, queries) shape of the level-0 batch kernels, forced
     through a tuning entry, gives every query's plain reply (family 3 stages the query
