@@ -142,6 +142,9 @@ def main():
                     help="skip the MAC autotuner (fixed kernel rules); it runs before any timed region")
     ap.add_argument("--tune-file", help="install this saved tuning table instead of tuning when it exists; "
                                         "else tune and save the table there")
+    ap.add_argument("--batch-fuse", type=int, default=1, choices=(0, 1),
+                    help="batched passes: level 0 of every database in fused persistent launches (1) or one "
+                         "launch per database chunk (0)")
     ap.add_argument("--host-io", action="store_true",
                     help="single/plain: also time replies from a host query to a host reply (PCIe included)")
     args = ap.parse_args()
@@ -1019,6 +1022,7 @@ def bench_batched_pass(args):
         ctx.tune_load(args.tune_file)
     order = np.random.default_rng(43).permutation(len(reqs))
     batch = [reqs[k] for k in order]
+    ctx.set_option("batch_fuse", args.batch_fuse)
     ctx.timing(2)
     ctx.timing_read_batched(reset=True)
     passes = max(1, args.steps)
@@ -1026,7 +1030,8 @@ def bench_batched_pass(args):
         ctx.reply_batch_dev(batch)
     ctx.sync()
     ms, launches, steps = ctx.timing_read_batched(reset=True)
-    print(json.dumps({"workload": "batched-pass", "passes": passes, "batch_kernel_ms_per_pass": ms / passes,
+    print(json.dumps({"workload": "batched-pass", "batch_fuse": args.batch_fuse, "passes": passes,
+                      "batch_kernel_ms_per_pass": ms / passes,
                       "batch_kernel_launches_per_pass": launches / passes,
                       "exact_steps_per_pass": steps / passes, "queries": len(reqs)}), flush=True)
     ctx.close()
@@ -1203,6 +1208,7 @@ def bench_metric(args):
         plain_lat.append((time.perf_counter() - t0) * 1e3)
     log(f"plain path: {plain_s:.3f} s for {len(reqs)} queries -> {total_raw / plain_s / 1e9:.1f} GB/s")
 
+    ctx.set_option("batch_fuse", args.batch_fuse)
     for _ in range(args.warmup):
         ctx.reply_batch_dev(batch)
     ctx.sync()
@@ -1221,7 +1227,27 @@ def bench_metric(args):
     pass_s = total / args.steps
     value = total_raw / pass_s / 1e9
     log(f"{args.steps} batched passes: {pass_s * 1e3:.1f} ms per pass -> {value:.1f} GB/s "
-        f"({plain_s / pass_s:.2f}x the plain path)")
+        f"({plain_s / pass_s:.2f}x the plain path), {k_launches / args.steps:.0f} level-0 launches per pass")
+    # the other level-0 launch structure beside it, on a few passes (same replies)
+    other = 1 - args.batch_fuse
+    ctx.set_option("batch_fuse", other)
+    ctx.reply_batch_dev(batch)
+    ctx.timing(2)
+    ctx.timing_read_batched(reset=True)
+    ctx.sync()
+    n_other = 3
+    t0 = time.perf_counter()
+    for _ in range(n_other):
+        ctx.reply_batch_dev(batch)
+    ctx.sync()
+    other_s = (time.perf_counter() - t0) / n_other
+    o_ms, o_launches, _ = ctx.timing_read_batched(reset=True)
+    ctx.timing(False)
+    ctx.set_option("batch_fuse", args.batch_fuse)
+    other_launches = {"batch_fuse": other, "ms_per_pass": other_s * 1e3, "gbs": total_raw / other_s / 1e9,
+                      "level0_kernel_ms_per_pass": o_ms / n_other, "level0_launches_per_pass": o_launches / n_other,
+                      "passes": n_other}
+    log(f"batch_fuse={other}: {other_s * 1e3:.1f} ms per pass, {o_launches / n_other:.0f} level-0 launches")
 
     # bit-exactness: every batched reply equals the plain reply; each database's golden
     # query equals the reference's reply bytes
@@ -1238,7 +1264,9 @@ def bench_metric(args):
     prof = batch_kernel_profile()
     steps_s = k_steps / (k_ms / 1e3)
     roofline = {"bound": "valu", "unit": "TOP/s", "peak": VALU_PEAK_TOPS,
-                "kernel": "mac_batch3_kernel (register-blocked level-0 MAC, every launch of the pass)",
+                "kernel": ("mac_fused_kernel (persistent register-blocked level-0 MAC over every database)"
+                           if args.batch_fuse else
+                           "mac_batch3_kernel (register-blocked level-0 MAC, every launch of the pass)"),
                 "kernel_ms_per_pass": k_ms / args.steps, "launches_per_pass": k_launches / args.steps,
                 "exact_steps_per_pass": k_steps / args.steps, "exact_steps_per_s": steps_s,
                 "peak_derivation": "256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 32-bit lane ops/s "
@@ -1302,6 +1330,8 @@ def bench_metric(args):
         "serving": latency,
         "plain_latency_ms": {"p50": float(np.percentile(plain_lat, 50)), "p99": float(np.percentile(plain_lat, 99))},
         "autotune_s": tune_s,
+        "level0_launch_structure": {"batch_fuse": args.batch_fuse, "level0_launches_per_pass": k_launches / args.steps,
+                                    "other": other_launches},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "single_query": single,

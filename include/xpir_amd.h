@@ -161,7 +161,14 @@ int xpg_reply_dev_noalloc(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_
 /* ---- batched replies: many queries over resident databases, the DB tiles of each
  *      database streamed from HBM once per query group.  No reference counterpart
  *      (the reference answers one query per generator); each reply is identical to
- *      xpg_reply_dev of the same query. ---- */
+ *      xpg_reply_dev of the same query.  Requests are grouped per database in chunks of
+ *      up to 64; level 0 of every chunk that runs register-blocked, over all the call's
+ *      databases, is one persistent launch (several when their intermediates exceed
+ *      "group_budget" or the launch's 24-segment table: a chunk takes one segment, two
+ *      when its query count is above 4 and not a multiple of 4), followed by each
+ *      chunk's recursion levels.  Option "batch_fuse" = 0 launches level 0 per chunk
+ *      instead (same bytes).  Not for HIP graph capture: the launch uploads its tile
+ *      table with an ordinary host-to-device copy. ---- */
 typedef struct {
   const xpg_db *db;
   const uint64_t *query_dev;

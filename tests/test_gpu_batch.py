@@ -25,9 +25,11 @@ def _setup_db(ctx, orc, prm, n, L, dims, seed):
     [(96, 200, [8, 4, 3], 20)],                       # d = 3: levels 1 and 2 grouped over requests
 ])
 @pytest.mark.parametrize("budget", [0, 1])             # 1 byte: one request per level group
-def test_batch_equals_plain_and_oracle(gpu_ctx, specs, budget):
+@pytest.mark.parametrize("fuse", [1, 0])               # level 0 fused over the databases, or per chunk
+def test_batch_equals_plain_and_oracle(gpu_ctx, specs, budget, fuse):
     import xpir_amd
     gpu_ctx.set_option("group_budget", budget)
+    gpu_ctx.set_option("batch_fuse", fuse)
     rng = np.random.default_rng(len(specs))
     dbs, reqs, expect = [], [], []
     for si, (n, L, dims, nq) in enumerate(specs):
@@ -75,6 +77,7 @@ def test_batch_equals_plain_and_oracle(gpu_ctx, specs, budget):
     for db in dbs:
         db.free()
     gpu_ctx.set_option("group_budget", 0)
+    gpu_ctx.set_option("batch_fuse", 1)
 
 
 SHAPES = [(2, g, q) for g, q in ((1, 1), (1, 2), (1, 4), (4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (16, 1))] + \
@@ -86,8 +89,10 @@ SHAPES = [(2, g, q) for g, q in ((1, 1), (1, 2), (1, 4), (4, 1), (4, 2), (4, 4),
 def test_every_register_block_shape_equals_plain(gpu_ctx, qn):
     """Each built (family, rows, queries) shape of the level-0 batch kernels, forced
     through a tuning entry, gives every query's plain reply (family 3 stages the query
-    words through LDS for a workgroup's four waves)."""
+    words through LDS for a workgroup's four waves).  Per-chunk launches (batch_fuse 0):
+    the tuning entry picks their shape."""
     import xpir_amd
+    gpu_ctx.set_option("batch_fuse", 0)
     n, L, dims = 600, 256, [30, 20]
     orc = Oracle(C2, dims[0])
     prm = xpir_amd.params(C2, dims[0])
@@ -111,7 +116,75 @@ def test_every_register_block_shape_equals_plain(gpu_ctx, qn):
                 assert np.array_equal(rd.download(), plain[k]), (fam, gr, qr, k)
     finally:
         gpu_ctx.tune_clear()
+        gpu_ctx.set_option("batch_fuse", 1)
         for _, qd, rd in reqs:
             qd.free()
             rd.free()
         db.free()
+
+
+# (n_elems, elem_bytes, dims, queries): every level 0 but one has enough rows (groups x
+# polys) for its chunk to run register-blocked and join a fused launch; the one with a
+# single query is answered alone.  Query counts give blocks of four plus ragged blocks of
+# 1, 2 and 3; 13 000 B elements take two polys; the d = 1 database has one group of 4
+# polys (its only row block is mostly rows past the level); one database has d = 3.
+# Fourteen fused chunks make 26 segments: two launches.
+FUSED_DBS = [
+    (400, 256, [20, 20], 5), (300, 13000, [15, 20], 7), (512, 1024, [16, 32], 5), (180, 600, [10, 18], 9),
+    (96, 200, [8, 4, 3], 9), (40, 40000, [40], 41), (250, 300, [25, 20], 2), (160, 4096, [10, 16], 3),
+    (330, 128, [30, 11], 10), (200, 5000, [10, 20], 6), (64, 2048, [4, 16], 1), (221, 777, [17, 13], 11),
+    (150, 64, [10, 16], 9), (100, 3000, [5, 20], 13), (120, 512, [6, 20], 6),
+]
+
+
+@pytest.mark.parametrize("budget", [0, 1 << 20])  # 1 MiB: one chunk per fused launch
+def test_fused_launch_over_many_databases(gpu_ctx, budget):
+    """Level 0 of fourteen databases' chunks in fused persistent launches (more segments
+    than one launch's table holds, or one chunk per launch under a small budget): every
+    reply equals the per-chunk launches' and the plain path's, and the oracle's for the
+    first query of each database."""
+    import xpir_amd
+    rng = np.random.default_rng(5)
+    dbs, reqs, expect = [], [], []
+    try:
+        for si, (n, L, dims, nq) in enumerate(FUSED_DBS):
+            orc = Oracle(C2, dims[0])
+            prm = xpir_amd.params(C2, dims[0])
+            db, data = _setup_db(gpu_ctx, orc, prm, n, L, dims, 300 + si)
+            dbs.append(db)
+            sk = orc.keygen(60 + si)
+            for qi in range(nq):
+                index = int(rng.integers(0, n))
+                q = np.concatenate(orc.make_query(sk, dims, index, 5000 + 100 * si + qi))
+                reqs.append((db, gpu_ctx.alloc(q.nbytes).upload(q), gpu_ctx.alloc(db.reply_words * 8)))
+                expect.append((orc, sk, dims, L, data[index * L:(index + 1) * L].tobytes(), q, qi))
+        print(f"fused: {len(FUSED_DBS)} databases, {len(reqs)} queries, budget {budget}", flush=True)
+        batch = [reqs[i] for i in rng.permutation(len(reqs))]
+        gpu_ctx.set_option("group_budget", budget)
+        gpu_ctx.reply_batch_dev(batch)
+        fused = [rd.download() for _, _, rd in reqs]
+        gpu_ctx.set_option("batch_fuse", 0)
+        for _, _, rd in reqs:
+            rd.upload(np.zeros(rd.nbytes // 8, dtype=np.uint64))
+        gpu_ctx.reply_batch_dev(batch)
+        for k, (db, qd, rd) in enumerate(reqs):
+            orc, sk, dims, L, element, q, qi = expect[k]
+            assert np.array_equal(fused[k], rd.download()), k
+            assert np.array_equal(fused[k], gpu_ctx.reply(db, q)), k
+            if qi == 0:
+                words = db.export_words()
+                levels, off = [], 0
+                for nl in dims:
+                    levels.append(q[off:off + nl * 2 * orc.mn])
+                    off += nl * 2 * orc.mn
+                want, _ = orc.generate_reply(words, db.polys, dims, levels)
+                assert np.array_equal(fused[k], want), k
+                assert extract_reply(orc, sk, fused[k], dims, L) == element
+    finally:
+        gpu_ctx.set_option("group_budget", 0)
+        gpu_ctx.set_option("batch_fuse", 1)
+        for _, qd, rd in reqs:
+            qd.free()
+            rd.free()
+        for db in dbs:
+            db.free()

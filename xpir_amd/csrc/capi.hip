@@ -427,6 +427,7 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
     if (ctx->level_buf[i]) (void)hipFree(ctx->level_buf[i]);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->batch_buf) (void)hipFree(ctx->batch_buf);
+  if (ctx->fused_buf) (void)hipFree(ctx->fused_buf);
   if (ctx->io_buf) (void)hipFree(ctx->io_buf);
   for (int b = 0; b < 2; b++) {
     if (ctx->pin[b]) (void)hipHostFree(ctx->pin[b]);
@@ -445,6 +446,10 @@ void *xpg_ctx_stream(xpg_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr;
 static int set_option(xpg_ctx *ctx, const char *key, int64_t value) {
   if (strcmp(key, "group_budget") == 0 && value >= 0) {
     ctx->group_budget = (uint64_t)value;
+    return XPG_OK;
+  }
+  if (strcmp(key, "batch_fuse") == 0 && (value == 0 || value == 1)) {
+    ctx->batch_fuse = (int)value;
     return XPG_OK;
   }
   if (strcmp(key, "alpha_zero_pad") == 0 && (value == 0 || value == 1)) {
@@ -925,6 +930,16 @@ static bool batch_pays(const xpg_ctx *ctx, const xpg_db *db, uint32_t qn) {
   return lanes * ((qn + 7) / 8) >= (1ull << 16);
 }
 
+// A workspace carve is returned when the scope that made it ends (later calls queue
+// behind this one on the stream, so they may reuse the bytes).
+struct WsMark {
+  xpg_ctx *c;
+  size_t used;
+  ~WsMark() {
+    if (c->ws) c->ws_used = used;
+  }
+};
+
 // Bytes of one buffer of a batched call: carved from the call's workspace when one is set
 // (kept until the call's end), else the context's grown buffer (which: 0 = level reply,
 // 1 = level database, 2 = level-0 outputs).
@@ -964,7 +979,9 @@ extern "C++" uint64_t batch_workspace_bytes(const xpg_ctx *ctx, const xpg_db *db
   const uint64_t lvl0 = db->d > 1 ? al256(k * prod_dims(db->d, db->dims, 1) * db->P0 * 2 * mn * 8) : 0;
   size_t per_db = 0, per_out = 0;
   batch_level_bytes(db, &per_db, &per_out);
-  const uint64_t batched = lvl0 + (db->d > 1 ? al256(per_db) + (per_out ? al256(per_out) : 0) : 0);
+  // (plus the fused launch's tile table)
+  const uint64_t batched =
+      lvl0 + al256(fused_table_bytes(k)) + (db->d > 1 ? al256(per_db) + (per_out ? al256(per_out) : 0) : 0);
   // one reply at a time: its level buffers and the largest level's scratch
   size_t need_reply = 0, need_db = 0;
   level_buf_sizes(db, 0, &need_reply, &need_db);
@@ -977,22 +994,9 @@ extern "C++" uint64_t batch_workspace_bytes(const xpg_ctx *ctx, const xpg_db *db
   return std::max<uint64_t>(batched, single + scr);
 }
 
-static int batch_one_db(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs) {
-  const uint32_t total = (uint32_t)reqs.size();
-  for (uint32_t s = 0; s < total; s += 64) {
-    const uint32_t cnt = std::min<uint32_t>(64, total - s);
-    std::vector<const xpg_request *> part(reqs.begin() + s, reqs.begin() + s + cnt);
-    int rc = XPG_OK;
-    if (batch_pays(ctx, db, cnt)) {
-      rc = batch_chunk(ctx, db, part);
-    } else {
-      for (const xpg_request *r : part)
-        if (!rc) rc = xpg_reply_dev(ctx, db, r->query_dev, r->reply_dev);
-    }
-    if (rc) return rc;
-  }
-  return XPG_OK;
-}
+// Levels 1..d-1 of a chunk's requests, whose level-0 outputs lie side by side at lvl0.
+static int batch_levels(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs,
+                        const uint64_t *lvl0);
 
 // One batched launch for level 0 of up to 64 requests, then each request's other levels.
 static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs) {
@@ -1002,13 +1006,7 @@ static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const x
   const uint32_t qn = (uint32_t)reqs.size();
   const uint64_t lvl0_words = groups0 * db->P0 * 2 * mn;
   // a workspace carve is returned at the call's end (later calls queue behind this one)
-  struct WsMark {
-    xpg_ctx *c;
-    size_t used;
-    ~WsMark() {
-      if (c->ws) c->ws_used = used;
-    }
-  } mark{ctx, ctx->ws_used};
+  WsMark mark{ctx, ctx->ws_used};
   // level-0 outputs: the final replies when d == 1, else per-request intermediates
   uint64_t *lvl0 = nullptr;
   if (db->d > 1) {
@@ -1023,18 +1021,28 @@ static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const x
   }
   int rc = launch_mac_batch(ctx, prm, db->words, qp.data(), op.data(), qn, db->dims[0], groups0, db->P0);
   if (rc || db->d == 1) return rc;
+  return batch_levels(ctx, db, reqs, lvl0);
+}
+
+static int batch_levels(xpg_ctx *ctx, const xpg_db *db, const std::vector<const xpg_request *> &reqs,
+                        const uint64_t *lvl0) {
   // Levels 1..d-1 of all the chunk's requests together: per level, one recursion import
   // over every request's previous-level reply (they lie side by side) and one MAC launch
   // over the requests' level databases side by side, each request reading its own query.
   // Requests are taken in sub-chunks whose level buffers fit the budget: the rest of the
   // workspace when one is set, else kGroupedBudget (option group_budget).
-  const uint32_t d = db->d;
+  const xpg_params *prm = &db->prm;
+  const uint64_t mn = (uint64_t)prm->m * prm->n;
+  const uint32_t qn = (uint32_t)reqs.size(), d = db->d;
+  const uint64_t lvl0_words = prod_dims(d, db->dims, 1) * db->P0 * 2 * mn;
+  WsMark mark{ctx, ctx->ws_used};
   size_t per_db = 0, per_out = 0;
   batch_level_bytes(db, &per_db, &per_out);
   const size_t per_req = al256(per_db) + (per_out ? al256(per_out) : 0);
   const uint64_t budget = ctx->ws ? ctx->ws_cap - ctx->ws_used : ctx->group_budget ? ctx->group_budget : kGroupedBudget;
   const uint32_t sub = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(qn, budget / per_req));
   uint64_t *lvl_db = nullptr, *lvl_out = nullptr;
+  int rc = XPG_OK;
   if (ctx->ws) {  // carved once, for the largest sub-chunk
     rc = batch_buffer(ctx, 1, sub * per_db, &lvl_db);
     if (!rc && per_out) rc = batch_buffer(ctx, 0, sub * per_out, &lvl_out);
@@ -1073,6 +1081,105 @@ static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const x
   return XPG_OK;
 }
 
+// Chunks (up to 64 requests of one database each) whose level 0 runs register-blocked,
+// in fused launches: as many chunks per launch as the launch's segment table and the
+// level-0 budget hold (the rest of the workspace when one is set, else kGroupedBudget),
+// then each chunk's remaining levels.
+struct BatchChunk {
+  const xpg_db *db;
+  std::vector<const xpg_request *> reqs;
+};
+
+static uint64_t lvl0_bytes(const xpg_db *db, uint32_t qn) {
+  return db->d > 1 ? al256((uint64_t)qn * prod_dims(db->d, db->dims, 1) * db->P0 * 2 * db->prm.m * db->prm.n * 8) : 0;
+}
+
+static int fused_table(xpg_ctx *ctx, size_t bytes, void **p) {
+  if (ctx->ws) {
+    uint64_t *w = nullptr;
+    const int rc = batch_buffer(ctx, 0, bytes, &w);  // carved (`which` only names grown buffers)
+    *p = w;
+    return rc;
+  }
+  if (ctx->fused_bytes < bytes) {
+    if (ctx->fused_buf) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipFree(ctx->fused_buf);
+    }
+    ctx->fused_buf = nullptr;
+    ctx->fused_bytes = 0;
+    ctx->epoch++;
+    const size_t want = std::max<size_t>(bytes, 64 << 10);
+    if (hipMalloc(&ctx->fused_buf, want) != hipSuccess) return set_err(ctx, XPG_ENOMEM, "fused tile table");
+    ctx->fused_bytes = want;
+  }
+  *p = ctx->fused_buf;
+  return XPG_OK;
+}
+
+static int batch_fused(xpg_ctx *ctx, const std::vector<BatchChunk> &chunks) {
+  for (size_t i = 0; i < chunks.size();) {
+    WsMark mark{ctx, ctx->ws_used};
+    const uint64_t avail =
+        ctx->ws ? ctx->ws_cap - ctx->ws_used : ctx->group_budget ? ctx->group_budget : kGroupedBudget;
+    uint32_t segs = 0, queries = 0;
+    uint64_t lvl0 = 0, levels = 0;
+    size_t j = i;
+    for (; j < chunks.size(); j++) {
+      const BatchChunk &c = chunks[j];
+      const uint32_t qn = (uint32_t)c.reqs.size();
+      size_t per_db = 0, per_out = 0;
+      batch_level_bytes(c.db, &per_db, &per_out);
+      const uint64_t lv = c.db->d > 1 ? al256(per_db) + (per_out ? al256(per_out) : 0) : 0;
+      // with a workspace, the table and one request's later levels come from it too
+      const uint64_t need = lvl0 + lvl0_bytes(c.db, qn) +
+                            (ctx->ws ? al256(fused_table_bytes(queries + qn)) + std::max(levels, lv) : 0);
+      if (j > i && (segs + fused_segments(qn) > kFusedMaxSegments || need > avail)) break;
+      segs += fused_segments(qn);
+      queries += qn;
+      lvl0 += lvl0_bytes(c.db, qn);
+      levels = std::max(levels, lv);
+    }
+    uint64_t *lvl0_base = nullptr;
+    int rc = lvl0 ? batch_buffer(ctx, 2, lvl0, &lvl0_base) : XPG_OK;
+    void *table = nullptr;
+    if (!rc) rc = fused_table(ctx, fused_table_bytes(queries), &table);
+    if (rc) return rc;
+    std::vector<FusedPart> parts;
+    std::vector<std::vector<const uint64_t *>> qp(j - i);
+    std::vector<std::vector<uint64_t *>> op(j - i);
+    std::vector<const uint64_t *> chunk_lvl0(j - i, nullptr);
+    uint64_t off = 0;  // words into lvl0_base
+    for (size_t k = i; k < j; k++) {
+      const xpg_db *db = chunks[k].db;
+      const uint32_t qn = (uint32_t)chunks[k].reqs.size();
+      const uint64_t words = prod_dims(db->d, db->dims, 1) * db->P0 * 2 * db->prm.m * db->prm.n;
+      chunk_lvl0[k - i] = lvl0_base + off;
+      for (uint32_t r = 0; r < qn; r++) {
+        qp[k - i].push_back(chunks[k].reqs[r]->query_dev);
+        op[k - i].push_back(db->d == 1 ? chunks[k].reqs[r]->reply_dev : lvl0_base + off + r * words);
+      }
+      off += lvl0_bytes(db, qn) / 8;
+      parts.push_back(FusedPart{&db->prm, db->words, qp[k - i].data(), op[k - i].data(), qn, db->dims[0],
+                                prod_dims(db->d, db->dims, 1), db->P0});
+    }
+    // the part with the most moduli first (every part's moduli are its first m)
+    size_t widest = 0;
+    for (size_t k = 1; k < parts.size(); k++)
+      if (parts[k].prm->m > parts[widest].prm->m) widest = k;
+    std::swap(parts[0], parts[widest]);
+    rc = launch_mac_fused(ctx, parts.data(), (uint32_t)parts.size(), table);
+    if (rc) return rc;
+    for (size_t k = i; k < j; k++)
+      if (chunks[k].db->d > 1) {
+        rc = batch_levels(ctx, chunks[k].db, chunks[k].reqs, chunk_lvl0[k - i]);
+        if (rc) return rc;
+      }
+    i = j;
+  }
+  return XPG_OK;
+}
+
 extern "C++" int reply_batch_in_workspace(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs, uint8_t *base,
                                          size_t bytes) {
   ctx->ws = base;
@@ -1095,11 +1202,27 @@ int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs) 
     if (!by_db.count(reqs[i].db)) order.push_back(reqs[i].db);
     by_db[reqs[i].db].push_back(&reqs[i]);
   }
+  // chunks of up to 64 requests per database: those whose level 0 runs register-blocked
+  // go to the fused launches (option batch_fuse, on by default), the others run alone
+  std::vector<BatchChunk> fused;
   for (const xpg_db *db : order) {
-    int rc = batch_one_db(ctx, db, by_db[db]);
-    if (rc) return rc;
+    const std::vector<const xpg_request *> &v = by_db[db];
+    for (size_t s = 0; s < v.size(); s += 64) {
+      BatchChunk c{db, std::vector<const xpg_request *>(v.begin() + s, v.begin() + std::min(v.size(), s + 64))};
+      const uint32_t cnt = (uint32_t)c.reqs.size();
+      int rc = XPG_OK;
+      if (!batch_pays(ctx, db, cnt)) {
+        for (const xpg_request *r : c.reqs)
+          if (!rc) rc = xpg_reply_dev(ctx, db, r->query_dev, r->reply_dev);
+      } else if (ctx->batch_fuse) {
+        fused.push_back(std::move(c));
+      } else {
+        rc = batch_chunk(ctx, db, c.reqs);
+      }
+      if (rc) return rc;
+    }
   }
-  return XPG_OK;
+  return batch_fused(ctx, fused);
 }
 
 int xpg_reply(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query, uint64_t query_words, uint64_t *reply,

@@ -87,6 +87,8 @@ struct xpg_ctx {
   size_t scratch_bytes = 0;
   uint64_t *batch_buf = nullptr;  // level-0 replies of a batch (d > 1)
   size_t batch_bytes = 0;
+  void *fused_buf = nullptr;      // tile counters and pointers of a fused level-0 launch
+  size_t fused_bytes = 0;
   uint64_t *level_buf[2] = {nullptr, nullptr};
   size_t level_bytes[2] = {0, 0};
   // a call's workspace: while set, a reply's level buffers and block summaries and a
@@ -111,6 +113,9 @@ struct xpg_ctx {
   int stream_pf = 0;               // streaming MAC: double-buffered chunk loads
   int cu_count = 256;              // compute units of the device
   uint64_t group_budget = 0;       // bytes for a batch's grouped levels (0 = default)
+  // batches: level 0 of every database chunk in fused persistent launches (1) or one
+  // launch per chunk (0)
+  int batch_fuse = 1;
   // a partial last alpha aggregate: 0 = the reference's bytes (readAggregatedStream's
   // bzero from the last real element's slot), 1 = the missing elements read as zeros
   int alpha_zero_pad = 0;
@@ -206,6 +211,26 @@ int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
 int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                          uint64_t *const *out, uint32_t qn, uint64_t n_l, uint64_t groups, uint32_t P, int gr,
                          int qr, int family);
+// Level 0 of several database chunks in one persistent launch (mac_fused_kernel): part i
+// is qn (1..64) queries over the level-0 database db (n_l elements per chain, `groups`
+// groups of P polys); q / out are host arrays of qn device pointers.  Every part's moduli
+// are the first part's first m.  dev_table is device memory of fused_table_bytes(total
+// queries) bytes that the launch uploads its tile counters and pointers to (stream
+// ordered: it may be reused by the next launch on the stream).  A chunk takes
+// fused_segments(qn) of the launch's kFusedMaxSegments segments.
+struct FusedPart {
+  const xpg_params *prm;
+  const uint64_t *db;
+  const uint64_t *const *q;
+  uint64_t *const *out;
+  uint32_t qn;
+  uint64_t n_l, groups;
+  uint32_t P;
+};
+constexpr uint32_t kFusedMaxSegments = 24;
+size_t fused_table_bytes(uint32_t queries);
+uint32_t fused_segments(uint32_t qn);
+int launch_mac_fused(xpg_ctx *ctx, const FusedPart *parts, uint32_t nparts, void *dev_table);
 // One level of up to 64 requests at once: their level databases side by side (request i's
 // groups are groups [i*gpr, (i+1)*gpr) of db), request i reading query q[i]; outputs to
 // out_per_req[i] (each request's own buffer) when given, else contiguous in out.

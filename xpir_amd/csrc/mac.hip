@@ -282,15 +282,25 @@ struct QPair {
   uint64_t y, yq;
 };
 
-template <int GR, int QR, int W>
-__global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
+// One level's database as a tile sees it (the per-database launch takes it from its
+// arguments, the fused launch from its segment table).
+struct TileLevel {
+  const uint64_t *db;
+  uint64_t n_l, groups;
+  uint32_t P, n, m, sblocks, qn;
+};
+
+// The tile (colgroup cg = (row block, column), query block qb) of one level: every wave
+// of the workgroup runs it; qbase[i] / obase[i] are the query and output of the block's
+// query i (the last query repeated past qn: its chains are computed and dropped).
+template <int GR, int QR, int W, int NQMAX>
+__device__ __forceinline__ void batch3_tile(const ModSet &ms, const TileLevel &a, const uint64_t *const (&qbase)[QR],
+                                            uint64_t *const (&obase)[QR], uint32_t nq, uint64_t cg,
+                                            QPair (&lds)[2][NQMAX][64]) {
   constexpr int NQ = kB3U * QR * 2;  // staged query rows (element, query, a|b) per chunk
   constexpr int NS = NQ / W;         // rows staged per wave
   static_assert(NQ % W == 0, "staged rows split evenly over the waves");
-  __shared__ QPair lds[2][NQ][64];
-  uint64_t cg;
-  uint32_t qb;
-  if (!batch_tile(a, blockIdx.x, cg, qb)) return;  // the whole workgroup
+  static_assert(NQ <= NQMAX, "staging buffer holds the block's rows");
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t sb = (uint32_t)(cg % a.sblocks);
@@ -300,8 +310,8 @@ __global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
   const uint64_t rows = a.groups * a.P;
   const uint64_t mn = (uint64_t)a.m * a.n;
   const uint32_t cs = c * a.n + sb * 64 + lane;  // this lane's word within a poly
-  const uint64_t p = a.ms.p[c];
-  const uint32_t delta = a.ms.delta[c];
+  const uint64_t p = ms.p[c];
+  const uint32_t delta = ms.delta[c];
   const StepConsts kc = step_consts(p, delta);
   const uint64_t xstride = (uint64_t)a.P * mn;  // next element of a row
   const uint64_t qstride = 2 * mn;              // next element of a query
@@ -316,7 +326,6 @@ __global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
     xr[g] = a.db + ((r / a.P) * a.n_l * a.P + (r % a.P)) * mn;
   }
   // staging: this wave loads the query rows qrow = w + W * s of every chunk
-  const uint32_t q0 = qb * QR;
   const uint64_t *sq[NS];
   uint32_t su[NS];
 #pragma unroll
@@ -324,7 +333,7 @@ __global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
     const uint32_t qrow = w + W * s;
     const uint32_t u = qrow / (QR * 2), rem = qrow % (QR * 2), i = rem >> 1, ab = rem & 1;
     su[s] = u;
-    sq[s] = a.q[min(q0 + i, a.qn - 1)] + ab * mn;
+    sq[s] = qbase[i] + ab * mn;
   }
   uint64_t acc_a[GR][QR], acc_b[GR][QR];
 #pragma unroll
@@ -394,7 +403,7 @@ __global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
     for (int g = 0; g < GR; g++) xv[g] = __builtin_nontemporal_load(lane_word(xr[g] + (uint64_t)e * xstride, cs8));
 #pragma unroll
     for (int i = 0; i < QR; i++) {
-      const uint64_t *qi = a.q[min(q0 + i, a.qn - 1)] + (uint64_t)e * qstride;
+      const uint64_t *qi = qbase[i] + (uint64_t)e * qstride;
       const uint64_t ya = *lane_word(qi, cs8), yb = *lane_word(qi + mn, cs8);
       const uint64_t yaq = shoup_quot(ya, p, delta), ybq = shoup_quot(yb, p, delta);
 #pragma unroll
@@ -409,10 +418,115 @@ __global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
     if (g >= rv) continue;
 #pragma unroll
     for (int i = 0; i < QR; i++) {
-      if (q0 + i < a.qn) {
-        uint64_t *o = a.out[q0 + i] + (r0 + g) * 2 * mn + cs;
+      if ((uint32_t)i < nq) {
+        uint64_t *o = obase[i] + (r0 + g) * 2 * mn + cs;
         o[0] = acc_a[g][i];
         o[mn] = acc_b[g][i];
+      }
+    }
+  }
+}
+
+template <int GR, int QR, int W>
+__global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
+  __shared__ QPair lds[2][kB3U * QR * 2][64];
+  uint64_t cg;
+  uint32_t qb;
+  if (!batch_tile(a, blockIdx.x, cg, qb)) return;  // the whole workgroup
+  const TileLevel lv{a.db, a.n_l, a.groups, a.P, a.n, a.m, a.sblocks, a.qn};
+  const uint32_t q0 = qb * QR;
+  const uint64_t *qbase[QR];
+  uint64_t *obase[QR];
+#pragma unroll
+  for (int i = 0; i < QR; i++) {
+    qbase[i] = a.q[min(q0 + i, a.qn - 1)];
+    obase[i] = a.out[min(q0 + i, a.qn - 1)];
+  }
+  batch3_tile<GR, QR, W>(a.ms, lv, qbase, obase, a.qn - q0, cg, lds);
+}
+
+// ------------------------------------------------------------------------------------
+// Level 0 of many databases in one persistent launch.  The launch's work is a table of
+// segments, one per (database chunk, query block width): a segment is the register-
+// blocked tiles (colgroup, query block) of one database's level 0 for its queries, in
+// blocks of QR = 4 queries and one block of the 1..3 left over.  Tiles are dealt to the
+// eight XCDs by colgroup (cg mod 8, as in the per-database launch, so the query blocks of
+// one colgroup share that XCD's L2), and each XCD's list takes the segments in the
+// order the host gives them (longest chains first).  The grid holds as many workgroups
+// as are resident at once; each takes tiles from its own XCD's list through an atomic
+// counter, and when that list is empty from the other XCDs' lists in turn, until every
+// list is empty (the only exit; no workgroup waits for another).  The words are the
+// same as the per-database launches': a tile is the same code on the same rows.
+constexpr int kFusedSegs = (int)kFusedMaxSegments;
+constexpr int kFusedGR = 4;
+constexpr int kFusedQR = 4;
+
+struct FusedSeg {
+  const uint64_t *db;
+  uint64_t n_l, groups;
+  uint32_t P, n, m, sblocks;
+  uint32_t qn, qr, qblocks;  // queries of the segment, per block (1..4), blocks
+  uint32_t ptr0;             // its query pointers at ptrs[ptr0 ..], outputs after them
+  uint32_t begin[8];         // first tile of the segment in each XCD's list
+};
+
+struct FusedArgs {
+  ModSet ms;
+  uint32_t *counters;        // [8], zero at launch: next tile of each XCD's list
+  const uint64_t *ptrs;      // query and output pointers of every segment
+  uint32_t nseg;
+  uint32_t tiles[8];         // tiles in each XCD's list
+  FusedSeg seg[kFusedSegs];
+};
+
+// a wave-uniform 64-bit value in scalar registers
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <int GR, int QR, int W, int NQMAX>
+__device__ __forceinline__ void fused_tile(const FusedArgs &a, const FusedSeg &g, uint64_t cg, uint32_t qb,
+                                           QPair (&lds)[2][NQMAX][64]) {
+  const TileLevel lv{g.db, g.n_l, g.groups, g.P, g.n, g.m, g.sblocks, g.qn};
+  const uint32_t q0 = qb * QR;
+  const uint64_t *qbase[QR];
+  uint64_t *obase[QR];
+#pragma unroll
+  for (int i = 0; i < QR; i++) {
+    const uint32_t qi = g.ptr0 + min(q0 + i, g.qn - 1);
+    qbase[i] = reinterpret_cast<const uint64_t *>(uniform64(a.ptrs[qi]));
+    obase[i] = reinterpret_cast<uint64_t *>(uniform64(a.ptrs[qi + g.qn]));
+  }
+  batch3_tile<GR, QR, W>(a.ms, lv, qbase, obase, g.qn - q0, cg, lds);
+}
+
+template <int W>
+__global__ void __launch_bounds__(64 * W) mac_fused_kernel(FusedArgs a) {
+  constexpr int GR = kFusedGR;
+  __shared__ QPair lds[2][kB3U * kFusedQR * 2][64];
+  __shared__ uint32_t slot;
+  const uint32_t home = blockIdx.x & 7;
+  for (uint32_t k = 0; k < 8; k++) {
+    const uint32_t x = (home + k) & 7;
+    const uint32_t ntiles = a.tiles[x];
+    for (;;) {
+      if (threadIdx.x == 0) slot = atomicAdd(&a.counters[x], 1u);
+      __syncthreads();
+      const uint32_t t = __builtin_amdgcn_readfirstlane(slot);
+      __syncthreads();  // every wave has read the slot before the next fetch writes it
+      if (t >= ntiles) break;
+      uint32_t s = 0;
+      while (s + 1 < a.nseg && a.seg[s + 1].begin[x] <= t) s++;
+      const FusedSeg &g = a.seg[s];
+      const uint32_t j = t - g.begin[x];
+      const uint64_t cg = x + 8ull * (j / g.qblocks);
+      const uint32_t qb = j % g.qblocks;
+      switch (g.qr) {
+        case 4: fused_tile<GR, 4, W>(a, g, cg, qb, lds); break;
+        case 3: fused_tile<GR, 3, W>(a, g, cg, qb, lds); break;
+        case 2: fused_tile<GR, 2, W>(a, g, cg, qb, lds); break;
+        default: fused_tile<GR, 1, W>(a, g, cg, qb, lds); break;
       }
     }
   }
@@ -980,6 +1094,116 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
     default: hipLaunchKernelGGL((mac_batch2_kernel<1, 1>), dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, b2); break;
   }
   const int rc = hipGetLastError() != hipSuccess ? set_err(ctx, XPG_EDEVICE, "batched MAC launch") : XPG_OK;
+  timing_end(ctx, k0, k1, 2);
+  timing_end(ctx, e0, e1);
+  return rc;
+}
+
+size_t fused_table_bytes(uint32_t queries) { return 64 + (size_t)queries * 2 * sizeof(uint64_t); }
+
+uint32_t fused_segments(uint32_t qn) { return qn > (uint32_t)kFusedQR && qn % kFusedQR ? 2u : 1u; }
+
+// Workgroups of the fused kernel resident at once on the device (its persistent grid).
+static uint32_t fused_grid(const xpg_ctx *ctx) {
+  static const int per_cu = [] {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(&mac_fused_kernel<4>), 256,
+                                                     0) != hipSuccess || nb < 1)
+      nb = 1;
+    return nb;
+  }();
+  return (uint32_t)(ctx->cu_count * per_cu + 7) / 8 * 8;
+}
+
+int launch_mac_fused(xpg_ctx *ctx, const FusedPart *parts, uint32_t nparts, void *dev_table) {
+  if (nparts == 0) return XPG_OK;
+  if (!parts || !dev_table) return set_err(ctx, XPG_EINVAL, "fused launch without parts or table");
+  constexpr int W = 4, GR = kFusedGR;
+  const xpg_params *prm0 = parts[0].prm;
+  // segments: a chunk's blocks of four queries, and its one block of the 1..3 left over
+  struct SegH {
+    FusedSeg s;
+    const FusedPart *part;
+    uint32_t q_first;
+  };
+  std::vector<SegH> segs;
+  uint32_t queries = 0;
+  uint64_t steps = 0;
+  for (uint32_t pi = 0; pi < nparts; pi++) {
+    const FusedPart &pt = parts[pi];
+    const xpg_params *prm = pt.prm;
+    if (pt.qn == 0 || pt.qn > (uint32_t)kMaxBatch) return set_err(ctx, XPG_EINVAL, "fused part of 1..64 queries");
+    if (prm->n < 64 || prm->n % 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n a multiple of 64");
+    if (pt.n_l == 0 || pt.n_l > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "fused chain length outside 32 bits");
+    if (prm->m > prm0->m) return set_err(ctx, XPG_EINVAL, "fused parts: the first part has the most moduli");
+    for (uint32_t c = 0; c < prm->m; c++)
+      if (prm->moduli[c] != prm0->moduli[c]) return set_err(ctx, XPG_EINVAL, "fused parts with other moduli");
+    const uint32_t q4 = pt.qn >= (uint32_t)kFusedQR ? pt.qn - pt.qn % kFusedQR : 0;
+    for (int half = 0; half < 2; half++) {
+      const uint32_t qfirst = half ? q4 : 0, qn = half ? pt.qn - q4 : q4;
+      if (qn == 0) continue;
+      SegH h{};
+      h.part = &pt;
+      h.q_first = qfirst;
+      h.s.db = pt.db;
+      h.s.n_l = pt.n_l;
+      h.s.groups = pt.groups;
+      h.s.P = pt.P;
+      h.s.n = prm->n;
+      h.s.m = prm->m;
+      h.s.sblocks = prm->n / 64;
+      h.s.qn = qn;
+      h.s.qr = half ? qn : (uint32_t)kFusedQR;
+      h.s.qblocks = (qn + h.s.qr - 1) / h.s.qr;
+      segs.push_back(h);
+    }
+    queries += pt.qn;
+    steps += (uint64_t)pt.qn * pt.groups * pt.P * prm->m * prm->n * 2 * pt.n_l;
+  }
+  if (segs.size() > (size_t)kFusedSegs) return set_err(ctx, XPG_EINVAL, "fused launch of more than 24 segments");
+  // longest tiles first in every XCD's list (a tile is W*GR rows x qr queries x n_l steps)
+  std::stable_sort(segs.begin(), segs.end(),
+                   [](const SegH &a, const SegH &b) { return a.s.n_l * a.s.qr > b.s.n_l * b.s.qr; });
+  FusedArgs fa{};
+  fa.ms = modset_of(prm0);
+  fa.nseg = (uint32_t)segs.size();
+  std::vector<uint64_t> image(8 + 2 * (size_t)queries, 0);  // 64 B of counters, then pointers
+  uint64_t tiles[8] = {0};
+  uint32_t ptr = 0;
+  for (size_t i = 0; i < segs.size(); i++) {
+    SegH &h = segs[i];
+    const uint64_t colgroups = ((h.s.groups * h.s.P + W * GR - 1) / (W * GR)) * h.s.m * h.s.sblocks;
+    for (int x = 0; x < 8; x++) {
+      h.s.begin[x] = (uint32_t)tiles[x];
+      tiles[x] += (colgroups > (uint64_t)x ? (colgroups - x + 7) / 8 : 0) * h.s.qblocks;
+      if (tiles[x] > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "fused grid too large");
+    }
+    h.s.ptr0 = ptr;
+    for (uint32_t q = 0; q < h.s.qn; q++) {
+      image[8 + ptr + q] = reinterpret_cast<uintptr_t>(h.part->q[h.q_first + q]);
+      image[8 + ptr + h.s.qn + q] = reinterpret_cast<uintptr_t>(h.part->out[h.q_first + q]);
+    }
+    ptr += 2 * h.s.qn;
+    fa.seg[i] = h.s;
+  }
+  uint64_t total = 0;
+  for (int x = 0; x < 8; x++) {
+    fa.tiles[x] = (uint32_t)tiles[x];
+    total += tiles[x];
+  }
+  fa.counters = reinterpret_cast<uint32_t *>(dev_table);
+  fa.ptrs = reinterpret_cast<const uint64_t *>(reinterpret_cast<char *>(dev_table) + 64);
+  if (hipMemcpyAsync(dev_table, image.data(), image.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream) !=
+      hipSuccess)
+    return set_err(ctx, XPG_EDEVICE, "fused tile table upload");
+  const uint32_t grid = std::max<uint32_t>(8, std::min<uint32_t>(fused_grid(ctx), (uint32_t)((total + 7) / 8 * 8)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  timing_begin(ctx, &e0, &e1);
+  hipEvent_t k0 = nullptr, k1 = nullptr;
+  timing_begin(ctx, &k0, &k1, 2);
+  if (ctx->timing && !ctx->ws) ctx->batch_steps += steps;
+  hipLaunchKernelGGL((mac_fused_kernel<W>), dim3(grid), dim3(64 * W), 0, ctx->stream, fa);
+  const int rc = hipGetLastError() != hipSuccess ? set_err(ctx, XPG_EDEVICE, "fused MAC launch") : XPG_OK;
   timing_end(ctx, k0, k1, 2);
   timing_end(ctx, e0, e1);
   return rc;
