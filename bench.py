@@ -15,7 +15,8 @@ another) on the same queries, and p50/p99 reply latency from a seeded Poisson st
 through the continuous-batching server over the same databases (plus d = 1 views of the
 two smallest).  Bit-exactness: each database's golden query against the reference's
 reply bytes (tests/golden/golden.json), every batched reply against the plain reply.
-roofline: the batched level-0 kernel (mac_batch3_kernel), integer-issue bound: exact
+roofline: the batched level-0 kernel (mac_fused_kernel: one persistent launch over all
+eight databases; --batch-fuse 0 times one launch per database chunk), integer-issue bound: exact
 chain steps/s (HIP events around its launches in the timed passes) x VALU lane
 instructions per step (rocprofv3 SQ_INSTS_VALU, profiles/) against 39.3 T lane ops/s.
 cpu_baseline: the reference's own arithmetic (oracle/_ref) on this host on a bounded
@@ -965,11 +966,13 @@ def bench_serving(args):
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # lane ops/s: CUs x SIMDs x lanes x clock = 39.3 T
 
 
-def batch_kernel_profile():
-    """VALU lane instructions per exact chain step of the batched level-0 kernel, and its
-    HBM bytes per pass, from the rocprofv3 counter runs committed under profiles/ (the
-    command that made them is recorded there); None when absent."""
-    path = os.path.join(ROOT, "profiles", "mac_batch3_kernel_batched_uneven_8dbs.json")
+def batch_kernel_profile(fused: bool = True):
+    """VALU lane instructions per exact chain step of the batched level-0 kernel (the fused
+    persistent launch, or the per-chunk launches), and its HBM bytes per pass, from the
+    rocprofv3 counter runs committed under profiles/ (the commands that made them are
+    recorded there); None when absent."""
+    name = "mac_fused_kernel" if fused else "mac_batch3_kernel"
+    path = os.path.join(ROOT, "profiles", f"{name}_batched_uneven_8dbs.json")
     try:
         return json.load(open(path))
     except (OSError, ValueError):
@@ -1261,7 +1264,7 @@ def bench_metric(args):
         d["golden_query"] = d["reqs"][0][1].download()
     log(f"bit-exact: batched == plain {exact_vs_plain}, golden queries {sum(golden_ok)}/{len(golden_ok)}")
 
-    prof = batch_kernel_profile()
+    prof = batch_kernel_profile(bool(args.batch_fuse))
     steps_s = k_steps / (k_ms / 1e3)
     roofline = {"bound": "valu", "unit": "TOP/s", "peak": VALU_PEAK_TOPS,
                 "kernel": ("mac_fused_kernel (persistent register-blocked level-0 MAC over every database)"
@@ -1278,7 +1281,8 @@ def bench_metric(args):
                         hbm_gbs=(prof["hbm_bytes_per_pass"] / (k_ms / args.steps / 1e3) / 1e9
                                  if prof.get("hbm_bytes_per_pass") else None),
                         valu_busy=prof.get("valu_busy"), sustained_clock_ghz=prof.get("clock_ghz_from_grbm"),
-                        counters_from="profiles/mac_batch3_kernel_batched_uneven_8dbs.json (rocprofv3 "
+                        counters_from=f"profiles/{'mac_fused_kernel' if args.batch_fuse else 'mac_batch3_kernel'}"
+                                      "_batched_uneven_8dbs.json (rocprofv3 "
                                       "SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU, GRBM_GUI_ACTIVE, FETCH_SIZE, WRITE_SIZE "
                                       "of the same pass; commands recorded there; traffic is those counters', "
                                       "not measured in this run)")
