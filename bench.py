@@ -29,6 +29,7 @@ import argparse
 import hashlib
 import json
 import os
+import shutil
 import sys
 import time
 
@@ -132,7 +133,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 3 for metric, else 20)")
     ap.add_argument("--workload", default="metric",
                     choices=["metric", "batched-pass", "single", "plain", "recursive", "batched", "serving",
-                             "session", "copy"])
+                             "session", "copy", "dbfile"])
     ap.add_argument("--scale", type=float, default=1.0, help="element-count scale of the batched/serving databases")
     ap.add_argument("--queries", type=int, default=None,
                     help="serving: queries in the Poisson stream (default 2000; metric: 3000)")
@@ -177,12 +178,75 @@ def main():
         return bench_recursive(args)
     if args.workload == "session":
         return bench_session(args)
+    if args.workload == "dbfile":
+        return bench_dbfile(args)
 
     cfg = C2_SINGLE if args.workload == "single" else C1_PLAIN
     gold = load_gold().get("c2_single" if args.workload == "single" else "c1_plain")
     ctx = xpir_amd.Context(0)
     out = run_single(ctx, args, cfg, gold)
     print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_dbfile(args):
+    """The NTT database through a file against importing it again: import from raw bytes
+    in HBM and from host memory, xpg_db_save (device -> file) and xpg_db_load (file ->
+    device, the file just written so in the page cache, plus the device checksum), on the
+    single-query database (65536 x 4 KiB, 2 GiB of NTT words) and the 100000 x 16 KiB
+    batched database (6.1 GiB).  Every loaded database's words equal the imported ones'."""
+    import tempfile
+
+    import xpir_amd
+    from xpir_amd.workloads import C2_SINGLE, batch_databases
+    ctx = xpir_amd.Context(0)
+    tmpdir = tempfile.mkdtemp(prefix="xpg_dbfile_")
+    rows = []
+    try:
+        for spec in (C2_SINGLE, batch_databases()[3]):
+            prm = xpir_amd.params(spec["crypto"], spec["dims"][0])
+            n, L, dims = spec["n_elems"], spec["elem_bytes"], spec["dims"]
+            raw = ctx.alloc(n * L + 16)
+            ctx.random_bytes_dev(spec["db_seed"], 0, raw, n * L)
+            db = ctx.import_db_dev(prm, raw, n, L, dims)  # builds the tables
+            db.free()
+            ctx.sync()
+            t0 = time.perf_counter()
+            db = ctx.import_db_dev(prm, raw, n, L, dims)
+            ctx.sync()
+            t_dev = time.perf_counter() - t0
+            host = raw.download(np.uint8)[: n * L]
+            raw.free()
+            t0 = time.perf_counter()
+            db_h = ctx.import_db(prm, host, n, L, dims)
+            ctx.sync()
+            t_host = time.perf_counter() - t0
+            db_h.free()
+            del host
+            path = os.path.join(tmpdir, "db.ntt")
+            digest = hashlib.sha256(f"{spec}".encode()).digest()
+            t0 = time.perf_counter()
+            ctx.save_db(db, path, digest)
+            t_save = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            db2 = ctx.load_db(prm, path, digest)
+            t_load = time.perf_counter() - t0
+            ntt = os.path.getsize(path) - 512
+            same = bool(np.array_equal(db2.export_words(), db.export_words()))
+            db2.free()
+            db.free()
+            os.remove(path)
+            row = {"db": f"{n} x {L} B {spec['crypto']} dims {dims}", "raw_bytes": n * L, "ntt_bytes": ntt,
+                   "import_from_hbm_ms": t_dev * 1e3, "import_from_host_ms": t_host * 1e3,
+                   "save_ms": t_save * 1e3, "load_ms": t_load * 1e3,
+                   "save_gbs": ntt / t_save / 1e9, "load_gbs": ntt / t_load / 1e9, "loaded_words_equal": same}
+            log(f"dbfile {row['db']}: import {t_dev * 1e3:.1f} ms (HBM) / {t_host * 1e3:.1f} ms (host); "
+                f"save {t_save * 1e3:.0f} ms ({row['save_gbs']:.2f} GB/s), load {t_load * 1e3:.0f} ms "
+                f"({row['load_gbs']:.2f} GB/s), words equal {same}")
+            rows.append(row)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
+    print(json.dumps({"workload": "dbfile", "rows": rows}), flush=True)
     ctx.close()
 
 

@@ -122,9 +122,25 @@ int xpg_db_import_ntt_words(xpg_ctx *ctx, const xpg_params *prm, const uint64_t 
                             const uint32_t *dims, xpg_db **out);
 /* copy the resident NTT words back (parity checks); cap in words */
 int xpg_db_export_ntt_words(xpg_ctx *ctx, const xpg_db *db, uint64_t *out, uint64_t cap_words);
+/* The NTT-domain database in a file, loaded straight back to HBM.  No reference
+ * counterpart: the reference imports at every start (importDataNFL,
+ * PIRReplyGeneratorNFL_internal.cpp:79-158) and keeps the imported database only in RAM
+ * (apps/server/PIRServer.cpp:71-104).  The file is a 512-byte header (parameters, grid,
+ * sizes, the partial-aggregate convention of the import, `digest`: 32 bytes the caller
+ * computes over the raw database, or zeros when NULL; an order-keyed checksum of the
+ * words) and the words as they lie in HBM; xpg_db_save writes it under path + ".tmp"
+ * and renames it when complete.  xpg_db_load refuses (XPG_EINVAL) a file of another
+ * format, made under other parameters than *prm, or with another digest (when digest is
+ * not NULL), or whose words fail the checksum (recomputed on the device after the
+ * copy); XPG_ESIZE for a file shorter or longer than its header says.  Host memory:
+ * the context's two pinned staging chunks. */
+int xpg_db_save(xpg_ctx *ctx, const xpg_db *db, const char *path, const uint8_t *digest);
+int xpg_db_load(xpg_ctx *ctx, const xpg_params *prm, const char *path, const uint8_t *digest, xpg_db **out);
 /* polys per unit at level 0, and the reply's ciphertext count */
 uint32_t xpg_db_polys(const xpg_db *db);
 uint32_t xpg_db_reply_polys(const xpg_db *db);
+/* the recursion depth d, and dims[0..min(d, cap)-1] copied to dims (a loaded database's grid) */
+uint32_t xpg_db_levels(const xpg_db *db, uint32_t *dims, uint32_t cap);
 /* query words the database expects: sum_l dims[l] * 2*m*n */
 uint64_t xpg_db_query_words(const xpg_db *db);
 /* raw database bytes (n_elems * elem_bytes), the numerator of DB GB/s */

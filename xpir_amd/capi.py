@@ -21,7 +21,8 @@ EXPORTS = [
     "xpg_ctx_destroy", "xpg_last_error", "xpg_ctx_stream", "xpg_ctx_set_option", "xpg_sync", "xpg_dev_alloc",
     "xpg_dev_free", "xpg_h2d", "xpg_d2h", "xpg_host_alloc", "xpg_host_free", "xpg_h2d_async",
     "xpg_db_import", "xpg_db_import_dev",
-    "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_polys", "xpg_db_reply_polys",
+    "xpg_db_import_ntt_words", "xpg_db_export_ntt_words", "xpg_db_save", "xpg_db_load", "xpg_db_polys",
+    "xpg_db_reply_polys", "xpg_db_levels",
     "xpg_db_query_words", "xpg_db_raw_bytes", "xpg_db_free", "xpg_reply", "xpg_reply_dev", "xpg_reply_dev_noalloc",
     "xpg_reply_batch_dev", "xpg_timing_enable", "xpg_timing_read", "xpg_timing_read_main", "xpg_timing_read_batched", "xpg_graph_stats", "xpg_autotune", "xpg_tune_set",
     "xpg_tune_get", "xpg_tune_clear", "xpg_keygen",
@@ -88,6 +89,9 @@ def lib() -> C.CDLL:
         "xpg_db_import_dev": (C.c_int, [vp, P, vp, u64, u64, u32, u32, C.POINTER(u32), C.POINTER(vp)]),
         "xpg_db_import_ntt_words": (C.c_int, [vp, P, vp, u64, u32, u64, u32, C.POINTER(u32), C.POINTER(vp)]),
         "xpg_db_export_ntt_words": (C.c_int, [vp, vp, vp, u64]),
+        "xpg_db_save": (C.c_int, [vp, vp, C.c_char_p, vp]),
+        "xpg_db_load": (C.c_int, [vp, P, C.c_char_p, vp, C.POINTER(vp)]),
+        "xpg_db_levels": (u32, [vp, C.POINTER(u32), u32]),
         "xpg_db_polys": (u32, [vp]),
         "xpg_db_reply_polys": (u32, [vp]),
         "xpg_db_query_words": (u64, [vp]),
@@ -288,6 +292,27 @@ class Context:
         self._check(lib().xpg_db_import_ntt_words(self.h, C.byref(prm), _ptr(words), units, P, raw_elem_bytes,
                                                   len(dims), self._dims(dims), C.byref(h)))
         return Database(self, h.value, prm, dims)
+
+    @staticmethod
+    def _digest(digest):
+        if digest is None:
+            return None
+        d = bytes(digest)
+        if len(d) != 32:
+            raise ValueError("digest is 32 bytes")
+        return d  # passed as a pointer to its bytes
+
+    def save_db(self, db: Database, path: str, digest: bytes = None):
+        """write the NTT database to `path` (xpg_db_save); digest: 32 bytes naming the raw database"""
+        self._check(lib().xpg_db_save(self.h, db.h, os.fsencode(path), self._digest(digest)))
+
+    def load_db(self, prm: XpgParams, path: str, digest: bytes = None) -> Database:
+        """a database from a file xpg_db_save wrote, checked against prm (and digest)"""
+        h = C.c_void_p()
+        self._check(lib().xpg_db_load(self.h, C.byref(prm), os.fsencode(path), self._digest(digest), C.byref(h)))
+        dims = (C.c_uint32 * 16)()
+        d = lib().xpg_db_levels(h.value, dims, 16)
+        return Database(self, h.value, prm, list(dims[:d]))
 
     def reply(self, db: Database, query: np.ndarray) -> np.ndarray:
         query = np.ascontiguousarray(query, dtype=np.uint64)

@@ -1,8 +1,12 @@
 // capi.hip -- host side of libxpir_amd.so: parameters, NTT tables, contexts, resident
 // databases and the d-level reply orchestration behind the C ABI (include/xpir_amd.h).
+#include <errno.h>
+#include <fcntl.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -574,6 +578,7 @@ static int db_setup(xpg_ctx *ctx, const xpg_params *prm, uint64_t n_elems, uint6
   db->n_elems = n_elems;
   db->elem_bytes = elem_bytes;
   db->alpha = alpha;
+  db->alpha_zero_pad = ctx->alpha_zero_pad;
   const uint64_t words = units * P0 * (uint64_t)prm->m * prm->n;
   (void)hipSetDevice(ctx->device);
   if (hipMalloc((void **)&db->words, words * 8) != hipSuccess) {
@@ -688,8 +693,215 @@ int xpg_db_export_ntt_words(xpg_ctx *ctx, const xpg_db *db, uint64_t *out, uint6
   return XPG_OK;
 }
 
+// ---- the NTT database in a file (no reference counterpart: the reference imports at
+// every start and keeps the imported database in RAM, apps/server/PIRServer.cpp:71-104) ----
+//
+// Layout: a 512-byte header (host byte order), then units x P0 x m x n words exactly as
+// they lie in HBM.  The header holds the parameters, the grid and sizes, the
+// partial-aggregate convention, the caller's 32-byte digest of the raw database, and an
+// order-keyed checksum of the words (launch_word_checksum) that the load recomputes on
+// the device.
+namespace {
+constexpr char kNttMagic[8] = {'X', 'P', 'G', 'N', 'T', 'T', 'D', 'B'};
+constexpr uint32_t kNttVersion = 1;
+constexpr size_t kNttHeader = 512;
+
+struct NttFileHeader {
+  char magic[8];
+  uint32_t version, header_bytes;
+  uint32_t k, n, logn, m, bits, Berr;
+  int32_t A;
+  uint32_t alpha;
+  uint64_t n0, moduli[4], abs_bytes, ct_bytes;
+  uint64_t n_elems, elem_bytes, units, real_units, words, checksum;
+  uint32_t d, P0, alpha_zero_pad, reserved;
+  uint32_t dims[XPG_MAX_LEVELS];
+  uint8_t digest[32];
+};
+static_assert(sizeof(NttFileHeader) <= kNttHeader, "header fits its block");
+
+bool same_params(const NttFileHeader &h, const xpg_params *p) {
+  if (h.k != p->k || h.n != p->n || h.logn != p->logn || h.m != p->m || h.bits != p->bits || h.Berr != p->Berr ||
+      h.A != p->A || h.n0 != p->n0 || h.abs_bytes != p->abs_bytes || h.ct_bytes != p->ct_bytes)
+    return false;
+  for (uint32_t c = 0; c < p->m && c < 4; c++)
+    if (h.moduli[c] != p->moduli[c]) return false;
+  return true;
+}
+
+// a checksum of words on the device, read back
+int device_checksum(xpg_ctx *ctx, const uint64_t *words, uint64_t n, uint64_t *out) {
+  uint64_t *dev = nullptr;
+  HIP_OK(hipMalloc((void **)&dev, 256), "checksum word");
+  int rc = launch_word_checksum(ctx, words, n, dev);
+  if (!rc && (hipMemcpyAsync(out, dev, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+              hipStreamSynchronize(ctx->stream) != hipSuccess))
+    rc = set_err(ctx, XPG_EDEVICE, "checksum read-back");
+  (void)hipFree(dev);
+  return rc;
+}
+
+struct Fd {
+  int fd = -1;
+  ~Fd() {
+    if (fd >= 0) close(fd);
+  }
+};
+
+bool write_all(int fd, const void *buf, size_t len) {
+  const char *p = static_cast<const char *>(buf);
+  while (len) {
+    const ssize_t w = write(fd, p, len);
+    if (w < 0 && errno == EINTR) continue;
+    if (w <= 0) return false;
+    p += w;
+    len -= (size_t)w;
+  }
+  return true;
+}
+
+// bytes read (short only at the end of the file), or -1 on an error
+ssize_t read_full(int fd, void *buf, size_t len, off_t at) {
+  char *p = static_cast<char *>(buf);
+  size_t got = 0;
+  while (got < len) {
+    const ssize_t r = pread(fd, p + got, len - got, at + (off_t)got);
+    if (r < 0 && errno == EINTR) continue;
+    if (r < 0) return -1;
+    if (r == 0) break;
+    got += (size_t)r;
+  }
+  return (ssize_t)got;
+}
+
+int ensure_pin(xpg_ctx *ctx) {
+  for (int b = 0; b < 2; b++) {
+    if (!ctx->pin[b]) HIP_OK(hipHostMalloc(&ctx->pin[b], kPinChunk, hipHostMallocDefault), "pinned staging chunk");
+    if (!ctx->pin_ev[b]) HIP_OK(hipEventCreateWithFlags(&ctx->pin_ev[b], hipEventDisableTiming), "staging event");
+  }
+  return XPG_OK;
+}
+}  // namespace
+
+int xpg_db_save(xpg_ctx *ctx, const xpg_db *db, const char *path, const uint8_t *digest) {
+  if (!ctx || !db || !path) return XPG_EINVAL;
+  if (db->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
+  NttFileHeader h{};
+  memcpy(h.magic, kNttMagic, 8);
+  h.version = kNttVersion;
+  h.header_bytes = (uint32_t)kNttHeader;
+  const xpg_params &p = db->prm;
+  h.k = p.k, h.n = p.n, h.logn = p.logn, h.m = p.m, h.bits = p.bits, h.Berr = p.Berr, h.A = p.A;
+  h.n0 = p.n0, h.abs_bytes = p.abs_bytes, h.ct_bytes = p.ct_bytes;
+  for (int c = 0; c < 4; c++) h.moduli[c] = p.moduli[c];
+  h.alpha = db->alpha;
+  h.n_elems = db->n_elems, h.elem_bytes = db->elem_bytes, h.units = db->units, h.real_units = db->real_units;
+  h.words = db->units * db->P0 * (uint64_t)p.m * p.n;
+  h.d = db->d, h.P0 = db->P0, h.alpha_zero_pad = (uint32_t)db->alpha_zero_pad;
+  for (uint32_t l = 0; l < db->d; l++) h.dims[l] = db->dims[l];
+  if (digest) memcpy(h.digest, digest, 32);
+  int rc = device_checksum(ctx, db->words, h.words, &h.checksum);
+  if (rc) return rc;
+  rc = ensure_pin(ctx);
+  if (rc) return rc;
+  // written under a temporary name and renamed once complete: a reader never sees a
+  // partial file under `path`
+  const std::string tmp = std::string(path) + ".tmp";
+  Fd f;
+  f.fd = open(tmp.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+  if (f.fd < 0) return set_err(ctx, XPG_EINVAL, std::string("cannot create ") + tmp + ": " + strerror(errno));
+  char block[kNttHeader] = {0};
+  memcpy(block, &h, sizeof(h));
+  bool ok = write_all(f.fd, block, kNttHeader);
+  // device -> two pinned chunks -> file: chunk i+1's copy runs while chunk i is written
+  const uint64_t bytes = h.words * 8;
+  const size_t nch = (size_t)((bytes + kPinChunk - 1) / kPinChunk);
+  auto enqueue = [&](size_t i) {
+    const size_t off = i * kPinChunk, len = std::min<uint64_t>(kPinChunk, bytes - off);
+    const int b = (int)(i & 1);
+    return hipMemcpyAsync(ctx->pin[b], reinterpret_cast<const char *>(db->words) + off, len, hipMemcpyDeviceToHost,
+                          ctx->stream) == hipSuccess &&
+           hipEventRecord(ctx->pin_ev[b], ctx->stream) == hipSuccess;
+  };
+  if (ok && nch) ok = enqueue(0);
+  for (size_t i = 0; ok && i < nch; i++) {
+    if (i + 1 < nch) ok = enqueue(i + 1);
+    const int b = (int)(i & 1);
+    const size_t off = i * kPinChunk, len = std::min<uint64_t>(kPinChunk, bytes - off);
+    ok = ok && hipEventSynchronize(ctx->pin_ev[b]) == hipSuccess && write_all(f.fd, ctx->pin[b], len);
+    // the buffer is free once written; the copy two chunks on reuses it
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  ok = ok && fsync(f.fd) == 0;
+  close(f.fd);
+  f.fd = -1;
+  if (!ok || rename(tmp.c_str(), path) != 0) {
+    (void)unlink(tmp.c_str());
+    return set_err(ctx, XPG_EDEVICE, std::string("writing ") + path + " failed");
+  }
+  return XPG_OK;
+}
+
+int xpg_db_load(xpg_ctx *ctx, const xpg_params *prm, const char *path, const uint8_t *digest, xpg_db **out) {
+  if (!ctx || !prm || !path || !out) return XPG_EINVAL;
+  *out = nullptr;
+  Fd f;
+  f.fd = open(path, O_RDONLY);
+  if (f.fd < 0) return set_err(ctx, XPG_EINVAL, std::string("cannot open ") + path + ": " + strerror(errno));
+  char block[kNttHeader];
+  if (read_full(f.fd, block, kNttHeader, 0) != (ssize_t)kNttHeader)
+    return set_err(ctx, XPG_ESIZE, "database file shorter than its header");
+  NttFileHeader h;
+  memcpy(&h, block, sizeof(h));
+  if (memcmp(h.magic, kNttMagic, 8) != 0 || h.version != kNttVersion || h.header_bytes != kNttHeader)
+    return set_err(ctx, XPG_EINVAL, "not a database file of this version");
+  if (!same_params(h, prm)) return set_err(ctx, XPG_EINVAL, "database file made under other parameters");
+  if (digest && memcmp(h.digest, digest, 32) != 0)
+    return set_err(ctx, XPG_EINVAL, "database file made from another database (digest differs)");
+  if (h.d == 0 || h.d > XPG_MAX_LEVELS || h.alpha == 0 || h.P0 == 0)
+    return set_err(ctx, XPG_EINVAL, "database file header out of range");
+  if (h.words != prod_dims(h.d, h.dims, 0) * h.P0 * (uint64_t)prm->m * prm->n)
+    return set_err(ctx, XPG_EINVAL, "database file header sizes disagree");
+  struct stat st;
+  if (fstat(f.fd, &st) != 0 || (uint64_t)st.st_size != kNttHeader + h.words * 8)
+    return set_err(ctx, XPG_ESIZE, "database file size differs from its header's");
+  xpg_db *db = nullptr;
+  int rc = db_setup(ctx, prm, h.n_elems, h.elem_bytes, h.alpha, h.d, h.dims, h.P0, h.real_units, &db);
+  if (rc) return rc;
+  db->alpha_zero_pad = (int)h.alpha_zero_pad;
+  rc = ensure_pin(ctx);
+  // file -> two pinned chunks -> device: chunk i is read while chunk i-1's copy runs
+  const uint64_t bytes = h.words * 8;
+  for (uint64_t off = 0, i = 0; !rc && off < bytes; off += kPinChunk, i++) {
+    const int b = (int)(i & 1);
+    const size_t len = std::min<uint64_t>(kPinChunk, bytes - off);
+    if (hipEventSynchronize(ctx->pin_ev[b]) != hipSuccess) rc = set_err(ctx, XPG_EDEVICE, "staging wait");
+    if (!rc && read_full(f.fd, ctx->pin[b], len, (off_t)(kNttHeader + off)) != (ssize_t)len)
+      rc = set_err(ctx, XPG_ESIZE, "database file read failed");
+    if (!rc && (hipMemcpyAsync(reinterpret_cast<char *>(db->words) + off, ctx->pin[b], len, hipMemcpyHostToDevice,
+                               ctx->stream) != hipSuccess ||
+                hipEventRecord(ctx->pin_ev[b], ctx->stream) != hipSuccess))
+      rc = set_err(ctx, XPG_EDEVICE, "H2D copy");
+  }
+  uint64_t sum = 0;
+  if (!rc) rc = device_checksum(ctx, db->words, h.words, &sum);
+  if (!rc && sum != h.checksum) rc = set_err(ctx, XPG_EINVAL, "database file checksum mismatch (corrupted words)");
+  if (rc) {
+    (void)hipStreamSynchronize(ctx->stream);
+    xpg_db_free(db);
+    return rc;
+  }
+  *out = db;
+  return XPG_OK;
+}
+
 uint32_t xpg_db_polys(const xpg_db *db) { return db ? db->P0 : 0; }
 uint32_t xpg_db_reply_polys(const xpg_db *db) { return db ? db->Pl[db->d] : 0; }
+uint32_t xpg_db_levels(const xpg_db *db, uint32_t *dims, uint32_t cap) {
+  if (!db) return 0;
+  for (uint32_t l = 0; dims && l < db->d && l < cap; l++) dims[l] = db->dims[l];
+  return db->d;
+}
 uint64_t xpg_db_raw_bytes(const xpg_db *db) { return db ? db->n_elems * db->elem_bytes : 0; }
 uint64_t xpg_db_query_words(const xpg_db *db) {
   if (!db) return 0;

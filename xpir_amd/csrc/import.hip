@@ -301,6 +301,16 @@ __global__ void fill_zero_kernel(uint64_t *dst, uint64_t words) {
     dst[i] = 0;
 }
 
+// Order-keyed checksum of n words: the sum over i of SplitMix64's output for word i
+// keyed by i (rand_word(w[i], i)), mod 2^64; one add per wave into *out (cleared before).
+__global__ void word_checksum_kernel(const uint64_t *w, uint64_t n, unsigned long long *out) {
+  uint64_t s = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    s += rand_word(w[i], i);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down((unsigned long long)s, off, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)s);
+}
+
 __global__ void random_bytes_kernel(uint64_t seed, uint64_t offset, uint8_t *dst, uint64_t nbytes) {
   // each thread writes the bytes of one stream word that overlap [offset, offset+nbytes)
   const uint64_t w0 = offset >> 3, w1 = (offset + nbytes + 7) >> 3;
@@ -416,6 +426,16 @@ int launch_encrypt(xpg_ctx *ctx, const xpg_params *prm, const EncSrc &src_in, ui
   if (blocks == 0) return XPG_OK;
   if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "query grid too large");
   return launch_ntt8<SRC_ENCRYPT>(ctx, prm->logn, blocks, a, bs, ac, es, out);
+}
+
+int launch_word_checksum(xpg_ctx *ctx, const uint64_t *words, uint64_t n, uint64_t *out_dev) {
+  int rc = launch_fill_zero(ctx, out_dev, 1);
+  if (rc || n == 0) return rc;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(word_checksum_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, words, n,
+                     reinterpret_cast<unsigned long long *>(out_dev));
+  return hipGetLastError() == hipSuccess ? XPG_OK : set_err(ctx, XPG_EDEVICE, "checksum launch");
 }
 
 int launch_random_bytes(xpg_ctx *ctx, uint64_t seed, uint64_t offset, uint8_t *dst, uint64_t nbytes) {

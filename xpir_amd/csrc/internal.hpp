@@ -59,6 +59,7 @@ struct xpg_db {
   uint32_t Pl[XPG_MAX_LEVELS + 1] = {0};  // polys per unit at each level; Pl[d] = reply polys
   uint64_t n_elems = 0, elem_bytes = 0;
   uint32_t alpha = 1;
+  int alpha_zero_pad = 0;   // the partial-aggregate convention the import used
   uint64_t *words = nullptr;  // units x P0 x m x n
   uint64_t serial = 0;        // unique per database ever created (graph cache key)
 };
@@ -194,6 +195,8 @@ int launch_encrypt(xpg_ctx *ctx, const xpg_params *prm, const EncSrc &src, uint6
                    uint64_t *out);
 int launch_random_bytes(xpg_ctx *ctx, uint64_t seed, uint64_t offset, uint8_t *dst, uint64_t nbytes);
 int launch_fill_zero(xpg_ctx *ctx, uint64_t *dst, uint64_t words);
+// *out_dev = sum_i rand_word(words[i], i) mod 2^64 (the NTT file's checksum), stream ordered.
+int launch_word_checksum(xpg_ctx *ctx, const uint64_t *words, uint64_t n, uint64_t *out_dev);
 
 // ---- launchers (mac.hip) ----
 // One reply level: out (groups x P x 2 x m x n) from db (groups*n_l x P x m x n) and
