@@ -128,8 +128,8 @@ int xpg_db_export_ntt_words(xpg_ctx *ctx, const xpg_db *db, uint64_t *out, uint6
  * (apps/server/PIRServer.cpp:71-104).  The file is a 512-byte header (parameters, grid,
  * sizes, the partial-aggregate convention of the import, `digest`: 32 bytes the caller
  * computes over the raw database, or zeros when NULL; an order-keyed checksum of the
- * words) and the words as they lie in HBM; xpg_db_save writes it under path + ".tmp"
- * and renames it when complete.  xpg_db_load refuses (XPG_EINVAL) a file of another
+ * words) and the words as they lie in HBM; xpg_db_save writes it under a temporary name
+ * of its own (path + ".tmp.<pid>.<n>") and renames it to path when complete.  xpg_db_load refuses (XPG_EINVAL) a file of another
  * format, made under other parameters than *prm, or with another digest (when digest is
  * not NULL), or whose words fail the checksum (recomputed on the device after the
  * copy); XPG_ESIZE for a file shorter or longer than its header says.  Host memory:

@@ -1,8 +1,9 @@
 // db_directory.cpp -- host-only checks of the facade's database handlers (no GPU):
 // DBDirectoryProcessor in one-file-per-element and file-splitting modes, its catalog
-// strings, zero padding of short files, DBHandler::readAggregatedStream, and two threads
-// reading one database through LockedDBHandler.
+// strings, zero padding of short files, DBHandler::readAggregatedStream, two threads
+// reading one database through LockedDBHandler, and the handlers' fingerprints.
 // usage: db_directory SCRATCH_DIR   (prints "ok" and exits 0 when every check holds)
+#include <fcntl.h>
 #include <stdio.h>
 #include <string.h>
 #include <sys/stat.h>
@@ -168,6 +169,27 @@ int main(int argc, char **argv) {
   t1.join();
   CHECK(bad[0] == 0 && bad[1] == 0);
   CHECK(!many.readElements(kFiles, 1, 0, kBytes, agg.data()));  // no such stream: refused
+
+  // fingerprints (what names a database file's contents): stable while nothing changes;
+  // new bytes under the same name and size change it (the modification time moves)
+  {
+    DBDirectoryProcessor a(dir), b(dir);
+    const std::string f0 = a.fingerprint();
+    CHECK(f0 == b.fingerprint());
+    CHECK(f0.compare(0, db.getCatalog(true).size(), db.getCatalog(true)) == 0);
+    struct timespec ts[2] = {{1000000000, 0}, {1000000000, 123}};  // atime, mtime
+    CHECK(utimensat(AT_FDCWD, (dir + "beta").c_str(), ts, 0) == 0);
+    CHECK(a.fingerprint() != f0);
+    std::vector<element_t> ev;
+    std::vector<char> e0 = pattern(40, 5);
+    ev.emplace_back("x", e0.size(), e0.data());
+    DBVectorProcessor v(ev);
+    const std::string v0 = v.fingerprint();
+    e0[17] ^= 1;  // same names and sizes, other bytes
+    CHECK(v.fingerprint() != v0);
+    DBGenerator g1(4, 16), g2(4, 16), g3(5, 16);
+    CHECK(g1.fingerprint() == g2.fingerprint() && g1.fingerprint() != g3.fingerprint());
+  }
 
   DBDirectoryProcessor too_many(2000, sdir);  // under one byte per element
   CHECK(too_many.getErrorStatus());

@@ -45,7 +45,7 @@ def test_loaded_database_answers_as_imported(gpu_ctx, tmp_path, desc, n, L, dims
     gpu_ctx.save_db(db, path, digest)
     words = db.export_words()
     assert os.path.getsize(path) == 512 + words.nbytes
-    assert not os.path.exists(path + ".tmp")
+    assert os.listdir(tmp_path) == ["db.ntt"]  # no temporary file left
     sk = orc.keygen(92)
     q = np.concatenate(orc.make_query(sk, dims, 7, 93))
     want = gpu_ctx.reply(db, q)

@@ -257,3 +257,46 @@ def test_concurrent_sessions_on_the_kept_database(tmp_path):
         q, want, elem, unit = expected[i]
         assert np.array_equal(results[i], want), i
         assert extract_reply(orc, sk, results[i], dims, unit) == elem, i
+
+
+def test_database_file_across_server_runs(tmp_path):
+    """--db-file: the first server run imports the directory and writes the database file;
+    a second run loads it instead of importing (same reply bytes); after one element file
+    changes, a third run imports again (the file names another database) and rewrites it."""
+    crypto, n, L, dims = "LWE:80:2048:120", 30, 4096, [6, 5]
+    orc = Oracle(crypto, dims[0])
+    db, files = _make_db(tmp_path, orc, n, L)
+    ntt = tmp_path / "db.ntt"
+    sk = orc.keygen(4)
+
+    def run(run_no, index, seed):
+        where = tmp_path / f"run{run_no}"  # the server's socket, log and working directory
+        where.mkdir()
+        with Server(where, "--db-dir", db, "--sessions", 1, "--no-pipeline", "--db-file", ntt) as srv:
+            c = srv.client(timeout=120)
+            c.hello()
+            names = _client_driven(c, orc, crypto, 2, 1, dims, files, L)
+            print(f"run {run_no}: index {index}", flush=True)
+            q, want, elem, unit = _expected(orc, files, names, L, 1, dims, sk, index, seed)
+            got = _exchange_query(c, orc, q, want)
+            c.close()
+            assert srv.wait(120) == 0, srv.log()
+            log = srv.log()
+        assert np.array_equal(got, want)
+        assert extract_reply(orc, sk, got, dims, unit) == elem
+        return log
+
+    log1 = run(1, 7, 3)
+    assert "loaded from the database file" not in log1 and ntt.exists()
+    size1, mtime1 = ntt.stat().st_size, ntt.stat().st_mtime_ns
+    log2 = run(2, 22, 5)
+    assert "loaded from the database file" in log2, log2
+    assert ntt.stat().st_mtime_ns == mtime1  # read, not rewritten
+    # one element changes (same name and size, new bytes): imported again, file rewritten
+    name = sorted(files)[3]
+    newb = bytes((b + 1) & 0xff for b in files[name])
+    (db / name).write_bytes(newb)
+    files[name] = newb
+    log3 = run(3, 3, 7)
+    assert "loaded from the database file" not in log3, log3
+    assert ntt.stat().st_size == size1 and ntt.stat().st_mtime_ns != mtime1

@@ -300,6 +300,38 @@ uint64_t DBVectorProcessor::readStream(uint64_t streamNb, char *buf, uint64_t si
 
 void DBVectorProcessor::closeStream(uint64_t streamNb) { offsets_.erase(streamNb); }
 
+// Four 64-bit FNV-1a lanes with distinct offset bases over the same bytes: a 32-byte
+// identity of a fingerprint string (not a cryptographic hash: it names the operator's own
+// database, it does not authenticate it).
+static void digest32(const std::string &text, uint8_t out[32]) {
+  for (int lane = 0; lane < 4; lane++) {
+    uint64_t h = 0xcbf29ce484222325ULL ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(lane + 1));
+    for (unsigned char c : text) h = (h ^ c) * 0x100000001b3ULL;
+    for (int b = 0; b < 8; b++) out[lane * 8 + b] = (uint8_t)(h >> (8 * b));
+  }
+}
+
+std::string DBVectorProcessor::fingerprint() {
+  std::string fp = getCatalog(true);
+  uint64_t h = 0xcbf29ce484222325ULL;
+  for (const element_t &e : elements_)
+    for (size_t i = 0; i < e.data_size; i++) h = (h ^ (unsigned char)e.data[i]) * 0x100000001b3ULL;
+  return fp + "bytes " + std::to_string(h) + "\n";
+}
+
+std::string DBDirectoryProcessor::fingerprint() {
+  std::string fp = getCatalog(true);
+  auto stamp = [&](const std::string &path) {
+    struct stat st;
+    if (stat(path.c_str(), &st) != 0) return std::string("missing\n");
+    return std::to_string((long long)st.st_size) + " " + std::to_string((long long)st.st_mtim.tv_sec) + "." +
+           std::to_string((long long)st.st_mtim.tv_nsec) + "\n";
+  };
+  if (splitting_) return fp + real_file_ + " " + stamp(real_file_);
+  for (const std::string &f : file_list_) fp += f + " " + stamp(directory_ + f);
+  return fp;
+}
+
 std::string DBVectorProcessor::getCatalog(bool typeOfCatalog) {
   if (!typeOfCatalog) return "1\n" + std::to_string(getNbStream()) + "\n" + std::to_string(max_bytes_) + "\n";
   std::string buf = "0\n" + std::to_string(getNbStream()) + "\n";
@@ -655,6 +687,44 @@ imported_database *PIRReplyGenerator::importData(uint64_t offset, uint64_t bytes
   out->nbElements = (nb + param_.alpha - 1) / param_.alpha;
   out->polysPerElement = xpg_db_polys(h);
   out->beforeImportElementBytesize = bytes_per_db_element;
+  return out;
+}
+
+imported_database *PIRReplyGenerator::importDataCached(uint64_t offset, uint64_t bytes_per_db_element,
+                                                       const std::string &path, bool *loaded) {
+  if (loaded) *loaded = false;
+  if (!ctx_) {
+    fail(XPG_EDEVICE, "no GPU context");
+    return nullptr;
+  }
+  std::string key = db_->fingerprint() + "offset " + std::to_string(offset) + " bytes " +
+                    std::to_string(bytes_per_db_element) + " alpha " + std::to_string(param_.alpha) + " d " +
+                    std::to_string(param_.d) + " dims";
+  for (unsigned l = 0; l < param_.d; l++) key += " " + std::to_string(param_.n[l]);
+  key += " crypto " + crypto_.descriptor() + "\n";
+  uint8_t digest[32];
+  digest32(key, digest);
+  xpg_db *h = nullptr;
+  if (xpg_db_load(ctx_, &prm_, path.c_str(), digest, &h) == XPG_OK) {
+    uint32_t dims[XPG_MAX_LEVELS];
+    const uint32_t d = xpg_db_levels(h, dims, XPG_MAX_LEVELS);
+    bool same = d == param_.d;
+    for (uint32_t l = 0; same && l < d; l++) same = dims[l] == param_.n[l];
+    if (same) {
+      imported_any_ = true;
+      imported_database *out = new imported_database();
+      out->imported_database_ptr = h;
+      out->nbElements = (db_->getNbStream() + param_.alpha - 1) / param_.alpha;
+      out->polysPerElement = xpg_db_polys(h);
+      out->beforeImportElementBytesize = bytes_per_db_element;
+      if (loaded) *loaded = true;
+      status_ = XPG_OK;
+      return out;
+    }
+    xpg_db_free(h);
+  }
+  imported_database *out = importData(offset, bytes_per_db_element);
+  if (out) (void)xpg_db_save(ctx_, out->imported_database_ptr, path.c_str(), digest);  // best effort
   return out;
 }
 

@@ -109,6 +109,12 @@ class DBHandler {
   // False when a stream does not open.  A handler shared between threads overrides this
   // to hold its lock across the whole sequence.  (No reference counterpart.)
   virtual bool readElements(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out);
+  // What identifies the database's contents for a file of its imported words
+  // (PIRReplyGenerator::importDataCached): this default is the full catalog (element
+  // names and sizes), which fixes the contents of generated databases; handlers whose
+  // bytes can change under the same names add what changes with them.  (No reference
+  // counterpart.)
+  virtual std::string fingerprint() { return getCatalog(true); }
   // The alpha streams of aggregate `streamNb`, each read from `offset` and zero-padded to
   // fileByteSize = min(bytes_per_file, maxFileBytesize - offset) bytes, into slot
   // (stream mod alpha) of rawBits; when the aggregate runs past the last stream, as many
@@ -131,6 +137,8 @@ class DBVectorProcessor : public DBHandler {
   // element names and sizes (DBVectorProcessor.cpp:50-66; the compact form there lacks the
   // newline after the count, which the client's parser needs: written with it here)
   std::string getCatalog(bool typeOfCatalog) override;
+  // the catalog and a hash of every element's bytes (they live in the caller's memory)
+  std::string fingerprint() override;
 
  private:
   std::vector<element_t> &elements_;
@@ -180,6 +188,8 @@ class DBDirectoryProcessor : public DBHandler {
   bool readStreams(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) override;
   static uint64_t getFileSize(const std::string &filePath);
   const std::vector<std::string> &fileList() const { return file_list_; }
+  // the catalog, and each file's size and modification time (the split file's in split mode)
+  std::string fingerprint() override;
 
  private:
   std::string directory_;
@@ -226,6 +236,10 @@ class LockedDBHandler : public DBHandler {
   bool readElements(uint64_t first, uint64_t count, uint64_t offset, uint64_t bytes, char *out) override {
     std::lock_guard<std::mutex> l(mu_);
     return in_->readElements(first, count, offset, bytes, out);
+  }
+  std::string fingerprint() override {
+    std::lock_guard<std::mutex> l(mu_);
+    return in_->fingerprint();
   }
 
  private:
@@ -313,6 +327,15 @@ class PIRReplyGenerator {
   // pushing may overlap importData on another thread) and its buffer is freed.
   void pushQuery(char *rawQuery);
   imported_database *importData(uint64_t offset, uint64_t bytes_per_db_element);
+  // importData through a file of the database's NTT words (xpg_db_save / xpg_db_load; no
+  // reference counterpart: the reference imports at every start).  When `path` holds
+  // this database -- same handler fingerprint, offset, element bytes, alpha, grid and
+  // cryptosystem -- it is loaded from there; otherwise (no file, another database or
+  // parameters, a damaged file) the database is imported and the file (re)written.  The
+  // file is an optimisation only: any failure to write it leaves the import in place.
+  // *loaded tells which happened.  Same ownership as importData.
+  imported_database *importDataCached(uint64_t offset, uint64_t bytes_per_db_element, const std::string &path,
+                                      bool *loaded = nullptr);
   void generateReply(const imported_database *database);
   void freeQueries();
   bool popReply(char **reply);

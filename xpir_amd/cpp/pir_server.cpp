@@ -50,6 +50,8 @@ void usage(const char *prog) {
           "      --db-dir DIR               database directory (default db/)\n"
           "      --optim-dir DIR            optimizer cache directory (default exp/)\n"
           "      --no-pipeline              import the database after the query upload\n"
+          "      --db-file FILE             keep the imported database's NTT words in FILE: load them from\n"
+          "                                 there when FILE holds this database, else import and write it\n"
           "      --sessions K               exit after K client sessions (default: serve until killed)\n"
           "      --concurrent N             serve up to N sessions at once (default 1)\n"
           "      --device N                 GPU (default 0)\n",
@@ -67,7 +69,7 @@ bool number(const char *s, uint64_t *out) {
 
 int main(int argc, char **argv) {
   int driven = -1;
-  std::string params_file = kDefaultParamsFile, db_dir = "db/", optim_dir = "exp/";
+  std::string params_file = kDefaultParamsFile, db_dir = "db/", optim_dir = "exp/", db_file;
   uint64_t split = 1, gen_n = 10, gen_l = 12800000, sessions = 0, device = 0, concurrent = 1;
   bool use_generator = false, no_pipeline = false;
   long listen_fd = -1, conn_fd = -1;
@@ -111,6 +113,8 @@ int main(int argc, char **argv) {
       text(&optim_dir);
     } else if (a == "--no-pipeline") {
       no_pipeline = true;
+    } else if (a == "--db-file") {
+      text(&db_file);
     } else if (a == "--sessions") {
       value(&sessions);
     } else if (a == "--concurrent") {
@@ -161,6 +165,7 @@ int main(int argc, char **argv) {
 
   session_option_t opt;
   opt.driven_mode = true;
+  opt.db_file = db_file;
   PIRParameters saved{};
   bool server_driven = false;
   if (driven == 0) {  // PIRServer::readPIRParamsFromFile (:116-125)
@@ -217,7 +222,9 @@ int main(int argc, char **argv) {
               "pir_server: session done, %llu reply ciphertexts; query upload %.3f ms, import %.3f ms%s, "
               "reply %.3f ms, send %.3f ms\n",
               (unsigned long long)s.repliesSent(), s.times().upload_ms, s.times().import_ms,
-              opt.got_preimported_database ? " (kept database)" : (no_pipeline ? "" : " (during the upload)"),
+              opt.got_preimported_database ? " (kept database)"
+              : s.times().db_loaded        ? " (loaded from the database file)"
+              : (no_pipeline ? "" : " (during the upload)"),
               s.times().reply_ms, s.times().send_ms);
     o.params = s.getPIRParams();
     o.kept = s.getSavedDatabase();

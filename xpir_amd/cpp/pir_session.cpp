@@ -249,7 +249,8 @@ bool PIRSession::queryAndReply(const session_option_t &opt) {
   std::string import_error;
   auto import = [&]() {
     const auto t0 = std::chrono::steady_clock::now();
-    db = gen.importData(0, db_->getmaxFileBytesize());
+    db = opt.db_file.empty() ? gen.importData(0, db_->getmaxFileBytesize())
+                             : gen.importDataCached(0, db_->getmaxFileBytesize(), opt.db_file, &times_.db_loaded);
     if (!db) import_error = gen.lastError();
     times_.import_ms = ms_since(t0);
   };

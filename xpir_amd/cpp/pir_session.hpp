@@ -66,6 +66,7 @@ struct SessionTimes {
   double import_ms = 0;  // database import (on its own thread, during the upload)
   double reply_ms = 0;   // generateReply: query to HBM, every level, reply back
   double send_ms = 0;    // reply ciphertexts written
+  bool db_loaded = false; // the database came from the database file (session_option_t::db_file)
 };
 
 // apps/server/PIRSession.hpp session_option_t
@@ -74,6 +75,10 @@ struct session_option_t {
   bool keep_database = false;             // keep this session's imported database
   bool got_preimported_database = false;  // `data` holds the database for these parameters
   imported_database *data = nullptr;
+  // a file of the imported database's NTT words: loaded instead of importing when it holds
+  // this database, else written after the import (PIRReplyGenerator::importDataCached;
+  // no reference counterpart)
+  std::string db_file;
 };
 
 class PIRSession {

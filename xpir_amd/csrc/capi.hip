@@ -804,9 +804,11 @@ int xpg_db_save(xpg_ctx *ctx, const xpg_db *db, const char *path, const uint8_t 
   if (rc) return rc;
   rc = ensure_pin(ctx);
   if (rc) return rc;
-  // written under a temporary name and renamed once complete: a reader never sees a
-  // partial file under `path`
-  const std::string tmp = std::string(path) + ".tmp";
+  // written under a temporary name of its own (process, call) and renamed once complete:
+  // a reader never sees a partial file under `path`, and two writers never share one
+  static std::atomic<uint64_t> saves{0};
+  const std::string tmp =
+      std::string(path) + ".tmp." + std::to_string((long long)getpid()) + "." + std::to_string(saves++);
   Fd f;
   f.fd = open(tmp.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
   if (f.fd < 0) return set_err(ctx, XPG_EINVAL, std::string("cannot create ") + tmp + ": " + strerror(errno));
