@@ -184,7 +184,7 @@ int xpg_reply_dev_noalloc(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_
  *      when its query count is above 4 and not a multiple of 4), followed by each
  *      chunk's recursion levels.  Option "batch_fuse" = 0 launches level 0 per chunk
  *      instead (same bytes).  Not for HIP graph capture: the launch uploads its tile
- *      table with an ordinary host-to-device copy. ---- */
+ *      table with a host-to-device copy (from the context's pinned table buffers). ---- */
 typedef struct {
   const xpg_db *db;
   const uint64_t *query_dev;

@@ -227,6 +227,25 @@ int copy_h2d_staged(xpg_ctx *ctx, void *dst, const void *src, size_t bytes) {
   return XPG_OK;
 }
 
+int copy_h2d_table(xpg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  const int b = ctx->tab_next;
+  ctx->tab_next ^= 1;
+  if (!ctx->tab_ev[b]) HIP_OK(hipEventCreateWithFlags(&ctx->tab_ev[b], hipEventDisableTiming), "table event");
+  HIP_OK(hipEventSynchronize(ctx->tab_ev[b]), "table wait");  // its last copy has read it
+  if (ctx->tab_bytes[b] < bytes) {
+    if (ctx->tab_pin[b]) (void)hipHostFree(ctx->tab_pin[b]);
+    ctx->tab_pin[b] = nullptr;
+    ctx->tab_bytes[b] = 0;
+    const size_t want = std::max<size_t>(bytes, 64 << 10);
+    HIP_OK(hipHostMalloc(&ctx->tab_pin[b], want, hipHostMallocDefault), "pinned table buffer");
+    ctx->tab_bytes[b] = want;
+  }
+  memcpy(ctx->tab_pin[b], src, bytes);
+  HIP_OK(hipMemcpyAsync(dst, ctx->tab_pin[b], bytes, hipMemcpyHostToDevice, ctx->stream), "table copy");
+  HIP_OK(hipEventRecord(ctx->tab_ev[b], ctx->stream), "table event");
+  return XPG_OK;
+}
+
 int ensure_io_buf(xpg_ctx *ctx, size_t bytes) {
   if (ctx->io_bytes >= bytes) return XPG_OK;
   if (ctx->io_buf) {
@@ -436,6 +455,8 @@ void xpg_ctx_destroy(xpg_ctx *ctx) {
   for (int b = 0; b < 2; b++) {
     if (ctx->pin[b]) (void)hipHostFree(ctx->pin[b]);
     if (ctx->pin_ev[b]) (void)hipEventDestroy(ctx->pin_ev[b]);
+    if (ctx->tab_pin[b]) (void)hipHostFree(ctx->tab_pin[b]);
+    if (ctx->tab_ev[b]) (void)hipEventDestroy(ctx->tab_ev[b]);
   }
   if (ctx->dec_buf) (void)hipFree(ctx->dec_buf);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);

@@ -106,6 +106,13 @@ struct xpg_ctx {
   // fill one while the DMA engine drains the other), each with the event of its last copy
   void *pin[2] = {nullptr, nullptr};
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  // two small pinned buffers through which launch tables reach the device without the
+  // stream drain a copy from pageable memory implies, taken in turn, each with the event of
+  // its last copy
+  void *tab_pin[2] = {nullptr, nullptr};
+  size_t tab_bytes[2] = {0, 0};
+  hipEvent_t tab_ev[2] = {nullptr, nullptr};
+  int tab_next = 0;
   uint64_t *dec_buf = nullptr;    // residues and lifted coefficients of xpg_decrypt_dev
   size_t dec_bytes = 0;
   // MAC kernel choice: 0 = by shape, 1 = one chain per lane, 3 = streaming block summaries
@@ -150,6 +157,9 @@ int ensure_level_buf(xpg_ctx *ctx, int which, size_t bytes);
 int ensure_batch_buf(xpg_ctx *ctx, size_t bytes);
 int ensure_io_buf(xpg_ctx *ctx, size_t bytes);
 int copy_h2d_staged(xpg_ctx *ctx, void *dst, const void *src, size_t bytes);
+// a small host table to device memory through the context's pinned table buffers,
+// ordered on its stream; src may be reused when the call returns
+int copy_h2d_table(xpg_ctx *ctx, void *dst, const void *src, size_t bytes);
 int ensure_dec_buf(xpg_ctx *ctx, size_t bytes);
 void graph_cache_clear(xpg_ctx *ctx, uint64_t db_serial);  // 0 = every entry
 ModSet modset_of(const xpg_params *prm);

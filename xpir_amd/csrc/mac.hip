@@ -1193,9 +1193,8 @@ int launch_mac_fused(xpg_ctx *ctx, const FusedPart *parts, uint32_t nparts, void
   }
   fa.counters = reinterpret_cast<uint32_t *>(dev_table);
   fa.ptrs = reinterpret_cast<const uint64_t *>(reinterpret_cast<char *>(dev_table) + 64);
-  if (hipMemcpyAsync(dev_table, image.data(), image.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream) !=
-      hipSuccess)
-    return set_err(ctx, XPG_EDEVICE, "fused tile table upload");
+  int rc = copy_h2d_table(ctx, dev_table, image.data(), image.size() * sizeof(uint64_t));
+  if (rc) return rc;
   const uint32_t grid = std::max<uint32_t>(8, std::min<uint32_t>(fused_grid(ctx), (uint32_t)((total + 7) / 8 * 8)));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timing_begin(ctx, &e0, &e1);
@@ -1203,7 +1202,7 @@ int launch_mac_fused(xpg_ctx *ctx, const FusedPart *parts, uint32_t nparts, void
   timing_begin(ctx, &k0, &k1, 2);
   if (ctx->timing && !ctx->ws) ctx->batch_steps += steps;
   hipLaunchKernelGGL((mac_fused_kernel<W>), dim3(grid), dim3(64 * W), 0, ctx->stream, fa);
-  const int rc = hipGetLastError() != hipSuccess ? set_err(ctx, XPG_EDEVICE, "fused MAC launch") : XPG_OK;
+  rc = hipGetLastError() != hipSuccess ? set_err(ctx, XPG_EDEVICE, "fused MAC launch") : XPG_OK;
   timing_end(ctx, k0, k1, 2);
   timing_end(ctx, e0, e1);
   return rc;
