@@ -188,3 +188,57 @@ def test_fused_launch_over_many_databases(gpu_ctx, budget):
             rd.free()
         for db in dbs:
             db.free()
+
+
+def test_fused_launch_mixes_cryptosystems(gpu_ctx):
+    """One fused level-0 launch over databases of three cryptosystems (n = 1024, 2048,
+    4096; one, two and three moduli): each segment runs at its own degree and modulus
+    count; every reply equals the plain path's and the oracle's."""
+    import xpir_amd
+    c1 = "LWE:80:1024:60"
+    # grids no wider in their second dimension than their first: A is sized from the first
+    # (setandgetAbsBitPerCiphertext(n0)), so a wider second level can exceed the noise budget
+    specs = [(c1, 1024, 5000, [32, 32], 7), (C2, 160, 1024, [16, 10], 9), ("LWE:100:4096:180", 90, 2500, [10, 10], 6),
+             (c1, 400, 9000, [40, 10], 12)]
+    rng = np.random.default_rng(11)
+    dbs, reqs, expect = [], [], []
+    try:
+        for si, (desc, n, L, dims, nq) in enumerate(specs):
+            orc = Oracle(desc, dims[0])
+            prm = xpir_amd.params(desc, dims[0])
+            db, data = _setup_db(gpu_ctx, orc, prm, n, L, dims, 700 + si)
+            dbs.append(db)
+            sk = orc.keygen(70 + si)
+            for qi in range(nq):
+                index = int(rng.integers(0, n))
+                q = np.concatenate(orc.make_query(sk, dims, index, 7000 + 100 * si + qi))
+                reqs.append((db, gpu_ctx.alloc(q.nbytes).upload(q), gpu_ctx.alloc(db.reply_words * 8)))
+                expect.append((orc, sk, dims, L, data[index * L:(index + 1) * L].tobytes(), q, qi))
+        print(f"fused mix: {len(specs)} databases of 3 cryptosystems, {len(reqs)} queries", flush=True)
+        gpu_ctx.timing(2)
+        gpu_ctx.timing_read_batched(reset=True)
+        gpu_ctx.reply_batch_dev([reqs[i] for i in rng.permutation(len(reqs))])
+        gpu_ctx.sync()
+        _, launches, _ = gpu_ctx.timing_read_batched(reset=True)
+        gpu_ctx.timing(False)
+        assert launches == 1  # every chunk in one fused launch
+        for k, (db, qd, rd) in enumerate(reqs):
+            orc, sk, dims, L, element, q, qi = expect[k]
+            got = rd.download()
+            assert np.array_equal(got, gpu_ctx.reply(db, q)), k
+            if qi == 0:
+                words = db.export_words()
+                levels, off = [], 0
+                for nl in dims:
+                    levels.append(q[off:off + nl * 2 * orc.mn])
+                    off += nl * 2 * orc.mn
+                want, _ = orc.generate_reply(words, db.polys, dims, levels)
+                assert np.array_equal(got, want), k
+            assert extract_reply(orc, sk, got, dims, L) == element
+    finally:
+        gpu_ctx.timing(False)
+        for _, qd, rd in reqs:
+            qd.free()
+            rd.free()
+        for db in dbs:
+            db.free()
