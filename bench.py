@@ -418,10 +418,10 @@ def main_kernel_name(tuned, n_l):
             k, a, b = e["kernel"], e["a"], e["b"]
             if k == 1:
                 return "mac_direct_kernel"
-            if k == 3:
-                return f"mac_stream_kernel<{a or 8},{'true' if b else 'false'}>"
+            if k == 3:  # b: the load schedule (0 load then compute, 1 double-buffered, 2 rolling)
+                return f"mac_stream_kernel<{a or 8},{b}>"
             return f"mac_batch2_kernel<{a},{b}>"
-    return "mac_stream_kernel<8,false>"
+    return "mac_stream_kernel<8,0>"
 
 
 def autotune_db(ctx, db, max_batch, args, label):

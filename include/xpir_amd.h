@@ -68,7 +68,9 @@ int xpg_sync(xpg_ctx *ctx);
 /* tuning knobs: "mac_kernel" = 0 (by shape), 1 (one chain per lane), 3 (streaming block
  * summaries; what 0 picks for few, long chains); "stream_e" = elements per streaming
  * block (0 = sized for one pass), "stream_u" = elements per load chunk 4|8|16 (0 = 8),
- * "stream_pf" = 1 to double-buffer the chunks; "graphs" = 1 to replay each reply's
+ * "stream_pf" = the chunks' load schedule: 0 by rule (rolling), 1 double-buffered, 2
+ * rolling (the next chunk's loads go into the registers each group of four elements
+ * frees), 3 a chunk's loads then its arithmetic; "graphs" = 1 to replay each reply's
  * launches (every level, the recursion between them) from a HIP graph captured on the
  * second call with the same database, query buffer and reply buffer (the first runs
  * eagerly and sizes the context buffers; any option change or buffer growth re-captures;

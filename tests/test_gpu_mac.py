@@ -90,8 +90,9 @@ def test_single_query_full_size_matches_golden(gpu_ctx):
         db.free()
 
 
-@pytest.mark.parametrize("pf", [0, 1])
-@pytest.mark.parametrize("u,e", [(4, 16), (8, 64), (8, 256), (16, 512), (8, 4096), (16, 48), (4, 1000)])
+@pytest.mark.parametrize("pf", [3, 1, 2])  # load then compute, double-buffered, rolling
+@pytest.mark.parametrize("u,e", [(4, 16), (8, 64), (8, 256), (16, 512), (8, 4096), (16, 48), (4, 1000), (8, 8),
+                                 (8, 24), (4, 12)])
 def test_streaming_block_shapes_match_oracle(gpu_ctx, u, e, pf):
     """The streaming MAC (block summaries + scan) at several block and chunk sizes: partial
     last blocks and chunks, one block per chain, many blocks per chain."""

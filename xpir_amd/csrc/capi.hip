@@ -490,7 +490,7 @@ static int set_option(xpg_ctx *ctx, const char *key, int64_t value) {
     ctx->mac_mode = (int)value;
     return XPG_OK;
   }
-  if (strcmp(key, "stream_pf") == 0 && (value == 0 || value == 1)) {
+  if (strcmp(key, "stream_pf") == 0 && value >= 0 && value <= 3) {
     ctx->stream_pf = (int)value;
     return XPG_OK;
   }

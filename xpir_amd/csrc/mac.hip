@@ -644,25 +644,27 @@ __device__ __forceinline__ void z_to_j_r(uint64_t z, int32_t zh, uint64_t p, uin
   R = S - b * p;
 }
 
-// Loads of one chunk of U elements (x, a, b words of this lane); the elements at or past
-// `valid` read as zero and are not summed.
+// Loads of one chunk of U elements (x, a, b words of this lane): xp / qp are the chunk's
+// wave-uniform row bases and lo the lane's byte offset in a row (scalar base plus one
+// 32-bit vector offset per load, no per-lane address arithmetic); the elements at or
+// past `valid` read as zero and are not summed.
 template <int U>
-__device__ __forceinline__ void stream_load(const uint64_t *xp, const uint64_t *qp, uint64_t xstride, uint64_t qstride,
-                                            uint64_t mn, int valid, uint64_t (&xv)[U], uint64_t (&av)[U],
-                                            uint64_t (&bv)[U]) {
+__device__ __forceinline__ void stream_load(const uint64_t *xp, const uint64_t *qp, uint32_t lo, uint64_t xstride,
+                                            uint64_t qstride, uint64_t mn, int valid, uint64_t (&xv)[U],
+                                            uint64_t (&av)[U], uint64_t (&bv)[U]) {
   if (valid >= U) {
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      xv[j] = __builtin_nontemporal_load(xp + j * xstride);
-      av[j] = __builtin_nontemporal_load(qp + j * qstride);
-      bv[j] = __builtin_nontemporal_load(qp + j * qstride + mn);
+      xv[j] = __builtin_nontemporal_load(lane_word(xp + j * xstride, lo));
+      av[j] = __builtin_nontemporal_load(lane_word(qp + j * qstride, lo));
+      bv[j] = __builtin_nontemporal_load(lane_word(qp + j * qstride + mn, lo));
     }
   } else {
 #pragma unroll
     for (int j = 0; j < U; j++) {
-      xv[j] = j < valid ? xp[j * xstride] : 0;
-      av[j] = j < valid ? qp[j * qstride] : 0;
-      bv[j] = j < valid ? qp[j * qstride + mn] : 0;
+      xv[j] = j < valid ? *lane_word(xp + j * xstride, lo) : 0;
+      av[j] = j < valid ? *lane_word(qp + j * qstride, lo) : 0;
+      bv[j] = j < valid ? *lane_word(qp + j * qstride + mn, lo) : 0;
     }
   }
 }
@@ -695,10 +697,13 @@ __device__ __forceinline__ void stream_sum(ChainSum &sa, ChainSum &sb, int valid
   }
 }
 
-// One wave = one (column, block), read in chunks of U elements.  PF: two register sets in
-// turn, so the next chunk's loads are in flight while the current chunk's products and
-// chain steps run.
-template <int U, bool PF>
+// One wave = one (column, block), read in chunks of U elements.  PF 0: a chunk's loads,
+// then its arithmetic (other waves fill the wait).  PF 1: two register sets in turn, so
+// the next chunk's loads are in flight while the current chunk's products and chain
+// steps run.  PF 2 (rolling): one register set; as each group of four elements' products
+// free their registers, the next chunk's loads for those four elements go out, so a wave
+// keeps about a chunk of loads in flight through its arithmetic at PF 0's register cost.
+template <int U, int PF>
 __global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
   const uint32_t lane = threadIdx.x & 63;
   // wave-uniform: global wave -> (block, column); adjacent waves take adjacent columns
@@ -715,28 +720,66 @@ __global__ void __launch_bounds__(256) mac_stream_kernel(StreamArgs a) {
   const uint64_t xstride = (uint64_t)a.P * mn, qstride = 2 * mn;
   const uint64_t e0 = (uint64_t)blk * a.E;
   const int64_t n = (int64_t)(min<uint64_t>(a.n_l, e0 + a.E) - e0);  // elements in this block
-  const uint64_t *xp = a.db + ((uint64_t)g * a.n_l + e0) * xstride + (uint64_t)cw * 64 + lane;
-  const uint64_t *qp = a.q + e0 * qstride + (uint64_t)c * a.n + sb * 64 + lane;
+  // wave-uniform row bases; the lane reads word `lane` of each 64-word row segment
+  const uint64_t *xp = a.db + ((uint64_t)g * a.n_l + e0) * xstride + (uint64_t)cw * 64;
+  const uint64_t *qp = a.q + e0 * qstride + (uint64_t)c * a.n + sb * 64;
+  const uint32_t lo = lane * 8;
   ChainSum sa{0, 0, 0, INT32_MIN}, sbs{0, 0, 0, INT32_MIN};
-  if (!PF) {
+  if (PF == 0) {
     // one register set: the chunk's loads, then its arithmetic (other waves fill the gap)
     for (int64_t e = 0; e < n; e += U) {
       uint64_t xv[U], av[U], bv[U];
       const int valid = (int)min<int64_t>(n - e, U);
-      stream_load<U>(xp + e * xstride, qp + e * qstride, xstride, qstride, mn, valid, xv, av, bv);
+      stream_load<U>(xp + e * xstride, qp + e * qstride, lo, xstride, qstride, mn, valid, xv, av, bv);
       stream_sum<U>(sa, sbs, valid, xv, av, bv, p, pv, delta);
+    }
+  } else if (PF == 2) {
+    uint64_t xv[U], av[U], bv[U];
+    stream_load<U>(xp, qp, lo, xstride, qstride, mn, (int)min<int64_t>(n, U), xv, av, bv);
+    int64_t e = 0;
+    // the registers hold chunk e (loaded or in flight); while it and the next are whole,
+    // each group of four elements: products, then the next chunk's loads into the freed
+    // registers, then the group's summary step
+    for (; e + 2 * U <= n; e += U) {
+      const uint64_t *xn = xp + (e + U) * xstride, *qn = qp + (e + U) * qstride;
+#pragma unroll
+      for (int j = 0; j < U; j += 4) {
+        uint64_t ra[4], rb[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          ra[i] = shoup_lazy_asm(xv[j + i], av[j + i], shoup_quot(av[j + i], p, delta), delta);
+          rb[i] = shoup_lazy_asm(xv[j + i], bv[j + i], shoup_quot(bv[j + i], p, delta), delta);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // the loads go out here, not at their first use
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          xv[j + i] = __builtin_nontemporal_load(lane_word(xn + (j + i) * xstride, lo));
+          av[j + i] = __builtin_nontemporal_load(lane_word(qn + (j + i) * qstride, lo));
+          bv[j + i] = __builtin_nontemporal_load(lane_word(qn + (j + i) * qstride + mn, lo));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        zsum_group4(sa, ra[0], ra[1], ra[2], ra[3], pv);
+        zsum_group4(sbs, rb[0], rb[1], rb[2], rb[3], pv);
+      }
+    }
+    // the last one or two chunks (the current whole or partial, the next partial or none)
+    for (; e < n; e += U) {
+      stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e, U), xv, av, bv, p, pv, delta);
+      if (e + U < n)
+        stream_load<U>(xp + (e + U) * xstride, qp + (e + U) * qstride, lo, xstride, qstride, mn,
+                       (int)min<int64_t>(n - e - U, U), xv, av, bv);
     }
   } else {
   uint64_t x0[U], a0[U], b0[U], x1[U], a1[U], b1[U];
-  stream_load<U>(xp, qp, xstride, qstride, mn, (int)min<int64_t>(n, U), x0, a0, b0);
+  stream_load<U>(xp, qp, lo, xstride, qstride, mn, (int)min<int64_t>(n, U), x0, a0, b0);
   for (int64_t e = 0; e < n; e += 2 * U) {
     if (e + U < n)
-      stream_load<U>(xp + (e + U) * xstride, qp + (e + U) * qstride, xstride, qstride, mn,
+      stream_load<U>(xp + (e + U) * xstride, qp + (e + U) * qstride, lo, xstride, qstride, mn,
                      (int)min<int64_t>(n - e - U, U), x1, a1, b1);
     stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e, U), x0, a0, b0, p, pv, delta);
     if (e + U >= n) break;
     if (e + 2 * U < n)
-      stream_load<U>(xp + (e + 2 * U) * xstride, qp + (e + 2 * U) * qstride, xstride, qstride, mn,
+      stream_load<U>(xp + (e + 2 * U) * xstride, qp + (e + 2 * U) * qstride, lo, xstride, qstride, mn,
                      (int)min<int64_t>(n - e - 2 * U, U), x0, a0, b0);
     stream_sum<U>(sa, sbs, (int)min<int64_t>(n - e - U, U), x1, a1, b1, p, pv, delta);
   }
@@ -828,7 +871,7 @@ __global__ void __launch_bounds__(128 * FIX_PARTS) mac_stream_fixup_kernel(Strea
 
 namespace {
 // Streaming launch: block summaries into scratch, then the per-column scan.
-template <int U, bool PF>
+template <int U, int PF>
 int launch_stream(xpg_ctx *ctx, StreamArgs &st) {
   const uint64_t recs = (uint64_t)st.nblk * st.columns * 128;
   const uint64_t waves = (uint64_t)st.nblk * st.columns;
@@ -871,9 +914,10 @@ uint64_t stream_block_elems(int cu_count, uint64_t n_l, uint64_t columns, int U,
   return (E + U - 1) / U * U;
 }
 
-// The streaming MAC for one level: U elements per load chunk, PF double-buffered chunks,
-// E elements per block (0 = auto_e).
-int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t groups, int U, bool pf, uint64_t E) {
+// The streaming MAC for one level: U elements per load chunk, pf the load schedule (0
+// load then compute, 1 double-buffered chunks, 2 rolling), E elements per block (0 =
+// auto_e).
+int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t groups, int U, int pf, uint64_t E) {
   StreamArgs st{};
   st.ms = a.ms;
   st.db = a.db;
@@ -889,14 +933,21 @@ int launch_stream_level(xpg_ctx *ctx, const MacArgs &a, uint32_t n, uint64_t gro
   if (groups * st.cpg > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.columns = (uint32_t)(groups * st.cpg);
   if (U != 4 && U != 8 && U != 16) return set_err(ctx, XPG_EINVAL, "streaming MAC chunk not built");
+  if (pf < 0 || pf > 2) return set_err(ctx, XPG_EINVAL, "streaming MAC load schedule not built");
   E = stream_block_elems(ctx->cu_count, a.n_l, st.columns, U, E);
   if (E > 0xffffffffULL || (a.n_l + E - 1) / E > 0xffffffffULL) return set_err(ctx, XPG_EINVAL, "MAC grid too large");
   st.E = (uint32_t)E;
   st.nblk = (uint32_t)((a.n_l + E - 1) / E);
-  switch (U) {
-    case 4: return pf ? launch_stream<4, true>(ctx, st) : launch_stream<4, false>(ctx, st);
-    case 16: return pf ? launch_stream<16, true>(ctx, st) : launch_stream<16, false>(ctx, st);
-    default: return pf ? launch_stream<8, true>(ctx, st) : launch_stream<8, false>(ctx, st);
+  switch (U * 4 + pf) {
+    case 16: return launch_stream<4, 0>(ctx, st);
+    case 17: return launch_stream<4, 1>(ctx, st);
+    case 18: return launch_stream<4, 2>(ctx, st);
+    case 64: return launch_stream<16, 0>(ctx, st);
+    case 65: return launch_stream<16, 1>(ctx, st);
+    case 66: return launch_stream<16, 2>(ctx, st);
+    case 33: return launch_stream<8, 1>(ctx, st);
+    case 34: return launch_stream<8, 2>(ctx, st);
+    default: return launch_stream<8, 0>(ctx, st);
   }
 }
 
@@ -964,7 +1015,9 @@ LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n
   } else {
     c.kernel = 3;
     c.a = ctx->stream_u ? ctx->stream_u : 8;
-    c.b = ctx->stream_pf;
+    // load schedule: option stream_pf 0 (unset) = the rule's rolling loads, 3 = load then
+    // compute, 1 / 2 as they are (LevelCfg.b: 0 load then compute, 1 double-buffered, 2 rolling)
+    c.b = ctx->stream_pf == 0 ? 2 : ctx->stream_pf == 3 ? 0 : ctx->stream_pf;
     c.c = ctx->stream_e;
   }
   return c;
@@ -993,7 +1046,7 @@ int launch_mac_level_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   timing_begin(ctx, &e0, &e1);
   int rc = XPG_OK;
   if (cfg.kernel == 3) {
-    rc = launch_stream_level(ctx, a, prm->n, groups, cfg.a ? cfg.a : 8, cfg.b != 0, (uint64_t)cfg.c);
+    rc = launch_stream_level(ctx, a, prm->n, groups, cfg.a ? cfg.a : 8, cfg.b, (uint64_t)cfg.c);
   } else if (cfg.kernel == 1) {
     const uint64_t lanes = groups * P * (uint64_t)prm->m * prm->n;
     const uint64_t blocks = (lanes + 255) / 256;

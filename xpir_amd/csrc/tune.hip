@@ -156,9 +156,11 @@ std::vector<LevelCfg> level_candidates(const xpg_ctx *ctx, const xpg_params *prm
   }
   if (prm->n >= 64 && n_l >= 2) {
     for (int u : {4, 8, 16})
-      for (int pf : {0, 1}) {
+      for (int pf : {0, 1, 2}) {
         const uint64_t e0 = stream_auto_e(ctx, prm, n_l, groups, P, u);
-        for (uint64_t e : {e0, 2 * e0, std::max<uint64_t>(u, e0 / 2)})
+        // one pass at four, two, one and eight waves per SIMD, and at three (the
+        // register-heavier schedules fit three)
+        for (uint64_t e : {e0, 2 * e0, 4 * e0, std::max<uint64_t>(u, e0 / 2), (e0 * 4 / 3 + u - 1) / u * u})
           if (e <= (1u << 20)) add(LevelCfg{3, u, pf, (int)e});
       }
   }
@@ -192,7 +194,7 @@ bool entry_valid(const xpg_tune_entry &e) {
   if (e.kind != 0 || e.qn != 0) return false;
   switch (e.kernel) {
     case 1: return true;
-    case 3: return (e.a == 4 || e.a == 8 || e.a == 16) && (e.b == 0 || e.b == 1) && e.c >= 0 && e.c <= (1 << 20);
+    case 3: return (e.a == 4 || e.a == 8 || e.a == 16) && e.b >= 0 && e.b <= 2 && e.c >= 0 && e.c <= (1 << 20);
     case 4: return (e.a == 0 || e.a == 1 || e.a == 4 || e.a == 8 || e.a == 16) && e.b == 1 &&
                    (e.c == 0 || e.c == 2 || e.c == 3) && (e.c != 3 || e.a == 4 || e.a == 8) && (e.a != 16 || e.c != 3);
     default: return false;
