@@ -1018,7 +1018,11 @@ LevelCfg default_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n
     // load schedule: option stream_pf 0 (unset) = the rule's rolling loads, 3 = load then
     // compute, 1 / 2 as they are (LevelCfg.b: 0 load then compute, 1 double-buffered, 2 rolling)
     c.b = ctx->stream_pf == 0 ? 2 : ctx->stream_pf == 3 ? 0 : ctx->stream_pf;
-    c.c = ctx->stream_e;
+    // elements per block: the option, else for the rolling loads twice the one-pass size
+    // (two waves per SIMD keep enough loads in flight, with longer runs per wave)
+    c.c = ctx->stream_e ? ctx->stream_e
+                        : c.b == 2 ? (int)std::min<uint64_t>(1u << 20, 2 * stream_auto_e(ctx, prm, n_l, groups, P, c.a))
+                                   : 0;
   }
   return c;
 }
