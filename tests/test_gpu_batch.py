@@ -242,3 +242,55 @@ def test_fused_launch_mixes_cryptosystems(gpu_ctx):
             rd.free()
         for db in dbs:
             db.free()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_fused_random_mixes_equal_plain(gpu_ctx, seed):
+    """Seeded random mixes: 5-8 databases of random element count and size, d = 1, 2 or 3
+    over random grids, 1..80 queries each (above 64, one database gives two chunks in the
+    same fused launch), interleaved.  Every reply of the fused launch equals the per-chunk
+    launches' and the plain path's."""
+    import xpir_amd
+    rng = np.random.default_rng(1000 + seed)
+    dbs, reqs = [], []
+    try:
+        for si in range(int(rng.integers(5, 9))):
+            desc = C2 if rng.random() < 0.7 else "LWE:80:1024:60"
+            d = int(rng.choice([1, 2, 2, 3]))
+            if d == 1:
+                dims = [int(rng.integers(8, 60))]
+            elif d == 2:
+                d0 = int(rng.integers(4, 24))
+                dims = [d0, int(rng.integers(2, d0 + 1))]
+            else:
+                d0 = int(rng.integers(4, 10))
+                dims = [d0, int(rng.integers(2, d0 + 1)), int(rng.integers(2, 4))]
+            n = int(rng.integers(max(1, int(np.prod(dims)) // 2), int(np.prod(dims)) + 1))
+            L = int(rng.integers(16, 6000))
+            orc = Oracle(desc, dims[0])
+            prm = xpir_amd.params(desc, dims[0])
+            db, _ = _setup_db(gpu_ctx, orc, prm, n, L, dims, 9000 + 100 * seed + si)
+            dbs.append(db)
+            sk = orc.keygen(90 + si)
+            nq = int(rng.integers(1, 81))
+            for qi in range(nq):
+                q = np.concatenate(orc.make_query(sk, dims, int(rng.integers(0, n)), 90000 + 1000 * si + qi))
+                reqs.append((db, gpu_ctx.alloc(q.nbytes).upload(q), gpu_ctx.alloc(db.reply_words * 8), q))
+        print(f"random mix {seed}: {len(dbs)} databases, {len(reqs)} queries", flush=True)
+        batch = [reqs[i][:3] for i in rng.permutation(len(reqs))]
+        gpu_ctx.reply_batch_dev(batch)
+        fused = [rd.download() for _, _, rd, _ in reqs]
+        gpu_ctx.set_option("batch_fuse", 0)
+        for _, _, rd, _ in reqs:
+            rd.upload(np.zeros(rd.nbytes // 8, dtype=np.uint64))
+        gpu_ctx.reply_batch_dev(batch)
+        for k, (db, qd, rd, q) in enumerate(reqs):
+            assert np.array_equal(fused[k], rd.download()), k
+            assert np.array_equal(fused[k], gpu_ctx.reply(db, q)), k
+    finally:
+        gpu_ctx.set_option("batch_fuse", 1)
+        for _, qd, rd, _ in reqs:
+            qd.free()
+            rd.free()
+        for db in dbs:
+            db.free()
