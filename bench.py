@@ -144,6 +144,10 @@ def main():
                     help="skip the MAC autotuner (fixed kernel rules); it runs before any timed region")
     ap.add_argument("--tune-file", help="install this saved tuning table instead of tuning when it exists; "
                                         "else tune and save the table there")
+    ap.add_argument("--workers", type=int, default=2, help="serving: worker contexts (HIP streams) of the server")
+    ap.add_argument("--heavy-steps", type=int, default=0,
+                    help="serving: exact chain steps above which a batch is heavy (one heavy batch at a time; "
+                         "0 = the server's default)")
     ap.add_argument("--batch-fuse", type=int, default=1, choices=(0, 1),
                     help="batched passes: level 0 of every database in fused persistent launches (1) or one "
                          "launch per database chunk (0)")
@@ -978,8 +982,10 @@ def bench_serving(args):
     pick = rng.integers(0, pool_n, size=n)
     use_d1 = rng.integers(0, 2, size=n)  # d in {1, 2} where a d = 1 view exists
     gaps = rng.exponential(1.0 / args.rate, size=n)
-    srv = xpir_amd.Server(0, reply_pool_bytes=8 << 30, workers=2, max_batch=64)
+    srv = xpir_amd.Server(0, reply_pool_bytes=8 << 30, workers=args.workers, max_batch=64,
+                          heavy_steps=args.heavy_steps)
     expected, lat, refused, served_raw, mismatches = {}, [], 0, 0, 0
+    per_db = {}
     i = 0
     t0 = time.perf_counter()
     t_next = t0
@@ -988,21 +994,24 @@ def bench_serving(args):
         now = time.perf_counter()
         while i < n and now >= t_next:
             d = dbs[choice[i]]
+            label = f"db{choice[i]}"
             if "d1" in d and use_d1[i]:
                 d = d["d1"]
+                label += "-d1"
             qd, digest = d["pool"][pick[i] % len(d["pool"])]
             t = srv.submit(d["db"], qd)
             if t is None:
                 refused += 1
                 break
-            expected[t] = (digest, d["raw"])
+            expected[t] = (digest, d["raw"], label)
             t_next += gaps[i]
             i += 1
         for c in srv.poll(timeout_us=500):
-            digest, raw = expected.pop(c.ticket)
+            digest, raw, label = expected.pop(c.ticket)
             got = ctx.download_ptr(c.reply_dev, c.reply_words)
             mismatches += hashlib.sha256(got.tobytes()).hexdigest() != digest
             lat.append((c.done_us - c.submit_us) / 1e3)
+            per_db.setdefault(label, []).append(lat[-1])
             served_raw += raw
             srv.release(c.ticket)
         if now - last_log > 10:
@@ -1016,12 +1025,15 @@ def bench_serving(args):
         "metric": "DB GB/s per reply, batched vs plain, 1 MI355X, bit-exact; p50/p99 reply ms",
         "workload": "serving", "value": served_raw / makespan / 1e9, "unit": "GB/s",
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "latency_ms_per_db": {k: {"n": len(v), "p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99))}
+                              for k, v in sorted(per_db.items())},
         "queries": n, "rate_per_s": args.rate, "makespan_s": makespan, "refused_submits": refused,
         "batches": st["batches"], "bit_exact": mismatches == 0, "mismatches": int(mismatches), "n_gpus": 1,
         "autotune_s": None if args.no_autotune else tune_s,
         "config": {"workload": "8 resident LWE:80:2048:120 databases, Zipf(1.1) choice, Poisson arrivals, "
                                "d=2 (d in {1,2} on the two smallest)",
-                   "scale": args.scale, "workers": 2, "max_batch": 64, "reply_pool_bytes": 8 << 30},
+                   "scale": args.scale, "workers": args.workers, "max_batch": 64, "reply_pool_bytes": 8 << 30,
+                   "heavy_steps": args.heavy_steps},
     }
     print(json.dumps(out), flush=True)
     ctx.close()

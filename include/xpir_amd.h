@@ -206,12 +206,18 @@ int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs);
  *      XPG_EBUSY (submit again after releasing replies); one whose intermediates alone
  *      exceed a worker's workspace can never run and gets XPG_ESIZE.  A batch takes as
  *      many of one database's pending requests as a workspace holds (at most
- *      max_batch). ---- */
+ *      max_batch).  Batches are taken in arrival order of their oldest request, except
+ *      that at most one heavy batch (more exact chain steps than heavy_steps) runs at a
+ *      time: while one does, an idle worker takes the oldest light batch instead, so
+ *      light requests never queue behind two heavy ones and heavy batches do not share
+ *      the GPU with each other. ---- */
 typedef struct xpg_server xpg_server;
 typedef struct {
   uint64_t reply_pool_bytes;  /* the server's device memory: slots + workspaces */
   uint32_t workers;           /* batches in flight (one stream each)   */
   uint32_t max_batch;         /* queries per batch                     */
+  uint64_t heavy_steps;       /* heavy batch threshold in exact chain steps (0 = 1e9,
+                                 UINT64_MAX = every batch light)        */
 } xpg_server_config;
 typedef struct {
   uint64_t ticket;

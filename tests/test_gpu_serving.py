@@ -245,3 +245,55 @@ def test_concurrent_submitters(gpu_ctx):
             for qd, _ in pool:
                 qd.free()
             db.free()
+
+
+def test_heavy_batches_run_one_at_a_time(gpu_ctx):
+    """heavy_steps: a burst over a heavy and a light database on two workers.  Batches of
+    the heavy database (each above the threshold) never run at the same time -- each
+    starts after the previous one's completion -- while light batches still run; every
+    reply equals the same query answered alone."""
+    import xpir_amd
+    rng = np.random.default_rng(17)
+    # (n, L, dims): about 18.7 M exact steps per heavy request, 2.2 M per light one
+    specs = [(2000, 4096, [50, 40]), (200, 256, [20, 10])]
+    dbs = []
+    for si, (n, L, dims) in enumerate(specs):
+        orc = Oracle(C2, dims[0])
+        prm = xpir_amd.params(C2, dims[0])
+        db = gpu_ctx.import_db(prm, orc.rand_bytes(800 + si, 0, n * L), n, L, dims)
+        sk = orc.keygen(810 + si)
+        pool = []
+        for qi in range(4):
+            q = np.concatenate(orc.make_query(sk, dims, int(rng.integers(0, n)), 8200 + 10 * si + qi))
+            pool.append((gpu_ctx.alloc(q.nbytes).upload(q), gpu_ctx.reply(db, q)))
+        dbs.append((db, pool))
+    srv = xpir_amd.Server(0, reply_pool_bytes=256 << 20, workers=2, max_batch=4, heavy_steps=10_000_000)
+    try:
+        expected, batches = {}, {}
+        for i in range(60):  # heavy and light requests interleaved, all at once
+            db, pool = dbs[i % 2]
+            qd, alone = pool[(i // 2) % 4]
+            t = srv.submit(db, qd)
+            assert t is not None
+            expected[t] = (i % 2, alone)
+        t0 = time.perf_counter()
+        while expected:
+            for c in srv.poll(timeout_us=1000):
+                assert c.status == 0
+                kind, alone = expected.pop(c.ticket)
+                assert np.array_equal(gpu_ctx.download_ptr(c.reply_dev, c.reply_words), alone)
+                batches.setdefault((kind, c.start_us), c.done_us)
+                srv.release(c.ticket)
+            assert time.perf_counter() - t0 < 60, "burst did not drain"
+        heavy = sorted((s, d) for (k, s), d in batches.items() if k == 0)
+        light = [s for (k, s) in batches if k == 1]
+        print(f"{len(heavy)} heavy batches, {len(light)} light batches", flush=True)
+        assert len(heavy) >= 8 and light
+        for (s0, d0), (s1, _) in zip(heavy, heavy[1:]):
+            assert s1 >= d0, "two heavy batches ran at the same time"
+    finally:
+        srv.close()
+        for db, pool in dbs:
+            for qd, _ in pool:
+                qd.free()
+            db.free()

@@ -42,7 +42,8 @@ class XpgRequest(C.Structure):
 
 
 class XpgServerConfig(C.Structure):
-    _fields_ = [("reply_pool_bytes", C.c_uint64), ("workers", C.c_uint32), ("max_batch", C.c_uint32)]
+    _fields_ = [("reply_pool_bytes", C.c_uint64), ("workers", C.c_uint32), ("max_batch", C.c_uint32),
+                ("heavy_steps", C.c_uint64)]
 
 
 class XpgCompletion(C.Structure):
@@ -447,8 +448,9 @@ def keygen(prm: XpgParams, seed: int) -> np.ndarray:
 class Server:
     """Continuous-batching reply server (xpg_server_*)."""
 
-    def __init__(self, device: int = 0, reply_pool_bytes: int = 1 << 30, workers: int = 2, max_batch: int = 64):
-        cfg = XpgServerConfig(reply_pool_bytes, workers, max_batch)
+    def __init__(self, device: int = 0, reply_pool_bytes: int = 1 << 30, workers: int = 2, max_batch: int = 64,
+                 heavy_steps: int = 0):
+        cfg = XpgServerConfig(reply_pool_bytes, workers, max_batch, heavy_steps)
         h = C.c_void_p()
         rc = lib().xpg_server_create(device, C.byref(cfg), C.byref(h))
         if rc != XPG_OK:

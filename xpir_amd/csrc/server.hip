@@ -36,6 +36,7 @@ struct Worker {
   xpg_ctx *ctx = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
+  bool heavy = false;     // its batch has more exact steps than the heavy threshold
   std::vector<Req> batch;
   uint8_t *ws = nullptr;  // this worker's share of the pool
 };
@@ -89,6 +90,7 @@ struct xpg_server {
   std::vector<Worker> workers;
   uint8_t *pool = nullptr;  // reply_pool_bytes: the reply slots, then the workspaces
   uint64_t ws_bytes = 0;    // each worker's workspace
+  uint64_t heavy_steps = 0; // heavy batch threshold (exact chain steps)
   Slab slab;                // the reply slots
   uint64_t budget_used = 0;
   uint64_t next_ticket = 1;
@@ -103,6 +105,18 @@ struct xpg_server {
   bool stop = false;
   uint64_t batches = 0, batched_requests = 0;
 };
+
+// exact chain steps of one request's reply, every level
+static uint64_t request_steps(const xpg_db *db) {
+  const uint64_t mn2 = 2ull * db->prm.m * db->prm.n;
+  uint64_t s = 0;
+  for (uint32_t l = 0; l < db->d; l++) {
+    uint64_t groups = 1;
+    for (uint32_t j = l + 1; j < db->d; j++) groups *= db->dims[j];
+    s += groups * db->Pl[l] * mn2 * db->dims[l];
+  }
+  return s;
+}
 
 // a request's reply slot (its intermediates come from the worker's workspace)
 static uint64_t request_footprint(const xpg_db *db, uint64_t *reply_words) {
@@ -139,13 +153,33 @@ static void dispatcher(xpg_server *s) {
       progressed = true;
     }
     if (progressed) s->done_cv.notify_all();
-    // dispatch: oldest pending request's database, up to max_batch of its requests and as
-    // many as the worker's workspace holds
+    // dispatch: the database of the oldest pending request, up to max_batch of its
+    // requests and as many as the worker's workspace holds -- unless that batch is heavy
+    // while another worker runs a heavy one: then the oldest light batch, or nothing
     for (Worker &w : s->workers) {
       if (w.busy || s->pending.empty()) continue;
-      const xpg_db *db = s->pending.front().db;
-      uint32_t fit = 1;
-      while (fit < s->cfg.max_batch && batch_workspace_bytes(w.ctx, db, fit + 1) <= s->ws_bytes) fit++;
+      bool heavy_running = false;
+      for (const Worker &o : s->workers) heavy_running |= o.busy && o.heavy;
+      const xpg_db *db = nullptr;
+      uint32_t fit = 0;
+      bool heavy = false;
+      std::map<const xpg_db *, uint32_t> waiting;  // pending requests per database
+      for (const Req &q : s->pending) waiting[q.db]++;
+      std::set<const xpg_db *> seen;
+      for (const Req &r : s->pending) {
+        if (seen.size() == waiting.size()) break;  // every database considered
+        if (!seen.insert(r.db).second) continue;
+        const uint32_t wait_n = waiting[r.db];
+        uint32_t f = 1;
+        while (f < s->cfg.max_batch && f < wait_n && batch_workspace_bytes(w.ctx, r.db, f + 1) <= s->ws_bytes) f++;
+        const bool h = request_steps(r.db) * f > s->heavy_steps;
+        if (h && heavy_running) continue;
+        db = r.db;
+        fit = f;
+        heavy = h;
+        break;
+      }
+      if (!db) continue;
       std::vector<Req> take;
       for (auto it = s->pending.begin(); it != s->pending.end() && take.size() < fit;) {
         if (it->db == db) {
@@ -177,6 +211,7 @@ static void dispatcher(xpg_server *s) {
       } else {
         w.batch = std::move(take);
         w.busy = true;
+        w.heavy = heavy;
       }
       progressed = true;
     }
@@ -198,6 +233,7 @@ int xpg_server_create(int device, const xpg_server_config *cfg, xpg_server **out
   xpg_server *s = new xpg_server();
   s->device = device;
   s->cfg = *cfg;
+  s->heavy_steps = cfg->heavy_steps ? cfg->heavy_steps : 1000000000ull;
   for (uint32_t i = 0; i < cfg->workers; i++) {
     Worker w;
     if (xpg_ctx_create(device, &w.ctx) != XPG_OK || hipEventCreateWithFlags(&w.done, hipEventDisableTiming) != hipSuccess) {
