@@ -155,7 +155,7 @@ def main():
                     help="single/plain: also time replies from a host query to a host reply (PCIe included)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path; run with --gpus 1",
+        print("bench.py measures one MI355X: the engine has no multi-GPU path; run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
