@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libxpir_amd.so")
 ARCH = os.environ.get("XPIR_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["capi.hip", "import.hip", "mac.hip", "server.hip", "decrypt.hip", "tune.hip"]
+SOURCES = ["capi.hip", "import.hip", "mac.hip", "mx.hip", "server.hip", "decrypt.hip", "tune.hip"]
 
 # -packed-fp32-ops: keep the compiler from forming packed fp32 VALU ops (none of this
 # code is fp32, the flag is a standing rule for every HIP file built here).

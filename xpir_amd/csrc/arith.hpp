@@ -107,34 +107,53 @@ __device__ __forceinline__ StepConsts step_consts(uint64_t p, uint32_t delta) {
 //        u is the new acc, else u + p.
 // Scratch v[2:9] and vcc; an s_nop separates a VALU write of vcc from its read as a lane
 // mask (a hazard on this GPU the compiler cannot see inside inline asm).
+#define XPG_MAC_STEP_ASM                                \
+  "v_mad_u64_u32 v[4:5], vcc, %[xh], %[ql], %[T]\n\t"   \
+  "v_mad_u64_u32 v[6:7], vcc, %[xl], %[qh], v[4:5]\n\t" \
+  "v_mul_lo_u32 v2, %[xl], %[yh]\n\t"                   \
+  "v_mov_b32 v8, v7\n\t"                                \
+  "v_cndmask_b32_e64 v9, 0, 1, vcc\n\t"                 \
+  "v_mad_u64_u32 v[4:5], vcc, %[xh], %[qh], v[8:9]\n\t" \
+  "v_mad_u64_u32 v[2:3], vcc, %[xh], %[yl], v[2:3]\n\t" \
+  "v_mad_u64_u32 v[6:7], vcc, %[xl], %[yl], %[negp]\n\t" \
+  "v_mad_u64_u32 v[6:7], vcc, v4, %[d], v[6:7]\n\t"     \
+  "v_mad_u64_u32 v[2:3], vcc, v5, %[d], v[2:3]\n\t"     \
+  "v_mad_u64_u32 v[2:3], vcc, v4, %[k29], v[2:3]\n\t"   \
+  "v_add_u32 v7, v7, v2\n\t"                            \
+  "v_lshl_add_u64 %[acc], %[acc], 0, v[6:7]\n\t"        \
+  "v_cmp_gt_u64_e64 vcc, %[negp], %[acc]\n\t"           \
+  "s_nop 0\n\t"                                         \
+  "v_cndmask_b32_e64 v2, %[plo], 0, vcc\n\t"            \
+  "v_cndmask_b32_e64 v3, %[phi], 0, vcc\n\t"            \
+  "v_lshl_add_u64 %[acc], v[2:3], 0, %[acc]"
+
 __device__ __forceinline__ void mac_step_asm(uint64_t &acc, uint64_t x, uint64_t y, uint64_t yq,
                                              const StepConsts &k) {
   const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
   const uint32_t ql = (uint32_t)yq, qh = (uint32_t)(yq >> 32);
   const uint64_t T = __umulhi(xl, ql);
-  asm("v_mad_u64_u32 v[4:5], vcc, %[xh], %[ql], %[T]\n\t"
-      "v_mad_u64_u32 v[6:7], vcc, %[xl], %[qh], v[4:5]\n\t"
-      "v_mul_lo_u32 v2, %[xl], %[yh]\n\t"
-      "v_mov_b32 v8, v7\n\t"
-      "v_cndmask_b32_e64 v9, 0, 1, vcc\n\t"
-      "v_mad_u64_u32 v[4:5], vcc, %[xh], %[qh], v[8:9]\n\t"
-      "v_mad_u64_u32 v[2:3], vcc, %[xh], %[yl], v[2:3]\n\t"
-      "v_mad_u64_u32 v[6:7], vcc, %[xl], %[yl], %[negp]\n\t"
-      "v_mad_u64_u32 v[6:7], vcc, v4, %[d], v[6:7]\n\t"
-      "v_mad_u64_u32 v[2:3], vcc, v5, %[d], v[2:3]\n\t"
-      "v_mad_u64_u32 v[2:3], vcc, v4, %[k29], v[2:3]\n\t"
-      "v_add_u32 v7, v7, v2\n\t"
-      "v_lshl_add_u64 %[acc], %[acc], 0, v[6:7]\n\t"
-      "v_cmp_gt_u64_e64 vcc, %[negp], %[acc]\n\t"
-      "s_nop 0\n\t"
-      "v_cndmask_b32_e64 v2, %[plo], 0, vcc\n\t"
-      "v_cndmask_b32_e64 v3, %[phi], 0, vcc\n\t"
-      "v_lshl_add_u64 %[acc], v[2:3], 0, %[acc]"
+  asm(XPG_MAC_STEP_ASM
       : [acc] "+v"(acc)
       : [xl] "v"(xl), [xh] "v"(xh), [yl] "v"(yl), [yh] "v"(yh), [ql] "v"(ql), [qh] "v"(qh), [T] "v"(T),
         [d] "s"(k.delta), [k29] "s"(k.k29), [negp] "s"(k.negp), [plo] "v"(k.plo), [phi] "v"(k.phi)
       : "v2", "v3", "v4", "v5", "v6", "v7", "v8", "v9", "vcc");
 }
+
+// mac_step_asm that also counts, in nsub, the steps that subtract p (acc + r >= p): the
+// compare's lane mask is still in vcc after the select, and one add-with-carry adds it.
+__device__ __forceinline__ void mac_step_count_asm(uint64_t &acc, uint32_t &nsub, uint64_t x, uint64_t y,
+                                                   uint64_t yq, const StepConsts &k) {
+  const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
+  const uint32_t ql = (uint32_t)yq, qh = (uint32_t)(yq >> 32);
+  const uint64_t T = __umulhi(xl, ql);
+  asm(XPG_MAC_STEP_ASM "\n\t"
+      "v_addc_co_u32 %[ns], vcc, 0, %[ns], vcc"
+      : [acc] "+v"(acc), [ns] "+v"(nsub)
+      : [xl] "v"(xl), [xh] "v"(xh), [yl] "v"(yl), [yh] "v"(yh), [ql] "v"(ql), [qh] "v"(qh), [T] "v"(T),
+        [d] "s"(k.delta), [k29] "s"(k.k29), [negp] "s"(k.negp), [plo] "v"(k.plo), [phi] "v"(k.phi)
+      : "v2", "v3", "v4", "v5", "v6", "v7", "v8", "v9", "vcc");
+}
+#undef XPG_MAC_STEP_ASM
 
 // The lazy Shoup product x*w - hi64(x*wq)*p (mod 2^64) for p = 2^61 - delta as 13 VALU
 // instructions (equal to shoup_lazy(x, w, wq, p) for every x, w, wq < 2^64): q as in

@@ -481,6 +481,10 @@ static int set_option(xpg_ctx *ctx, const char *key, int64_t value) {
     ctx->alpha_zero_pad = (int)value;
     return XPG_OK;
   }
+  if (strcmp(key, "db_layout") == 0 && (value == 0 || value == 1)) {
+    ctx->db_layout = (int)value;
+    return XPG_OK;
+  }
   if (strcmp(key, "graphs") == 0 && (value == 0 || value == 1)) {
     ctx->graphs = value != 0;
     if (!ctx->graphs) graph_cache_clear(ctx, 0);
@@ -647,6 +651,40 @@ static int import_partial_aggregate(xpg_ctx *ctx, const xpg_params *prm, const u
   return rc;
 }
 
+// The database in the matrix layout when the context asks for it (option db_layout) and
+// its level 0 has one (mx_geometry): level 0's digit fragments and tail rows are made from
+// the row-major words, which are then freed.  A database whose level 0 is too short keeps
+// its row-major words.
+static int db_to_matrix(xpg_ctx *ctx, xpg_db *db) {
+  if (ctx->db_layout != 1) return XPG_OK;
+  MxGeom g;
+  if (!mx_geometry(&db->prm, db->dims[0], prod_dims(db->d, db->dims, 1), db->P0, &g)) return XPG_OK;
+  uint32_t *bad = nullptr;
+  HIP_OK(hipMalloc((void **)&db->mx, mx_frag_bytes(g)), "matrix layout allocation");
+  HIP_OK(hipMalloc((void **)&db->mx_tail, mx_tail_words(g) * 8), "tail rows allocation");
+  HIP_OK(hipMalloc((void **)&bad, 256), "flag allocation");
+  int rc = XPG_OK;
+  if (hipMemsetAsync(bad, 0, 4, ctx->stream) != hipSuccess) rc = set_err(ctx, XPG_EDEVICE, "flag clear");
+  if (!rc) rc = launch_mx_pack_db(ctx, &db->prm, g, db->words, db->mx, db->mx_tail, bad);
+  uint32_t hbad = 0;
+  if (!rc && hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+    rc = set_err(ctx, XPG_EDEVICE, "flag copy");
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = set_err(ctx, XPG_EDEVICE, "stream sync");
+  (void)hipFree(bad);
+  if (!rc && hbad) rc = set_err(ctx, XPG_EINVAL, "a database word is too large for the matrix layout (>= 2^63 - 2^56)");
+  if (rc) {
+    (void)hipFree(db->mx);
+    (void)hipFree(db->mx_tail);
+    db->mx = nullptr;
+    db->mx_tail = nullptr;
+    return rc;
+  }
+  (void)hipFree(db->words);
+  db->words = nullptr;
+  db->mxg = g;
+  return XPG_OK;
+}
+
 int xpg_db_import_dev(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems_dev, uint64_t n_elems,
                       uint64_t elem_bytes, uint32_t alpha, uint32_t d, const uint32_t *dims, xpg_db **out) {
   if (!ctx || !prm || !out) return XPG_EINVAL;
@@ -667,6 +705,7 @@ int xpg_db_import_dev(xpg_ctx *ctx, const xpg_params *prm, const uint8_t *elems_
     rc = launch_import_bytes(ctx, prm, src, real_units - 1, real_units - 1, P0, (*out)->words);
     if (!rc) rc = import_partial_aggregate(ctx, prm, elems_dev, n_elems, elem_bytes, alpha, P0, *out);
   }
+  if (!rc) rc = db_to_matrix(ctx, *out);
   if (rc) {
     xpg_db_free(*out);
     *out = nullptr;
@@ -698,6 +737,7 @@ int xpg_db_import_ntt_words(xpg_ctx *ctx, const xpg_params *prm, const uint64_t 
   rc = copy_h2d_staged(ctx, db->words, words, units * per * 8);
   if (!rc) rc = launch_fill_zero(ctx, db->words + units * per, (db->units - units) * per);
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = set_err(ctx, XPG_EDEVICE, "sync");
+  if (!rc) rc = db_to_matrix(ctx, db);
   if (rc) {
     xpg_db_free(db);
     *out = nullptr;
@@ -707,6 +747,7 @@ int xpg_db_import_ntt_words(xpg_ctx *ctx, const xpg_params *prm, const uint64_t 
 
 int xpg_db_export_ntt_words(xpg_ctx *ctx, const xpg_db *db, uint64_t *out, uint64_t cap_words) {
   if (!ctx || !db || !out) return XPG_EINVAL;
+  if (!db->words) return set_err(ctx, XPG_EINVAL, "a database in the matrix layout has no row-major words");
   const uint64_t words = db->units * db->P0 * (uint64_t)db->prm.m * db->prm.n;
   if (cap_words < words) return set_err(ctx, XPG_ESIZE, "export buffer too small");
   HIP_OK(hipMemcpyAsync(out, db->words, words * 8, hipMemcpyDeviceToHost, ctx->stream), "D2H copy");
@@ -806,6 +847,7 @@ int ensure_pin(xpg_ctx *ctx) {
 
 int xpg_db_save(xpg_ctx *ctx, const xpg_db *db, const char *path, const uint8_t *digest) {
   if (!ctx || !db || !path) return XPG_EINVAL;
+  if (!db->words) return set_err(ctx, XPG_EINVAL, "a database in the matrix layout has no row-major words to save");
   if (db->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
   NttFileHeader h{};
   memcpy(h.magic, kNttMagic, 8);
@@ -909,6 +951,7 @@ int xpg_db_load(xpg_ctx *ctx, const xpg_params *prm, const char *path, const uin
   uint64_t sum = 0;
   if (!rc) rc = device_checksum(ctx, db->words, h.words, &sum);
   if (!rc && sum != h.checksum) rc = set_err(ctx, XPG_EINVAL, "database file checksum mismatch (corrupted words)");
+  if (!rc) rc = db_to_matrix(ctx, db);
   if (rc) {
     (void)hipStreamSynchronize(ctx->stream);
     xpg_db_free(db);
@@ -942,6 +985,8 @@ void xpg_db_free(xpg_db *db) {
     db->ctx->dbs.erase(db);
   }
   if (db->words) (void)hipFree(db->words);
+  if (db->mx) (void)hipFree(db->mx);
+  if (db->mx_tail) (void)hipFree(db->mx_tail);
   delete db;
 }
 
@@ -1013,7 +1058,14 @@ static int run_levels_in(xpg_ctx *ctx, const xpg_db *db, uint32_t first, const u
     const uint32_t P = db->Pl[l];
     const bool last = (l + 1 == db->d);
     uint64_t *out = last ? reply_dev : lvl_reply;
-    int rc = launch_mac_level(ctx, prm, data, q, db->dims[l], groups, P, out);
+    int rc = XPG_OK;
+    if (l == 0 && db->mx) {
+      void *work = nullptr;
+      rc = get_scratch(ctx, mx_work_bytes(db->mxg, 1), &work);
+      if (!rc) rc = launch_mx_level(ctx, prm, db->mxg, db->mx, db->mx_tail, &q, &out, 1, work);
+    } else {
+      rc = launch_mac_level(ctx, prm, data, q, db->dims[l], groups, P, out);
+    }
     if (rc) return rc;
     q += (uint64_t)db->dims[l] * 2 * mn;
     if (!last) {
@@ -1120,6 +1172,10 @@ int xpg_reply_dev_noalloc(xpg_ctx *ctx, const xpg_db *db, const uint64_t *query_
   size_t scr = 0;
   for (uint32_t l = 0; l < db->d; l++) {
     const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
+    if (l == 0 && db->mx) {
+      scr = std::max<size_t>(scr, mx_work_bytes(db->mxg, 1));
+      continue;
+    }
     const LevelCfg cfg = resolve_level_cfg(ctx, prm, db->dims[l], groups, db->Pl[l]);
     scr = std::max<size_t>(scr, level_scratch_bytes(ctx, prm, db->dims[l], groups, db->Pl[l], cfg));
   }
@@ -1215,14 +1271,20 @@ extern "C++" uint64_t batch_workspace_bytes(const xpg_ctx *ctx, const xpg_db *db
   size_t per_db = 0, per_out = 0;
   batch_level_bytes(db, &per_db, &per_out);
   // (plus the fused launch's tile table)
-  const uint64_t batched =
-      lvl0 + al256(fused_table_bytes(k)) + (db->d > 1 ? al256(per_db) + (per_out ? al256(per_out) : 0) : 0);
+  // (plus, in the matrix layout, the launch's query fragments and staged residues)
+  const uint64_t batched = lvl0 + al256(fused_table_bytes(k)) +
+                           std::max<uint64_t>(db->mx ? al256(mx_work_bytes(db->mxg, k)) : 0,
+                                              db->d > 1 ? al256(per_db) + (per_out ? al256(per_out) : 0) : 0);
   // one reply at a time: its level buffers and the largest level's scratch
   size_t need_reply = 0, need_db = 0;
   level_buf_sizes(db, 0, &need_reply, &need_db);
   uint64_t single = need_db ? al256(need_reply) + al256(need_db) : 0, scr = 0;
   for (uint32_t l = 0; l < db->d; l++) {
     const uint64_t groups = prod_dims(db->d, db->dims, l + 1);
+    if (l == 0 && db->mx) {
+      scr = std::max<uint64_t>(scr, mx_work_bytes(db->mxg, 1));
+      continue;
+    }
     const LevelCfg cfg = resolve_level_cfg(ctx, prm, db->dims[l], groups, db->Pl[l]);
     scr = std::max<uint64_t>(scr, level_scratch_bytes(ctx, prm, db->dims[l], groups, db->Pl[l], cfg));
   }
@@ -1254,7 +1316,15 @@ static int batch_chunk(xpg_ctx *ctx, const xpg_db *db, const std::vector<const x
     qp[i] = reqs[i]->query_dev;
     op[i] = db->d == 1 ? reqs[i]->reply_dev : lvl0 + i * lvl0_words;
   }
-  int rc = launch_mac_batch(ctx, prm, db->words, qp.data(), op.data(), qn, db->dims[0], groups0, db->P0);
+  int rc = XPG_OK;
+  if (db->mx) {
+    // the launch's workspace after the level-0 outputs (not kept: the levels below reuse it)
+    void *work = nullptr;
+    rc = get_scratch(ctx, mx_work_bytes(db->mxg, qn), &work);
+    if (!rc) rc = launch_mx_level(ctx, prm, db->mxg, db->mx, db->mx_tail, qp.data(), op.data(), qn, work);
+  } else {
+    rc = launch_mac_batch(ctx, prm, db->words, qp.data(), op.data(), qn, db->dims[0], groups0, db->P0);
+  }
   if (rc || db->d == 1) return rc;
   return batch_levels(ctx, db, reqs, lvl0);
 }
@@ -1446,7 +1516,9 @@ int xpg_reply_batch_dev(xpg_ctx *ctx, const xpg_request *reqs, uint32_t n_reqs) 
       BatchChunk c{db, std::vector<const xpg_request *>(v.begin() + s, v.begin() + std::min(v.size(), s + 64))};
       const uint32_t cnt = (uint32_t)c.reqs.size();
       int rc = XPG_OK;
-      if (!batch_pays(ctx, db, cnt)) {
+      if (db->mx) {
+        rc = batch_chunk(ctx, db, c.reqs);  // the matrix layout: one launch for any count
+      } else if (!batch_pays(ctx, db, cnt)) {
         for (const xpg_request *r : c.reqs)
           if (!rc) rc = xpg_reply_dev(ctx, db, r->query_dev, r->reply_dev);
       } else if (ctx->batch_fuse) {

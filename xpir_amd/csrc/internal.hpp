@@ -45,6 +45,14 @@ struct EncSrc {
   uint64_t skq_unused;
 };
 
+// Level 0 in the matrix layout (mx.hip): S = m*n slots, U = groups*P units in UT tiles of
+// 4, n_l elements in EB blocks of 32, of which the first E0 (EBP blocks) run on the matrix
+// cores and the last L = n_l - E0 (40..71) with the reference's step.
+struct MxGeom {
+  uint32_t S = 0, U = 0, UT = 0, EB = 0, EBP = 0, L = 0, P = 0;
+  uint64_t n_l = 0, groups = 0, E0 = 0;
+};
+
 struct xpg_db {
   // the importing context while it lives (xpg_ctx_destroy detaches its databases: a
   // database outlives its context and serves any context of the same GPU)
@@ -60,7 +68,12 @@ struct xpg_db {
   uint64_t n_elems = 0, elem_bytes = 0;
   uint32_t alpha = 1;
   int alpha_zero_pad = 0;   // the partial-aggregate convention the import used
-  uint64_t *words = nullptr;  // units x P0 x m x n
+  uint64_t *words = nullptr;  // units x P0 x m x n (null in the matrix layout)
+  // matrix layout (context option db_layout = 1 at import): level 0's digit fragments and
+  // each group's last L elements row-major (the chains' tails); `words` is then freed
+  uint8_t *mx = nullptr;
+  uint64_t *mx_tail = nullptr;
+  MxGeom mxg;
   uint64_t serial = 0;        // unique per database ever created (graph cache key)
 };
 
@@ -127,6 +140,8 @@ struct xpg_ctx {
   // a partial last alpha aggregate: 0 = the reference's bytes (readAggregatedStream's
   // bzero from the last real element's slot), 1 = the missing elements read as zeros
   int alpha_zero_pad = 0;
+  // databases imported from now on: 0 = row-major NTT words, 1 = the matrix layout
+  int db_layout = 0;
   // MAC timing
   int timing = 0;  // 0 off, 1 every MAC level and main kernel, 2 main kernels alone
   std::vector<hipEvent_t> ev_pool;
@@ -250,6 +265,29 @@ int launch_mac_fused(xpg_ctx *ctx, const FusedPart *parts, uint32_t nparts, void
 int launch_mac_level_multi(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                            uint32_t nreq, uint64_t n_l, uint64_t gpr, uint32_t P, uint64_t *const *out_per_req,
                            uint64_t *out);
+// The last L steps of level-0 chains continued from the residues already in out (the
+// matrix-core level 0's tails; mac.hip): tail_db holds each group's last L elements
+// (groups*L rows x P x m x n), q[i] query i's first tail element.
+int launch_mac_tail(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *tail_db, const uint64_t *const *q,
+                    uint64_t *const *out, uint32_t qn, uint64_t L, uint64_t groups, uint32_t P);
+
+// ---- matrix-core level 0 (mx.hip) ----
+// Whether level 0 of (n_l, groups, P) can take the matrix layout (n a multiple of 64 and
+// n_l >= 72), and its geometry.
+bool mx_geometry(const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, MxGeom *g);
+uint64_t mx_frag_bytes(const MxGeom &g);
+uint64_t mx_tail_words(const MxGeom &g);
+// The fragments of row-major level-0 words and the tails' rows; *bad_dev (zero before the
+// launch) becomes nonzero when a word is at least 2^63 - 2^56 (no 8-digit form).
+int launch_mx_pack_db(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const uint64_t *words, uint8_t *frag,
+                      uint64_t *tail, uint32_t *bad_dev);
+// device workspace of one launch of qn (1..64) queries
+uint64_t mx_work_bytes(const MxGeom &g, uint32_t qn);
+// Level 0 of qn queries (q[i]: level-0 query words; out[i]: groups x P x 2 x m x n) on a
+// database in the matrix layout, the same words as launch_mac_level / launch_mac_batch.
+int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const uint8_t *frag, const uint64_t *tail,
+                    const uint64_t *const *q, uint64_t *const *out, uint32_t qn, void *work);
+
 // what launch_mac_level runs for this shape: the tuning table's choice unless an option
 // pins a kernel, else default_level_cfg
 LevelCfg resolve_level_cfg(const xpg_ctx *ctx, const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P);

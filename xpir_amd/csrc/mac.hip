@@ -293,7 +293,21 @@ struct TileLevel {
 // The tile (colgroup cg = (row block, column), query block qb) of one level: every wave
 // of the workgroup runs it; qbase[i] / obase[i] are the query and output of the block's
 // query i (the last query repeated past qn: its chains are computed and dropped).
-template <int GR, int QR, int W, int NQMAX>
+// The word a tail chain leaves: acc = rho + j*p after steps of which `kept` did not
+// subtract p.  Started from j = 0, its j is max(D, M) where the true start j0 <= 8 ends at
+// max(j0 + D, M), D = j - kept; both agree when kept >= j + 8.  Otherwise bit 63 (never set
+// in a chain word, which stays below 2^62) asks for the whole chain.
+__device__ __forceinline__ uint64_t tail_word(uint64_t acc, uint32_t kept, uint64_t p) {
+  uint32_t j = 0;
+  for (uint64_t t = acc; t >= p && j < 16; t -= p) j++;
+  return kept >= j + 8 ? acc : acc | (1ull << 63);
+}
+
+// TAIL: the chains continue from the words already in the outputs instead of zero, and a
+// chain whose steps do not show that its start's excess multiple of p is forgotten gets
+// bit 63 set in its word (for a whole recomputation): see the matrix-core level 0, whose
+// chains' last steps this runs.
+template <int GR, int QR, int W, bool TAIL, int NQMAX>
 __device__ __forceinline__ void batch3_tile(const ModSet &ms, const TileLevel &a, const uint64_t *const (&qbase)[QR],
                                             uint64_t *const (&obase)[QR], uint32_t nq, uint64_t cg,
                                             QPair (&lds)[2][NQMAX][64]) {
@@ -336,10 +350,28 @@ __device__ __forceinline__ void batch3_tile(const ModSet &ms, const TileLevel &a
     sq[s] = qbase[i] + ab * mn;
   }
   uint64_t acc_a[GR][QR], acc_b[GR][QR];
+  uint32_t ns_a[GR][QR], ns_b[GR][QR];  // TAIL: steps that subtracted p
 #pragma unroll
   for (int g = 0; g < GR; g++)
 #pragma unroll
-    for (int i = 0; i < QR; i++) acc_a[g][i] = acc_b[g][i] = 0;
+    for (int i = 0; i < QR; i++) {
+      acc_a[g][i] = acc_b[g][i] = 0;
+      ns_a[g][i] = ns_b[g][i] = 0;
+      if (TAIL && g < rv && (uint32_t)i < nq) {
+        const uint64_t *o = obase[i] + (r0 + g) * 2 * mn + cs;
+        acc_a[g][i] = o[0];
+        acc_b[g][i] = o[mn];
+      }
+    }
+  auto step2 = [&](int g, int i, uint64_t x, uint64_t ya, uint64_t yaq, uint64_t yb, uint64_t ybq) {
+    if constexpr (TAIL) {
+      mac_step_count_asm(acc_a[g][i], ns_a[g][i], x, ya, yaq, kc);
+      mac_step_count_asm(acc_b[g][i], ns_b[g][i], x, yb, ybq, kc);
+    } else {
+      mac_step_asm(acc_a[g][i], x, ya, yaq, kc);
+      mac_step_asm(acc_b[g][i], x, yb, ybq, kc);
+    }
+  };
   // element indices in 32 bits, so that the clamps and offsets below stay scalar
   const uint32_t nl = (uint32_t)a.n_l, nfull = nl / kB3U, last = nl - 1;
   const uint32_t cs8 = cs * 8;  // this lane's byte offset within a row
@@ -370,10 +402,7 @@ __device__ __forceinline__ void batch3_tile(const ModSet &ms, const TileLevel &a
       for (int i = 0; i < QR; i++) {
         const QPair ya = lds[buf][(u * QR + i) * 2][lane], yb = lds[buf][(u * QR + i) * 2 + 1][lane];
 #pragma unroll
-        for (int g = 0; g < GR; g++) {
-          mac_step_asm(acc_a[g][i], xv[u][g], ya.y, ya.yq, kc);
-          mac_step_asm(acc_b[g][i], xv[u][g], yb.y, yb.yq, kc);
-        }
+        for (int g = 0; g < GR; g++) step2(g, i, xv[u][g], ya.y, ya.yq, yb.y, yb.yq);
       }
   };
 
@@ -407,10 +436,7 @@ __device__ __forceinline__ void batch3_tile(const ModSet &ms, const TileLevel &a
       const uint64_t ya = *lane_word(qi, cs8), yb = *lane_word(qi + mn, cs8);
       const uint64_t yaq = shoup_quot(ya, p, delta), ybq = shoup_quot(yb, p, delta);
 #pragma unroll
-      for (int g = 0; g < GR; g++) {
-        mac_step_asm(acc_a[g][i], xv[g], ya, yaq, kc);
-        mac_step_asm(acc_b[g][i], xv[g], yb, ybq, kc);
-      }
+      for (int g = 0; g < GR; g++) step2(g, i, xv[g], ya, yaq, yb, ybq);
     }
   }
 #pragma unroll
@@ -420,14 +446,19 @@ __device__ __forceinline__ void batch3_tile(const ModSet &ms, const TileLevel &a
     for (int i = 0; i < QR; i++) {
       if ((uint32_t)i < nq) {
         uint64_t *o = obase[i] + (r0 + g) * 2 * mn + cs;
-        o[0] = acc_a[g][i];
-        o[mn] = acc_b[g][i];
+        if constexpr (TAIL) {
+          o[0] = tail_word(acc_a[g][i], nl - ns_a[g][i], p);
+          o[mn] = tail_word(acc_b[g][i], nl - ns_b[g][i], p);
+        } else {
+          o[0] = acc_a[g][i];
+          o[mn] = acc_b[g][i];
+        }
       }
     }
   }
 }
 
-template <int GR, int QR, int W>
+template <int GR, int QR, int W, bool TAIL = false>
 __global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
   __shared__ QPair lds[2][kB3U * QR * 2][64];
   uint64_t cg;
@@ -442,7 +473,7 @@ __global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
     qbase[i] = a.q[min(q0 + i, a.qn - 1)];
     obase[i] = a.out[min(q0 + i, a.qn - 1)];
   }
-  batch3_tile<GR, QR, W>(a.ms, lv, qbase, obase, a.qn - q0, cg, lds);
+  batch3_tile<GR, QR, W, TAIL>(a.ms, lv, qbase, obase, a.qn - q0, cg, lds);
 }
 
 // ------------------------------------------------------------------------------------
@@ -498,7 +529,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, const FusedSeg &g
     qbase[i] = reinterpret_cast<const uint64_t *>(uniform64(a.ptrs[qi]));
     obase[i] = reinterpret_cast<uint64_t *>(uniform64(a.ptrs[qi + g.qn]));
   }
-  batch3_tile<GR, QR, W>(a.ms, lv, qbase, obase, g.qn - q0, cg, lds);
+  batch3_tile<GR, QR, W, false>(a.ms, lv, qbase, obase, g.qn - q0, cg, lds);
 }
 
 template <int W>
@@ -1154,6 +1185,50 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
   timing_end(ctx, k0, k1, 2);
   timing_end(ctx, e0, e1);
   return rc;
+}
+
+// The last L steps of the level-0 chains of qn (1..64) queries whose outputs already hold
+// the residues of their first elements (the matrix-core level 0): tail_db holds each
+// group's last L elements (groups*L rows x P x m x n), q[i] points at query i's first
+// tail element.  Register blocks of 4 rows x QR queries with the query words shared
+// through LDS (mac_batch3_kernel in its TAIL form).
+int launch_mac_tail(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *tail_db, const uint64_t *const *q,
+                    uint64_t *const *out, uint32_t qn, uint64_t L, uint64_t groups, uint32_t P) {
+  if (qn == 0 || qn > (uint32_t)kMaxBatch) return set_err(ctx, XPG_EINVAL, "batch of 1..64 queries per launch");
+  if (prm->n < 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n >= 64");
+  constexpr int GR = 4, W = 4;
+  const uint32_t QR = qn >= 4 ? 4 : qn >= 2 ? 2 : 1;
+  const uint32_t rest = qn % QR;
+  if (rest && qn > QR) {
+    const int rc = launch_mac_tail(ctx, prm, tail_db, q, out, qn - rest, L, groups, P);
+    if (rc) return rc;
+    return launch_mac_tail(ctx, prm, tail_db, q + (qn - rest), out + (qn - rest), rest, L, groups, P);
+  }
+  Batch2Args b2{};
+  b2.ms = modset_of(prm);
+  b2.db = tail_db;
+  for (uint32_t i = 0; i < qn; i++) {
+    b2.q[i] = q[i];
+    b2.out[i] = out[i];
+  }
+  b2.n_l = L;
+  b2.groups = groups;
+  b2.P = P;
+  b2.n = prm->n;
+  b2.m = prm->m;
+  b2.sblocks = prm->n / 64;
+  b2.qn = qn;
+  b2.qblocks = (qn + QR - 1) / QR;
+  b2.colgroups = ((groups * P + W * GR - 1) / (W * GR)) * prm->m * b2.sblocks;
+  const uint64_t blocks = ((b2.colgroups + 7) / 8) * 8 * b2.qblocks;
+  if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "tail MAC grid too large");
+  const dim3 wg(64 * W), grid((uint32_t)blocks);
+  switch (QR) {
+    case 4: hipLaunchKernelGGL((mac_batch3_kernel<GR, 4, W, true>), grid, wg, 0, ctx->stream, b2); break;
+    case 2: hipLaunchKernelGGL((mac_batch3_kernel<GR, 2, W, true>), grid, wg, 0, ctx->stream, b2); break;
+    default: hipLaunchKernelGGL((mac_batch3_kernel<GR, 1, W, true>), grid, wg, 0, ctx->stream, b2); break;
+  }
+  return hipGetLastError() != hipSuccess ? set_err(ctx, XPG_EDEVICE, "tail MAC launch") : XPG_OK;
 }
 
 size_t fused_table_bytes(uint32_t queries) { return 64 + (size_t)queries * 2 * sizeof(uint64_t); }

@@ -1,0 +1,537 @@
+// mx.hip -- level 0 of the reply on the matrix cores: the residues of the chains' sums by
+// int8 MFMA over signed byte digits, the chains' last steps by the reference's own step.
+//
+// Reference path: the level-0 chains of PIRReplyGeneratorNFL_internal::generateReply
+// (pir/replyGenerator/PIRReplyGeneratorNFL_internal.cpp:511-595) through
+// NFLlib::mulandaddShoup (crypto/NFLlib.hpp:303-321).  A chain (query q, half s, unit u =
+// group * P + poly, slot sl = modulus k * n + coefficient c) runs, from acc = 0, over the
+// elements e = 0 .. n_l - 1:
+//     acc = acc + r_e,  r_e = x_e*y_e - hi64(x_e*y'_e)*p  (in [0, 2p));  if (acc >= p) acc -= p.
+// Write acc = rho + j*p (rho < p) and r_e = u_e + c_e*p (u_e = x_e*y_e mod p).  Then rho is
+// the residue of the sum of x*y over the elements so far, and j evolves by
+// j' = max(j + b_e + c_e - 1, 0) with b_e the residue wrap [rho + u_e >= p] (the map the
+// streaming MAC in mac.hip composes).  j stays at most 8 (acc < 2^64 < 9p).  So:
+//   1. The residue of the first E0 elements (whole blocks of 32, leaving a tail of 40..71)
+//      is an exact integer dot product reduced mod p: the matrix cores compute it.
+//   2. The tail's steps run with the reference's step from acc = that residue (j = 0),
+//      counting the steps that do not subtract p (`kept`).  A start at j0 ends at
+//      max(j0 + D, M), a start at 0 at max(D, M) = j with D = j - kept, so the two agree
+//      for every j0 <= 8 when kept >= j + 8 (mac.hip: tail_word).
+//   3. Chains whose tail does not settle this (bit 63 of their word), and every chain of a
+//      query with a word not below its modulus (outside the model: its r_e may leave
+//      [0, 2p)), are recomputed whole with the reference's step (mx_fixup_kernel).
+// The words are the reference's: same residue, same excess multiple of p.
+//
+// Digits: a word x < 2^63 - 2^56 is x = sum_j d_j 256^j with d_j in [-128, 127] the bytes
+// of (x + B) ^ B, B = 0x8080808080808080 (the bytes of x + B are d_j + 128).  For a slot,
+//     sum_e y_e x_e = sum_{i,j} 2^(8(i+j)) S_ij,   S_ij = sum_e dy_i(e) dx_j(e),
+// and v_mfma_i32_32x32x32_i8 accumulates S_ij for row (query pair, half, digit i) and
+// column (unit of four, digit j), 32 elements per instruction, exactly in int32 for up to
+// 2^16 elements (|dy dx| <= 2^14).  Longer prefixes are cut into passes of 2^16.
+//
+// Fragments (1 KiB, one instruction operand): lane L holds row (or column) L % 32 and the
+// 16 elements 16*(L/32) .. +15 of a block of 32, one byte each (the instruction pairs A's
+// and B's bytes by lane half and byte position).  The database's fragments, laid out once
+// when it is imported ("matrix layout"), are [slot][unit tile of 4][element block]; a
+// launch lays its queries' out as [slot][row tile of 2 queries][element block].
+#include <algorithm>
+
+#include "internal.hpp"
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr uint64_t kDigitBias = 0x8080808080808080ull;
+constexpr uint64_t kDigitMax = 0x7F7F7F7F7F7F7F80ull;  // words below it have 8 signed digits
+constexpr uint64_t kM61 = (1ull << 61) - 1;
+constexpr uint32_t kTailMin = 40;                      // fewest tail steps
+constexpr uint32_t kPassBlocks = 2048;                 // element blocks per int32 pass (2^16)
+constexpr int kNT = 2;                                 // column tiles per wave
+
+__device__ __forceinline__ uint64_t digits_of(uint64_t x) { return (x + kDigitBias) ^ kDigitBias; }
+
+// 4x4 byte transpose: y_k = (x0.b_k, x1.b_k, x2.b_k, x3.b_k)
+__device__ __forceinline__ void bt4(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint32_t (&y)[4]) {
+  const uint32_t a = __builtin_amdgcn_perm(x1, x0, 0x05010400u), b = __builtin_amdgcn_perm(x1, x0, 0x07030602u);
+  const uint32_t c = __builtin_amdgcn_perm(x3, x2, 0x05010400u), d = __builtin_amdgcn_perm(x3, x2, 0x07030602u);
+  y[0] = __builtin_amdgcn_perm(c, a, 0x05040100u);
+  y[1] = __builtin_amdgcn_perm(c, a, 0x07060302u);
+  y[2] = __builtin_amdgcn_perm(d, b, 0x05040100u);
+  y[3] = __builtin_amdgcn_perm(d, b, 0x07060302u);
+}
+
+// 16 digit words w[t] -> the eight 16-byte lane chunks of one (row or unit, lane half):
+// chunk j holds digit j of w[0..15], written to dst + 16*j.
+__device__ __forceinline__ void store_chunks(const uint64_t (&w)[16], uint8_t *dst) {
+  uint32_t o[8][4];  // [digit][dword]
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      uint32_t y[4];
+      bt4((uint32_t)(w[4 * q] >> (32 * half)), (uint32_t)(w[4 * q + 1] >> (32 * half)),
+          (uint32_t)(w[4 * q + 2] >> (32 * half)), (uint32_t)(w[4 * q + 3] >> (32 * half)), y);
+#pragma unroll
+      for (int k = 0; k < 4; k++) o[4 * half + k][q] = y[k];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    v4i v = {(int)o[j][0], (int)o[j][1], (int)o[j][2], (int)o[j][3]};
+    *reinterpret_cast<v4i *>(dst + 16 * j) = v;
+  }
+}
+
+// ---- the database in the matrix layout ----
+struct PackDbArgs {
+  const uint64_t *words;  // row-major: ((g*n_l + e)*P + pl)*m*n + sl
+  uint8_t *frag;
+  uint32_t *bad;          // set when a word has no 8-digit form
+  uint64_t n_l, total;    // threads
+  uint32_t S, U, UT, EB, P;
+};
+
+// thread (sl fastest, lane half h, unit u of the padded tiles, element block eb): digits of
+// its 16 words into lanes (u % 4) * 8 + j + 32 h of fragment (sl, u / 4, eb)
+__global__ void __launch_bounds__(256) mx_pack_db_kernel(PackDbArgs a) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= a.total) return;
+  const uint32_t sl = (uint32_t)(t % a.S);
+  uint64_t r = t / a.S;
+  const uint32_t h = (uint32_t)(r & 1);
+  r >>= 1;
+  const uint32_t u = (uint32_t)(r % (a.UT * 4));
+  const uint32_t eb = (uint32_t)(r / (a.UT * 4));
+  const uint64_t mn = a.S;
+  uint64_t w[16];
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint64_t e = (uint64_t)eb * 32 + 16 * h + k;
+    uint64_t x = 0;
+    if (u < a.U && e < a.n_l) {
+      const uint64_t g = u / a.P, pl = u % a.P;
+      x = __builtin_nontemporal_load(a.words + ((g * a.n_l + e) * a.P + pl) * mn + sl);
+    }
+    bad |= x >= kDigitMax;
+    w[k] = digits_of(x);
+  }
+  if (bad) *a.bad = 1;
+  uint8_t *dst = a.frag + (((uint64_t)sl * a.UT + u / 4) * a.EB + eb) * 1024 + 16 * ((u % 4) * 8 + 32 * h);
+  store_chunks(w, dst);
+}
+
+// dst[g*per + i] = src[g*stride + i] for i < per (each group's last elements)
+__global__ void __launch_bounds__(256) mx_tail_copy_kernel(const uint64_t *src, uint64_t *dst, uint64_t per,
+                                                          uint64_t stride, uint64_t total) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= total) return;
+  dst[t] = src[(t / per) * stride + t % per];
+}
+
+// ---- the queries of one launch ----
+constexpr int kMaxQ = 64;
+
+struct PackQArgs {
+  const uint64_t *q[kMaxQ];  // level-0 query words: (e*2 + s)*m*n + sl
+  uint8_t *frag;
+  uint32_t *qbad;            // [kMaxQ]: a word of the query is not below its modulus
+  ModSet ms;
+  uint64_t n_l, total;
+  uint32_t S, n, qn, RTp, EB, EBP;
+};
+
+// thread (sl fastest, h, half s, query row qq < 2*RTp, element block eb < EB): digits of its
+// 16 words into lanes (qq % 2) * 16 + s * 8 + j + 32 h of fragment (sl, qq / 2, eb) for the
+// prefix blocks eb < EBP; every block's words are checked against the modulus
+__global__ void __launch_bounds__(256) mx_pack_q_kernel(PackQArgs a) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= a.total) return;
+  const uint32_t sl = (uint32_t)(t % a.S);
+  uint64_t r = t / a.S;
+  const uint32_t h = (uint32_t)(r & 1), s = (uint32_t)((r >> 1) & 1);
+  r >>= 2;
+  const uint32_t qq = (uint32_t)(r % (a.RTp * 2));
+  const uint32_t eb = (uint32_t)(r / (a.RTp * 2));
+  const uint64_t p = a.ms.p[sl / a.n];
+  const uint64_t mn = a.S;
+  uint64_t w[16];
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint64_t e = (uint64_t)eb * 32 + 16 * h + k;
+    uint64_t y = 0;
+    if (qq < a.qn && e < a.n_l) y = __builtin_nontemporal_load(a.q[qq] + (e * 2 + s) * mn + sl);
+    bad |= y >= p;
+    w[k] = digits_of(y);
+  }
+  if (bad) a.qbad[qq] = 1;
+  if (eb >= a.EBP) return;
+  uint8_t *dst = a.frag + (((uint64_t)sl * a.RTp + qq / 2) * a.EBP + eb) * 1024 + 16 * ((qq % 2) * 16 + s * 8 + 32 * h);
+  store_chunks(w, dst);
+}
+
+__global__ void mx_clear_kernel(uint32_t *p, uint32_t n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// ---- the matrix-core MAC ----
+struct MacArgsMx {
+  const uint8_t *A;  // query fragments [S][RTp][EBP]
+  const uint8_t *B;  // database fragments [S][UT][EB]
+  uint64_t *stage;   // per item MT*NT*16 words: (tile i, tile j, row group a, unit ul)
+  ModSet ms;
+  uint64_t items, per_xcd;
+  uint32_t n, RTp, UT, EB, EBP, RB, CB;
+};
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+  const int lo = __shfl_xor((int)(uint32_t)v, m, 64), hi = __shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Item = (slot, column block cb, row block rb), rb fastest: the row blocks of one column
+// block run back to back on one XCD (its database fragments come from HBM once), and the
+// XCD's items stay within few slots (that slot's query fragments stay in its L2).
+template <int MT>
+__global__ void __launch_bounds__(64) mx_mac_kernel(MacArgsMx a) {
+  constexpr int NT = kNT;
+  const uint32_t bid = blockIdx.x;
+  const uint64_t item = (uint64_t)(bid & 7) * a.per_xcd + (bid >> 3);
+  if (item >= a.items) return;
+  const uint32_t rb = (uint32_t)(item % a.RB);
+  const uint64_t t = item / a.RB;
+  const uint32_t cb = (uint32_t)(t % a.CB);
+  const uint32_t sl = (uint32_t)(t / a.CB);
+  const uint32_t lane = threadIdx.x;
+  const uint32_t k = sl / a.n;
+  const uint64_t p = a.ms.p[k];
+  const uint32_t delta = a.ms.delta[k];
+  const uint32_t ntv = min((uint32_t)NT, a.UT - cb * NT);  // column tiles in the database
+  const v4i *A[MT];
+  const v4i *B[NT];
+#pragma unroll
+  for (int i = 0; i < MT; i++)
+    A[i] = reinterpret_cast<const v4i *>(a.A + (((uint64_t)sl * a.RTp + rb * MT + i) * a.EBP) * 1024) + lane;
+#pragma unroll
+  for (int j = 0; j < NT; j++)
+    B[j] = reinterpret_cast<const v4i *>(a.B + (((uint64_t)sl * a.UT + cb * NT + min((uint32_t)j, ntv - 1)) * a.EB) * 1024) + lane;
+  // the epilogue's per-lane weight 2^(8(4h + lx)) mod p and its Shoup quotient
+  const uint32_t h = lane >> 5, lx = lane & 7, ul = (lane >> 3) & 3;
+  const uint32_t wexp = 8 * (4 * h + lx);
+  const uint64_t pw = wexp < 64 ? (1ull << wexp) : ((uint64_t)delta << (wexp - 61));
+  const uint64_t pwq = shoup_quot(pw, p, delta);
+  uint64_t *st = a.stage + item * (MT * NT * 16);
+  for (uint32_t c0 = 0; c0 < a.EBP; c0 += kPassBlocks) {
+    const uint32_t c1 = min(a.EBP, c0 + kPassBlocks);
+    v16i acc[MT][NT];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+      for (int j = 0; j < NT; j++) acc[i][j] = v16i{};
+    v4i fa[2][MT], fb[2][NT];
+    auto load = [&](int buf, uint32_t eb) {
+#pragma unroll
+      for (int i = 0; i < MT; i++) fa[buf][i] = A[i][(uint64_t)eb * 64];
+#pragma unroll
+      for (int j = 0; j < NT; j++) fb[buf][j] = B[j][(uint64_t)eb * 64];
+    };
+    auto mma = [&](int buf) {
+#pragma unroll
+      for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[buf][i], fb[buf][j], acc[i][j], 0, 0, 0);
+    };
+    load(0, c0);
+    uint32_t eb = c0;
+    for (; eb + 2 <= c1; eb += 2) {
+      load(1, eb + 1);
+      mma(0);
+      if (eb + 2 < c1) load(0, eb + 2);
+      mma(1);
+    }
+    if (eb < c1) mma(0);
+    // epilogue: per tile and row group a (query of the pair, half), the lane's column
+    // (unit ul, digit lx) and rows (digits 4h + b) give sum_b S * 2^(8b) times the lane's
+    // weight, reduced mod p; the sum over the 16 lanes of the unit is the output
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+      for (int j = 0; j < NT; j++) {
+        uint64_t V[4];
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          const int64_t D = (int64_t)acc[i][j][4 * g] + (int64_t)acc[i][j][4 * g + 1] * 256 +
+                            (int64_t)acc[i][j][4 * g + 2] * 65536 + (int64_t)acc[i][j][4 * g + 3] * 16777216;
+          uint64_t w = shoup_lazy_sf((uint64_t)(D + (int64_t)p), pw, pwq, delta);
+          w = w >= p ? w - p : w;
+          w += shfl_xor64(w, 1);
+          w += shfl_xor64(w, 2);
+          w += shfl_xor64(w, 4);  // below 8p < 2^64
+          w = (w & kM61) + (w >> 61) * delta;
+          w = w >= p ? w - p : w;
+          w += shfl_xor64(w, 32);
+          V[g] = w >= p ? w - p : w;
+        }
+        if (h == 0 && lx < 4) {
+          uint64_t v = lx == 0 ? V[0] : lx == 1 ? V[1] : lx == 2 ? V[2] : V[3];
+          uint64_t *o = st + ((i * NT + j) * 4 + lx) * 4 + ul;
+          if (c0 > 0) {
+            v += *o;
+            v = v >= p ? v - p : v;
+          }
+          *o = v;
+        }
+      }
+  }
+}
+
+// ---- staged residues to the level's outputs ----
+struct UnstageArgs {
+  const uint64_t *stage;
+  uint64_t *out[kMaxQ];  // (u*2 + s)*m*n + sl
+  uint32_t S, U, qn, MT, RB, CB;
+};
+
+// block (32-slot block sb, column block cb, row block rb): the 32 items' words through LDS,
+// out as rows of 32 consecutive slots
+constexpr uint32_t kUnstageSlots = 32;
+
+__global__ void __launch_bounds__(256) mx_unstage_kernel(UnstageArgs a) {
+  __shared__ uint64_t t[kUnstageSlots][4 * kNT * 16 + 1];
+  const uint32_t wpi = a.MT * kNT * 16;
+  const uint32_t rb = blockIdx.x % a.RB;
+  const uint32_t r = blockIdx.x / a.RB;
+  const uint32_t cb = r % a.CB, sb = r / a.CB;
+  for (uint32_t i = threadIdx.x; i < kUnstageSlots * wpi; i += 256) {
+    const uint32_t s = i / wpi, w = i % wpi;
+    const uint64_t item = ((uint64_t)(sb * kUnstageSlots + s) * a.CB + cb) * a.RB + rb;
+    t[s][w] = a.stage[item * wpi + w];
+  }
+  __syncthreads();
+  const uint64_t mn = a.S;
+  for (uint32_t i = threadIdx.x; i < kUnstageSlots * wpi; i += 256) {
+    const uint32_t w = i / kUnstageSlots, s = i % kUnstageSlots;
+    const uint32_t ul = w % 4, g = (w / 4) % 4, j = (w / 16) % kNT, ti = w / (16 * kNT);
+    const uint32_t q = 2 * (rb * a.MT + ti) + (g >> 1), half = g & 1;
+    const uint32_t u = 4 * (cb * kNT + j) + ul;
+    if (q < a.qn && u < a.U) a.out[q][((uint64_t)u * 2 + half) * mn + sb * kUnstageSlots + s] = t[s][w];
+  }
+}
+
+// ---- whole chains where the tail did not settle them ----
+struct FixArgs {
+  const uint8_t *frag;       // database fragments [S][UT][EB]
+  const uint64_t *q[kMaxQ];  // level-0 query words
+  uint64_t *out[kMaxQ];
+  const uint32_t *qbad;
+  ModSet ms;
+  uint64_t n_l, total;
+  uint32_t S, n, U, UT, EB, P;
+};
+
+__global__ void __launch_bounds__(256) mx_fixup_kernel(FixArgs a) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= a.total) return;
+  const uint32_t sl = (uint32_t)(t % a.S);
+  uint64_t r = t / a.S;
+  const uint32_t s = (uint32_t)(r & 1);
+  r >>= 1;
+  const uint32_t u = (uint32_t)(r % a.U), q = (uint32_t)(r / a.U);
+  const uint64_t mn = a.S;
+  uint64_t *o = a.out[q] + ((uint64_t)u * 2 + s) * mn + sl;
+  if (!(*o >> 63) && !a.qbad[q]) return;
+  const uint32_t k = sl / a.n;
+  const uint64_t p = a.ms.p[k];
+  const uint32_t delta = a.ms.delta[k];
+  const uint8_t *f0 = a.frag + ((uint64_t)sl * a.UT + u / 4) * a.EB * 1024 + 16 * ((u % 4) * 8);
+  const uint64_t *y = a.q[q] + (uint64_t)s * mn + sl;
+  uint64_t acc = 0;
+  for (uint64_t e = 0; e < a.n_l; e++) {
+    const uint8_t *f = f0 + (e / 32) * 1024 + 16 * 32 * ((e % 32) / 16) + (e % 16);
+    uint64_t b = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) b |= (uint64_t)f[16 * j] << (8 * j);
+    const uint64_t x = (b ^ kDigitBias) - kDigitBias;
+    const uint64_t yv = y[e * 2 * mn];
+    acc = mac_step(acc, x, yv, shoup_quot(yv, p, delta), p, delta);
+  }
+  *o = acc;
+}
+
+uint64_t al256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
+
+int mt_for(uint32_t qn) { return qn == 1 ? 1 : qn <= 4 ? 2 : 4; }
+
+}  // namespace
+
+bool mx_geometry(const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, MxGeom *g) {
+  if (prm->n % 64 != 0 || n_l < kTailMin + 32 || n_l > (1ull << 31)) return false;
+  g->S = prm->m * prm->n;
+  g->U = (uint32_t)(groups * P);
+  g->UT = (g->U + 3) / 4;
+  g->EB = (uint32_t)((n_l + 31) / 32);
+  g->E0 = (n_l - kTailMin) / 32 * 32;
+  g->EBP = (uint32_t)(g->E0 / 32);
+  g->L = (uint32_t)(n_l - g->E0);
+  g->n_l = n_l;
+  g->groups = groups;
+  g->P = P;
+  return groups * P < (1ull << 31);
+}
+
+uint64_t mx_frag_bytes(const MxGeom &g) { return (uint64_t)g.S * g.UT * g.EB * 1024; }
+
+uint64_t mx_tail_words(const MxGeom &g) { return g.groups * g.L * g.P * (uint64_t)g.S; }
+
+int launch_mx_pack_db(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const uint64_t *words, uint8_t *frag,
+                      uint64_t *tail, uint32_t *bad_dev) {
+  PackDbArgs a{};
+  a.words = words;
+  a.frag = frag;
+  a.bad = bad_dev;
+  a.n_l = g.n_l;
+  a.S = g.S;
+  a.U = g.U;
+  a.UT = g.UT;
+  a.EB = g.EB;
+  a.P = g.P;
+  a.total = (uint64_t)g.S * g.UT * 4 * 2 * g.EB;
+  const uint64_t blocks = (a.total + 255) / 256;
+  if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "database too large for the matrix layout");
+  hipLaunchKernelGGL(mx_pack_db_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "matrix layout launch");
+  // each group's last L elements, row-major, for the chains' tails
+  const uint64_t row = (uint64_t)g.P * g.S, per_group = g.L * row, tw = g.groups * per_group;
+  const uint64_t tblocks = (tw + 255) / 256;
+  if (tblocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "database too large for the matrix layout");
+  hipLaunchKernelGGL(mx_tail_copy_kernel, dim3((uint32_t)tblocks), dim3(256), 0, ctx->stream, words + g.E0 * row,
+                     tail, per_group, g.n_l * row, tw);
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "tail rows copy");
+  (void)prm;
+  return XPG_OK;
+}
+
+// workspace of one launch of qn queries: query fragments, staged residues, query flags
+static void mx_work_parts(const MxGeom &g, uint32_t qn, uint64_t *a_bytes, uint64_t *st_bytes) {
+  const int MT = mt_for(qn);
+  const uint32_t RT = (qn + 1) / 2, RB = (RT + MT - 1) / MT, RTp = RB * MT;
+  const uint32_t CB = (g.UT + kNT - 1) / kNT;
+  *a_bytes = (uint64_t)g.S * RTp * g.EBP * 1024;
+  *st_bytes = (uint64_t)g.S * CB * RB * MT * kNT * 16 * 8;
+}
+
+uint64_t mx_work_bytes(const MxGeom &g, uint32_t qn) {
+  uint64_t ab = 0, sb = 0;
+  mx_work_parts(g, qn, &ab, &sb);
+  return al256(ab) + al256(sb) + 256;
+}
+
+int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const uint8_t *frag, const uint64_t *tail,
+                    const uint64_t *const *q, uint64_t *const *out, uint32_t qn, void *work) {
+  if (qn == 0 || qn > (uint32_t)kMaxQ) return set_err(ctx, XPG_EINVAL, "batch of 1..64 queries per launch");
+  const int MT = mt_for(qn);
+  const uint32_t RT = (qn + 1) / 2, RB = (RT + MT - 1) / MT, RTp = RB * MT;
+  const uint32_t CB = (g.UT + kNT - 1) / kNT;
+  uint64_t ab = 0, sb = 0;
+  mx_work_parts(g, qn, &ab, &sb);
+  uint8_t *A = static_cast<uint8_t *>(work);
+  uint64_t *stage = reinterpret_cast<uint64_t *>(A + al256(ab));
+  uint32_t *qbad = reinterpret_cast<uint32_t *>(A + al256(ab) + al256(sb));
+  const ModSet ms = modset_of(prm);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  timing_begin(ctx, &e0, &e1);
+  hipLaunchKernelGGL(mx_clear_kernel, dim3(1), dim3(64), 0, ctx->stream, qbad, (uint32_t)kMaxQ);
+  {
+    PackQArgs a{};
+    for (uint32_t i = 0; i < qn; i++) a.q[i] = q[i];
+    a.frag = A;
+    a.qbad = qbad;
+    a.ms = ms;
+    a.n_l = g.n_l;
+    a.S = g.S;
+    a.n = prm->n;
+    a.qn = qn;
+    a.RTp = RTp;
+    a.EB = g.EB;
+    a.EBP = g.EBP;
+    a.total = (uint64_t)g.S * 4 * RTp * 2 * g.EB;
+    const uint64_t blocks = (a.total + 255) / 256;
+    if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "query layout grid too large");
+    hipLaunchKernelGGL(mx_pack_q_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
+  }
+  {
+    MacArgsMx a{};
+    a.A = A;
+    a.B = frag;
+    a.stage = stage;
+    a.ms = ms;
+    a.n = prm->n;
+    a.RTp = RTp;
+    a.UT = g.UT;
+    a.EB = g.EB;
+    a.EBP = g.EBP;
+    a.RB = RB;
+    a.CB = CB;
+    a.items = (uint64_t)g.S * CB * RB;
+    a.per_xcd = (a.items + 7) / 8;
+    const uint64_t blocks = a.per_xcd * 8;
+    if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "matrix MAC grid too large");
+    hipEvent_t k0 = nullptr, k1 = nullptr;
+    timing_begin(ctx, &k0, &k1, 2);
+    if (ctx->timing && !ctx->ws) ctx->batch_steps += (uint64_t)qn * g.U * g.S * 2 * g.n_l;
+    switch (MT) {
+      case 4: hipLaunchKernelGGL(mx_mac_kernel<4>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
+      case 2: hipLaunchKernelGGL(mx_mac_kernel<2>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
+      default: hipLaunchKernelGGL(mx_mac_kernel<1>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
+    }
+    timing_end(ctx, k0, k1, 2);
+  }
+  {
+    UnstageArgs a{};
+    a.stage = stage;
+    for (uint32_t i = 0; i < qn; i++) a.out[i] = out[i];
+    a.S = g.S;
+    a.U = g.U;
+    a.qn = qn;
+    a.MT = (uint32_t)MT;
+    a.RB = RB;
+    a.CB = CB;
+    const uint64_t blocks = (uint64_t)(g.S / kUnstageSlots) * CB * RB;
+    hipLaunchKernelGGL(mx_unstage_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
+  }
+  if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "matrix MAC launch");
+  {
+    std::vector<const uint64_t *> qt(qn);
+    for (uint32_t i = 0; i < qn; i++) qt[i] = q[i] + g.E0 * 2 * (uint64_t)g.S;
+    const int rc = launch_mac_tail(ctx, prm, tail, qt.data(), out, qn, g.L, g.groups, g.P);
+    if (rc) return rc;
+  }
+  {
+    FixArgs a{};
+    a.frag = frag;
+    for (uint32_t i = 0; i < qn; i++) {
+      a.q[i] = q[i];
+      a.out[i] = out[i];
+    }
+    a.qbad = qbad;
+    a.ms = ms;
+    a.n_l = g.n_l;
+    a.S = g.S;
+    a.n = prm->n;
+    a.U = g.U;
+    a.UT = g.UT;
+    a.EB = g.EB;
+    a.P = g.P;
+    a.total = (uint64_t)qn * g.U * 2 * g.S;
+    const uint64_t blocks = (a.total + 255) / 256;
+    if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "fixup grid too large");
+    hipLaunchKernelGGL(mx_fixup_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
+  }
+  const int rc = hipGetLastError() != hipSuccess ? set_err(ctx, XPG_EDEVICE, "matrix MAC launch") : XPG_OK;
+  timing_end(ctx, e0, e1);
+  return rc;
+}

@@ -308,6 +308,7 @@ int xpg_autotune(xpg_ctx *ctx, const xpg_db *db, uint32_t max_batch, xpg_tune_en
   if (!ctx || !db || (!out && cap)) return XPG_EINVAL;
   if (db->device != ctx->device) return set_err(ctx, XPG_EINVAL, "database lives on another GPU");
   if (max_batch > 64) return set_err(ctx, XPG_EINVAL, "batches hold at most 64 queries");
+  if (db->mx) return set_err(ctx, XPG_EINVAL, "a database in the matrix layout has one level-0 kernel: nothing to tune");
   (void)hipSetDevice(ctx->device);
   TimingOff quiet(ctx);
   const xpg_params *prm = &db->prm;
