@@ -151,6 +151,9 @@ def main():
     ap.add_argument("--batch-fuse", type=int, default=1, choices=(0, 1),
                     help="batched passes: level 0 of every database in fused persistent launches (1) or one "
                          "launch per database chunk (0)")
+    ap.add_argument("--db-layout", type=int, default=1, choices=(0, 1),
+                    help="batched/serving databases: level 0 on the matrix cores (1, the matrix layout) or "
+                         "the row-major words with the VALU kernels (0)")
     ap.add_argument("--host-io", action="store_true",
                     help="single/plain: also time replies from a host query to a host reply (PCIe included)")
     args = ap.parse_args()
@@ -454,11 +457,13 @@ def autotune_db(ctx, db, max_batch, args, label):
     return {"seconds": dt, "entries": entries}
 
 
-def _load_batch_dbs(ctx, scale, d_override=None):
-    """The eight uneven databases, imported on the GPU and resident (d = 2 grids)."""
+def _load_batch_dbs(ctx, scale, d_override=None, layout=0):
+    """The eight uneven databases, imported on the GPU and resident (d = 2 grids), in the
+    row-major (layout 0) or the matrix layout (1)."""
     import xpir_amd
     from xpir_amd.workloads import batch_databases
     dbs = []
+    ctx.set_option("db_layout", layout)
     for spec in batch_databases(scale):
         dims = spec["dims"]
         prm = xpir_amd.params(spec["crypto"], dims[0])
@@ -474,7 +479,8 @@ def _load_batch_dbs(ctx, scale, d_override=None):
             f"({raw / dt / 1e9:.2f} GB/s raw), NTT {db.polys * dims[0] * dims[1] * prm.m * prm.n * 8 / 2**30:.2f} GiB")
         sk = xpir_amd.keygen(prm, spec["key_seed"])
         skd = ctx.alloc(sk.nbytes).upload(sk)
-        dbs.append(dict(spec=spec, db=db, prm=prm, raw=raw, skd=skd, import_s=dt))
+        dbs.append(dict(spec=spec, db=db, prm=prm, raw=raw, skd=skd, import_s=dt, matrix=layout == 1))
+    ctx.set_option("db_layout", 0)
     return dbs
 
 
@@ -857,7 +863,7 @@ def bench_batched(args):
     import xpir_amd
     from xpir_amd.workloads import batch_query_counts
     ctx = xpir_amd.Context(0)
-    dbs = _load_batch_dbs(ctx, args.scale)
+    dbs = _load_batch_dbs(ctx, args.scale, layout=args.db_layout)
     counts = batch_query_counts()
     rng = np.random.default_rng(42)
     reqs = []
@@ -877,7 +883,8 @@ def bench_batched(args):
         bucket = 1
         while bucket < min(nq, 64):
             bucket *= 2
-        tune[i] = autotune_db(ctx, d["db"], bucket if bucket >= 2 else 0, args, f"db{i}")
+        tune[i] = None if d.get("matrix") else autotune_db(ctx, d["db"], bucket if bucket >= 2 else 0, args,
+                                                           f"db{i}")
     # plain: one query per database, alone (best of 2)
     plain_t = []
     for d in dbs:
@@ -931,7 +938,7 @@ def bench_serving(args):
     same query answered alone."""
     import xpir_amd
     ctx = xpir_amd.Context(0)
-    dbs = _load_batch_dbs(ctx, args.scale)
+    dbs = _load_batch_dbs(ctx, args.scale, layout=args.db_layout)
     # d = 1 views of the two smallest databases (their d = 1 queries fit the budget; the
     # absorption size depends on the first dimension, so each view is its own transform)
     for base in dbs[:2]:
@@ -948,7 +955,7 @@ def bench_serving(args):
     tune_s = 0.0
     for i, d in enumerate(dbs):
         for label, view in (("d2", d), ("d1", d.get("d1"))):
-            if view is not None:
+            if view is not None and not view.get("matrix"):  # the matrix layout has nothing to tune
                 r = autotune_db(ctx, view["db"], 8, args, f"db{i}-{label}")
                 tune_s += r["seconds"] if r else 0.0
     rng = np.random.default_rng(51)
@@ -1055,6 +1062,34 @@ def batch_kernel_profile(fused: bool = True):
         return None
 
 
+MFMA_I8_PEAK_TOPS = 5000.0  # MI355X dense int8 matrix-core peak: 5.0 POP/s
+
+
+def matrix_roofline(kernel_ms, launches, steps):
+    """The matrix layout's level-0 kernel (mx_mac_kernel: the residues of the chains'
+    prefixes) against the dense int8 matrix-core peak.  An exact step of a prefix is the 8 x
+    8 byte-digit products of its two words: 64 multiply-adds, 128 int8 ops.  HBM traffic
+    from the kernel's committed counter profile when present."""
+    path = os.path.join(ROOT, "profiles", "mx_mac_kernel_batched_uneven_8dbs.json")
+    try:
+        prof = json.load(open(path))
+    except (OSError, ValueError):
+        prof = None
+    achieved = steps * 128 / (kernel_ms / 1e3) / 1e12
+    r = {"bound": "mfma", "unit": "TOP/s", "peak": MFMA_I8_PEAK_TOPS,
+         "kernel": "mx_mac_kernel (int8 MFMA residues of the level-0 chains' prefixes, one launch per database "
+                   "chunk)",
+         "kernel_ms_per_pass": kernel_ms, "launches_per_pass": launches, "prefix_steps_per_pass": steps,
+         "int8_ops_per_step": 128, "achieved": achieved, "frac": achieved / MFMA_I8_PEAK_TOPS,
+         "peak_derivation": "AMD's published dense int8 MFMA rate of MI355X (5.0 POP/s, no sparsity)",
+         "traffic": prof.get("hbm_bytes_per_pass") if prof else None}
+    if prof:
+        r.update(hbm_gbs=prof["hbm_bytes_per_pass"] / (kernel_ms / 1e3) / 1e9,
+                 counters_from="profiles/mx_mac_kernel_batched_uneven_8dbs.json (rocprofv3 FETCH_SIZE, WRITE_SIZE "
+                               "of the same pass; commands recorded there)")
+    return r
+
+
 def make_batch_queries(ctx, dbs, counts, rng):
     """Per database: its golden query first, then count-1 seeded queries (indices from
     rng, noise seeds 9000 + 100 i + j), all generated on the GPU."""
@@ -1078,6 +1113,9 @@ def make_batch_queries(ctx, dbs, counts, rng):
 def tune_batch_dbs(ctx, dbs, counts, args):
     tuned, secs = [], 0.0
     for i, (d, nq) in enumerate(zip(dbs, counts)):
+        if d.get("matrix"):  # one level-0 kernel: nothing to tune
+            tuned.append(None)
+            continue
         bucket = 1
         while bucket < min(nq, 64):
             bucket *= 2
@@ -1094,7 +1132,7 @@ def bench_batched_pass(args):
     import xpir_amd
     from xpir_amd.workloads import batch_query_counts
     ctx = xpir_amd.Context(0)
-    dbs = _load_batch_dbs(ctx, args.scale)
+    dbs = _load_batch_dbs(ctx, args.scale, layout=args.db_layout)
     counts = batch_query_counts()
     reqs = make_batch_queries(ctx, dbs, counts, np.random.default_rng(42))
     if args.tune_file and os.path.exists(args.tune_file):
@@ -1250,6 +1288,35 @@ def cpu_baseline_batched(dbs, counts, golden_replies, budget_steps=4.0e8):
                       "counts; threads = polys per element (the reference's OpenMP split)"}
 
 
+def valu_roofline(args, k_ms, k_launches, k_steps, steps_s):
+    """The row-major layout's level-0 kernel against the VALU issue rate, with the counters
+    of its committed profile."""
+    prof = batch_kernel_profile(bool(args.batch_fuse))
+    roofline = {"bound": "valu", "unit": "TOP/s", "peak": VALU_PEAK_TOPS,
+                "kernel": ("mac_fused_kernel (persistent register-blocked level-0 MAC over every database)"
+                           if args.batch_fuse else
+                           "mac_batch3_kernel (register-blocked level-0 MAC, every launch of the pass)"),
+                "kernel_ms_per_pass": k_ms / args.steps, "launches_per_pass": k_launches / args.steps,
+                "exact_steps_per_pass": k_steps / args.steps, "exact_steps_per_s": steps_s,
+                "peak_derivation": "256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 32-bit lane ops/s "
+                                   "(= 157.3 TFLOP/s fp32 / 2 flops per FMA / 2 for packed fp32)"}
+    if prof:
+        vps = prof["valu_lane_ops_per_step"]
+        roofline.update(achieved=steps_s * vps / 1e12, frac=steps_s * vps / 1e12 / VALU_PEAK_TOPS,
+                        valu_lane_ops_per_step=vps, traffic=prof.get("hbm_bytes_per_pass"),
+                        hbm_gbs=(prof["hbm_bytes_per_pass"] / (k_ms / args.steps / 1e3) / 1e9
+                                 if prof.get("hbm_bytes_per_pass") else None),
+                        valu_busy=prof.get("valu_busy"), sustained_clock_ghz=prof.get("clock_ghz_from_grbm"),
+                        counters_from=f"profiles/{'mac_fused_kernel' if args.batch_fuse else 'mac_batch3_kernel'}"
+                                      "_batched_uneven_8dbs.json (rocprofv3 "
+                                      "SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU, GRBM_GUI_ACTIVE, FETCH_SIZE, WRITE_SIZE "
+                                      "of the same pass; commands recorded there; traffic is those counters', "
+                                      "not measured in this run)")
+    else:
+        roofline.update(achieved=None, frac=None, traffic=None)
+    return roofline
+
+
 def bench_metric(args):
     """The metric: batched vs plain over the eight uneven databases, p50/p99 latency of a
     serving stream over them, bit-exactness against the reference's replies."""
@@ -1258,7 +1325,7 @@ def bench_metric(args):
     gold = load_gold()
     ctx = xpir_amd.Context(0)
     t_setup = time.perf_counter()
-    dbs = _load_batch_dbs(ctx, args.scale)
+    dbs = _load_batch_dbs(ctx, args.scale, layout=args.db_layout)
     counts = batch_query_counts()
     reqs = make_batch_queries(ctx, dbs, counts, np.random.default_rng(42))
     log(f"{len(reqs)} queries over {len(dbs)} databases, {sum(r[1].nbytes for r in reqs) / 2**30:.1f} GiB of queries "
@@ -1307,26 +1374,29 @@ def bench_metric(args):
     value = total_raw / pass_s / 1e9
     log(f"{args.steps} batched passes: {pass_s * 1e3:.1f} ms per pass -> {value:.1f} GB/s "
         f"({plain_s / pass_s:.2f}x the plain path), {k_launches / args.steps:.0f} level-0 launches per pass")
-    # the other level-0 launch structure beside it, on a few passes (same replies)
-    other = 1 - args.batch_fuse
-    ctx.set_option("batch_fuse", other)
-    ctx.reply_batch_dev(batch)
-    ctx.timing(2)
-    ctx.timing_read_batched(reset=True)
-    ctx.sync()
-    n_other = 3
-    t0 = time.perf_counter()
-    for _ in range(n_other):
+    # row-major layout: the other level-0 launch structure beside it, on a few passes (same
+    # replies); the matrix layout has one launch structure (a launch per database chunk)
+    other_launches = None
+    if not args.db_layout:
+        other = 1 - args.batch_fuse
+        ctx.set_option("batch_fuse", other)
         ctx.reply_batch_dev(batch)
-    ctx.sync()
-    other_s = (time.perf_counter() - t0) / n_other
-    o_ms, o_launches, _ = ctx.timing_read_batched(reset=True)
-    ctx.timing(False)
-    ctx.set_option("batch_fuse", args.batch_fuse)
-    other_launches = {"batch_fuse": other, "ms_per_pass": other_s * 1e3, "gbs": total_raw / other_s / 1e9,
-                      "level0_kernel_ms_per_pass": o_ms / n_other, "level0_launches_per_pass": o_launches / n_other,
-                      "passes": n_other}
-    log(f"batch_fuse={other}: {other_s * 1e3:.1f} ms per pass, {o_launches / n_other:.0f} level-0 launches")
+        ctx.timing(2)
+        ctx.timing_read_batched(reset=True)
+        ctx.sync()
+        n_other = 3
+        t0 = time.perf_counter()
+        for _ in range(n_other):
+            ctx.reply_batch_dev(batch)
+        ctx.sync()
+        other_s = (time.perf_counter() - t0) / n_other
+        o_ms, o_launches, _ = ctx.timing_read_batched(reset=True)
+        ctx.timing(False)
+        ctx.set_option("batch_fuse", args.batch_fuse)
+        other_launches = {"batch_fuse": other, "ms_per_pass": other_s * 1e3, "gbs": total_raw / other_s / 1e9,
+                          "level0_kernel_ms_per_pass": o_ms / n_other,
+                          "level0_launches_per_pass": o_launches / n_other, "passes": n_other}
+        log(f"batch_fuse={other}: {other_s * 1e3:.1f} ms per pass, {o_launches / n_other:.0f} level-0 launches")
 
     # bit-exactness: every batched reply equals the plain reply; each database's golden
     # query equals the reference's reply bytes
@@ -1340,31 +1410,11 @@ def bench_metric(args):
         d["golden_query"] = d["reqs"][0][1].download()
     log(f"bit-exact: batched == plain {exact_vs_plain}, golden queries {sum(golden_ok)}/{len(golden_ok)}")
 
-    prof = batch_kernel_profile(bool(args.batch_fuse))
     steps_s = k_steps / (k_ms / 1e3)
-    roofline = {"bound": "valu", "unit": "TOP/s", "peak": VALU_PEAK_TOPS,
-                "kernel": ("mac_fused_kernel (persistent register-blocked level-0 MAC over every database)"
-                           if args.batch_fuse else
-                           "mac_batch3_kernel (register-blocked level-0 MAC, every launch of the pass)"),
-                "kernel_ms_per_pass": k_ms / args.steps, "launches_per_pass": k_launches / args.steps,
-                "exact_steps_per_pass": k_steps / args.steps, "exact_steps_per_s": steps_s,
-                "peak_derivation": "256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 32-bit lane ops/s "
-                                   "(= 157.3 TFLOP/s fp32 / 2 flops per FMA / 2 for packed fp32)"}
-    if prof:
-        vps = prof["valu_lane_ops_per_step"]
-        roofline.update(achieved=steps_s * vps / 1e12, frac=steps_s * vps / 1e12 / VALU_PEAK_TOPS,
-                        valu_lane_ops_per_step=vps, traffic=prof.get("hbm_bytes_per_pass"),
-                        hbm_gbs=(prof["hbm_bytes_per_pass"] / (k_ms / args.steps / 1e3) / 1e9
-                                 if prof.get("hbm_bytes_per_pass") else None),
-                        valu_busy=prof.get("valu_busy"), sustained_clock_ghz=prof.get("clock_ghz_from_grbm"),
-                        counters_from=f"profiles/{'mac_fused_kernel' if args.batch_fuse else 'mac_batch3_kernel'}"
-                                      "_batched_uneven_8dbs.json (rocprofv3 "
-                                      "SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU, GRBM_GUI_ACTIVE, FETCH_SIZE, WRITE_SIZE "
-                                      "of the same pass; commands recorded there; traffic is those counters', "
-                                      "not measured in this run)")
+    if args.db_layout:
+        roofline = matrix_roofline(k_ms / args.steps, k_launches / args.steps, k_steps / args.steps)
     else:
-        roofline.update(achieved=None, frac=None, traffic=None)
-
+        roofline = valu_roofline(args, k_ms, k_launches, k_steps, steps_s)
     latency = serving_latency(ctx, dbs, args)
     log(f"serving stream: p50 {latency['p50']:.2f} ms p99 {latency['p99']:.2f} ms, "
         f"{latency['served_gbs']:.1f} GB/s, exact {latency['bit_exact_vs_alone']}")
@@ -1410,8 +1460,9 @@ def bench_metric(args):
         "serving": latency,
         "plain_latency_ms": {"p50": float(np.percentile(plain_lat, 50)), "p99": float(np.percentile(plain_lat, 99))},
         "autotune_s": tune_s,
-        "level0_launch_structure": {"batch_fuse": args.batch_fuse, "level0_launches_per_pass": k_launches / args.steps,
-                                    "other": other_launches},
+        "db_layout": "matrix (level 0 on the int8 matrix cores)" if args.db_layout else "row-major",
+        "level0_launch_structure": {"batch_fuse": None if args.db_layout else args.batch_fuse,
+                                    "level0_launches_per_pass": k_launches / args.steps, "other": other_launches},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "single_query": single,

@@ -26,8 +26,9 @@
 // of (x + B) ^ B, B = 0x8080808080808080 (the bytes of x + B are d_j + 128).  For a slot,
 //     sum_e y_e x_e = sum_{i,j} 2^(8(i+j)) S_ij,   S_ij = sum_e dy_i(e) dx_j(e),
 // and v_mfma_i32_32x32x32_i8 accumulates S_ij for row (query pair, half, digit i) and
-// column (unit of four, digit j), 32 elements per instruction, exactly in int32 for up to
-// 2^16 elements (|dy dx| <= 2^14).  Longer prefixes are cut into passes of 2^16.
+// column (unit of four, digit j), 32 elements per instruction, exactly in int32 (|dy dx| <=
+// 2^14).  Prefixes are cut into passes of 2^15 elements, so |S_ij| <= 2^29 and the
+// epilogue may pair two lanes' sums exactly in 64 bits before reducing mod p.
 //
 // Fragments (1 KiB, one instruction operand): lane L holds row (or column) L % 32 and the
 // 16 elements 16*(L/32) .. +15 of a block of 32, one byte each (the instruction pairs A's
@@ -47,7 +48,7 @@ constexpr uint64_t kDigitBias = 0x8080808080808080ull;
 constexpr uint64_t kDigitMax = 0x7F7F7F7F7F7F7F80ull;  // words below it have 8 signed digits
 constexpr uint64_t kM61 = (1ull << 61) - 1;
 constexpr uint32_t kTailMin = 40;                      // fewest tail steps
-constexpr uint32_t kPassBlocks = 2048;                 // element blocks per int32 pass (2^16)
+constexpr uint32_t kPassBlocks = 1024;                 // element blocks per pass (2^15 elements)
 constexpr int kNT = 2;                                 // column tiles per wave
 
 __device__ __forceinline__ uint64_t digits_of(uint64_t x) { return (x + kDigitBias) ^ kDigitBias; }
@@ -178,10 +179,18 @@ __global__ void mx_clear_kernel(uint32_t *p, uint32_t n) {
 }
 
 // ---- the matrix-core MAC ----
+//
+// A workgroup of four waves takes an item (slot, column block cb, row block rb): TR = WR*MT
+// row tiles x TC = WC*NT column tiles (WR x WC waves, each MT x NT tiles of 32 x 32).  Per
+// block of 32 elements it stages the item's TR + TC fragments in LDS (each wave loads a
+// quarter of them), and every wave reads its MT + NT from there: each fragment leaves L2
+// once per item instead of once per wave.  Items run rb fastest: the row blocks of one
+// column block follow one another on one XCD (its database fragments come from HBM once),
+// and an XCD's items stay within few slots (their query fragments stay in its L2).
 struct MacArgsMx {
   const uint8_t *A;  // query fragments [S][RTp][EBP]
   const uint8_t *B;  // database fragments [S][UT][EB]
-  uint64_t *stage;   // per item MT*NT*16 words: (tile i, tile j, row group a, unit ul)
+  uint64_t *stage;   // per item 4 * MT*NT*16 words: (wave, tile i, tile j, row group, unit)
   ModSet ms;
   uint64_t items, per_xcd;
   uint32_t n, RTp, UT, EB, EBP, RB, CB;
@@ -192,38 +201,103 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
   return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
-// Item = (slot, column block cb, row block rb), rb fastest: the row blocks of one column
-// block run back to back on one XCD (its database fragments come from HBM once), and the
-// XCD's items stay within few slots (that slot's query fragments stay in its L2).
-template <int MT>
-__global__ void __launch_bounds__(64) mx_mac_kernel(MacArgsMx a) {
-  constexpr int NT = kNT;
+// lane exchanges within quads and half rows by DPP (one VALU instruction per dword)
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kQuadRev = 0x1B, kHalfMirror = 0x141;
+
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)v, CTRL, 0xF, 0xF, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, true);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// a * b + c (signed 32 x 32 + 64 bits), one v_mad_i64_i32
+__device__ __forceinline__ uint64_t mad_i64(int32_t a, int32_t b, uint64_t c) {
+  uint64_t d;
+  asm("v_mad_i64_i32 %0, vcc, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c) : "vcc");
+  return d;
+}
+
+__device__ __forceinline__ uint64_t canon(uint64_t v, uint64_t p) { return v >= p ? v - p : v; }
+
+// Epilogue of one tile: lane L holds S for column (unit ul = (L>>3)&3, digit lx = L&7) and
+// rows 8g + 4h + b (g: query of the pair and half; digits 4h + b), h = L>>5, in acc[4g+b].
+// Returns, in the lanes with h = 0 and lx < 4, the output of row group g = lx for unit ul:
+// sum over the 16 lanes of unit ul of 2^(8(4h+lx)) * sum_b 2^(8b) S, reduced mod p.
+//   D_g = sum_b 2^(8b) S (|D| < 2^53.01: |S| <= 2^29 for passes of 2^15 elements);
+//   xor 1, exact: the even lane keeps groups 0 and 2, the odd lane 1 and 3, each as
+//     P = D(even lane) + 2^8 D(odd lane)  (|P| < 2^61.02);
+//   weights: P + 4p (positive) times 2^(8(4h + (lx & ~1))) mod p, a Shoup product;
+//   xor 2: the lanes with bit 1 clear keep groups {0, 1}, the others {2, 3} (so lane lx
+//     holds group lx & 3); xor 4 and xor 32 add the partners' partial sums mod p.
+// xor 1, 2 and 4 are DPP moves (xor 4 = quad reversal, then half-row mirror); xor 32 a
+// permute through LDS.
+__device__ __forceinline__ uint64_t tile_epilogue(const v16i &acc, uint32_t lane, uint64_t p, uint32_t delta,
+                                                  uint64_t pw, uint64_t pwq, uint64_t p4) {
+  uint64_t D[4];
+#pragma unroll
+  for (int g = 0; g < 4; g++)
+    D[g] = mad_i64(acc[4 * g + 3], 1 << 24,
+                   mad_i64(acc[4 * g + 2], 1 << 16, mad_i64(acc[4 * g + 1], 1 << 8, (uint64_t)(int64_t)acc[4 * g])));
+  const bool odd = lane & 1, hi2 = (lane >> 1) & 1;
+  // xor 1: send the groups the partner keeps
+  const uint64_t r0 = dpp64<kQuadXor1>(odd ? D[0] : D[1]);
+  const uint64_t r1 = dpp64<kQuadXor1>(odd ? D[2] : D[3]);
+  const uint64_t k0 = odd ? D[1] : D[0], k1 = odd ? D[3] : D[2];
+  const uint64_t P0 = (odd ? r0 : k0) + ((odd ? k0 : r0) << 8);  // groups odd, 2 + odd
+  const uint64_t P1 = (odd ? r1 : k1) + ((odd ? k1 : r1) << 8);
+  const uint64_t w0 = canon(shoup_lazy_asm(P0 + p4, pw, pwq, delta), p);
+  const uint64_t w1 = canon(shoup_lazy_asm(P1 + p4, pw, pwq, delta), p);
+  // xor 2: keep group odd (bit 1 clear) or 2 + odd (bit 1 set)
+  uint64_t v = canon((hi2 ? w1 : w0) + dpp64<kQuadXor2>(hi2 ? w0 : w1), p);
+  v = canon(v + dpp64<kHalfMirror>(dpp64<kQuadRev>(v)), p);
+  return canon(v + shfl_xor64(v, 32), p);
+}
+
+template <int WR, int MT>
+__global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
+  constexpr int WC = 4 / WR, NT = kNT, TR = WR * MT, TC = WC * NT, NF = TR + TC, LPT = (NF + 3) / 4;
+  __shared__ v4i lds[2][NF][64];
   const uint32_t bid = blockIdx.x;
   const uint64_t item = (uint64_t)(bid & 7) * a.per_xcd + (bid >> 3);
-  if (item >= a.items) return;
+  if (item >= a.items) return;  // the whole workgroup
   const uint32_t rb = (uint32_t)(item % a.RB);
   const uint64_t t = item / a.RB;
   const uint32_t cb = (uint32_t)(t % a.CB);
   const uint32_t sl = (uint32_t)(t / a.CB);
-  const uint32_t lane = threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t wr = wave / WC, wc = wave % WC;
   const uint32_t k = sl / a.n;
   const uint64_t p = a.ms.p[k];
   const uint32_t delta = a.ms.delta[k];
-  const uint32_t ntv = min((uint32_t)NT, a.UT - cb * NT);  // column tiles in the database
-  const v4i *A[MT];
-  const v4i *B[NT];
+  // the fragments this thread stages: f = wave + 4 i (row tiles first, then column tiles;
+  // column tiles past the database read its last tile, whose results are dropped), each
+  // from a wave-uniform base plus the lane's 32-bit offset (one offset for all of them)
+  const uint8_t *src[LPT];
 #pragma unroll
-  for (int i = 0; i < MT; i++)
-    A[i] = reinterpret_cast<const v4i *>(a.A + (((uint64_t)sl * a.RTp + rb * MT + i) * a.EBP) * 1024) + lane;
+  for (int i = 0; i < LPT; i++) {
+    const uint32_t f = min((uint32_t)(wave + 4 * i), (uint32_t)NF - 1);
+    src[i] = f < (uint32_t)TR ? a.A + ((uint64_t)sl * a.RTp + rb * TR + f) * a.EBP * 1024
+                              : a.B + ((uint64_t)sl * a.UT + min(cb * TC + f - TR, a.UT - 1)) * a.EB * 1024;
+  }
+  auto stage_ok = [&](int i) { return wave + 4 * i < (uint32_t)NF; };
+  auto load = [&](v4i (&r)[LPT], uint32_t eb) {
+    const uint32_t off = eb * 1024 + lane * 16;
 #pragma unroll
-  for (int j = 0; j < NT; j++)
-    B[j] = reinterpret_cast<const v4i *>(a.B + (((uint64_t)sl * a.UT + cb * NT + min((uint32_t)j, ntv - 1)) * a.EB) * 1024) + lane;
-  // the epilogue's per-lane weight 2^(8(4h + lx)) mod p and its Shoup quotient
-  const uint32_t h = lane >> 5, lx = lane & 7, ul = (lane >> 3) & 3;
-  const uint32_t wexp = 8 * (4 * h + lx);
+    for (int i = 0; i < LPT; i++)
+      if (stage_ok(i)) r[i] = *reinterpret_cast<const v4i *>(src[i] + off);
+  };
+  auto put = [&](int buf, const v4i (&r)[LPT]) {
+#pragma unroll
+    for (int i = 0; i < LPT; i++)
+      if (stage_ok(i)) lds[buf][wave + 4 * i][lane] = r[i];
+  };
+  // the epilogue's per-lane weight 2^(8(4h + (lx & ~1))) mod p and its Shoup quotient
+  const uint32_t wexp = 8 * (4 * (lane >> 5) + (lane & 6));
   const uint64_t pw = wexp < 64 ? (1ull << wexp) : ((uint64_t)delta << (wexp - 61));
   const uint64_t pwq = shoup_quot(pw, p, delta);
-  uint64_t *st = a.stage + item * (MT * NT * 16);
+  uint64_t *st = a.stage + (item * 4 + wave) * (MT * NT * 16);
   for (uint32_t c0 = 0; c0 < a.EBP; c0 += kPassBlocks) {
     const uint32_t c1 = min(a.EBP, c0 + kPassBlocks);
     v16i acc[MT][NT];
@@ -231,54 +305,43 @@ __global__ void __launch_bounds__(64) mx_mac_kernel(MacArgsMx a) {
     for (int i = 0; i < MT; i++)
 #pragma unroll
       for (int j = 0; j < NT; j++) acc[i][j] = v16i{};
-    v4i fa[2][MT], fb[2][NT];
-    auto load = [&](int buf, uint32_t eb) {
+    // two register sets run two blocks ahead of the LDS buffer being read: at the top of
+    // block eb, LDS buffer (eb - c0) & 1 holds eb, one set eb + 1 and the other eb + 2
+    v4i R0[LPT], R1[LPT];
+    load(R0, c0);
+    put(0, R0);
+    if (c0 + 1 < c1) load(R0, c0 + 1);
+    if (c0 + 2 < c1) load(R1, c0 + 2);
+    __syncthreads();
+    for (uint32_t eb = c0; eb < c1; eb += 2) {
 #pragma unroll
-      for (int i = 0; i < MT; i++) fa[buf][i] = A[i][(uint64_t)eb * 64];
+      for (int half = 0; half < 2; half++) {
+        if (half == 0 || eb + 1 < c1) {
+          v4i fa[MT], fb[NT];
 #pragma unroll
-      for (int j = 0; j < NT; j++) fb[buf][j] = B[j][(uint64_t)eb * 64];
-    };
-    auto mma = [&](int buf) {
+          for (int i = 0; i < MT; i++) fa[i] = lds[half][wr * MT + i][lane];
 #pragma unroll
-      for (int i = 0; i < MT; i++)
+          for (int j = 0; j < NT; j++) fb[j] = lds[half][TR + wc * NT + j][lane];
 #pragma unroll
-        for (int j = 0; j < NT; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[buf][i], fb[buf][j], acc[i][j], 0, 0, 0);
-    };
-    load(0, c0);
-    uint32_t eb = c0;
-    for (; eb + 2 <= c1; eb += 2) {
-      load(1, eb + 1);
-      mma(0);
-      if (eb + 2 < c1) load(0, eb + 2);
-      mma(1);
+          for (int i = 0; i < MT; i++)
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+              acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+          if (eb + half + 1 < c1) {
+            put(half ^ 1, half ? R1 : R0);
+            if (eb + half + 3 < c1) load(half ? R1 : R0, eb + half + 3);
+          }
+        }
+        __syncthreads();
+      }
     }
-    if (eb < c1) mma(0);
-    // epilogue: per tile and row group a (query of the pair, half), the lane's column
-    // (unit ul, digit lx) and rows (digits 4h + b) give sum_b S * 2^(8b) times the lane's
-    // weight, reduced mod p; the sum over the 16 lanes of the unit is the output
 #pragma unroll
     for (int i = 0; i < MT; i++)
 #pragma unroll
       for (int j = 0; j < NT; j++) {
-        uint64_t V[4];
-#pragma unroll
-        for (int g = 0; g < 4; g++) {
-          const int64_t D = (int64_t)acc[i][j][4 * g] + (int64_t)acc[i][j][4 * g + 1] * 256 +
-                            (int64_t)acc[i][j][4 * g + 2] * 65536 + (int64_t)acc[i][j][4 * g + 3] * 16777216;
-          uint64_t w = shoup_lazy_sf((uint64_t)(D + (int64_t)p), pw, pwq, delta);
-          w = w >= p ? w - p : w;
-          w += shfl_xor64(w, 1);
-          w += shfl_xor64(w, 2);
-          w += shfl_xor64(w, 4);  // below 8p < 2^64
-          w = (w & kM61) + (w >> 61) * delta;
-          w = w >= p ? w - p : w;
-          w += shfl_xor64(w, 32);
-          V[g] = w >= p ? w - p : w;
-        }
-        if (h == 0 && lx < 4) {
-          uint64_t v = lx == 0 ? V[0] : lx == 1 ? V[1] : lx == 2 ? V[2] : V[3];
-          uint64_t *o = st + ((i * NT + j) * 4 + lx) * 4 + ul;
+        uint64_t v = tile_epilogue(acc[i][j], lane, p, delta, pw, pwq, 4 * p);
+        if (lane < 32 && (lane & 7) < 4) {
+          uint64_t *o = st + ((i * NT + j) * 4 + (lane & 7)) * 4 + ((lane >> 3) & 3);
           if (c0 > 0) {
             v += *o;
             v = v >= p ? v - p : v;
@@ -293,16 +356,18 @@ __global__ void __launch_bounds__(64) mx_mac_kernel(MacArgsMx a) {
 struct UnstageArgs {
   const uint64_t *stage;
   uint64_t *out[kMaxQ];  // (u*2 + s)*m*n + sl
-  uint32_t S, U, qn, MT, RB, CB;
+  uint32_t S, U, qn, WR, MT, RB, CB;
 };
 
-// block (32-slot block sb, column block cb, row block rb): the 32 items' words through LDS,
-// out as rows of 32 consecutive slots
-constexpr uint32_t kUnstageSlots = 32;
+// block (8-slot block sb, column block cb, row block rb): the 8 items' words through LDS,
+// out as rows of 8 consecutive slots
+constexpr uint32_t kUnstageSlots = 8;
+constexpr uint32_t kItemWordsMax = 4 * 4 * kNT * 16;
 
 __global__ void __launch_bounds__(256) mx_unstage_kernel(UnstageArgs a) {
-  __shared__ uint64_t t[kUnstageSlots][4 * kNT * 16 + 1];
-  const uint32_t wpi = a.MT * kNT * 16;
+  __shared__ uint64_t t[kUnstageSlots][kItemWordsMax + 1];
+  const uint32_t per_wave = a.MT * kNT * 16, wpi = 4 * per_wave;
+  const uint32_t WC = 4 / a.WR, TR = a.WR * a.MT, TC = WC * kNT;
   const uint32_t rb = blockIdx.x % a.RB;
   const uint32_t r = blockIdx.x / a.RB;
   const uint32_t cb = r % a.CB, sb = r / a.CB;
@@ -315,9 +380,11 @@ __global__ void __launch_bounds__(256) mx_unstage_kernel(UnstageArgs a) {
   const uint64_t mn = a.S;
   for (uint32_t i = threadIdx.x; i < kUnstageSlots * wpi; i += 256) {
     const uint32_t w = i / kUnstageSlots, s = i % kUnstageSlots;
-    const uint32_t ul = w % 4, g = (w / 4) % 4, j = (w / 16) % kNT, ti = w / (16 * kNT);
-    const uint32_t q = 2 * (rb * a.MT + ti) + (g >> 1), half = g & 1;
-    const uint32_t u = 4 * (cb * kNT + j) + ul;
+    const uint32_t wave = w / per_wave, rem = w % per_wave;
+    const uint32_t ul = rem % 4, g = (rem / 4) % 4, j = (rem / 16) % kNT, ti = rem / (16 * kNT);
+    const uint32_t rt = rb * TR + (wave / WC) * a.MT + ti, ct = cb * TC + (wave % WC) * kNT + j;
+    const uint32_t q = 2 * rt + (g >> 1), half = g & 1;
+    const uint32_t u = 4 * ct + ul;
     if (q < a.qn && u < a.U) a.out[q][((uint64_t)u * 2 + half) * mn + sb * kUnstageSlots + s] = t[s][w];
   }
 }
@@ -364,12 +431,21 @@ __global__ void __launch_bounds__(256) mx_fixup_kernel(FixArgs a) {
 
 uint64_t al256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
 
-int mt_for(uint32_t qn) { return qn == 1 ? 1 : qn <= 4 ? 2 : 4; }
+// workgroup shape by row tiles (query pairs): WR waves along the rows, MT row tiles each
+struct WgShape {
+  int WR, MT;
+  uint32_t TR, TC;
+};
+WgShape shape_for(uint32_t qn) {
+  const uint32_t RT = (qn + 1) / 2;
+  const int WR = RT > 4 ? 2 : 1, MT = RT > 2 ? 4 : (int)RT;
+  return WgShape{WR, MT, (uint32_t)(WR * MT), (uint32_t)((4 / WR) * kNT)};
+}
 
 }  // namespace
 
 bool mx_geometry(const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, MxGeom *g) {
-  if (prm->n % 64 != 0 || n_l < kTailMin + 32 || n_l > (1ull << 31)) return false;
+  if (prm->n % 64 != 0 || n_l < kTailMin + 32 || n_l >= (1ull << 27)) return false;  // 32-bit fragment offsets
   g->S = prm->m * prm->n;
   g->U = (uint32_t)(groups * P);
   g->UT = (g->U + 3) / 4;
@@ -417,11 +493,11 @@ int launch_mx_pack_db(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, cons
 
 // workspace of one launch of qn queries: query fragments, staged residues, query flags
 static void mx_work_parts(const MxGeom &g, uint32_t qn, uint64_t *a_bytes, uint64_t *st_bytes) {
-  const int MT = mt_for(qn);
-  const uint32_t RT = (qn + 1) / 2, RB = (RT + MT - 1) / MT, RTp = RB * MT;
-  const uint32_t CB = (g.UT + kNT - 1) / kNT;
+  const WgShape w = shape_for(qn);
+  const uint32_t RT = (qn + 1) / 2, RB = (RT + w.TR - 1) / w.TR, RTp = RB * w.TR;
+  const uint32_t CB = (g.UT + w.TC - 1) / w.TC;
   *a_bytes = (uint64_t)g.S * RTp * g.EBP * 1024;
-  *st_bytes = (uint64_t)g.S * CB * RB * MT * kNT * 16 * 8;
+  *st_bytes = (uint64_t)g.S * CB * RB * 4 * w.MT * kNT * 16 * 8;
 }
 
 uint64_t mx_work_bytes(const MxGeom &g, uint32_t qn) {
@@ -433,9 +509,9 @@ uint64_t mx_work_bytes(const MxGeom &g, uint32_t qn) {
 int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const uint8_t *frag, const uint64_t *tail,
                     const uint64_t *const *q, uint64_t *const *out, uint32_t qn, void *work) {
   if (qn == 0 || qn > (uint32_t)kMaxQ) return set_err(ctx, XPG_EINVAL, "batch of 1..64 queries per launch");
-  const int MT = mt_for(qn);
-  const uint32_t RT = (qn + 1) / 2, RB = (RT + MT - 1) / MT, RTp = RB * MT;
-  const uint32_t CB = (g.UT + kNT - 1) / kNT;
+  const WgShape w = shape_for(qn);
+  const uint32_t RT = (qn + 1) / 2, RB = (RT + w.TR - 1) / w.TR, RTp = RB * w.TR;
+  const uint32_t CB = (g.UT + w.TC - 1) / w.TC;
   uint64_t ab = 0, sb = 0;
   mx_work_parts(g, qn, &ab, &sb);
   uint8_t *A = static_cast<uint8_t *>(work);
@@ -482,12 +558,13 @@ int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const 
     if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "matrix MAC grid too large");
     hipEvent_t k0 = nullptr, k1 = nullptr;
     timing_begin(ctx, &k0, &k1, 2);
-    if (ctx->timing && !ctx->ws) ctx->batch_steps += (uint64_t)qn * g.U * g.S * 2 * g.n_l;
-    switch (MT) {
-      case 4: hipLaunchKernelGGL(mx_mac_kernel<4>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
-      case 2: hipLaunchKernelGGL(mx_mac_kernel<2>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
-      default: hipLaunchKernelGGL(mx_mac_kernel<1>, dim3((uint32_t)blocks), dim3(64), 0, ctx->stream, a); break;
-    }
+    // the timed kernel's chain steps: the prefixes (the tails run after it)
+    if (ctx->timing && !ctx->ws) ctx->batch_steps += (uint64_t)qn * g.U * g.S * 2 * g.E0;
+    const dim3 grid((uint32_t)blocks), wg(256);
+    if (w.WR == 2) hipLaunchKernelGGL((mx_mac_kernel<2, 4>), grid, wg, 0, ctx->stream, a);
+    else if (w.MT == 4) hipLaunchKernelGGL((mx_mac_kernel<1, 4>), grid, wg, 0, ctx->stream, a);
+    else if (w.MT == 2) hipLaunchKernelGGL((mx_mac_kernel<1, 2>), grid, wg, 0, ctx->stream, a);
+    else hipLaunchKernelGGL((mx_mac_kernel<1, 1>), grid, wg, 0, ctx->stream, a);
     timing_end(ctx, k0, k1, 2);
   }
   {
@@ -497,7 +574,8 @@ int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const 
     a.S = g.S;
     a.U = g.U;
     a.qn = qn;
-    a.MT = (uint32_t)MT;
+    a.WR = (uint32_t)w.WR;
+    a.MT = (uint32_t)w.MT;
     a.RB = RB;
     a.CB = CB;
     const uint64_t blocks = (uint64_t)(g.S / kUnstageSlots) * CB * RB;
