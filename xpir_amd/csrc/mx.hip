@@ -220,6 +220,12 @@ __device__ __forceinline__ uint64_t mad_i64(int32_t a, int32_t b, uint64_t c) {
 
 __device__ __forceinline__ uint64_t canon(uint64_t v, uint64_t p) { return v >= p ? v - p : v; }
 
+// a wave-uniform 64-bit value in scalar registers
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Epilogue of one tile: lane L holds S for column (unit ul = (L>>3)&3, digit lx = L&7) and
 // rows 8g + 4h + b (g: query of the pair and half; digits 4h + b), h = L>>5, in acc[4g+b].
 // Returns, in the lanes with h = 0 and lx < 4, the output of row group g = lx for unit ul:
@@ -240,17 +246,19 @@ __device__ __forceinline__ uint64_t tile_epilogue(const v16i &acc, uint32_t lane
     D[g] = mad_i64(acc[4 * g + 3], 1 << 24,
                    mad_i64(acc[4 * g + 2], 1 << 16, mad_i64(acc[4 * g + 1], 1 << 8, (uint64_t)(int64_t)acc[4 * g])));
   const bool odd = lane & 1, hi2 = (lane >> 1) & 1;
+  const uint32_t sk = odd ? 8 : 0, sr = 8 - sk;  // P = (even lane's D) + 2^8 (odd lane's D)
   // xor 1: send the groups the partner keeps
   const uint64_t r0 = dpp64<kQuadXor1>(odd ? D[0] : D[1]);
   const uint64_t r1 = dpp64<kQuadXor1>(odd ? D[2] : D[3]);
-  const uint64_t k0 = odd ? D[1] : D[0], k1 = odd ? D[3] : D[2];
-  const uint64_t P0 = (odd ? r0 : k0) + ((odd ? k0 : r0) << 8);  // groups odd, 2 + odd
-  const uint64_t P1 = (odd ? r1 : k1) + ((odd ? k1 : r1) << 8);
-  const uint64_t w0 = canon(shoup_lazy_asm(P0 + p4, pw, pwq, delta), p);
-  const uint64_t w1 = canon(shoup_lazy_asm(P1 + p4, pw, pwq, delta), p);
-  // xor 2: keep group odd (bit 1 clear) or 2 + odd (bit 1 set)
-  uint64_t v = canon((hi2 ? w1 : w0) + dpp64<kQuadXor2>(hi2 ? w0 : w1), p);
-  v = canon(v + dpp64<kHalfMirror>(dpp64<kQuadRev>(v)), p);
+  const uint64_t P0 = ((odd ? D[1] : D[0]) << sk) + (r0 << sr);  // group odd
+  const uint64_t P1 = ((odd ? D[3] : D[2]) << sk) + (r1 << sr);  // group 2 + odd
+  // weighted, in [0, 2p)
+  const uint64_t w0 = shoup_lazy_sf(P0 + p4, pw, pwq, delta);
+  const uint64_t w1 = shoup_lazy_sf(P1 + p4, pw, pwq, delta);
+  // xor 2 (keep group odd when bit 1 is clear, else 2 + odd) and xor 4: below 8p < 2^64
+  uint64_t v = (hi2 ? w1 : w0) + dpp64<kQuadXor2>(hi2 ? w0 : w1);
+  v += dpp64<kHalfMirror>(dpp64<kQuadRev>(v));
+  v = canon((v & kM61) + (v >> 61) * delta, p);
   return canon(v + shfl_xor64(v, 32), p);
 }
 
@@ -278,10 +286,13 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
 #pragma unroll
   for (int i = 0; i < LPT; i++) {
     const uint32_t f = min((uint32_t)(wave + 4 * i), (uint32_t)NF - 1);
-    src[i] = f < (uint32_t)TR ? a.A + ((uint64_t)sl * a.RTp + rb * TR + f) * a.EBP * 1024
-                              : a.B + ((uint64_t)sl * a.UT + min(cb * TC + f - TR, a.UT - 1)) * a.EB * 1024;
+    const uint64_t off = f < (uint32_t)TR ? ((uint64_t)sl * a.RTp + rb * TR + f) * a.EBP * 1024
+                                          : ((uint64_t)sl * a.UT + min(cb * TC + f - TR, a.UT - 1)) * a.EB * 1024;
+    src[i] = (f < (uint32_t)TR ? a.A : a.B) + uniform64(off);
   }
-  auto stage_ok = [&](int i) { return wave + 4 * i < (uint32_t)NF; };
+  // every wave stages LPT fragments when four divide NF; otherwise the last round's
+  // fragments past NF are skipped (a wave-uniform test)
+  auto stage_ok = [&](int i) { return NF % 4 == 0 || i + 1 < LPT || wave + 4 * i < (uint32_t)NF; };
   auto load = [&](v4i (&r)[LPT], uint32_t eb) {
     const uint32_t off = eb * 1024 + lane * 16;
 #pragma unroll
@@ -306,12 +317,15 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
 #pragma unroll
       for (int j = 0; j < NT; j++) acc[i][j] = v16i{};
     // two register sets run two blocks ahead of the LDS buffer being read: at the top of
-    // block eb, LDS buffer (eb - c0) & 1 holds eb, one set eb + 1 and the other eb + 2
+    // block eb, LDS buffer (eb - c0) & 1 holds eb, one set eb + 1 and the other eb + 2.
+    // Block indices past the pass are clamped to its last block (loads without branches,
+    // so each wait names only the older set); those copies are never read.
+    const uint32_t last = c1 - 1;
     v4i R0[LPT], R1[LPT];
     load(R0, c0);
     put(0, R0);
-    if (c0 + 1 < c1) load(R0, c0 + 1);
-    if (c0 + 2 < c1) load(R1, c0 + 2);
+    load(R0, min(c0 + 1, last));
+    load(R1, min(c0 + 2, last));
     __syncthreads();
     for (uint32_t eb = c0; eb < c1; eb += 2) {
 #pragma unroll
@@ -327,11 +341,9 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
 #pragma unroll
             for (int j = 0; j < NT; j++)
               acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
-          if (eb + half + 1 < c1) {
-            put(half ^ 1, half ? R1 : R0);
-            if (eb + half + 3 < c1) load(half ? R1 : R0, eb + half + 3);
-          }
         }
+        put(half ^ 1, half ? R1 : R0);
+        load(half ? R1 : R0, min(eb + half + 3, last));
         __syncthreads();
       }
     }
