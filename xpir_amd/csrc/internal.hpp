@@ -46,8 +46,8 @@ struct EncSrc {
 };
 
 // Level 0 in the matrix layout (mx.hip): S = m*n slots, U = groups*P units in UT tiles of
-// 4, n_l elements in EB blocks of 32, of which the first E0 (EBP blocks) run on the matrix
-// cores and the last L = n_l - E0 (40..71) with the reference's step.
+// 4, n_l elements in EB blocks of 32, of which the first E0 = n_l - 40 (in EBP blocks) run
+// on the matrix cores and the last L = 40 with the reference's step.
 struct MxGeom {
   uint32_t S = 0, U = 0, UT = 0, EB = 0, EBP = 0, L = 0, P = 0;
   uint64_t n_l = 0, groups = 0, E0 = 0;
@@ -272,8 +272,8 @@ int launch_mac_tail(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *tail_db
                     uint64_t *const *out, uint32_t qn, uint64_t L, uint64_t groups, uint32_t P);
 
 // ---- matrix-core level 0 (mx.hip) ----
-// Whether level 0 of (n_l, groups, P) can take the matrix layout (n a multiple of 64 and
-// n_l >= 72), and its geometry.
+// Whether level 0 of (n_l, groups, P) can take the matrix layout (n a multiple of 64, m*n
+// of 256, 72 <= n_l < 2^27), and its geometry.
 bool mx_geometry(const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, MxGeom *g);
 uint64_t mx_frag_bytes(const MxGeom &g);
 uint64_t mx_tail_words(const MxGeom &g);

@@ -11,8 +11,9 @@
 // the residue of the sum of x*y over the elements so far, and j evolves by
 // j' = max(j + b_e + c_e - 1, 0) with b_e the residue wrap [rho + u_e >= p] (the map the
 // streaming MAC in mac.hip composes).  j stays at most 8 (acc < 2^64 < 9p).  So:
-//   1. The residue of the first E0 elements (whole blocks of 32, leaving a tail of 40..71)
-//      is an exact integer dot product reduced mod p: the matrix cores compute it.
+//   1. The residue of the first E0 = n_l - 40 elements is an exact integer dot product
+//      reduced mod p: the matrix cores compute it (over whole blocks of 32 elements, the
+//      query digits past E0 zero).
 //   2. The tail's steps run with the reference's step from acc = that residue (j = 0),
 //      counting the steps that do not subtract p (`kept`).  A start at j0 ends at
 //      max(j0 + D, M), a start at 0 at max(D, M) = j with D = j - kept, so the two agree
@@ -50,6 +51,7 @@ constexpr uint64_t kM61 = (1ull << 61) - 1;
 constexpr uint32_t kTailMin = 40;                      // fewest tail steps
 constexpr uint32_t kPassBlocks = 1024;                 // element blocks per pass (2^15 elements)
 constexpr int kNT = 2;                                 // column tiles per wave
+constexpr int kStageBlocks = 2;                        // element blocks per LDS stage
 
 __device__ __forceinline__ uint64_t digits_of(uint64_t x) { return (x + kDigitBias) ^ kDigitBias; }
 
@@ -140,13 +142,14 @@ struct PackQArgs {
   uint8_t *frag;
   uint32_t *qbad;            // [kMaxQ]: a word of the query is not below its modulus
   ModSet ms;
-  uint64_t n_l, total;
+  uint64_t n_l, E0, total;
   uint32_t S, n, qn, RTp, EB, EBP;
 };
 
 // thread (sl fastest, h, half s, query row qq < 2*RTp, element block eb < EB): digits of its
 // 16 words into lanes (qq % 2) * 16 + s * 8 + j + 32 h of fragment (sl, qq / 2, eb) for the
-// prefix blocks eb < EBP; every block's words are checked against the modulus
+// prefix blocks eb < EBP (elements from E0 on as zero digits); every element's word is
+// checked against the modulus
 __global__ void __launch_bounds__(256) mx_pack_q_kernel(PackQArgs a) {
   const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= a.total) return;
@@ -166,7 +169,7 @@ __global__ void __launch_bounds__(256) mx_pack_q_kernel(PackQArgs a) {
     uint64_t y = 0;
     if (qq < a.qn && e < a.n_l) y = __builtin_nontemporal_load(a.q[qq] + (e * 2 + s) * mn + sl);
     bad |= y >= p;
-    w[k] = digits_of(y);
+    w[k] = e < a.E0 ? digits_of(y) : 0;
   }
   if (bad) a.qbad[qq] = 1;
   if (eb >= a.EBP) return;
@@ -213,12 +216,16 @@ __device__ __forceinline__ uint64_t dpp64(uint64_t v) {
 
 // a * b + c (signed 32 x 32 + 64 bits), one v_mad_i64_i32
 __device__ __forceinline__ uint64_t mad_i64(int32_t a, int32_t b, uint64_t c) {
-  uint64_t d;
-  asm("v_mad_i64_i32 %0, vcc, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c) : "vcc");
+  uint64_t d, carry;  // the carry-out goes to a scalar pair of its own (never read)
+  asm("v_mad_i64_i32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(carry) : "v"(a), "s"(b), "v"(c));
   return d;
 }
 
-__device__ __forceinline__ uint64_t canon(uint64_t v, uint64_t p) { return v >= p ? v - p : v; }
+// v mod p for v < 2p, with negp = 2^64 - p in scalar registers (an add, a compare, two selects)
+__device__ __forceinline__ uint64_t canon(uint64_t v, uint64_t p, uint64_t negp) {
+  const uint64_t t = v + negp;
+  return v >= p ? t : v;
+}
 
 // a wave-uniform 64-bit value in scalar registers
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
@@ -238,13 +245,13 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 //     holds group lx & 3); xor 4 and xor 32 add the partners' partial sums mod p.
 // xor 1, 2 and 4 are DPP moves (xor 4 = quad reversal, then half-row mirror); xor 32 a
 // permute through LDS.
-__device__ __forceinline__ uint64_t tile_epilogue(const v16i &acc, uint32_t lane, uint64_t p, uint32_t delta,
-                                                  uint64_t pw, uint64_t pwq, uint64_t p4) {
+__device__ __forceinline__ uint64_t tile_epilogue(const v16i &acc, uint32_t lane, uint64_t p, uint64_t negp,
+                                                  uint32_t delta, uint64_t pw, uint64_t pwq, uint64_t p4) {
   uint64_t D[4];
 #pragma unroll
   for (int g = 0; g < 4; g++)
     D[g] = mad_i64(acc[4 * g + 3], 1 << 24,
-                   mad_i64(acc[4 * g + 2], 1 << 16, mad_i64(acc[4 * g + 1], 1 << 8, (uint64_t)(int64_t)acc[4 * g])));
+                   mad_i64(acc[4 * g + 2], 1 << 16, mad_i64(acc[4 * g + 1], 1 << 8, mad_i64(acc[4 * g], 1, 0))));
   const bool odd = lane & 1, hi2 = (lane >> 1) & 1;
   const uint32_t sk = odd ? 8 : 0, sr = 8 - sk;  // P = (even lane's D) + 2^8 (odd lane's D)
   // xor 1: send the groups the partner keeps
@@ -253,19 +260,20 @@ __device__ __forceinline__ uint64_t tile_epilogue(const v16i &acc, uint32_t lane
   const uint64_t P0 = ((odd ? D[1] : D[0]) << sk) + (r0 << sr);  // group odd
   const uint64_t P1 = ((odd ? D[3] : D[2]) << sk) + (r1 << sr);  // group 2 + odd
   // weighted, in [0, 2p)
-  const uint64_t w0 = shoup_lazy_sf(P0 + p4, pw, pwq, delta);
-  const uint64_t w1 = shoup_lazy_sf(P1 + p4, pw, pwq, delta);
+  const uint64_t w0 = shoup_lazy_asm(P0 + p4, pw, pwq, delta);
+  const uint64_t w1 = shoup_lazy_asm(P1 + p4, pw, pwq, delta);
   // xor 2 (keep group odd when bit 1 is clear, else 2 + odd) and xor 4: below 8p < 2^64
   uint64_t v = (hi2 ? w1 : w0) + dpp64<kQuadXor2>(hi2 ? w0 : w1);
   v += dpp64<kHalfMirror>(dpp64<kQuadRev>(v));
-  v = canon((v & kM61) + (v >> 61) * delta, p);
-  return canon(v + shfl_xor64(v, 32), p);
+  v = canon((v & kM61) + (v >> 61) * delta, p, negp);
+  return canon(v + shfl_xor64(v, 32), p, negp);
 }
 
 template <int WR, int MT>
 __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
   constexpr int WC = 4 / WR, NT = kNT, TR = WR * MT, TC = WC * NT, NF = TR + TC, LPT = (NF + 3) / 4;
-  __shared__ v4i lds[2][NF][64];
+  constexpr int KB = kStageBlocks;
+  __shared__ v4i lds[2][KB][NF][64];
   const uint32_t bid = blockIdx.x;
   const uint64_t item = (uint64_t)(bid & 7) * a.per_xcd + (bid >> 3);
   if (item >= a.items) return;  // the whole workgroup
@@ -293,16 +301,22 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
   // every wave stages LPT fragments when four divide NF; otherwise the last round's
   // fragments past NF are skipped (a wave-uniform test)
   auto stage_ok = [&](int i) { return NF % 4 == 0 || i + 1 < LPT || wave + 4 * i < (uint32_t)NF; };
-  auto load = [&](v4i (&r)[LPT], uint32_t eb) {
-    const uint32_t off = eb * 1024 + lane * 16;
+  // a stage is KB consecutive blocks (indices past `last` read block `last` again)
+  auto load = [&](v4i (&r)[KB][LPT], uint32_t eb, uint32_t last) {
 #pragma unroll
-    for (int i = 0; i < LPT; i++)
-      if (stage_ok(i)) r[i] = *reinterpret_cast<const v4i *>(src[i] + off);
+    for (int kb = 0; kb < KB; kb++) {
+      const uint32_t off = min(eb + kb, last) * 1024 + lane * 16;
+#pragma unroll
+      for (int i = 0; i < LPT; i++)
+        if (stage_ok(i)) r[kb][i] = *reinterpret_cast<const v4i *>(src[i] + off);
+    }
   };
-  auto put = [&](int buf, const v4i (&r)[LPT]) {
+  auto put = [&](int buf, const v4i (&r)[KB][LPT]) {
 #pragma unroll
-    for (int i = 0; i < LPT; i++)
-      if (stage_ok(i)) lds[buf][wave + 4 * i][lane] = r[i];
+    for (int kb = 0; kb < KB; kb++)
+#pragma unroll
+      for (int i = 0; i < LPT; i++)
+        if (stage_ok(i)) lds[buf][kb][wave + 4 * i][lane] = r[kb][i];
   };
   // the epilogue's per-lane weight 2^(8(4h + (lx & ~1))) mod p and its Shoup quotient
   const uint32_t wexp = 8 * (4 * (lane >> 5) + (lane & 6));
@@ -316,34 +330,47 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
     for (int i = 0; i < MT; i++)
 #pragma unroll
       for (int j = 0; j < NT; j++) acc[i][j] = v16i{};
-    // two register sets run two blocks ahead of the LDS buffer being read: at the top of
-    // block eb, LDS buffer (eb - c0) & 1 holds eb, one set eb + 1 and the other eb + 2.
-    // Block indices past the pass are clamped to its last block (loads without branches,
-    // so each wait names only the older set); those copies are never read.
+    // two register sets run two stages ahead of the LDS buffer being read: at the top of
+    // stage eb (blocks eb .. eb + KB - 1), LDS buffer ((eb - c0) / KB) & 1 holds it, one set
+    // the next stage and the other the one after (in flight).  Block indices past the pass
+    // are clamped to its last block (loads without branches, so each wait names only the
+    // older set); such copies are never multiplied.
     const uint32_t last = c1 - 1;
-    v4i R0[LPT], R1[LPT];
-    load(R0, c0);
+    v4i R0[KB][LPT], R1[KB][LPT];
+    load(R0, c0, last);
     put(0, R0);
-    load(R0, min(c0 + 1, last));
-    load(R1, min(c0 + 2, last));
+    load(R0, c0 + KB, last);
+    load(R1, c0 + 2 * KB, last);
     __syncthreads();
-    for (uint32_t eb = c0; eb < c1; eb += 2) {
+    for (uint32_t eb = c0; eb < c1; eb += 2 * KB) {
 #pragma unroll
       for (int half = 0; half < 2; half++) {
-        if (half == 0 || eb + 1 < c1) {
-          v4i fa[MT], fb[NT];
+        const uint32_t e0 = eb + half * KB;
+        // the stage's blocks from LDS, each block's fragments read while the previous
+        // block's products run
+        v4i fa[2][MT], fb[2][NT];
 #pragma unroll
-          for (int i = 0; i < MT; i++) fa[i] = lds[half][wr * MT + i][lane];
+        for (int i = 0; i < MT; i++) fa[0][i] = lds[half][0][wr * MT + i][lane];
 #pragma unroll
-          for (int j = 0; j < NT; j++) fb[j] = lds[half][TR + wc * NT + j][lane];
+        for (int j = 0; j < NT; j++) fb[0][j] = lds[half][0][TR + wc * NT + j][lane];
 #pragma unroll
-          for (int i = 0; i < MT; i++)
+        for (int kb = 0; kb < KB; kb++) {
+          if (kb + 1 < KB) {
 #pragma unroll
-            for (int j = 0; j < NT; j++)
-              acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < MT; i++) fa[(kb + 1) & 1][i] = lds[half][kb + 1][wr * MT + i][lane];
+#pragma unroll
+            for (int j = 0; j < NT; j++) fb[(kb + 1) & 1][j] = lds[half][kb + 1][TR + wc * NT + j][lane];
+          }
+          if (e0 + kb < c1) {
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+#pragma unroll
+              for (int j = 0; j < NT; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[kb & 1][i], fb[kb & 1][j], acc[i][j], 0, 0, 0);
+          }
         }
         put(half ^ 1, half ? R1 : R0);
-        load(half ? R1 : R0, min(eb + half + 3, last));
+        load(half ? R1 : R0, e0 + 3 * KB, last);
         __syncthreads();
       }
     }
@@ -351,13 +378,10 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
     for (int i = 0; i < MT; i++)
 #pragma unroll
       for (int j = 0; j < NT; j++) {
-        uint64_t v = tile_epilogue(acc[i][j], lane, p, delta, pw, pwq, 4 * p);
+        uint64_t v = tile_epilogue(acc[i][j], lane, p, 0 - p, delta, pw, pwq, 4 * p);
         if (lane < 32 && (lane & 7) < 4) {
           uint64_t *o = st + ((i * NT + j) * 4 + (lane & 7)) * 4 + ((lane >> 3) & 3);
-          if (c0 > 0) {
-            v += *o;
-            v = v >= p ? v - p : v;
-          }
+          if (c0 > 0) v = canon(v + *o, p, 0 - p);
           *o = v;
         }
       }
@@ -368,33 +392,33 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
 struct UnstageArgs {
   const uint64_t *stage;
   uint64_t *out[kMaxQ];  // (u*2 + s)*m*n + sl
-  uint32_t S, U, qn, WR, MT, RB, CB;
+  uint32_t S, U, qn, RB, CB;
 };
 
 // block (8-slot block sb, column block cb, row block rb): the 8 items' words through LDS,
-// out as rows of 8 consecutive slots
+// out as rows of 8 consecutive slots; the item layout (WR x WC waves of MT x NT tiles of
+// 4 x 4 words) is a template so that its decoding is shifts and masks
 constexpr uint32_t kUnstageSlots = 8;
-constexpr uint32_t kItemWordsMax = 4 * 4 * kNT * 16;
 
+template <int WR, int MT>
 __global__ void __launch_bounds__(256) mx_unstage_kernel(UnstageArgs a) {
-  __shared__ uint64_t t[kUnstageSlots][kItemWordsMax + 1];
-  const uint32_t per_wave = a.MT * kNT * 16, wpi = 4 * per_wave;
-  const uint32_t WC = 4 / a.WR, TR = a.WR * a.MT, TC = WC * kNT;
+  constexpr uint32_t WC = 4 / WR, TR = WR * MT, TC = WC * kNT, PER_WAVE = MT * kNT * 16, WPI = 4 * PER_WAVE;
+  __shared__ uint64_t t[kUnstageSlots][WPI + 1];
   const uint32_t rb = blockIdx.x % a.RB;
   const uint32_t r = blockIdx.x / a.RB;
   const uint32_t cb = r % a.CB, sb = r / a.CB;
-  for (uint32_t i = threadIdx.x; i < kUnstageSlots * wpi; i += 256) {
-    const uint32_t s = i / wpi, w = i % wpi;
+  for (uint32_t i = threadIdx.x; i < kUnstageSlots * WPI; i += 256) {
+    const uint32_t s = i / WPI, w = i % WPI;
     const uint64_t item = ((uint64_t)(sb * kUnstageSlots + s) * a.CB + cb) * a.RB + rb;
-    t[s][w] = a.stage[item * wpi + w];
+    t[s][w] = a.stage[item * WPI + w];
   }
   __syncthreads();
   const uint64_t mn = a.S;
-  for (uint32_t i = threadIdx.x; i < kUnstageSlots * wpi; i += 256) {
+  for (uint32_t i = threadIdx.x; i < kUnstageSlots * WPI; i += 256) {
     const uint32_t w = i / kUnstageSlots, s = i % kUnstageSlots;
-    const uint32_t wave = w / per_wave, rem = w % per_wave;
+    const uint32_t wave = w / PER_WAVE, rem = w % PER_WAVE;
     const uint32_t ul = rem % 4, g = (rem / 4) % 4, j = (rem / 16) % kNT, ti = rem / (16 * kNT);
-    const uint32_t rt = rb * TR + (wave / WC) * a.MT + ti, ct = cb * TC + (wave % WC) * kNT + j;
+    const uint32_t rt = rb * TR + (wave / WC) * MT + ti, ct = cb * TC + (wave % WC) * kNT + j;
     const uint32_t q = 2 * rt + (g >> 1), half = g & 1;
     const uint32_t u = 4 * ct + ul;
     if (q < a.qn && u < a.U) a.out[q][((uint64_t)u * 2 + half) * mn + sb * kUnstageSlots + s] = t[s][w];
@@ -412,14 +436,14 @@ struct FixArgs {
   uint32_t S, n, U, UT, EB, P;
 };
 
+// block: 256 consecutive slots of one output row (query q, unit u, half s); the row
+// index is decoded once per block
 __global__ void __launch_bounds__(256) mx_fixup_kernel(FixArgs a) {
-  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= a.total) return;
-  const uint32_t sl = (uint32_t)(t % a.S);
-  uint64_t r = t / a.S;
-  const uint32_t s = (uint32_t)(r & 1);
-  r >>= 1;
-  const uint32_t u = (uint32_t)(r % a.U), q = (uint32_t)(r / a.U);
+  const uint32_t blocks_per_row = a.S / 256;
+  const uint64_t row = blockIdx.x / blocks_per_row;
+  const uint32_t sl = (uint32_t)(blockIdx.x % blocks_per_row) * 256 + threadIdx.x;
+  const uint32_t s = (uint32_t)(row & 1);
+  const uint32_t u = (uint32_t)((row >> 1) % a.U), q = (uint32_t)((row >> 1) / a.U);
   const uint64_t mn = a.S;
   uint64_t *o = a.out[q] + ((uint64_t)u * 2 + s) * mn + sl;
   if (!(*o >> 63) && !a.qbad[q]) return;
@@ -457,14 +481,15 @@ WgShape shape_for(uint32_t qn) {
 }  // namespace
 
 bool mx_geometry(const xpg_params *prm, uint64_t n_l, uint64_t groups, uint32_t P, MxGeom *g) {
-  if (prm->n % 64 != 0 || n_l < kTailMin + 32 || n_l >= (1ull << 27)) return false;  // 32-bit fragment offsets
+  // slots in whole 256-lane blocks; 32-bit fragment offsets
+  if (prm->n % 64 != 0 || (prm->m * prm->n) % 256 != 0 || n_l < kTailMin + 32 || n_l >= (1ull << 27)) return false;
   g->S = prm->m * prm->n;
   g->U = (uint32_t)(groups * P);
   g->UT = (g->U + 3) / 4;
   g->EB = (uint32_t)((n_l + 31) / 32);
-  g->E0 = (n_l - kTailMin) / 32 * 32;
-  g->EBP = (uint32_t)(g->E0 / 32);
-  g->L = (uint32_t)(n_l - g->E0);
+  g->E0 = n_l - kTailMin;
+  g->EBP = (uint32_t)((g->E0 + 31) / 32);
+  g->L = kTailMin;
   g->n_l = n_l;
   g->groups = groups;
   g->P = P;
@@ -540,6 +565,7 @@ int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const 
     a.qbad = qbad;
     a.ms = ms;
     a.n_l = g.n_l;
+    a.E0 = g.E0;
     a.S = g.S;
     a.n = prm->n;
     a.qn = qn;
@@ -586,12 +612,13 @@ int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const 
     a.S = g.S;
     a.U = g.U;
     a.qn = qn;
-    a.WR = (uint32_t)w.WR;
-    a.MT = (uint32_t)w.MT;
     a.RB = RB;
     a.CB = CB;
-    const uint64_t blocks = (uint64_t)(g.S / kUnstageSlots) * CB * RB;
-    hipLaunchKernelGGL(mx_unstage_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
+    const dim3 grid((uint32_t)((uint64_t)(g.S / kUnstageSlots) * CB * RB)), wg(256);
+    if (w.WR == 2) hipLaunchKernelGGL((mx_unstage_kernel<2, 4>), grid, wg, 0, ctx->stream, a);
+    else if (w.MT == 4) hipLaunchKernelGGL((mx_unstage_kernel<1, 4>), grid, wg, 0, ctx->stream, a);
+    else if (w.MT == 2) hipLaunchKernelGGL((mx_unstage_kernel<1, 2>), grid, wg, 0, ctx->stream, a);
+    else hipLaunchKernelGGL((mx_unstage_kernel<1, 1>), grid, wg, 0, ctx->stream, a);
   }
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "matrix MAC launch");
   {
