@@ -39,10 +39,18 @@ M32, M64 = (1 << 32) - 1, (1 << 64) - 1
 
 def _asm_lines(fn):
     src = open(ARITH).read()
-    i = src.index(f"{fn}(uint64_t")
-    j = src.index("asm(", i)
+    i = src.index(f"{fn}(uint64_t") + len(fn)
+    j = re.search(r"(?<!\w)asm\(", src[i:]).start() + i  # the asm statement, not the name
     k = re.search(r"\n\s*:", src[j:]).start() + j  # the first operand list
-    text = "".join(re.findall(r'"((?:[^"\\]|\\.)*)"', src[j:k]))
+    text = ""
+    m = re.match(r"asm\((\w+)", src[j:])
+    if m:  # the template starts with a macro: its string literals first
+        d = src.index(f"#define {m.group(1)}")
+        e = src.index("\n", d)
+        while src[e - 1] == "\\":
+            e = src.index("\n", e + 1)
+        text = "".join(re.findall(r'"((?:[^"\\]|\\.)*)"', src[d:e]))
+    text += "".join(re.findall(r'"((?:[^"\\]|\\.)*)"', src[j:k]))
     return [s.strip() for s in text.split("\\n\\t") if s.strip()]
 
 
@@ -105,6 +113,11 @@ class _Lane:
                 self._write(a[0], self._read(a[2], 32) if self._read(a[3], 1) else self._read(a[1], 32), 32)
             elif op == "v_add_u32":
                 self._write(a[0], self._read(a[1], 32) + self._read(a[2], 32), 32)
+            elif op == "v_addc_co_u32":  # VOP2: dst = src0 + src1 + vcc, carry-out to vcc
+                assert a[1] == "vcc" and a[4] == "vcc"
+                res = self._read(a[2], 32) + self._read(a[3], 32) + self._read("vcc", 1)
+                self._write(a[0], res, 32)
+                self._set_vcc(res >> 32)
             elif op == "v_add3_u32":
                 self._write(a[0], self._read(a[1], 32) + self._read(a[2], 32) + self._read(a[3], 32), 32)
             elif op == "v_lshl_add_u64":
@@ -145,8 +158,11 @@ def test_asm_sequences_compute_the_textbook_values():
     L = Oracle.lib()
     rng = random.Random(20261204)
     mac, lazy, add, sub = (_asm_lines(f) for f in ("mac_step_asm", "shoup_lazy_asm", "add_mod2p_asm", "sub_mod2p_asm"))
-    # 17 and 12 VALU instructions and one s_nop each; with T's v_mul_hi_u32 from C, 18 and 13
+    mac_count = _asm_lines("mac_step_count_asm")
+    # 17 and 12 VALU instructions and one s_nop each; with T's v_mul_hi_u32 from C, 18 and 13;
+    # the counting step adds one add-with-carry
     assert sum(ln.startswith("v_") for ln in mac) == 17 and sum(ln.startswith("v_") for ln in lazy) == 12
+    assert mac_count[:-1] == mac and mac_count[-1].startswith("v_addc_co_u32")
     for i in range(17):
         p = L.xo_modulus(i)
         d = (1 << 61) - p
@@ -163,8 +179,13 @@ def test_asm_sequences_compute_the_textbook_values():
                        plo=[p & M32, 32], phi=[p >> 32, 32])
             got = _Lane(ops, rng).run(mac).ops["acc"][0]
             want = (acc + _lazy(x, y, yq, p)) & M64
+            subtracted = want >= p
             want = want - p if want >= p else want
             assert got == want, (i, x, y, acc)
+            # the counting form: the same word, and the count grows exactly when p was subtracted
+            ns = rng.randrange(1000)
+            lane = _Lane(dict(ops, acc=[acc, 64], ns=[ns, 32]), rng).run(mac_count)
+            assert lane.ops["acc"][0] == want and lane.ops["ns"][0] == ns + subtracted, (i, x, y, acc)
             # lazy product
             ops = dict(r=[0, 64], xl=[xl, 32], xh=[xh, 32], wl=[yl, 32], wh=[yh, 32], ql=[ql, 32], qh=[qh, 32],
                        T=[T, 64], d=[d, 32], k29=[0xE0000000, 32])

@@ -293,14 +293,14 @@ struct TileLevel {
 // The tile (colgroup cg = (row block, column), query block qb) of one level: every wave
 // of the workgroup runs it; qbase[i] / obase[i] are the query and output of the block's
 // query i (the last query repeated past qn: its chains are computed and dropped).
-// The word a tail chain leaves: acc = rho + j*p after steps of which `kept` did not
-// subtract p.  Started from j = 0, its j is max(D, M) where the true start j0 <= 8 ends at
-// max(j0 + D, M), D = j - kept; both agree when kept >= j + 8.  Otherwise bit 63 (never set
-// in a chain word, which stays below 2^62) asks for the whole chain.
-__device__ __forceinline__ uint64_t tail_word(uint64_t acc, uint32_t kept, uint64_t p) {
-  uint32_t j = 0;
-  for (uint64_t t = acc; t >= p && j < 16; t -= p) j++;
-  return kept >= j + 8 ? acc : acc | (1ull << 63);
+// The word a tail chain leaves after steps of which `kept` did not subtract p.  Written
+// acc = rho + j*p, a step maps j to max(j + w, 0) (w in {-1, 0, 1}); over the tail, a start
+// at j0 ends at max(j0 + D, M) and the start at 0 at M, with D = M - kept, so every start
+// j0 <= kept ends where the start at 0 does.  The chain's true start has j0 <= 8 (its word
+// is below 2^64 < 9p), so kept >= 8 settles it; otherwise bit 63 (never set in a chain
+// word, which stays below 2^62) asks for the whole chain.
+__device__ __forceinline__ uint64_t tail_word(uint64_t acc, uint32_t kept) {
+  return kept >= 8 ? acc : acc | (1ull << 63);
 }
 
 // TAIL: the chains continue from the words already in the outputs instead of zero, and a
@@ -447,8 +447,8 @@ __device__ __forceinline__ void batch3_tile(const ModSet &ms, const TileLevel &a
       if ((uint32_t)i < nq) {
         uint64_t *o = obase[i] + (r0 + g) * 2 * mn + cs;
         if constexpr (TAIL) {
-          o[0] = tail_word(acc_a[g][i], nl - ns_a[g][i], p);
-          o[mn] = tail_word(acc_b[g][i], nl - ns_b[g][i], p);
+          o[0] = tail_word(acc_a[g][i], nl - ns_a[g][i]);
+          o[mn] = tail_word(acc_b[g][i], nl - ns_b[g][i]);
         } else {
           o[0] = acc_a[g][i];
           o[mn] = acc_b[g][i];

@@ -15,9 +15,9 @@
 //      reduced mod p: the matrix cores compute it (over whole blocks of 32 elements, the
 //      query digits past E0 zero).
 //   2. The tail's steps run with the reference's step from acc = that residue (j = 0),
-//      counting the steps that do not subtract p (`kept`).  A start at j0 ends at
-//      max(j0 + D, M), a start at 0 at max(D, M) = j with D = j - kept, so the two agree
-//      for every j0 <= 8 when kept >= j + 8 (mac.hip: tail_word).
+//      counting the steps that do not subtract p (`kept`).  Over the tail a start at j0
+//      ends at max(j0 + D, M) and the start at 0 at M, with D = M - kept, so every start
+//      j0 <= kept ends alike: kept >= 8 covers every true start (mac.hip: tail_word).
 //   3. Chains whose tail does not settle this (bit 63 of their word), and every chain of a
 //      query with a word not below its modulus (outside the model: its r_e may leave
 //      [0, 2p)), are recomputed whole with the reference's step (mx_fixup_kernel).
