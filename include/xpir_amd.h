@@ -79,7 +79,22 @@ int xpg_sync(xpg_ctx *ctx);
  * at once (requests are grouped under it; 0 = 24 GiB).  "alpha_zero_pad" = 1: a partial
  * last alpha aggregate holds its real elements and zeros (the default 0 gives it the
  * reference's bytes, see xpg_db_import); this one changes reply bytes when n_elems is
- * not a multiple of alpha. */
+ * not a multiple of alpha.
+ * "db_layout" = 1: databases imported (or loaded) from then on keep level 0 in the matrix
+ * layout, for the int8 matrix cores: each word as eight signed byte digits in the
+ * fragments of v_mfma_i32_32x32x32_i8, plus each group's last 40 elements as NTT words;
+ * the row-major words are freed after the conversion (both exist during it).  Replies
+ * on such a database take level 0 as the exact residue of all but the last 40 products
+ * of each chain (matrix cores), then the reference's step for the last 40 from that
+ * residue; a chain whose last steps cannot settle its excess multiple of p, and every
+ * chain of a query holding a word not below its modulus, is recomputed whole with the
+ * reference's step: the reply bytes are the row-major layout's.  Applies when
+ * 72 <= dims[0] < 2^27 and m*n is a multiple of 256 (other databases keep the row-major
+ * words); the conversion fails with XPG_EINVAL for a word >= 2^63 - 2^56 (none that an
+ * import makes).  Such a database has no row-major words: xpg_db_export_ntt_words,
+ * xpg_db_save and xpg_autotune refuse it (XPG_EINVAL); xpg_reply*, xpg_reply_batch_dev,
+ * xpg_reply_dev_noalloc (when its scratch holds the launch's query fragments) and the
+ * serving loop answer from it.  No reference counterpart. */
 int xpg_ctx_set_option(xpg_ctx *ctx, const char *key, int64_t value);
 
 /* device memory plumbing for callers without a GPU runtime of their own */

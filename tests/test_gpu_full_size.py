@@ -84,19 +84,24 @@ def test_recursive_full_size_matches_golden(gpu_ctx):
         db.free()
 
 
-@pytest.fixture(scope="module")
-def batched(gpu_ctx):
+@pytest.fixture(scope="module", params=[0, 1], ids=["row_major", "matrix_layout"])
+def batched(gpu_ctx, request):
     """The eight batched databases imported at full size and resident (109 GiB of NTT
-    words), with each one's secret key on the device."""
+    words), in the row-major layout or the matrix layout (option db_layout; one layout
+    at a time on the GPU), with each one's secret key on the device."""
     import xpir_amd
     from xpir_amd.workloads import batch_databases
     out = []
-    print("full size: importing the eight batched databases", flush=True)
-    for spec in batch_databases():
-        prm = xpir_amd.params(spec["crypto"], spec["dims"][0])
-        db = _import(gpu_ctx, prm, spec)
-        sk = xpir_amd.keygen(prm, spec["key_seed"])
-        out.append(dict(spec=spec, prm=prm, db=db, skd=gpu_ctx.alloc(sk.nbytes).upload(sk)))
+    print(f"full size: importing the eight batched databases, db_layout {request.param}", flush=True)
+    gpu_ctx.set_option("db_layout", request.param)
+    try:
+        for spec in batch_databases():
+            prm = xpir_amd.params(spec["crypto"], spec["dims"][0])
+            db = _import(gpu_ctx, prm, spec)
+            sk = xpir_amd.keygen(prm, spec["key_seed"])
+            out.append(dict(spec=spec, prm=prm, db=db, skd=gpu_ctx.alloc(sk.nbytes).upload(sk)))
+    finally:
+        gpu_ctx.set_option("db_layout", 0)
     gpu_ctx.sync()
     yield out
     for d in out:
