@@ -1040,7 +1040,7 @@ def bench_serving(args):
         "config": {"workload": "8 resident LWE:80:2048:120 databases, Zipf(1.1) choice, Poisson arrivals, "
                                "d=2 (d in {1,2} on the two smallest)",
                    "scale": args.scale, "workers": args.workers, "max_batch": 64, "reply_pool_bytes": 8 << 30,
-                   "heavy_steps": args.heavy_steps},
+                   "heavy_steps": args.heavy_steps, "db_layout": "matrix" if args.db_layout else "row-major"},
     }
     print(json.dumps(out), flush=True)
     ctx.close()
