@@ -183,13 +183,14 @@ __global__ void mx_clear_kernel(uint32_t *p, uint32_t n) {
 
 // ---- the matrix-core MAC ----
 //
-// A workgroup of four waves takes an item (slot, column block cb, row block rb): TR = WR*MT
-// row tiles x TC = WC*NT column tiles (WR x WC waves, each MT x NT tiles of 32 x 32).  Per
-// block of 32 elements it stages the item's TR + TC fragments in LDS (each wave loads a
-// quarter of them), and every wave reads its MT + NT from there: each fragment leaves L2
-// once per item instead of once per wave.  Items run rb fastest: the row blocks of one
-// column block follow one another on one XCD (its database fragments come from HBM once),
-// and an XCD's items stay within few slots (their query fragments stay in its L2).
+// A workgroup of four waves takes an item (slot, column block cb, row block rb): MT row
+// tiles shared by the four waves and 4 x NT column tiles, NT per wave (tiles of 32 x 32).
+// Per stage of two blocks of 32 elements the row fragments go through LDS (wave w loads
+// row fragment w) and each wave loads its own column fragments straight into registers:
+// LDS carries only what several waves read (its bandwidth, not the matrix cores, bounded
+// a layout whose waves shared both kinds).  Items run rb fastest: the row blocks of one
+// column block follow one another on one XCD (its database fragments come from HBM
+// once), and an XCD's items stay within few slots (their query fragments stay in its L2).
 struct MacArgsMx {
   const uint8_t *A;  // query fragments [S][RTp][EBP]
   const uint8_t *B;  // database fragments [S][UT][EB]
@@ -269,11 +270,10 @@ __device__ __forceinline__ uint64_t tile_epilogue(const v16i &acc, uint32_t lane
   return canon(v + shfl_xor64(v, 32), p, negp);
 }
 
-template <int WR, int MT>
+template <int MT>
 __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
-  constexpr int WC = 4 / WR, NT = kNT, TR = WR * MT, TC = WC * NT, NF = TR + TC, LPT = (NF + 3) / 4;
-  constexpr int KB = kStageBlocks;
-  __shared__ v4i lds[2][KB][NF][64];
+  constexpr int NT = kNT, TR = MT, TC = 4 * NT, KB = kStageBlocks;
+  __shared__ v4i lds[2][KB][MT][64];
   const uint32_t bid = blockIdx.x;
   const uint64_t item = (uint64_t)(bid & 7) * a.per_xcd + (bid >> 3);
   if (item >= a.items) return;  // the whole workgroup
@@ -283,40 +283,39 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
   const uint32_t sl = (uint32_t)(t / a.CB);
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t wr = wave / WC, wc = wave % WC;
   const uint32_t k = sl / a.n;
   const uint64_t p = a.ms.p[k];
   const uint32_t delta = a.ms.delta[k];
-  // the fragments this thread stages: f = wave + 4 i (row tiles first, then column tiles;
-  // column tiles past the database read its last tile, whose results are dropped), each
-  // from a wave-uniform base plus the lane's 32-bit offset (one offset for all of them)
-  const uint8_t *src[LPT];
+  // wave w stages row fragment w (when w < MT) and loads its own NT column fragments
+  // (column tiles past the database read its last tile, whose results are dropped), each
+  // from a wave-uniform base plus the lane's 32-bit offset
+  const bool stager = MT == 4 || wave < (uint32_t)MT;
+  const uint8_t *srcA = a.A + uniform64(((uint64_t)sl * a.RTp + rb * TR + min(wave, (uint32_t)MT - 1)) * a.EBP * 1024);
+  const uint8_t *srcB[NT];
 #pragma unroll
-  for (int i = 0; i < LPT; i++) {
-    const uint32_t f = min((uint32_t)(wave + 4 * i), (uint32_t)NF - 1);
-    const uint64_t off = f < (uint32_t)TR ? ((uint64_t)sl * a.RTp + rb * TR + f) * a.EBP * 1024
-                                          : ((uint64_t)sl * a.UT + min(cb * TC + f - TR, a.UT - 1)) * a.EB * 1024;
-    src[i] = (f < (uint32_t)TR ? a.A : a.B) + uniform64(off);
-  }
-  // every wave stages LPT fragments when four divide NF; otherwise the last round's
-  // fragments past NF are skipped (a wave-uniform test)
-  auto stage_ok = [&](int i) { return NF % 4 == 0 || i + 1 < LPT || wave + 4 * i < (uint32_t)NF; };
+  for (int j = 0; j < NT; j++)
+    srcB[j] = a.B + uniform64(((uint64_t)sl * a.UT + min(cb * TC + wave * NT + j, a.UT - 1)) * a.EB * 1024);
   // a stage is KB consecutive blocks (indices past `last` read block `last` again)
-  auto load = [&](v4i (&r)[KB][LPT], uint32_t eb, uint32_t last) {
+  auto loadA = [&](v4i (&r)[KB], uint32_t eb, uint32_t last) {
+    if (stager) {
+#pragma unroll
+      for (int kb = 0; kb < KB; kb++)
+        r[kb] = *reinterpret_cast<const v4i *>(srcA + min(eb + kb, last) * 1024 + lane * 16);
+    }
+  };
+  auto putA = [&](int buf, const v4i (&r)[KB]) {
+    if (stager) {
+#pragma unroll
+      for (int kb = 0; kb < KB; kb++) lds[buf][kb][wave][lane] = r[kb];
+    }
+  };
+  auto loadB = [&](v4i (&r)[KB][NT], uint32_t eb, uint32_t last) {
 #pragma unroll
     for (int kb = 0; kb < KB; kb++) {
       const uint32_t off = min(eb + kb, last) * 1024 + lane * 16;
 #pragma unroll
-      for (int i = 0; i < LPT; i++)
-        if (stage_ok(i)) r[kb][i] = *reinterpret_cast<const v4i *>(src[i] + off);
+      for (int j = 0; j < NT; j++) r[kb][j] = *reinterpret_cast<const v4i *>(srcB[j] + off);
     }
-  };
-  auto put = [&](int buf, const v4i (&r)[KB][LPT]) {
-#pragma unroll
-    for (int kb = 0; kb < KB; kb++)
-#pragma unroll
-      for (int i = 0; i < LPT; i++)
-        if (stage_ok(i)) lds[buf][kb][wave + 4 * i][lane] = r[kb][i];
   };
   // the epilogue's per-lane weight 2^(8(4h + (lx & ~1))) mod p and its Shoup quotient
   const uint32_t wexp = 8 * (4 * (lane >> 5) + (lane & 6));
@@ -330,47 +329,43 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
     for (int i = 0; i < MT; i++)
 #pragma unroll
       for (int j = 0; j < NT; j++) acc[i][j] = v16i{};
-    // two register sets run two stages ahead of the LDS buffer being read: at the top of
-    // stage eb (blocks eb .. eb + KB - 1), LDS buffer ((eb - c0) / KB) & 1 holds it, one set
-    // the next stage and the other the one after (in flight).  Block indices past the pass
-    // are clamped to its last block (loads without branches, so each wait names only the
-    // older set); such copies are never multiplied.
+    // Row fragments: two register sets run two stages ahead of the LDS buffer being read
+    // (at the top of stage eb, LDS buffer ((eb - c0) / KB) & 1 holds it, one set the next
+    // stage and the other the one after).  Column fragments: two register sets, the
+    // current stage's and the next's, each reloaded two stages ahead once its products
+    // are issued.  Block indices past the pass are clamped to its last block (loads
+    // without branches, so each wait names only the older loads); such copies are never
+    // multiplied.
     const uint32_t last = c1 - 1;
-    v4i R0[KB][LPT], R1[KB][LPT];
-    load(R0, c0, last);
-    put(0, R0);
-    load(R0, c0 + KB, last);
-    load(R1, c0 + 2 * KB, last);
+    v4i RA0[KB], RA1[KB], B0[KB][NT], B1[KB][NT];
+    loadA(RA0, c0, last);
+    loadB(B0, c0, last);
+    putA(0, RA0);
+    loadA(RA0, c0 + KB, last);
+    loadA(RA1, c0 + 2 * KB, last);
+    loadB(B1, c0 + KB, last);
     __syncthreads();
     for (uint32_t eb = c0; eb < c1; eb += 2 * KB) {
 #pragma unroll
       for (int half = 0; half < 2; half++) {
         const uint32_t e0 = eb + half * KB;
-        // the stage's blocks from LDS, each block's fragments read while the previous
-        // block's products run
-        v4i fa[2][MT], fb[2][NT];
-#pragma unroll
-        for (int i = 0; i < MT; i++) fa[0][i] = lds[half][0][wr * MT + i][lane];
-#pragma unroll
-        for (int j = 0; j < NT; j++) fb[0][j] = lds[half][0][TR + wc * NT + j][lane];
+        v4i (&B)[KB][NT] = half ? B1 : B0;
 #pragma unroll
         for (int kb = 0; kb < KB; kb++) {
-          if (kb + 1 < KB) {
-#pragma unroll
-            for (int i = 0; i < MT; i++) fa[(kb + 1) & 1][i] = lds[half][kb + 1][wr * MT + i][lane];
-#pragma unroll
-            for (int j = 0; j < NT; j++) fb[(kb + 1) & 1][j] = lds[half][kb + 1][TR + wc * NT + j][lane];
-          }
           if (e0 + kb < c1) {
+            v4i fa[MT];
+#pragma unroll
+            for (int i = 0; i < MT; i++) fa[i] = lds[half][kb][i][lane];
 #pragma unroll
             for (int i = 0; i < MT; i++)
 #pragma unroll
               for (int j = 0; j < NT; j++)
-                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[kb & 1][i], fb[kb & 1][j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], B[kb][j], acc[i][j], 0, 0, 0);
           }
         }
-        put(half ^ 1, half ? R1 : R0);
-        load(half ? R1 : R0, e0 + 3 * KB, last);
+        putA(half ^ 1, half ? RA1 : RA0);
+        loadA(half ? RA1 : RA0, e0 + 3 * KB, last);
+        loadB(B, e0 + 2 * KB, last);
         __syncthreads();
       }
     }
@@ -474,8 +469,8 @@ struct WgShape {
 };
 WgShape shape_for(uint32_t qn) {
   const uint32_t RT = (qn + 1) / 2;
-  const int WR = RT > 4 ? 2 : 1, MT = RT > 2 ? 4 : (int)RT;
-  return WgShape{WR, MT, (uint32_t)(WR * MT), (uint32_t)((4 / WR) * kNT)};
+  const int MT = RT > 2 ? 4 : (int)RT;
+  return WgShape{1, MT, (uint32_t)MT, (uint32_t)(4 * kNT)};
 }
 
 }  // namespace
@@ -599,10 +594,9 @@ int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const 
     // the timed kernel's chain steps: the prefixes (the tails run after it)
     if (ctx->timing && !ctx->ws) ctx->batch_steps += (uint64_t)qn * g.U * g.S * 2 * g.E0;
     const dim3 grid((uint32_t)blocks), wg(256);
-    if (w.WR == 2) hipLaunchKernelGGL((mx_mac_kernel<2, 4>), grid, wg, 0, ctx->stream, a);
-    else if (w.MT == 4) hipLaunchKernelGGL((mx_mac_kernel<1, 4>), grid, wg, 0, ctx->stream, a);
-    else if (w.MT == 2) hipLaunchKernelGGL((mx_mac_kernel<1, 2>), grid, wg, 0, ctx->stream, a);
-    else hipLaunchKernelGGL((mx_mac_kernel<1, 1>), grid, wg, 0, ctx->stream, a);
+    if (w.MT == 4) hipLaunchKernelGGL((mx_mac_kernel<4>), grid, wg, 0, ctx->stream, a);
+    else if (w.MT == 2) hipLaunchKernelGGL((mx_mac_kernel<2>), grid, wg, 0, ctx->stream, a);
+    else hipLaunchKernelGGL((mx_mac_kernel<1>), grid, wg, 0, ctx->stream, a);
     timing_end(ctx, k0, k1, 2);
   }
   {
@@ -615,8 +609,7 @@ int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const 
     a.RB = RB;
     a.CB = CB;
     const dim3 grid((uint32_t)((uint64_t)(g.S / kUnstageSlots) * CB * RB)), wg(256);
-    if (w.WR == 2) hipLaunchKernelGGL((mx_unstage_kernel<2, 4>), grid, wg, 0, ctx->stream, a);
-    else if (w.MT == 4) hipLaunchKernelGGL((mx_unstage_kernel<1, 4>), grid, wg, 0, ctx->stream, a);
+    if (w.MT == 4) hipLaunchKernelGGL((mx_unstage_kernel<1, 4>), grid, wg, 0, ctx->stream, a);
     else if (w.MT == 2) hipLaunchKernelGGL((mx_unstage_kernel<1, 2>), grid, wg, 0, ctx->stream, a);
     else hipLaunchKernelGGL((mx_unstage_kernel<1, 1>), grid, wg, 0, ctx->stream, a);
   }
