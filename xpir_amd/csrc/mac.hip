@@ -112,6 +112,62 @@ __global__ void __launch_bounds__(256) mac_direct_multi_kernel(MultiArgs m) {
   o[mn] = acc_b;
 }
 
+// The same launch with one lane per (group, slot) carrying all PR polys of the group: the
+// element's query words are loaded and their Shoup quotients derived once for the PR polys
+// (one chain per lane reloads them per poly); the next element's words load while this
+// one's steps run.  Each chain still takes its elements in order through the same step.
+template <int PR>
+__global__ void __launch_bounds__(256) mac_multi_rows_kernel(MultiArgs m) {
+  const MacArgs &a = m.base;
+  const uint64_t mn = (uint64_t)a.m * a.n;
+  const uint64_t lane = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (lane >= a.groups * mn) return;
+  const uint64_t cs = lane % mn;
+  const uint32_t c = (uint32_t)(cs >> a.logn);
+  const uint64_t g = lane / mn;
+  const uint64_t req = g / m.gpr;
+  const uint64_t *q = m.q[req] + cs;
+  const uint64_t p = a.ms.p[c];
+  const uint32_t delta = a.ms.delta[c];
+  const StepConsts kc = step_consts(p, delta);
+  const uint64_t *x = a.db + g * a.n_l * PR * mn + cs;  // element e, poly k at (e * PR + k) * mn
+  uint64_t acc_a[PR], acc_b[PR];
+#pragma unroll
+  for (int k = 0; k < PR; k++) acc_a[k] = acc_b[k] = 0;
+  uint64_t xv[2][PR], ya[2], yb[2];
+  auto load = [&](int b, uint64_t e) {
+#pragma unroll
+    for (int k = 0; k < PR; k++) xv[b][k] = __builtin_nontemporal_load(x + (e * PR + k) * mn);
+    ya[b] = q[e * 2 * mn];
+    yb[b] = q[e * 2 * mn + mn];
+  };
+  auto steps = [&](int b) {
+    const uint64_t yaq = shoup_quot(ya[b], p, delta), ybq = shoup_quot(yb[b], p, delta);
+#pragma unroll
+    for (int k = 0; k < PR; k++) {
+      mac_step_asm(acc_a[k], xv[b][k], ya[b], yaq, kc);
+      mac_step_asm(acc_b[k], xv[b][k], yb[b], ybq, kc);
+    }
+  };
+  const uint64_t nl = a.n_l;
+  load(0, 0);
+  uint64_t e = 0;
+  for (; e + 2 <= nl; e += 2) {
+    load(1, e + 1);
+    steps(0);
+    if (e + 2 < nl) load(0, e + 2);
+    steps(1);
+  }
+  if (e < nl) steps(0);
+  const uint64_t first = (g - req * m.gpr) * PR;
+#pragma unroll
+  for (int k = 0; k < PR; k++) {
+    uint64_t *o = m.per_req_out ? m.out[req] + (first + k) * 2 * mn : a.out + (g * PR + k) * 2 * mn;
+    o[cs] = acc_a[k];
+    o[cs + mn] = acc_b[k];
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // Levels with several groups, one or many queries.
 constexpr int kMaxBatch = 64;  // queries per launch; pointers travel in the kernel arguments
@@ -1368,7 +1424,14 @@ int launch_mac_level_multi(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *
   timing_begin(ctx, &e0, &e1);
   hipEvent_t k0 = nullptr, k1 = nullptr;
   timing_begin(ctx, &k0, &k1, 1);
-  hipLaunchKernelGGL(mac_direct_multi_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, m);
+  // every poly of a group in one lane where the poly count has a build (a lane per group
+  // and slot), else one chain pair per lane
+  const uint64_t row_blocks = (m.base.groups * (uint64_t)prm->m * prm->n + 255) / 256;
+  switch (P) {
+    case 6: hipLaunchKernelGGL(mac_multi_rows_kernel<6>, dim3((uint32_t)row_blocks), dim3(256), 0, ctx->stream, m); break;
+    case 12: hipLaunchKernelGGL(mac_multi_rows_kernel<12>, dim3((uint32_t)row_blocks), dim3(256), 0, ctx->stream, m); break;
+    default: hipLaunchKernelGGL(mac_direct_multi_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, m); break;
+  }
   timing_end(ctx, k0, k1, 1);
   timing_end(ctx, e0, e1);
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "grouped MAC launch");

@@ -431,33 +431,45 @@ struct FixArgs {
   uint32_t S, n, U, UT, EB, P;
 };
 
-// block: 256 consecutive slots of one output row (query q, unit u, half s); the row
-// index is decoded once per block
+// block: 1024 consecutive slots of one output row (query q, unit u, half s), four per
+// thread; the row index is decoded once per block.  A marked word, or
+// every word of a query with a word not below its modulus, takes the whole chain.
+constexpr uint32_t kFixSlots = 1024;
+
 __global__ void __launch_bounds__(256) mx_fixup_kernel(FixArgs a) {
-  const uint32_t blocks_per_row = a.S / 256;
+  const uint32_t blocks_per_row = (a.S + kFixSlots - 1) / kFixSlots;
   const uint64_t row = blockIdx.x / blocks_per_row;
-  const uint32_t sl = (uint32_t)(blockIdx.x % blocks_per_row) * 256 + threadIdx.x;
+  const uint32_t sl0 = (uint32_t)(blockIdx.x % blocks_per_row) * kFixSlots + threadIdx.x * 4;
+  if (sl0 >= a.S) return;
   const uint32_t s = (uint32_t)(row & 1);
   const uint32_t u = (uint32_t)((row >> 1) % a.U), q = (uint32_t)((row >> 1) / a.U);
   const uint64_t mn = a.S;
-  uint64_t *o = a.out[q] + ((uint64_t)u * 2 + s) * mn + sl;
-  if (!(*o >> 63) && !a.qbad[q]) return;
-  const uint32_t k = sl / a.n;
-  const uint64_t p = a.ms.p[k];
-  const uint32_t delta = a.ms.delta[k];
-  const uint8_t *f0 = a.frag + ((uint64_t)sl * a.UT + u / 4) * a.EB * 1024 + 16 * ((u % 4) * 8);
-  const uint64_t *y = a.q[q] + (uint64_t)s * mn + sl;
-  uint64_t acc = 0;
-  for (uint64_t e = 0; e < a.n_l; e++) {
-    const uint8_t *f = f0 + (e / 32) * 1024 + 16 * 32 * ((e % 32) / 16) + (e % 16);
-    uint64_t b = 0;
+  uint64_t *o = a.out[q] + ((uint64_t)u * 2 + s) * mn + sl0;
+  const bool all = a.qbad[q] != 0;
+  uint64_t w[4];  // (caller buffers are 8-byte aligned: four word loads)
 #pragma unroll
-    for (int j = 0; j < 8; j++) b |= (uint64_t)f[16 * j] << (8 * j);
-    const uint64_t x = (b ^ kDigitBias) - kDigitBias;
-    const uint64_t yv = y[e * 2 * mn];
-    acc = mac_step(acc, x, yv, shoup_quot(yv, p, delta), p, delta);
+  for (int i = 0; i < 4; i++) w[i] = sl0 + i < a.S ? o[i] : 0;
+  if (!((w[0] | w[1] | w[2] | w[3]) >> 63) && !all) return;
+  for (int i = 0; i < 4; i++) {
+    if (sl0 + i >= a.S || (!(w[i] >> 63) && !all)) continue;
+    const uint32_t sl = sl0 + i;
+    const uint32_t k = sl / a.n;
+    const uint64_t p = a.ms.p[k];
+    const uint32_t delta = a.ms.delta[k];
+    const uint8_t *f0 = a.frag + ((uint64_t)sl * a.UT + u / 4) * a.EB * 1024 + 16 * ((u % 4) * 8);
+    const uint64_t *y = a.q[q] + (uint64_t)s * mn + sl;
+    uint64_t acc = 0;
+    for (uint64_t e = 0; e < a.n_l; e++) {
+      const uint8_t *f = f0 + (e / 32) * 1024 + 16 * 32 * ((e % 32) / 16) + (e % 16);
+      uint64_t b = 0;
+#pragma unroll
+      for (int j = 0; j < 8; j++) b |= (uint64_t)f[16 * j] << (8 * j);
+      const uint64_t x = (b ^ kDigitBias) - kDigitBias;
+      const uint64_t yv = y[e * 2 * mn];
+      acc = mac_step(acc, x, yv, shoup_quot(yv, p, delta), p, delta);
+    }
+    o[i] = acc;
   }
-  *o = acc;
 }
 
 uint64_t al256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
@@ -637,7 +649,7 @@ int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const 
     a.EB = g.EB;
     a.P = g.P;
     a.total = (uint64_t)qn * g.U * 2 * g.S;
-    const uint64_t blocks = (a.total + 255) / 256;
+    const uint64_t blocks = (uint64_t)qn * g.U * 2 * ((g.S + kFixSlots - 1) / kFixSlots);
     if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "fixup grid too large");
     hipLaunchKernelGGL(mx_fixup_kernel, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
   }
