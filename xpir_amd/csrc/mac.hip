@@ -186,6 +186,9 @@ struct Batch2Args {
   uint64_t *out[kMaxBatch];
   uint64_t n_l, groups, colgroups;  // colgroups = row blocks * m * sblocks
   uint32_t P, n, m, sblocks, qn, qblocks;
+  // tail chains: their starting residues in the matrix kernel's staging (null: in out)
+  const uint64_t *stage;
+  uint32_t st_cb, st_rb, st_mt, qfirst;  // qfirst: the launch's first query in the chunk
 };
 
 // Word at byte offset `lane_off` from a wave-uniform base: written this way the load
@@ -344,6 +347,10 @@ struct TileLevel {
   const uint64_t *db;
   uint64_t n_l, groups;
   uint32_t P, n, m, sblocks, qn;
+  // TAIL: the matrix kernel's staged residues (items of CB column x RB row blocks, MT row
+  // tiles per wave) and the tile's first query; null: the residues are in the outputs
+  const uint64_t *stage = nullptr;
+  uint32_t st_cb = 0, st_rb = 0, st_mt = 0, q0 = 0;
 };
 
 // The tile (colgroup cg = (row block, column), query block qb) of one level: every wave
@@ -414,9 +421,22 @@ __device__ __forceinline__ void batch3_tile(const ModSet &ms, const TileLevel &a
       acc_a[g][i] = acc_b[g][i] = 0;
       ns_a[g][i] = ns_b[g][i] = 0;
       if (TAIL && g < rv && (uint32_t)i < nq) {
-        const uint64_t *o = obase[i] + (r0 + g) * 2 * mn + cs;
-        acc_a[g][i] = o[0];
-        acc_b[g][i] = o[mn];
+        if (a.stage) {
+          // the residue of (slot cs, unit u, query q, half) in its staged item: item
+          // (slot, unit block of 32, query block of 2 MT), within it wave (u % 32) / 8,
+          // row tile (q / 2) % MT, column tile (u % 8) / 4, row group (q % 2) * 2 + half,
+          // unit u % 4 (mx.hip: the matrix kernel's epilogue)
+          const uint32_t u = (uint32_t)(r0 + g), q = a.q0 + i, mt = a.st_mt;
+          const uint64_t item = ((uint64_t)cs * a.st_cb + u / 32) * a.st_rb + (q / 2) / mt;
+          const uint64_t w = item * (mt * 128) + ((u % 32) / 8) * (mt * 32) + (((q / 2) % mt) * 2 + (u % 8) / 4) * 16 +
+                             (q % 2) * 8 + (u % 4);
+          acc_a[g][i] = a.stage[w];
+          acc_b[g][i] = a.stage[w + 4];
+        } else {
+          const uint64_t *o = obase[i] + (r0 + g) * 2 * mn + cs;
+          acc_a[g][i] = o[0];
+          acc_b[g][i] = o[mn];
+        }
       }
     }
   auto step2 = [&](int g, int i, uint64_t x, uint64_t ya, uint64_t yaq, uint64_t yb, uint64_t ybq) {
@@ -520,8 +540,9 @@ __global__ void __launch_bounds__(64 * W) mac_batch3_kernel(Batch2Args a) {
   uint64_t cg;
   uint32_t qb;
   if (!batch_tile(a, blockIdx.x, cg, qb)) return;  // the whole workgroup
-  const TileLevel lv{a.db, a.n_l, a.groups, a.P, a.n, a.m, a.sblocks, a.qn};
   const uint32_t q0 = qb * QR;
+  const TileLevel lv{a.db,    a.n_l,    a.groups, a.P,     a.n,     a.m,           a.sblocks,
+                     a.qn,    a.stage,  a.st_cb,  a.st_rb, a.st_mt, a.qfirst + q0};
   const uint64_t *qbase[QR];
   uint64_t *obase[QR];
 #pragma unroll
@@ -1249,16 +1270,18 @@ int launch_mac_batch_cfg(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db
 // tail element.  Register blocks of 4 rows x QR queries with the query words shared
 // through LDS (mac_batch3_kernel in its TAIL form).
 int launch_mac_tail(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *tail_db, const uint64_t *const *q,
-                    uint64_t *const *out, uint32_t qn, uint64_t L, uint64_t groups, uint32_t P) {
+                    uint64_t *const *out, uint32_t qn, uint64_t L, uint64_t groups, uint32_t P, const TailStage &ts,
+                    uint32_t qfirst) {
   if (qn == 0 || qn > (uint32_t)kMaxBatch) return set_err(ctx, XPG_EINVAL, "batch of 1..64 queries per launch");
   if (prm->n < 64) return set_err(ctx, XPG_EINVAL, "MAC kernel needs n >= 64");
   constexpr int GR = 4, W = 4;
   const uint32_t QR = qn >= 4 ? 4 : qn >= 2 ? 2 : 1;
   const uint32_t rest = qn % QR;
   if (rest && qn > QR) {
-    const int rc = launch_mac_tail(ctx, prm, tail_db, q, out, qn - rest, L, groups, P);
+    const int rc = launch_mac_tail(ctx, prm, tail_db, q, out, qn - rest, L, groups, P, ts, qfirst);
     if (rc) return rc;
-    return launch_mac_tail(ctx, prm, tail_db, q + (qn - rest), out + (qn - rest), rest, L, groups, P);
+    return launch_mac_tail(ctx, prm, tail_db, q + (qn - rest), out + (qn - rest), rest, L, groups, P, ts,
+                           qfirst + (qn - rest));
   }
   Batch2Args b2{};
   b2.ms = modset_of(prm);
@@ -1276,6 +1299,11 @@ int launch_mac_tail(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *tail_db
   b2.qn = qn;
   b2.qblocks = (qn + QR - 1) / QR;
   b2.colgroups = ((groups * P + W * GR - 1) / (W * GR)) * prm->m * b2.sblocks;
+  b2.stage = ts.stage;
+  b2.st_cb = ts.cb;
+  b2.st_rb = ts.rb;
+  b2.st_mt = ts.mt;
+  b2.qfirst = qfirst;
   const uint64_t blocks = ((b2.colgroups + 7) / 8) * 8 * b2.qblocks;
   if (blocks > 0x7fffffffULL) return set_err(ctx, XPG_EINVAL, "tail MAC grid too large");
   const dim3 wg(64 * W), grid((uint32_t)blocks);

@@ -383,43 +383,6 @@ __global__ void __launch_bounds__(256, 2) mx_mac_kernel(MacArgsMx a) {
   }
 }
 
-// ---- staged residues to the level's outputs ----
-struct UnstageArgs {
-  const uint64_t *stage;
-  uint64_t *out[kMaxQ];  // (u*2 + s)*m*n + sl
-  uint32_t S, U, qn, RB, CB;
-};
-
-// block (8-slot block sb, column block cb, row block rb): the 8 items' words through LDS,
-// out as rows of 8 consecutive slots; the item layout (WR x WC waves of MT x NT tiles of
-// 4 x 4 words) is a template so that its decoding is shifts and masks
-constexpr uint32_t kUnstageSlots = 8;
-
-template <int WR, int MT>
-__global__ void __launch_bounds__(256) mx_unstage_kernel(UnstageArgs a) {
-  constexpr uint32_t WC = 4 / WR, TR = WR * MT, TC = WC * kNT, PER_WAVE = MT * kNT * 16, WPI = 4 * PER_WAVE;
-  __shared__ uint64_t t[kUnstageSlots][WPI + 1];
-  const uint32_t rb = blockIdx.x % a.RB;
-  const uint32_t r = blockIdx.x / a.RB;
-  const uint32_t cb = r % a.CB, sb = r / a.CB;
-  for (uint32_t i = threadIdx.x; i < kUnstageSlots * WPI; i += 256) {
-    const uint32_t s = i / WPI, w = i % WPI;
-    const uint64_t item = ((uint64_t)(sb * kUnstageSlots + s) * a.CB + cb) * a.RB + rb;
-    t[s][w] = a.stage[item * WPI + w];
-  }
-  __syncthreads();
-  const uint64_t mn = a.S;
-  for (uint32_t i = threadIdx.x; i < kUnstageSlots * WPI; i += 256) {
-    const uint32_t w = i / kUnstageSlots, s = i % kUnstageSlots;
-    const uint32_t wave = w / PER_WAVE, rem = w % PER_WAVE;
-    const uint32_t ul = rem % 4, g = (rem / 4) % 4, j = (rem / 16) % kNT, ti = rem / (16 * kNT);
-    const uint32_t rt = rb * TR + (wave / WC) * MT + ti, ct = cb * TC + (wave % WC) * kNT + j;
-    const uint32_t q = 2 * rt + (g >> 1), half = g & 1;
-    const uint32_t u = 4 * ct + ul;
-    if (q < a.qn && u < a.U) a.out[q][((uint64_t)u * 2 + half) * mn + sb * kUnstageSlots + s] = t[s][w];
-  }
-}
-
 // ---- whole chains where the tail did not settle them ----
 struct FixArgs {
   const uint8_t *frag;       // database fragments [S][UT][EB]
@@ -611,25 +574,17 @@ int launch_mx_level(xpg_ctx *ctx, const xpg_params *prm, const MxGeom &g, const 
     else hipLaunchKernelGGL((mx_mac_kernel<1>), grid, wg, 0, ctx->stream, a);
     timing_end(ctx, k0, k1, 2);
   }
-  {
-    UnstageArgs a{};
-    a.stage = stage;
-    for (uint32_t i = 0; i < qn; i++) a.out[i] = out[i];
-    a.S = g.S;
-    a.U = g.U;
-    a.qn = qn;
-    a.RB = RB;
-    a.CB = CB;
-    const dim3 grid((uint32_t)((uint64_t)(g.S / kUnstageSlots) * CB * RB)), wg(256);
-    if (w.MT == 4) hipLaunchKernelGGL((mx_unstage_kernel<1, 4>), grid, wg, 0, ctx->stream, a);
-    else if (w.MT == 2) hipLaunchKernelGGL((mx_unstage_kernel<1, 2>), grid, wg, 0, ctx->stream, a);
-    else hipLaunchKernelGGL((mx_unstage_kernel<1, 1>), grid, wg, 0, ctx->stream, a);
-  }
   if (hipGetLastError() != hipSuccess) return set_err(ctx, XPG_EDEVICE, "matrix MAC launch");
   {
     std::vector<const uint64_t *> qt(qn);
     for (uint32_t i = 0; i < qn; i++) qt[i] = q[i] + g.E0 * 2 * (uint64_t)g.S;
-    const int rc = launch_mac_tail(ctx, prm, tail, qt.data(), out, qn, g.L, g.groups, g.P);
+    // the tails start from the staged residues (no pass of their own to the outputs)
+    TailStage ts;
+    ts.stage = stage;
+    ts.cb = CB;
+    ts.rb = RB;
+    ts.mt = (uint32_t)w.MT;
+    const int rc = launch_mac_tail(ctx, prm, tail, qt.data(), out, qn, g.L, g.groups, g.P, ts, 0);
     if (rc) return rc;
   }
   {
