@@ -265,12 +265,12 @@ int launch_mac_fused(xpg_ctx *ctx, const FusedPart *parts, uint32_t nparts, void
 int launch_mac_level_multi(xpg_ctx *ctx, const xpg_params *prm, const uint64_t *db, const uint64_t *const *q,
                            uint32_t nreq, uint64_t n_l, uint64_t gpr, uint32_t P, uint64_t *const *out_per_req,
                            uint64_t *out);
-// The last L steps of level-0 chains continued from residues the matrix-core level 0
-// computed (mac.hip): tail_db holds each group's last L elements (groups*L rows x P x m x
-// n), q[i] query i's first tail element; the words go to out.
+// (mac.hip) The last L steps of level-0 chains continued from residues the matrix-core
+// level 0 computed (mx.hip): tail_db holds each group's last L elements (groups*L rows x
+// P x m x n), q[i] query i's first tail element; the words go to out.
 // The chains' starting residues come from the matrix kernel's staging buffer when
-// stage.stage is set (items of cb x rb blocks, mt row tiles per wave; qfirst: the index of
-// q[0] in the staged launch), else from out.
+// stage.stage is set (items of cb column blocks x rb row blocks, each wave holding mt row
+// tiles; qfirst: the index of q[0] in the staged launch), else from out.
 struct TailStage {
   const uint64_t *stage = nullptr;
   uint32_t cb = 0, rb = 0, mt = 0;
